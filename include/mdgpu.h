@@ -1,0 +1,123 @@
+/*
+ * mdgpu.h — C ABI of the MI355X (gfx950) hot path for MDOODZ 7.0.
+ *
+ * One context owns all device buffers for one grid.  Every call is
+ * synchronous on return (the context's stream is synchronised) and returns
+ * MDG_OK or a negative MDG_ERR_* code; mdg_last_error() gives the message.
+ * Array arguments may be host pointers or device pointers: the library
+ * detects device allocations and then works in place without copies.
+ *
+ * Layouts are MDOODZ's own (row-major, x index fastest):
+ *   vertices Nx*Nz, centres (Nx-1)*(Nz-1), Vx nodes Nx*(Nz+1),
+ *   Vz nodes (Nx+1)*Nz;   Nx, Nz count vertices.
+ * Boundary tags are MDOODZ's (Setup.c:206-230): -1 free, 0 and 11 Dirichlet,
+ * 13 free slip, 2 stress, -2/-12 periodic, 30 air, 31 surface pressure.
+ * References below are to files of the MDOODZ 7.0 source tree (MDLIB/).
+ */
+#ifndef MDGPU_H
+#define MDGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDG_OK               0
+#define MDG_ERR_ARG         -1   /* bad argument or size                    */
+#define MDG_ERR_HIP         -2   /* HIP runtime / allocation failure        */
+#define MDG_ERR_UNSUPPORTED -3   /* option outside the implemented subset   */
+#define MDG_ERR_STATE       -4   /* call order (e.g. no markers set)        */
+#define MDG_ERR_NOT_SPD     -5   /* non-positive diagonal (Solvers.c:1246)  */
+#define MDG_ERR_DIVERGED    -6   /* solver did not reach its tolerance      */
+#define MDG_ERR_NAN         -7   /* NaN residual (StokesRoutines.c:947-953) */
+
+typedef struct mdg_ctx mdg_ctx;
+
+/* Grid description: replaces the coordinate set-up of SetGridCoordinates
+ * (GridRoutines.c:246-330); coordinates are built as the same running sums. */
+typedef struct {
+    int    Nx, Nz;                 /* vertices                               */
+    double xmin, xmax, zmin, zmax;
+    int    periodic_x;             /* must be 0 in this build                */
+} mdg_grid_desc;
+
+/* Context lifetime; device 0 of the calling process. */
+int         mdg_create(const mdg_grid_desc *desc, mdg_ctx **out);
+void        mdg_destroy(mdg_ctx *ctx);
+const char *mdg_last_error(const mdg_ctx *ctx);
+
+/* Device memory owned by the caller but allocated through the library's HIP
+ * runtime, for callers that keep marker and grid arrays resident on the GPU
+ * between calls (the reference has no counterpart: its arrays are host
+ * arrays).  mdg_memcpy copies in any direction and is synchronous. */
+int mdg_dev_alloc(mdg_ctx *ctx, unsigned long long bytes, void **ptr);
+int mdg_dev_free(mdg_ctx *ctx, void *ptr);
+int mdg_memcpy(mdg_ctx *ctx, void *dst, const void *src, unsigned long long bytes);
+
+/* Coordinates as computed by the library (host copies), lengths Nx, Nz,
+ * Nx-1, Nz-1, Nx+1, Nz+1.  Mirrors mesh->xg_coord ... (GridRoutines.c:246-330). */
+int mdg_grid_coords(const mdg_ctx *ctx, double *xg, double *zg, double *xc, double *zc,
+                    double *xvz, double *zvx);
+
+/* Boundary tags and values on the four node sets (mesh->BCu/BCv/BCp/BCg,
+ * filled by SetBCs, Setup.c:206-~784).  BCu_v/BCv_v hold 2*value for tag 11
+ * as SetBCs stores them. */
+int mdg_set_bc(mdg_ctx *ctx, const signed char *BCu_t, const double *BCu_v,
+               const signed char *BCv_t, const double *BCv_v, const signed char *BCp_t,
+               const signed char *BCg_t);
+
+/* ------------------------------------------------------------------ markers
+ * Replaces the marker arrays handed to P2Mastah (the `markers` struct of
+ * mdoodz-private.h:30-49: x, z, phase, Nb_part).  The library keeps a
+ * device copy sorted by (cell, quarter cell); phase == -1 marks inactive
+ * markers (ParticleRoutines.c:2292).  Capacity grows as needed.           */
+int mdg_markers_set(mdg_ctx *ctx, int n, const double *x, const double *z, const int *phase);
+
+/* Marker -> node index of ParticleRoutines.c:2300-2310 for node set
+ * `centroid` (1 centres, 0 vertices, -1 Vx, -2 Vz), in the caller's marker
+ * order.  Bit-exact with the reference expression ceil(d/dx + 0.5) - 1. */
+int mdg_marker_index(mdg_ctx *ctx, int centroid, int *ip, int *jp);
+
+/* Device copy of a per-marker field in the library's sorted order, for
+ * repeated P2G calls on device-resident data (MDG_SRC_SORTED below).
+ * src is in the caller's marker order; dst must be a device array of n. */
+int mdg_markers_gather(mdg_ctx *ctx, const double *src, double *dst_sorted);
+
+/* ---------------------------------------------------------------------- P2G
+ * P2Mastah (ParticleRoutines.c:2170-2665, prototype mdoodz-private.h:350):
+ *   flag 0: mat_prop is a per-phase table (mat_prop[phase]);
+ *   flag 1: mat_prop is a per-marker array in the caller's marker order
+ *           (or, with flag MDG_SRC_SORTED added, in the library's sorted
+ *           order, device resident);
+ *   avg 0/1/2: arithmetic / harmonic / geometric weighted mean;
+ *   prop 1: phase fractions into phase_perc (nb_phases node arrays,
+ *           centroid 0 or 1), node_field unused;
+ *   centroid 1/0/-1/-2, interp_stencil 1 or 9.
+ * Nodes with total weight < 1e-30 or tagged 30/31 in node_type keep their
+ * previous value.  node_type may be NULL (no tag test).  Results agree with
+ * the reference to round-off (summation order differs), not bit for bit. */
+#define MDG_SRC_SORTED 0x100
+int mdg_p2mastah(mdg_ctx *ctx, const double *mat_prop, int n_mat_prop, double *node_field,
+                 const signed char *node_type, int flag, int avg, int prop, int centroid,
+                 int interp_stencil, int nb_phases, double *const *phase_perc);
+
+/* P2Mastah_Fused (ParticleRoutines.c:1895-2163, field descriptor
+ * mdoodz-private.h:311-319): nfields (<= 16) fields, one marker pass. */
+typedef struct {
+    const double      *src;        /* table (flag 0) or per-marker (flag 1) */
+    int                n_src;      /* entries in src                        */
+    double            *dst;        /* node array                            */
+    const signed char *node_type;  /* may be NULL                           */
+    int                flag;       /* 0, 1, or 1|MDG_SRC_SORTED             */
+    int                avg;        /* 0, 1, 2                               */
+} mdg_p2g_field;
+int mdg_p2mastah_fused(mdg_ctx *ctx, int centroid, int interp_stencil,
+                       const mdg_p2g_field *fields, int nfields);
+
+/* Time of the last P2G kernel (ms, HIP events on the context stream) and
+ * the number of markers it processed. */
+int mdg_last_kernel_ms(const mdg_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

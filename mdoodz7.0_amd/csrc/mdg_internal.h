@@ -1,0 +1,99 @@
+// Internal state of an mdg_ctx and helpers shared by the HIP sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mdgpu.h"
+
+// Device buffer with capacity growth; freed with the context.
+template <typename T>
+struct dbuf {
+    T     *p   = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, (n ? n : 1) * sizeof(T));
+        if (e == hipSuccess) cap = n;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct mdg_ctx {
+    // grid
+    int    Nx = 0, Nz = 0, Ncx = 0, Ncz = 0;
+    double dx = 0, dz = 0, xmin = 0, xmax = 0, zmin = 0, zmax = 0;
+    std::vector<double> h_xg, h_zg, h_xc, h_zc, h_xvz, h_zvx;
+    dbuf<double> xg, zg, xc, zc, xvz, zvx;
+    // boundary tags / values
+    dbuf<signed char> BCu_t, BCv_t, BCp_t, BCg_t;
+    dbuf<double>      BCu_v, BCv_v;
+    bool              have_bc = false;
+    // markers, sorted by (cell, quarter cell)
+    int               n_markers = 0;
+    dbuf<double>      mx, mz;        // sorted coordinates
+    dbuf<int>         mphase;        // sorted phase
+    dbuf<int>         perm;          // sorted slot -> caller index
+    dbuf<unsigned>    qoff;          // 4*ncells+1 quarter-cell offsets
+    dbuf<unsigned>    keys, keys2;   // sort scratch
+    dbuf<int>         idx;           // sort scratch
+    dbuf<unsigned char> sort_tmp;
+    // P2G scratch
+    dbuf<double>      field_tmp;     // per-marker field in sorted order
+    dbuf<double>      node_tmp;      // device node arrays for host callers
+    dbuf<double>      table_tmp;     // per-phase tables
+    dbuf<signed char> ntype_tmp;
+    dbuf<int>         flag_dev;      // stray / status flags
+    // runtime
+    hipStream_t  stream = nullptr;
+    hipEvent_t   ev0 = nullptr, ev1 = nullptr;
+    double       last_ms = 0.0;
+    std::string  err;
+};
+
+int mdg_fail(mdg_ctx *c, int code, const char *fmt, ...);
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return mdg_fail((ctx), MDG_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// True when p is a device (or managed) allocation visible to the GPU.
+bool mdg_is_device_ptr(const void *p);
+
+// Copy n elements of a possibly-host array into a device buffer (or return
+// the device pointer itself when it already is one).
+template <typename T>
+int mdg_stage_in(mdg_ctx *c, const T *src, size_t n, dbuf<T> &tmp, const T **out) {
+    if (!src) { *out = nullptr; return MDG_OK; }
+    if (mdg_is_device_ptr(src)) { *out = src; return MDG_OK; }
+    HIPCHK(c, tmp.reserve(n));
+    HIPCHK(c, hipMemcpyAsync(tmp.p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    *out = tmp.p;
+    return MDG_OK;
+}
+
+static inline int mdg_grid1d(long n, int block) { return (int)((n + block - 1) / block); }
+
+// Node-set geometry for P2Mastah's `centroid` argument (ParticleRoutines.c:2190-2213).
+struct node_set {
+    int nx, nz;
+    const double *X, *Z;     // device coordinate vectors
+    double X0, Z0;           // first coordinates (host copies)
+};
+int mdg_node_set(mdg_ctx *c, int centroid, node_set *ns);
+
+// Frees the solver's device state (defined with the solver).
+void mdg_solver_release(mdg_ctx *c);

@@ -1,0 +1,320 @@
+// Marker -> grid interpolation (P2Mastah, ParticleRoutines.c:2170-2665 and
+// P2Mastah_Fused, ParticleRoutines.c:1895-2163) on cell-sorted markers.
+//
+// One thread owns one node and walks the quarter-cell ranges of the markers
+// whose nearest node it is (plus the 3x3 neighbour homes for the 9-cell
+// stencil), summing weights and weighted values in a fixed order: no
+// atomics, bit-reproducible from run to run.  Each marker's nearest node is
+// recomputed with the reference expression; a marker whose exact node differs
+// from the one its sort key implies (only possible within round-off of a
+// cell edge for the Vx/Vz node sets) raises a flag and the call is redone
+// with a one-node wider search that tests every marker exactly.
+#include "mdg_internal.h"
+#include "p2g_common.h"
+
+#define P2G_MAXF 4
+#define P2G_MAXPH 32
+#define P2G_MAXTAB 1024
+
+struct p2g_fld {
+    const double      *src;   // per-phase table (flag 0) or sorted per-marker (flag 1)
+    double            *dst;
+    const signed char *ntype;
+    int                flag, avg, n_src;
+};
+struct p2g_set {
+    p2g_fld f[P2G_MAXF];
+};
+
+struct p2g_geo {
+    int nx, nz, Ncx, Ncz;
+    p2g_axes ax;
+    const double *X, *Z;
+    double X0, Z0, dx, dz, dx_itp, dz_itp;
+    int R, S;   // stencil radius (0: 1-cell, 1: 9-cell) and search radius
+};
+
+template <typename F>
+__device__ __forceinline__ void p2g_visit(const p2g_geo &g, const unsigned *__restrict__ qoff,
+                                          const double *__restrict__ mx,
+                                          const double *__restrict__ mz, int k, int l,
+                                          int *__restrict__ stray, F &&acc) {
+    for (int b = -g.S; b <= g.S; b++) {
+        const int hl = l + b;
+        if (hl < 0 || hl >= g.nz) continue;
+        for (int a = -g.S; a <= g.S; a++) {
+            const int hk = k + a;
+            if (hk < 0 || hk >= g.nx) continue;
+            for_node_ranges(qoff, g.Ncx, g.Ncz, g.ax, hk, hl, [&](unsigned s, unsigned e) {
+                for (unsigned i = s; i < e; i++) {
+                    const double x = mx[i], z = mz[i];
+                    const int ip = near_index(x, g.X0, g.dx, g.nx);
+                    const int jp = near_index(z, g.Z0, g.dz, g.nz);
+                    if (g.S == g.R && (ip != hk || jp != hl)) { *stray = 1; continue; }
+                    const int ox = k - ip, oz = l - jp;
+                    if (ox < -g.R || ox > g.R || oz < -g.R || oz > g.R) continue;
+                    // distances exactly as the reference forms them:
+                    // |X[ip] - x| for the home node, |X[ip] +- dx - x| otherwise
+                    const double dxm = (ox == 0) ? fabs(g.X[ip] - x) : fabs(g.X[ip] + ox * g.dx - x);
+                    const double dzm = (oz == 0) ? fabs(g.Z[jp] - z) : fabs(g.Z[jp] + oz * g.dz - z);
+                    acc(i, (1.0 - dxm / g.dx_itp) * (1.0 - dzm / g.dz_itp));
+                }
+            });
+        }
+    }
+}
+
+__device__ __forceinline__ double mark_val(const p2g_fld &f, int p, unsigned i) {
+    // a phase outside the table reads as NaN instead of out of bounds
+    double v = (f.flag == 0) ? ((p < f.n_src) ? f.src[p] : __builtin_nan("")) : f.src[i];
+    if (f.avg == 1) v = 1.0 / v;
+    if (f.avg == 2) v = log(v);
+    return v;
+}
+
+template <int NF>
+__global__ void __launch_bounds__(256) k_p2g(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff,
+                                             const double *__restrict__ mx,
+                                             const double *__restrict__ mz,
+                                             const int *__restrict__ mph, int *__restrict__ stray) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= g.nx * g.nz) return;
+    const int k = node % g.nx, l = node / g.nx;
+    double W = 0.0, BW[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) BW[f] = 0.0;
+    p2g_visit(g, qoff, mx, mz, k, l, stray, [&](unsigned i, double w) {
+        const int p = mph[i];
+        W += w;
+#pragma unroll
+        for (int f = 0; f < NF; f++) BW[f] += mark_val(fs.f[f], p, i) * w;
+    });
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        const p2g_fld &F = fs.f[f];
+        if (fabs(W) < 1e-30) continue;
+        if (F.ntype && (F.ntype[node] == 30 || F.ntype[node] == 31)) continue;
+        double v = BW[f] / W;
+        if (F.avg == 1) v = 1.0 / v;
+        if (F.avg == 2) v = exp(v);
+        F.dst[node] = v;
+    }
+}
+
+// Phase fractions (P2Mastah with prop = 1): per-thread phase accumulators in LDS.
+__global__ void __launch_bounds__(128) k_p2g_phase(p2g_geo g, int nph, const signed char *ntype,
+                                                   const unsigned *__restrict__ qoff,
+                                                   const double *__restrict__ mx,
+                                                   const double *__restrict__ mz,
+                                                   const int *__restrict__ mph,
+                                                   double *__restrict__ perc,
+                                                   int *__restrict__ stray) {
+    __shared__ double acc[P2G_MAXPH * 128];
+    const int t = threadIdx.x;
+    const int node = blockIdx.x * blockDim.x + t;
+    for (int p = 0; p < nph; p++) acc[p * 128 + t] = 0.0;
+    if (node >= g.nx * g.nz) return;
+    const int k = node % g.nx, l = node / g.nx;
+    double W = 0.0;
+    p2g_visit(g, qoff, mx, mz, k, l, stray, [&](unsigned i, double w) {
+        W += w;
+        const int p = mph[i];
+        if (p >= 0 && p < nph) acc[p * 128 + t] += w;
+    });
+    const size_t nn = (size_t)g.nx * g.nz;
+    const bool skip = fabs(W) < 1e-30 || (ntype && (ntype[node] == 30 || ntype[node] == 31));
+    for (int p = 0; p < nph; p++) {
+        const double a = acc[p * 128 + t];
+        perc[p * nn + node] = skip ? a : a / W;
+    }
+}
+
+extern int mdg_gather_sorted(mdg_ctx *c, const double *src, double *dst);
+
+static int setup_geo(mdg_ctx *c, int centroid, int stencil, p2g_geo *g) {
+    node_set ns;
+    int rc = mdg_node_set(c, centroid, &ns);
+    if (rc) return rc;
+    if (stencil != 1 && stencil != 9) return mdg_fail(c, MDG_ERR_ARG, "interp_stencil must be 1 or 9");
+    if (!c->qoff.p) return mdg_fail(c, MDG_ERR_STATE, "mdg_markers_set has not been called");
+    g->nx = ns.nx; g->nz = ns.nz; g->Ncx = c->Ncx; g->Ncz = c->Ncz;
+    g->ax = p2g_axes_of(centroid);
+    g->X = ns.X; g->Z = ns.Z; g->X0 = ns.X0; g->Z0 = ns.Z0; g->dx = c->dx; g->dz = c->dz;
+    g->dx_itp = (stencil == 9) ? 3.0 * c->dx / 2.0 : c->dx / 2.0;   // ParticleRoutines.c:2218-2221
+    g->dz_itp = (stencil == 9) ? 3.0 * c->dz / 2.0 : c->dz / 2.0;
+    g->R = (stencil == 9) ? 1 : 0;
+    g->S = g->R;
+    return MDG_OK;
+}
+
+static int launch_fields(mdg_ctx *c, p2g_geo g, const p2g_set &s, int nf) {
+    const int nn = g.nx * g.nz;
+    dim3 grid(mdg_grid1d(nn, 256)), block(256);
+    switch (nf) {
+    case 1: hipLaunchKernelGGL(k_p2g<1>, grid, block, 0, c->stream, g, s, c->qoff.p, c->mx.p, c->mz.p, c->mphase.p, c->flag_dev.p); break;
+    case 2: hipLaunchKernelGGL(k_p2g<2>, grid, block, 0, c->stream, g, s, c->qoff.p, c->mx.p, c->mz.p, c->mphase.p, c->flag_dev.p); break;
+    case 3: hipLaunchKernelGGL(k_p2g<3>, grid, block, 0, c->stream, g, s, c->qoff.p, c->mx.p, c->mz.p, c->mphase.p, c->flag_dev.p); break;
+    default: hipLaunchKernelGGL(k_p2g<4>, grid, block, 0, c->stream, g, s, c->qoff.p, c->mx.p, c->mz.p, c->mphase.p, c->flag_dev.p); break;
+    }
+    HIPCHK(c, hipGetLastError());
+    return MDG_OK;
+}
+
+// Runs `launch` once; if a stray marker was flagged, reruns it with the
+// wide search.  The kernel time of the final launch is kept in last_ms.
+template <typename L>
+static int run_checked(mdg_ctx *c, p2g_geo &g, L &&launch) {
+    for (int pass = 0; pass < 2; pass++) {
+        HIPCHK(c, hipMemsetAsync(c->flag_dev.p, 0, sizeof(int), c->stream));
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        int rc = launch(g);
+        if (rc) return rc;
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        int stray = 0;
+        HIPCHK(c, hipMemcpyAsync(&stray, c->flag_dev.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->last_ms = ms;
+        if (!stray) return MDG_OK;
+        g.S = g.R + 1;
+    }
+    return MDG_OK;
+}
+
+struct staged_field {
+    p2g_fld f;
+    double *host_dst;     // non-null when results go back to a host array
+    size_t  n;
+};
+
+// Resolve one field's source, destination and node tags onto the device.
+static int stage_field(mdg_ctx *c, const double *src, int n_src, double *dst,
+                       const signed char *ntype, int flag, int avg, size_t nn, size_t slot,
+                       staged_field *sf, dbuf<double> &fld, dbuf<double> &tab,
+                       dbuf<double> &nodes, dbuf<signed char> &tags) {
+    const int base_flag = flag & ~MDG_SRC_SORTED;
+    if (!src || !dst) return mdg_fail(c, MDG_ERR_ARG, "null field array");
+    if (avg < 0 || avg > 2) return mdg_fail(c, MDG_ERR_ARG, "avg must be 0, 1 or 2");
+    if (base_flag == 0) {
+        if (n_src < 1 || n_src > P2G_MAXTAB)
+            return mdg_fail(c, MDG_ERR_ARG, "phase table size must be 1..%d", P2G_MAXTAB);
+        // tables are tiny: always copy (host or device source)
+        HIPCHK(c, hipMemcpyAsync(tab.p + slot * P2G_MAXTAB, src, sizeof(double) * n_src,
+                                 hipMemcpyDefault, c->stream));
+        sf->f.src = tab.p + slot * P2G_MAXTAB;
+    } else if (base_flag == 1) {
+        if (n_src < c->n_markers) return mdg_fail(c, MDG_ERR_ARG, "per-marker field shorter than markers");
+        if (flag & MDG_SRC_SORTED) {
+            if (!mdg_is_device_ptr(src)) return mdg_fail(c, MDG_ERR_ARG, "sorted source must be device memory");
+            sf->f.src = src;
+        } else {
+            double *d = fld.p + slot * (size_t)(c->n_markers + 1);
+            int rc = mdg_gather_sorted(c, src, d);
+            if (rc) return rc;
+            sf->f.src = d;
+        }
+    } else {
+        return mdg_fail(c, MDG_ERR_UNSUPPORTED, "flag %d (anisotropy flags are not implemented)", flag);
+    }
+    sf->f.flag = base_flag;
+    sf->f.n_src = n_src;
+    sf->f.avg = avg;
+    sf->n = nn;
+    if (mdg_is_device_ptr(dst)) {
+        sf->f.dst = dst;
+        sf->host_dst = nullptr;
+    } else {
+        sf->f.dst = nodes.p + slot * nn;
+        sf->host_dst = dst;
+        HIPCHK(c, hipMemcpyAsync(sf->f.dst, dst, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    sf->f.ntype = nullptr;
+    if (ntype) {
+        if (mdg_is_device_ptr(ntype)) sf->f.ntype = ntype;
+        else {
+            HIPCHK(c, hipMemcpyAsync(tags.p + slot * nn, ntype, nn, hipMemcpyHostToDevice, c->stream));
+            sf->f.ntype = tags.p + slot * nn;
+        }
+    }
+    return MDG_OK;
+}
+
+extern "C" int mdg_p2mastah_fused(mdg_ctx *c, int centroid, int stencil,
+                                  const mdg_p2g_field *fields, int nfields) {
+    if (!c) return MDG_ERR_ARG;
+    if (nfields < 1 || nfields > 16 || !fields) return mdg_fail(c, MDG_ERR_ARG, "nfields must be 1..16");
+    p2g_geo g;
+    int rc = setup_geo(c, centroid, stencil, &g);
+    if (rc) return rc;
+    const size_t nn = (size_t)g.nx * g.nz, N = (size_t)c->n_markers + 1;
+    HIPCHK(c, c->field_tmp.reserve(N * P2G_MAXF));
+    HIPCHK(c, c->table_tmp.reserve(P2G_MAXTAB * P2G_MAXF));
+    HIPCHK(c, c->node_tmp.reserve(nn * P2G_MAXF));
+    HIPCHK(c, c->ntype_tmp.reserve(nn * P2G_MAXF));
+    for (int f0 = 0; f0 < nfields; f0 += P2G_MAXF) {
+        const int nf = (nfields - f0 < P2G_MAXF) ? nfields - f0 : P2G_MAXF;
+        staged_field sf[P2G_MAXF];
+        p2g_set s = {};
+        for (int q = 0; q < nf; q++) {
+            const mdg_p2g_field &F = fields[f0 + q];
+            rc = stage_field(c, F.src, F.n_src, F.dst, F.node_type, F.flag, F.avg, nn, q, &sf[q],
+                             c->field_tmp, c->table_tmp, c->node_tmp, c->ntype_tmp);
+            if (rc) return rc;
+            s.f[q] = sf[q].f;
+        }
+        rc = run_checked(c, g, [&](p2g_geo &gg) { return launch_fields(c, gg, s, nf); });
+        if (rc) return rc;
+        g.S = g.R;
+        for (int q = 0; q < nf; q++)
+            if (sf[q].host_dst)
+                HIPCHK(c, hipMemcpyAsync(sf[q].host_dst, sf[q].f.dst, nn * sizeof(double),
+                                         hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDG_OK;
+}
+
+extern "C" int mdg_p2mastah(mdg_ctx *c, const double *mat_prop, int n_mat_prop, double *node_field,
+                            const signed char *node_type, int flag, int avg, int prop,
+                            int centroid, int stencil, int nb_phases, double *const *phase_perc) {
+    if (!c) return MDG_ERR_ARG;
+    if (prop == 0) {
+        mdg_p2g_field f = {mat_prop, n_mat_prop, node_field, node_type, flag, avg};
+        return mdg_p2mastah_fused(c, centroid, stencil, &f, 1);
+    }
+    if (prop != 1) return mdg_fail(c, MDG_ERR_ARG, "prop must be 0 or 1");
+    if (centroid != 0 && centroid != 1)
+        return mdg_fail(c, MDG_ERR_ARG, "phase fractions exist on centres and vertices only");
+    if (nb_phases < 1 || nb_phases > P2G_MAXPH || !phase_perc)
+        return mdg_fail(c, MDG_ERR_ARG, "nb_phases must be 1..%d", P2G_MAXPH);
+    p2g_geo g;
+    int rc = setup_geo(c, centroid, stencil, &g);
+    if (rc) return rc;
+    const size_t nn = (size_t)g.nx * g.nz;
+    HIPCHK(c, c->node_tmp.reserve(nn * nb_phases));
+    HIPCHK(c, c->ntype_tmp.reserve(nn));
+    const signed char *nt = nullptr;
+    if (node_type) {
+        if (mdg_is_device_ptr(node_type)) nt = node_type;
+        else {
+            HIPCHK(c, hipMemcpyAsync(c->ntype_tmp.p, node_type, nn, hipMemcpyHostToDevice, c->stream));
+            nt = c->ntype_tmp.p;
+        }
+    }
+    rc = run_checked(c, g, [&](p2g_geo &gg) {
+        hipLaunchKernelGGL(k_p2g_phase, dim3(mdg_grid1d((long)nn, 128)), dim3(128), 0, c->stream, gg,
+                           nb_phases, nt, c->qoff.p, c->mx.p, c->mz.p, c->mphase.p, c->node_tmp.p,
+                           c->flag_dev.p);
+        HIPCHK(c, hipGetLastError());
+        return MDG_OK;
+    });
+    if (rc) return rc;
+    for (int p = 0; p < nb_phases; p++) {
+        if (!phase_perc[p]) return mdg_fail(c, MDG_ERR_ARG, "phase_perc[%d] is null", p);
+        HIPCHK(c, hipMemcpyAsync(phase_perc[p], c->node_tmp.p + p * nn, nn * sizeof(double),
+                                 hipMemcpyDefault, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDG_OK;
+}

@@ -1,0 +1,69 @@
+// Device helpers shared by the marker and P2G kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+// Nearest node along one axis, ties to the lower index, clamped to [0, n-1]:
+// the reference expression ceil(distance/dx + 0.5) - 1 of
+// ParticleRoutines.c:2300-2310.  Division and ceil are correctly rounded on
+// both host and device and no contraction applies, so the index is
+// bit-identical to the reference.
+__device__ __forceinline__ int near_index(double pos, double origin, double h, int n) {
+    const double distance = pos - origin;
+    int i = (int)ceil(distance / h + 0.5) - 1;
+    i = i < 0 ? 0 : i;
+    return i > n - 1 ? n - 1 : i;
+}
+
+// How a node set lines up with the cells along one axis:
+//   CELL_AXIS: node index k covers cell k + off (both halves);
+//   VERT_AXIS: node index k covers the east half of cell k-1 and the west
+//              half of cell k.
+enum { CELL_AXIS = 0, VERT_AXIS = 1 };
+
+struct p2g_axes {
+    int xkind, xoff, zkind, zoff;
+};
+
+// centroid 1 centres, 0 vertices, -1 Vx nodes, -2 Vz nodes.
+__host__ __device__ inline p2g_axes p2g_axes_of(int centroid) {
+    switch (centroid) {
+    case 1:  return {CELL_AXIS, 0, CELL_AXIS, 0};
+    case 0:  return {VERT_AXIS, 0, VERT_AXIS, 0};
+    case -1: return {VERT_AXIS, 0, CELL_AXIS, -1};
+    default: return {CELL_AXIS, -1, VERT_AXIS, 0};
+    }
+}
+
+// Visit the marker ranges [b, e) (in sorted order) whose expected nearest
+// node is (hk, hl).  Ranges are quarter cells; halves of one cell that are
+// adjacent in the key order are merged.
+template <typename F>
+__device__ __forceinline__ void for_node_ranges(const unsigned *__restrict__ qoff, int Ncx, int Ncz,
+                                                p2g_axes ax, int hk, int hl, F &&f) {
+    int cxs[2], hxs[2], nxr;   // (cell column, half: 0 west, 1 east, 2 both)
+    if (ax.xkind == CELL_AXIS) { cxs[0] = hk + ax.xoff; hxs[0] = 2; nxr = 1; }
+    else { cxs[0] = hk - 1; hxs[0] = 1; cxs[1] = hk; hxs[1] = 0; nxr = 2; }
+    int czs[2], hzs[2], nzr;
+    if (ax.zkind == CELL_AXIS) { czs[0] = hl + ax.zoff; hzs[0] = 2; nzr = 1; }
+    else { czs[0] = hl - 1; hzs[0] = 1; czs[1] = hl; hzs[1] = 0; nzr = 2; }
+    for (int a = 0; a < nzr; a++) {
+        const int cz = czs[a];
+        if (cz < 0 || cz >= Ncz) continue;
+        for (int b = 0; b < nxr; b++) {
+            const int cx = cxs[b];
+            if (cx < 0 || cx >= Ncx) continue;
+            const unsigned base = 4u * ((unsigned)cx + (unsigned)cz * (unsigned)Ncx);
+            if (hzs[a] == 2 && hxs[b] == 2) {
+                f(qoff[base], qoff[base + 4]);
+            } else if (hzs[a] == 2) {          // both z halves, one x half
+                f(qoff[base + hxs[b]], qoff[base + hxs[b] + 1]);
+                f(qoff[base + 2 + hxs[b]], qoff[base + 3 + hxs[b]]);
+            } else if (hxs[b] == 2) {          // both x halves of one z half
+                f(qoff[base + 2 * hzs[a]], qoff[base + 2 * hzs[a] + 2]);
+            } else {
+                const unsigned q = base + hxs[b] + 2 * hzs[a];
+                f(qoff[q], qoff[q + 1]);
+            }
+        }
+    }
+}

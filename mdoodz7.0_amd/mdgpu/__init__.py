@@ -1,0 +1,232 @@
+"""Python host side of the MI355X hot path (ctypes over libmdgpu.so).
+
+The names mirror the reference call sites: Context.p2mastah is P2Mastah
+(ParticleRoutines.c:2170), Context.p2mastah_fused is P2Mastah_Fused
+(ParticleRoutines.c:1895), Context.killer_solver replaces KillerSolver
+(Solvers.c:1054) at the lin_solver switch (StokesRoutines.c:845-848).
+Arguments keep their reference meaning; failures raise MdgError carrying the
+library's message instead of calling exit().
+
+Arrays may be numpy arrays (host) or DeviceArray objects (device memory
+allocated through the library's own HIP runtime); device arrays are used in
+place.  There is no CPU fallback: without the built library or a GPU every
+call raises.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libmdgpu.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "mdgpu.h")
+_LIB = None
+
+MDG_SRC_SORTED = 0x100
+
+
+class MdgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("mdgpu error %d: %s" % (code, msg))
+        self.code = code
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("Nx", C.c_int), ("Nz", C.c_int), ("xmin", C.c_double), ("xmax", C.c_double),
+                ("zmin", C.c_double), ("zmax", C.c_double), ("periodic_x", C.c_int)]
+
+
+class P2GField(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("n_src", C.c_int), ("dst", C.c_void_p),
+                ("node_type", C.c_void_p), ("flag", C.c_int), ("avg", C.c_int)]
+
+
+def declared_functions():
+    """Names of every function declared in include/mdgpu.h."""
+    txt = open(HEADER_PATH).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mdg_[a-z0-9_]+)\s*\(", txt)))
+
+
+def lib():
+    """Load libmdgpu.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmdgpu.so not built: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.mdg_create.argtypes = [C.POINTER(GridDesc), C.POINTER(vp)]
+        L.mdg_destroy.argtypes = [vp]
+        L.mdg_destroy.restype = None
+        L.mdg_last_error.argtypes = [vp]
+        L.mdg_last_error.restype = C.c_char_p
+        L.mdg_grid_coords.argtypes = [vp] + [vp] * 6
+        L.mdg_set_bc.argtypes = [vp] * 7
+        L.mdg_markers_set.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.mdg_marker_index.argtypes = [vp, C.c_int, vp, vp]
+        L.mdg_markers_gather.argtypes = [vp, vp, vp]
+        L.mdg_p2mastah.argtypes = [vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, C.c_int, vp]
+        L.mdg_p2mastah_fused.argtypes = [vp, C.c_int, C.c_int, C.POINTER(P2GField), C.c_int]
+        L.mdg_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double)]
+        L.mdg_dev_alloc.argtypes = [vp, C.c_ulonglong, C.POINTER(vp)]
+        L.mdg_dev_free.argtypes = [vp, vp]
+        L.mdg_memcpy.argtypes = [vp, vp, vp, C.c_ulonglong]
+        _extra_signatures(L)
+        _LIB = L
+    return _LIB
+
+
+def _extra_signatures(L):
+    """Signatures of the Stokes/solver entry points (defined further down)."""
+    vp = C.c_void_p
+    if hasattr(L, "mdg_stokes_residual"):
+        L.mdg_stokes_residual.argtypes = [vp] * 10 + [vp, vp]
+    if hasattr(L, "mdg_stokes_apply"):
+        L.mdg_stokes_apply.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp]
+
+
+def ptr(a):
+    """Raw address of a numpy array or DeviceArray (None passes NULL)."""
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a.p
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"], "arrays must be C-contiguous"
+        return a.ctypes.data
+    raise TypeError("expected numpy array or torch tensor, got %r" % type(a))
+
+
+class DeviceArray:
+    """A flat device array allocated with mdg_dev_alloc on a context."""
+
+    def __init__(self, ctx, n, dtype=np.float64, host=None):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        h = C.c_void_p()
+        ctx._check(ctx._L.mdg_dev_alloc(ctx.h, self.n * self.dtype.itemsize, C.byref(h)))
+        self.p = h.value
+        if host is not None:
+            self.upload(host)
+
+    def __len__(self):
+        return self.n
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, self.dtype)
+        assert host.size == self.n
+        self.ctx._check(self.ctx._L.mdg_memcpy(self.ctx.h, self.p, host.ctypes.data, host.nbytes))
+        return self
+
+    def numpy(self):
+        out = np.empty(self.n, self.dtype)
+        self.ctx._check(self.ctx._L.mdg_memcpy(self.ctx.h, out.ctypes.data, self.p, out.nbytes))
+        return out
+
+    def free(self):
+        if self.p and self.ctx.h:
+            self.ctx._L.mdg_dev_free(self.ctx.h, self.p)
+        self.p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One grid on the GPU (mdg_create / mdg_destroy)."""
+
+    def __init__(self, Nx, Nz, xmin, xmax, zmin, zmax):
+        self._L = lib()
+        self.Nx, self.Nz = Nx, Nz
+        h = C.c_void_p()
+        rc = self._L.mdg_create(C.byref(GridDesc(Nx, Nz, xmin, xmax, zmin, zmax, 0)), C.byref(h))
+        if rc != 0:
+            raise MdgError(rc, "mdg_create failed (is a GPU visible?)")
+        self.h = h
+        self.n_markers = 0
+
+    def device(self, n, dtype=np.float64, host=None):
+        """Allocate a DeviceArray of n elements (optionally filled from host)."""
+        return DeviceArray(self, n, dtype, host)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.mdg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MdgError(rc, self._L.mdg_last_error(self.h).decode())
+
+    def node_count(self, centroid):
+        return {1: (self.Nx - 1) * (self.Nz - 1), 0: self.Nx * self.Nz,
+                -1: self.Nx * (self.Nz + 1), -2: (self.Nx + 1) * self.Nz}[centroid]
+
+    def coords(self):
+        Nx, Nz = self.Nx, self.Nz
+        out = dict(xg=np.zeros(Nx), zg=np.zeros(Nz), xc=np.zeros(Nx - 1), zc=np.zeros(Nz - 1),
+                   xvz=np.zeros(Nx + 1), zvx=np.zeros(Nz + 1))
+        self._check(self._L.mdg_grid_coords(self.h, *[ptr(out[k]) for k in
+                                                      ("xg", "zg", "xc", "zc", "xvz", "zvx")]))
+        return out
+
+    def set_bc(self, BCu_t, BCu_v, BCv_t, BCv_v, BCp_t, BCg_t):
+        self._check(self._L.mdg_set_bc(self.h, ptr(BCu_t), ptr(BCu_v), ptr(BCv_t), ptr(BCv_v),
+                                       ptr(BCp_t), ptr(BCg_t)))
+
+    def markers_set(self, x, z, phase):
+        self._check(self._L.mdg_markers_set(self.h, len(x), ptr(x), ptr(z), ptr(phase)))
+        self.n_markers = len(x)
+
+    def marker_index(self, centroid):
+        ip = np.zeros(self.n_markers, np.int32)
+        jp = np.zeros(self.n_markers, np.int32)
+        self._check(self._L.mdg_marker_index(self.h, centroid, ptr(ip), ptr(jp)))
+        return ip, jp
+
+    def markers_gather(self, src, dst_sorted):
+        self._check(self._L.mdg_markers_gather(self.h, ptr(src), ptr(dst_sorted)))
+
+    def p2mastah(self, mat_prop, node_field=None, node_type=None, flag=0, avg=0, prop=0,
+                 centroid=1, interp_stencil=1, nb_phases=0, phase_perc=None):
+        """P2Mastah (ParticleRoutines.c:2170).  Returns node_field, or the
+        phase fraction arrays when prop == 1."""
+        nn = self.node_count(centroid)
+        if prop == 1:
+            if phase_perc is None:
+                phase_perc = [np.zeros(nn) for _ in range(nb_phases)]
+            arr = (C.c_void_p * nb_phases)(*[ptr(a) for a in phase_perc])
+            self._check(self._L.mdg_p2mastah(self.h, ptr(mat_prop), 0, None, ptr(node_type),
+                                             flag, avg, 1, centroid, interp_stencil, nb_phases,
+                                             C.cast(arr, C.c_void_p)))
+            return phase_perc
+        if node_field is None:
+            node_field = np.zeros(nn)
+        self._check(self._L.mdg_p2mastah(self.h, ptr(mat_prop), len(mat_prop), ptr(node_field),
+                                         ptr(node_type), flag, avg, 0, centroid, interp_stencil,
+                                         0, None))
+        return node_field
+
+    def p2mastah_fused(self, centroid, interp_stencil, fields):
+        """P2Mastah_Fused; fields = list of dicts(src, dst, flag, avg, node_type)."""
+        arr = (P2GField * len(fields))()
+        for i, f in enumerate(fields):
+            arr[i] = P2GField(ptr(f["src"]), len(f["src"]), ptr(f["dst"]),
+                              ptr(f.get("node_type")), f.get("flag", 0), f.get("avg", 0))
+        self._check(self._L.mdg_p2mastah_fused(self.h, centroid, interp_stencil, arr, len(fields)))
+
+    def last_kernel_ms(self):
+        v = C.c_double()
+        self._check(self._L.mdg_last_kernel_ms(self.h, C.byref(v)))
+        return v.value

@@ -1,0 +1,218 @@
+"""CPU FP64 oracle for the MDOODZ hot path (ctypes wrapper over liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product path.  Each wrapped C
+function cites the reference file:line it restates (see mdoracle.h).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_dp = C.POINTER(C.c_double)
+_bp = C.POINTER(C.c_int8)
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("Nx", C.c_int), ("Nz", C.c_int), ("dx", C.c_double), ("dz", C.c_double),
+                ("xg", _dp), ("zg", _dp), ("xc", _dp), ("zc", _dp), ("xvz", _dp), ("zvx", _dp),
+                ("BCu_t", _bp), ("BCv_t", _bp), ("BCp_t", _bp), ("BCg_t", _bp),
+                ("BCu_v", _dp), ("BCv_v", _dp)]
+
+
+class KillerOpts(C.Structure):
+    _fields_ = [("penalty", C.c_double), ("celvol", C.c_double), ("max_its_PH", C.c_int),
+                ("rel_tol_KSP", C.c_double), ("max_its_KSP", C.c_int),
+                ("lin_abs_div", C.c_double), ("lin_rel_div", C.c_double),
+                ("lin_abs_mom", C.c_double), ("lin_rel_mom", C.c_double),
+                ("its_PH", C.c_int), ("its_KSP_tot", C.c_int),
+                ("maxdiv", C.c_double), ("maxdiv0", C.c_double),
+                ("maxru", C.c_double), ("maxru0", C.c_double), ("status", C.c_int)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        gp = C.POINTER(OrcGrid)
+        L.orc_grid_coords.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, _dp, _dp, f64p, f64p, f64p, f64p, f64p, f64p]
+        L.orc_bc_pure_shear.argtypes = [gp, C.c_double]
+        L.orc_eqn_numbering.argtypes = [gp, i32p, i32p, i32p, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int)]
+        L.orc_eqn_numbering.restype = C.c_int
+        L.orc_marker_index.argtypes = [gp, C.c_int, C.c_int, f64p, f64p, i32p, i32p]
+        L.orc_p2mastah.argtypes = [gp, C.c_int, f64p, f64p, i32p, f64p, C.c_int, C.c_int,
+                                   C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, f64p,
+                                   C.c_int, C.c_void_p, C.c_int, C.c_int]
+        L.orc_c2v.argtypes = [gp, C.c_int, f64p, f64p]
+        L.orc_apply_bc.argtypes = [gp, f64p, f64p]
+        L.orc_strain_rates.argtypes = [gp, C.c_int, f64p, f64p, f64p, f64p, f64p, f64p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_eval_rhs.argtypes = [gp, C.c_double, C.c_double, f64p, f64p, f64p, f64p]
+        L.orc_residual.argtypes = [gp, i32p, i32p, i32p, C.c_int, f64p, f64p, f64p, f64p,
+                                   f64p, f64p, f64p, C.c_void_p, C.c_void_p, f64p, f64p]
+        L.orc_assemble_picard.argtypes = [gp, i32p, i32p, i32p, C.c_int, C.c_int, f64p, f64p,
+                                          f64p, i32p, i32p, f64p, i32p, i32p, f64p, i32p,
+                                          i32p, f64p]
+        L.orc_killer_solve.argtypes = [C.c_int, C.c_int, i32p, i32p, f64p, i32p, i32p, f64p,
+                                       i32p, i32p, f64p, f64p, f64p, f64p, f64p,
+                                       C.POINTER(KillerOpts)]
+        L.orc_killer_solve.restype = C.c_int
+        L.orc_set_band_order.argtypes = [gp, i32p, i32p, C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+class Grid:
+    """Staggered grid + boundary tags in the reference layout (GridRoutines.c:246-330)."""
+
+    def __init__(self, Nx, Nz, xmin, xmax, zmin, zmax):
+        self.Nx, self.Nz = Nx, Nz
+        self.xmin, self.xmax, self.zmin, self.zmax = xmin, xmax, zmin, zmax
+        self.xg = np.zeros(Nx); self.zg = np.zeros(Nz)
+        self.xc = np.zeros(Nx - 1); self.zc = np.zeros(Nz - 1)
+        self.xvz = np.zeros(Nx + 1); self.zvx = np.zeros(Nz + 1)
+        dx, dz = C.c_double(), C.c_double()
+        lib().orc_grid_coords(Nx, Nz, xmin, zmin, xmax, zmax, C.byref(dx), C.byref(dz),
+                              self.xg, self.zg, self.xc, self.zc, self.xvz, self.zvx)
+        self.dx, self.dz = dx.value, dz.value
+        self.BCu_t = np.full(Nx * (Nz + 1), -1, np.int8)
+        self.BCv_t = np.full((Nx + 1) * Nz, -1, np.int8)
+        self.BCp_t = np.full((Nx - 1) * (Nz - 1), -1, np.int8)
+        self.BCg_t = np.full(Nx * Nz, -1, np.int8)
+        self.BCu_v = np.zeros(Nx * (Nz + 1))
+        self.BCv_v = np.zeros((Nx + 1) * Nz)
+        self._c = None
+
+    @property
+    def c(self):
+        g = OrcGrid(self.Nx, self.Nz, self.dx, self.dz, _ptr(self.xg), _ptr(self.zg),
+                    _ptr(self.xc), _ptr(self.zc), _ptr(self.xvz), _ptr(self.zvx),
+                    _ptr(self.BCu_t, _bp), _ptr(self.BCv_t, _bp), _ptr(self.BCp_t, _bp),
+                    _ptr(self.BCg_t, _bp), _ptr(self.BCu_v), _ptr(self.BCv_v))
+        self._c = g
+        return C.byref(g)
+
+    def pure_shear(self, ebg):
+        lib().orc_bc_pure_shear(self.c, ebg)
+        return self
+
+    def numbering(self):
+        eu = np.zeros(self.Nx * (self.Nz + 1), np.int32)
+        ev = np.zeros((self.Nx + 1) * self.Nz, np.int32)
+        ep = np.zeros((self.Nx - 1) * (self.Nz - 1), np.int32)
+        nm, nc = C.c_int(), C.c_int()
+        neq = lib().orc_eqn_numbering(self.c, eu, ev, ep, C.byref(nm), C.byref(nc))
+        return eu, ev, ep, nm.value, nc.value, neq
+
+    def node_count(self, centroid):
+        return {1: (self.Nx - 1) * (self.Nz - 1), 0: self.Nx * self.Nz,
+                -1: self.Nx * (self.Nz + 1), -2: (self.Nx + 1) * self.Nz}[centroid]
+
+
+def marker_index(g, centroid, x, z):
+    ip = np.zeros(len(x), np.int32); jp = np.zeros(len(x), np.int32)
+    lib().orc_marker_index(g.c, centroid, len(x), np.ascontiguousarray(x),
+                           np.ascontiguousarray(z), ip, jp)
+    return ip, jp
+
+
+def p2mastah(g, x, z, phase, mat_prop, flag=0, avg=0, prop=0, centroid=1, stencil=1,
+             periodic_x=0, node_type=None, node_field=None, nb_phases=0, mode=0, nthreads=1):
+    """ParticleRoutines.c:2170-2665.  Returns node_field (prop=0) or phase_perc (prop=1)."""
+    nn = g.node_count(centroid)
+    if node_field is None:
+        node_field = np.zeros(nn)
+    perc = np.zeros(max(nb_phases, 1) * nn) if prop == 1 else None
+    nt = node_type.ctypes.data if node_type is not None else None
+    lib().orc_p2mastah(g.c, len(x), np.ascontiguousarray(x, np.float64),
+                       np.ascontiguousarray(z, np.float64), np.ascontiguousarray(phase, np.int32),
+                       np.ascontiguousarray(mat_prop, np.float64), flag, avg, prop, centroid,
+                       stencil, periodic_x, nt, node_field, nb_phases,
+                       perc.ctypes.data if perc is not None else None, mode, nthreads)
+    if prop == 1:
+        return perc.reshape(nb_phases, nn)
+    return node_field
+
+
+def c2v(g, cent, periodic_x=0):
+    v = np.zeros(g.Nx * g.Nz)
+    lib().orc_c2v(g.c, periodic_x, np.ascontiguousarray(cent), v)
+    return v
+
+
+def apply_bc(g, u, v):
+    lib().orc_apply_bc(g.c, u, v)
+
+
+def strain_rates(g, u, v, compressible=0):
+    nc = (g.Nx - 1) * (g.Nz - 1); nv = g.Nx * g.Nz
+    out = {k: np.zeros(nc) for k in ("div", "exxd", "ezzd")}
+    out["exz"] = np.zeros(nv); out["wxz"] = np.zeros(nv)
+    lib().orc_strain_rates(g.c, compressible, u, v, out["div"], out["exxd"], out["ezzd"],
+                           out["exz"], out["wxz"].ctypes.data, None, None, None)
+    return out
+
+
+def eval_rhs(g, gx, gz, rho_n, rho_s):
+    rx = np.zeros(g.Nx * (g.Nz + 1)); rz = np.zeros((g.Nx + 1) * g.Nz)
+    lib().orc_eval_rhs(g.c, gx, gz, rho_n, rho_s, rx, rz)
+    return rx, rz
+
+
+def residual(g, num, sxxd, szzd, sxz, p_corr, div, roger_x, roger_z, d=None):
+    eu, ev, ep, nm, nc, neq = num
+    F = np.zeros(neq)
+    norms = np.zeros(3)
+    lib().orc_residual(g.c, eu, ev, ep, nm, sxxd, szzd, sxz, p_corr, div, roger_x, roger_z,
+                       None, d.ctypes.data if d is not None else None, F, norms)
+    return F, norms
+
+
+def assemble_picard(g, num, eta_n, eta_s):
+    """CSR blocks A (nm x nm), B (nm x nc), C (nc x nm) of the Picard operator."""
+    eu, ev, ep, nm, nc, neq = num
+    D11 = np.ascontiguousarray(2.0 * eta_n); D33 = np.ascontiguousarray(eta_s)
+    Ap = np.zeros(nm + 1, np.int32); Aj = np.zeros(9 * nm + 1, np.int32); Ax = np.zeros(9 * nm + 1)
+    Bp = np.zeros(nm + 1, np.int32); Bj = np.zeros(2 * nm + 1, np.int32); Bx = np.zeros(2 * nm + 1)
+    Cp = np.zeros(nc + 1, np.int32); Cj = np.zeros(4 * nc + 1, np.int32); Cx = np.zeros(4 * nc + 1)
+    lib().orc_assemble_picard(g.c, eu, ev, ep, nm, nc, D11, D11, D33, Ap, Aj, Ax, Bp, Bj, Bx,
+                              Cp, Cj, Cx)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((Ax[:Ap[-1]], Aj[:Ap[-1]], Ap), shape=(nm, nm))
+    B = sp.csr_matrix((Bx[:Bp[-1]], Bj[:Bp[-1]], Bp), shape=(nm, nc))
+    Cm = sp.csr_matrix((Cx[:Cp[-1]], Cj[:Cp[-1]], Cp), shape=(nc, nm))
+    return A, B, Cm
+
+
+def killer_solve(g, num, A, B, Cm, bu, bp, penalty=1e5, max_its_PH=10, rel_tol_KSP=1e-4,
+                 max_its_KSP=100, lin_abs_div=1e-9, lin_rel_div=1e-5, lin_abs_mom=1e-9,
+                 lin_rel_mom=1e-5):
+    """Solvers.c:1054-1454 (PH + GCR(6), exact band-Cholesky preconditioner)."""
+    eu, ev, ep, nm, nc, neq = num
+    lib().orc_set_band_order(g.c, eu, ev, nm)
+    o = KillerOpts(penalty, g.dx * g.dz, max_its_PH, rel_tol_KSP, max_its_KSP,
+                   lin_abs_div, lin_rel_div, lin_abs_mom, lin_rel_mom, 0, 0, 0, 0, 0, 0, 0)
+    du = np.zeros(nm); dp = np.zeros(nc)
+
+    def csr(M):
+        return (np.ascontiguousarray(M.indptr, np.int32), np.ascontiguousarray(M.indices, np.int32),
+                np.ascontiguousarray(M.data, np.float64))
+    a, b, c = csr(A), csr(B), csr(Cm)
+    st = lib().orc_killer_solve(nm, nc, *a, *b, *c, np.ascontiguousarray(bu),
+                                np.ascontiguousarray(bp), du, dp, C.byref(o))
+    return du, dp, o, st

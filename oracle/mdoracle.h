@@ -1,0 +1,122 @@
+/*
+ * mdoracle — CPU FP64 restatement of the MDOODZ 7.0 per-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path
+ * (libmdgpu.so).  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product path never calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the MDOODZ7.0 source tree).  Grid layouts are the reference's own:
+ *   vertices   Nx*Nz        index k + l*Nx          (xg, zg)
+ *   centres    (Nx-1)*(Nz-1) index k + l*(Nx-1)      (xc, zc)
+ *   Vx nodes   Nx*(Nz+1)    index k + l*Nx          (xg, zvx)
+ *   Vz nodes   (Nx+1)*Nz    index k + l*(Nx+1)      (xvz, zg)
+ * BC tags follow Setup.c:206-230 (-1 free, 0/11 Dirichlet, 13 free slip,
+ * 2 stress, -2/-12 periodic, 30 air, 31 surface pressure).
+ */
+#ifndef MDORACLE_H
+#define MDORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int Nx, Nz;                 /* vertices (MDOODZ convention)          */
+    double dx, dz;
+    double *xg, *zg;            /* Nx, Nz                                */
+    double *xc, *zc;            /* Nx-1, Nz-1                            */
+    double *xvz, *zvx;          /* Nx+1, Nz+1                            */
+    signed char *BCu_t, *BCv_t, *BCp_t, *BCg_t;
+    double *BCu_v, *BCv_v;
+} orc_grid;
+
+/* GridRoutines.c:246-330 (running-sum coordinates) */
+void orc_grid_coords(int Nx, int Nz, double xmin, double zmin, double xmax, double zmax,
+                     double *dx, double *dz, double *xg, double *zg, double *xc, double *zc,
+                     double *xvz, double *zvx);
+
+/* Setup.c:206-600 with SetPureShearBCVx/Vz (Setup.c:907-981) */
+void orc_bc_pure_shear(orc_grid *g, double bkg_strain_rate);
+
+/* StokesRoutines.c:1162-1220 ; returns neq, writes neq_mom / neq_cont */
+int orc_eqn_numbering(const orc_grid *g, int *eqn_u, int *eqn_v, int *eqn_p,
+                      int *neq_mom, int *neq_cont);
+
+/* ParticleRoutines.c:2300-2310 marker -> node index for one node set */
+void orc_marker_index(const orc_grid *g, int centroid, int n, const double *x,
+                      const double *z, int *ip, int *jp);
+
+/* ParticleRoutines.c:2170-2665 (P2Mastah).  mode 0: thread-private buffers
+ * and serial-order reduction; mode 3: atomic scatter.  phase_perc is
+ * nb_phases contiguous node arrays (used when prop==1). */
+void orc_p2mastah(const orc_grid *g, int n, const double *x, const double *z, const int *phase,
+                  const double *mat_prop, int flag, int avg, int prop, int centroid,
+                  int stencil, int periodic_x, const signed char *node_type,
+                  double *node_field, int nb_phases, double *phase_perc, int mode,
+                  int nthreads);
+
+/* GridRoutines.c:108-212 */
+void orc_c2v(const orc_grid *g, int periodic_x, const double *cent, double *vert);
+
+/* StokesRoutines.c:415-~500 (ghost values from BC tags) */
+void orc_apply_bc(const orc_grid *g, double *u, double *v);
+
+/* RheologyDensity.c:2001-2097 */
+void orc_strain_rates(const orc_grid *g, int compressible, const double *u, const double *v,
+                      double *div, double *exxd, double *ezzd, double *exz, double *wxz,
+                      double *exxd_s, double *ezzd_s, double *div_s);
+
+/* StokesRoutines.c:961-1160 */
+void orc_eval_rhs(const orc_grid *g, double gx, double gz, const double *rho_n,
+                  const double *rho_s, double *roger_x, double *roger_z);
+
+/* StokesAssemblyDecoupled.c:184-211,688-724,1128-1144 (Assemble=0) and
+ * StokesRoutines.c:867-955 (scaling by d and interior RMS norms).
+ * F is in equation numbering (neq entries); d may be NULL (=1). */
+void orc_residual(const orc_grid *g, const int *eqn_u, const int *eqn_v, const int *eqn_p,
+                  int neq_mom, const double *sxxd, const double *szzd, const double *sxz,
+                  const double *p_corr, const double *div, const double *roger_x,
+                  const double *roger_z, const double *rhs_p, const double *d, double *F,
+                  double norms[3]);
+
+/* StokesAssemblyDecoupled.c:62-212,562-725,1090-1145,1295-1565 (Assemble=1),
+ * incompressible, no stabilisation.  CSR (row pointers, cols, vals), columns in
+ * block-local numbering.  Capacity: A 9 per row, B/C 4 per row. Returns nnz. */
+void orc_assemble_picard(const orc_grid *g, const int *eqn_u, const int *eqn_v,
+                         const int *eqn_p, int neq_mom, int neq_cont, const double *D11,
+                         const double *D22, const double *D33,
+                         int *Ap, int *Aj, double *Ax, int *Bp, int *Bj, double *Bx,
+                         int *Cp, int *Cj, double *Cx);
+
+typedef struct {
+    double penalty;            /* model.penalty (gamma)                  */
+    double celvol;             /* dx*dz                                  */
+    int    max_its_PH;         /* InputOutput.c:1184 default 10          */
+    double rel_tol_KSP;        /* default 1e-4                           */
+    int    max_its_KSP;        /* default 1000                           */
+    double lin_abs_div, lin_rel_div, lin_abs_mom, lin_rel_mom;
+    /* outputs */
+    int    its_PH, its_KSP_tot;
+    double maxdiv, maxdiv0, maxru, maxru0;
+    int    status;             /* 0 ok, -1 not SPD, -2 PH abort          */
+} orc_killer_opts;
+
+/* Solvers.c:124-230 (kspgcr) and Solvers.c:1054-1454 (KillerSolver) for the
+ * incompressible case.  The CHOLMOD factor of Jts = A - B*(penalty*I)*C is
+ * replaced by an exact banded Cholesky factor of the same matrix. */
+int orc_killer_solve(int neq_mom, int neq_cont,
+                     const int *Ap, const int *Aj, const double *Ax,
+                     const int *Bp, const int *Bj, const double *Bx,
+                     const int *Cp, const int *Cj, const double *Cx,
+                     const double *rhs_mom, const double *rhs_cont, double *du, double *dp,
+                     orc_killer_opts *opts);
+
+/* Row-interleaved momentum DOF order (Vx row l, then Vz row l) used to
+ * give the exact factor of orc_killer_solve a narrow band. */
+void orc_set_band_order(const orc_grid *g, const int *eqn_u, const int *eqn_v, int neq_mom);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
