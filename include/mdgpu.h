@@ -113,6 +113,71 @@ typedef struct {
 int mdg_p2mastah_fused(mdg_ctx *ctx, int centroid, int interp_stencil,
                        const mdg_p2g_field *fields, int nfields);
 
+/* ------------------------------------------------------------ Stokes operator
+ * Picard operator K = [A B; C 0] of the isotropic viscous Stokes equations
+ * (StokesAssemblyDecoupled.c:62-212, 562-725, 1090-1145 with Assemble = 1),
+ * applied matrix-free: y = K x with x, y in MDOODZ's equation numbering
+ * (StokesRoutines.c:1162-1220; Vx rows, Vz rows, then cells).  D11_n, D22_n
+ * (cells) and D33_s (vertices) are RheologicalOperators' Picard coefficients
+ * (StokesRoutines.c:113-122,195-198: D11 = D22 = 2 eta_n, D33 = eta_s).  The
+ * supported boundary layout is MDOODZ's closed box: walls tag 0 (normal
+ * velocity), ghost rows/columns tag 13 (free slip) or 11 (no slip). */
+int mdg_eqn_count(const mdg_ctx *ctx, int *neq, int *neq_mom, int *neq_cont);
+int mdg_stokes_apply(mdg_ctx *ctx, const double *D11_n, const double *D22_n,
+                     const double *D33_s, const double *x, double *y);
+
+/* Ghost velocities from the tags (ApplyBC, StokesRoutines.c:415-500), in place
+ * on the Vx (Nx*(Nz+1)) and Vz ((Nx+1)*Nz) arrays. */
+int mdg_apply_bc(mdg_ctx *ctx, double *u, double *v);
+
+/* StrainRateComponents (RheologyDensity.c:2001-2097, incompressible): div,
+ * exxd, ezzd on cells and exz on vertices from u_in, v_in. */
+int mdg_strain_rates(mdg_ctx *ctx, const double *u, const double *v, double *div,
+                     double *exxd, double *ezzd, double *exz);
+
+/* EvaluateRHS body force (StokesRoutines.c:961-1160): roger_x = gx * mean of
+ * the two vertex densities, roger_z = gz * mean of the two cell densities. */
+int mdg_eval_rhs(mdg_ctx *ctx, double gx, double gz, const double *rho_n, const double *rho_s,
+                 double *roger_x, double *roger_z);
+
+/* Residual of EvaluateStokesResidualDecoupled (StokesRoutines.c:867-955)
+ * from the grid stresses: F (equation numbering) and the interior RMS norms
+ * norms[0..2] = resx, resz, resp.  d (equation numbering) is the diagonal
+ * scaling of ExtractDiagonalScale, or NULL for none. */
+int mdg_stokes_residual(mdg_ctx *ctx, const double *sxxd, const double *szzd, const double *sxz,
+                        const double *p_corr, const double *div, const double *roger_x,
+                        const double *roger_z, const double *d, double *F, double norms[3]);
+
+/* --------------------------------------------------------------- Stokes solve
+ * Replaces KillerSolver (Solvers.c:1054-1454; called from
+ * SolveStokesDefectDecoupled, StokesRoutines.c:845-848) for the Picard
+ * operator: solves K sol = [rhs_mom; rhs_cont] (equation numbering) to a
+ * relative 2-norm residual rel_tol.  Outer flexible GCR (FP64) on the
+ * augmented system with a viscosity-weighted penalty gamma*eta_n/dV
+ * (Powell-Hestenes term of KillerSolver with Dcm = penalty/dV), inner CG on the
+ * augmented velocity block preconditioned by a staggered geometric multigrid
+ * V-cycle, in FP64 or FP32 (precision).  Zero fields take defaults. */
+#define MDG_PREC_FP64  0
+#define MDG_PREC_MIXED 1
+typedef struct {
+    double penalty;        /* gamma of the viscosity-weighted penalty (1)      */
+    double rel_tol;        /* ||r||/||r0|| target (1e-8)                       */
+    int    max_its;        /* outer iterations (500)                           */
+    int    restart;        /* GCR restart length (20)                          */
+    double inner_tol;      /* relative tolerance of the velocity CG (1e-3)     */
+    int    inner_max_its;  /* (60)                                             */
+    int    cheb_degree;    /* Chebyshev smoothing degree per sweep (3)         */
+    int    coarse_cells;   /* coarsen until <= this many cells per side (8)    */
+    int    precision;      /* MDG_PREC_FP64 or MDG_PREC_MIXED                  */
+    int    precond;        /* reserved: 0 = multigrid                          */
+    /* outputs */
+    int    its, inner_its, levels, status;
+    double rel_res, time_ms;
+} mdg_solver_opts;
+int mdg_killer_solver(mdg_ctx *ctx, mdg_solver_opts *opts, const double *D11_n,
+                      const double *D22_n, const double *D33_s, const double *rhs_mom,
+                      const double *rhs_cont, double *sol);
+
 /* Time of the last P2G kernel (ms, HIP events on the context stream) and
  * the number of markers it processed. */
 int mdg_last_kernel_ms(const mdg_ctx *ctx, double *ms);

@@ -1,5 +1,5 @@
 // Context lifetime, grid coordinates, boundary tags, error reporting.
-#include "mdg_internal.h"
+#include "stokes_internal.h"
 
 int mdg_fail(mdg_ctx *c, int code, const char *fmt, ...) {
     if (c) {
@@ -83,6 +83,9 @@ extern "C" void mdg_destroy(mdg_ctx *c) {
     c->qoff.release(); c->keys.release(); c->keys2.release(); c->idx.release();
     c->sort_tmp.release(); c->field_tmp.release(); c->node_tmp.release();
     c->table_tmp.release(); c->ntype_tmp.release(); c->flag_dev.release();
+    c->eqn_u.release(); c->eqn_v.release(); c->eqn_p.release(); c->eta_n.release();
+    c->eta_s.release(); c->tmp_a.release(); c->tmp_b.release(); c->tmp_c.release();
+    c->tmp_d.release(); c->gx.release(); c->gy.release(); c->eqbuf.release(); c->part.release();
     mdg_solver_release(c);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -136,13 +139,54 @@ extern "C" int mdg_set_bc(mdg_ctx *c, const signed char *BCu_t, const double *BC
         (rc = copy_any(c, c->BCp_t, BCp_t, np)) || (rc = copy_any(c, c->BCg_t, BCg_t, ng)))
         return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::vector<signed char> hp(np), hg(ng);
+    std::vector<signed char> hu(nu), hv(nv), hp(np), hg(ng);
+    HIPCHK(c, hipMemcpy(hu.data(), c->BCu_t.p, nu, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hv.data(), c->BCv_t.p, nv, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(hp.data(), c->BCp_t.p, np, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(hg.data(), c->BCg_t.p, ng, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < np; i++)
         if (hp[i] != -1) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "BCp_t must be -1 (no air cells)");
     for (size_t i = 0; i < ng; i++)
         if (hg[i] == 30) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "BCg_t 30 (air) not implemented");
+    // The matrix-free stencils assume MDOODZ's closed-box layout: interior
+    // -1, Dirichlet normal velocity on the walls (tag 0), and one tangential
+    // condition per side on the ghost rows/columns (13 free slip or 11 no slip).
+    const int Nx = c->Nx, Nz = c->Nz;
+    int side[4] = {0, 0, 0, 0};   // S, N, W, E: 1 = tag 13, 2 = tag 11
+    auto kind = [](signed char t) { return t == 13 ? 1 : (t == 11 ? 2 : 0); };
+    for (int l = 0; l < Nz + 1; l++)
+        for (int k = 0; k < Nx; k++) {
+            const signed char t = hu[k + l * Nx];
+            const bool wall = (k == 0 || k == Nx - 1), ghost = (l == 0 || l == Nz);
+            if (ghost && !wall) {
+                const int s = (l == 0) ? 0 : 1, kd = kind(t);
+                if (!kd || (side[s] && side[s] != kd))
+                    return mdg_fail(c, MDG_ERR_UNSUPPORTED, "Vx ghost row %d: tag %d (need one of 13/11 per side)", l, (int)t);
+                side[s] = kd;
+            } else if (wall && !ghost) {
+                if (t != 0) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "Vx wall node (%d,%d) tag %d (need 0)", k, l, (int)t);
+            } else if (!wall && !ghost && t != -1) {
+                return mdg_fail(c, MDG_ERR_UNSUPPORTED, "interior Vx node (%d,%d) tag %d", k, l, (int)t);
+            }
+        }
+    for (int l = 0; l < Nz; l++)
+        for (int k = 0; k < Nx + 1; k++) {
+            const signed char t = hv[k + l * (Nx + 1)];
+            const bool wall = (l == 0 || l == Nz - 1), ghost = (k == 0 || k == Nx);
+            if (ghost && !wall) {
+                const int s = (k == 0) ? 2 : 3, kd = kind(t);
+                if (!kd || (side[s] && side[s] != kd))
+                    return mdg_fail(c, MDG_ERR_UNSUPPORTED, "Vz ghost column %d: tag %d (need one of 13/11 per side)", k, (int)t);
+                side[s] = kd;
+            } else if (wall && !ghost) {
+                if (t != 0) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "Vz wall node (%d,%d) tag %d (need 0)", k, l, (int)t);
+            } else if (!wall && !ghost && t != -1) {
+                return mdg_fail(c, MDG_ERR_UNSUPPORTED, "interior Vz node (%d,%d) tag %d", k, l, (int)t);
+            }
+        }
+    c->sg = sgrid{Nx, Nz, c->dx, c->dz, side[0], side[1], side[2], side[3]};
+    int rc2 = mdg_build_numbering(c, hu, hv, hp);
+    if (rc2) return rc2;
     c->have_bc = true;
     return MDG_OK;
 }

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/mdgpu.h"
+#include "stencil.h"
 
 // Device buffer with capacity growth; freed with the context.
 template <typename T>
@@ -55,6 +56,15 @@ struct mdg_ctx {
     dbuf<double>      table_tmp;     // per-phase tables
     dbuf<signed char> ntype_tmp;
     dbuf<int>         flag_dev;      // stray / status flags
+    // Stokes: grid descriptor for the stencils, DOF numbering, coefficients
+    sgrid             sg = {};
+    int               neq = 0, neq_mom = 0, neq_cont = 0, ndof_norm[2] = {0, 0};
+    dbuf<int>         eqn_u, eqn_v, eqn_p;
+    dbuf<double>      eta_n, eta_s;          // Picard coefficients (D11/2, D33)
+    dbuf<double>      tmp_a, tmp_b, tmp_c, tmp_d;
+    dbuf<double>      gx, gy, eqbuf;         // grid-layout work vectors
+    dbuf<double>      part;                  // reduction partials
+    struct solver_state *solver = nullptr;
     // runtime
     hipStream_t  stream = nullptr;
     hipEvent_t   ev0 = nullptr, ev1 = nullptr;
@@ -97,3 +107,4 @@ int mdg_node_set(mdg_ctx *c, int centroid, node_set *ns);
 
 // Frees the solver's device state (defined with the solver).
 void mdg_solver_release(mdg_ctx *c);
+int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x);

@@ -79,13 +79,32 @@ def lib():
     return _LIB
 
 
+class SolverOpts(C.Structure):
+    """mdg_solver_opts (include/mdgpu.h)."""
+    _fields_ = [("penalty", C.c_double), ("rel_tol", C.c_double), ("max_its", C.c_int),
+                ("restart", C.c_int), ("inner_tol", C.c_double), ("inner_max_its", C.c_int),
+                ("cheb_degree", C.c_int), ("coarse_cells", C.c_int), ("precision", C.c_int),
+                ("precond", C.c_int), ("its", C.c_int), ("inner_its", C.c_int),
+                ("levels", C.c_int), ("status", C.c_int), ("rel_res", C.c_double),
+                ("time_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+MDG_PREC_FP64, MDG_PREC_MIXED = 0, 1
+
+
 def _extra_signatures(L):
-    """Signatures of the Stokes/solver entry points (defined further down)."""
+    """Signatures of the Stokes and solver entry points."""
     vp = C.c_void_p
-    if hasattr(L, "mdg_stokes_residual"):
-        L.mdg_stokes_residual.argtypes = [vp] * 10 + [vp, vp]
-    if hasattr(L, "mdg_stokes_apply"):
-        L.mdg_stokes_apply.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp]
+    L.mdg_eqn_count.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.mdg_stokes_apply.argtypes = [vp] * 6
+    L.mdg_apply_bc.argtypes = [vp, vp, vp]
+    L.mdg_strain_rates.argtypes = [vp] * 7
+    L.mdg_eval_rhs.argtypes = [vp, C.c_double, C.c_double, vp, vp, vp, vp]
+    L.mdg_stokes_residual.argtypes = [vp] * 11
+    L.mdg_killer_solver.argtypes = [vp, C.POINTER(SolverOpts), vp, vp, vp, vp, vp, vp]
 
 
 def ptr(a):
@@ -225,6 +244,56 @@ class Context:
             arr[i] = P2GField(ptr(f["src"]), len(f["src"]), ptr(f["dst"]),
                               ptr(f.get("node_type")), f.get("flag", 0), f.get("avg", 0))
         self._check(self._L.mdg_p2mastah_fused(self.h, centroid, interp_stencil, arr, len(fields)))
+
+    # ------------------------------------------------------------- Stokes
+    def eqn_count(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._check(self._L.mdg_eqn_count(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def stokes_apply(self, D11, D22, D33, x):
+        """y = K x (Picard operator, equation numbering)."""
+        y = np.zeros(len(x))
+        self._check(self._L.mdg_stokes_apply(self.h, ptr(D11), ptr(D22), ptr(D33),
+                                             ptr(np.ascontiguousarray(x)), ptr(y)))
+        return y
+
+    def apply_bc(self, u, v):
+        self._check(self._L.mdg_apply_bc(self.h, ptr(u), ptr(v)))
+
+    def strain_rates(self, u, v):
+        nc = (self.Nx - 1) * (self.Nz - 1)
+        out = dict(div=np.zeros(nc), exxd=np.zeros(nc), ezzd=np.zeros(nc),
+                   exz=np.zeros(self.Nx * self.Nz))
+        self._check(self._L.mdg_strain_rates(self.h, ptr(u), ptr(v), ptr(out["div"]),
+                                             ptr(out["exxd"]), ptr(out["ezzd"]), ptr(out["exz"])))
+        return out
+
+    def eval_rhs(self, gx, gz, rho_n, rho_s):
+        rx = np.zeros(self.Nx * (self.Nz + 1)); rz = np.zeros((self.Nx + 1) * self.Nz)
+        self._check(self._L.mdg_eval_rhs(self.h, gx, gz, ptr(rho_n), ptr(rho_s), ptr(rx), ptr(rz)))
+        return rx, rz
+
+    def stokes_residual(self, sxxd, szzd, sxz, p_corr, div, roger_x, roger_z, d=None):
+        neq = self.eqn_count()[0]
+        F = np.zeros(neq)
+        norms = np.zeros(3)
+        self._check(self._L.mdg_stokes_residual(self.h, ptr(sxxd), ptr(szzd), ptr(sxz),
+                                                ptr(p_corr), ptr(div), ptr(roger_x),
+                                                ptr(roger_z), ptr(d), ptr(F), ptr(norms)))
+        return F, norms
+
+    def killer_solver(self, D11, D22, D33, rhs_mom, rhs_cont, **opts):
+        """Replacement of KillerSolver: returns (sol, SolverOpts)."""
+        o = SolverOpts()
+        for k, v in opts.items():
+            setattr(o, k, v)
+        sol = np.zeros(len(rhs_mom) + len(rhs_cont))
+        rc = self._L.mdg_killer_solver(self.h, C.byref(o), ptr(D11), ptr(D22), ptr(D33),
+                                       ptr(np.ascontiguousarray(rhs_mom)),
+                                       ptr(np.ascontiguousarray(rhs_cont)), ptr(sol))
+        self._check(rc)
+        return sol, o
 
     def last_kernel_ms(self):
         v = C.c_double()

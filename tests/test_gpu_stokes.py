@@ -1,0 +1,104 @@
+"""GPU parity of the Stokes operator, residual and solve against the oracle."""
+import numpy as np
+import pytest
+
+import mdgpu
+from mdgpu import setups as S
+from conftest import solvi_bcs
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(oracle_mod, n=41, bc="pure_shear", contrast=3.0, seed=3):
+    box = (-0.5, 0.5, -0.5, 0.5)
+    g = oracle_mod.Grid(n, n, *box)
+    if bc == "pure_shear":
+        g.pure_shear(1.0)
+    else:
+        solvi_bcs(g, S)
+    x, z = S.put_markers(g.xg, g.zg, g.dx, g.dz, 4, 4, seed=seed)
+    ph = ((x * x + z * z) < 0.2 ** 2).astype(np.int32)
+    eta = np.array([1.0, 10.0 ** contrast])
+    eta_n = oracle_mod.p2mastah(g, x, z, ph, eta, flag=0, avg=0, centroid=1)
+    eta_s = oracle_mod.p2mastah(g, x, z, ph, eta, flag=0, avg=0, centroid=0)
+    ctx = mdgpu.Context(n, n, *box)
+    ctx.set_bc(g.BCu_t, g.BCu_v, g.BCv_t, g.BCv_v, g.BCp_t, g.BCg_t)
+    return g, ctx, eta_n, eta_s
+
+
+@pytest.mark.parametrize("bc", ["pure_shear", "solvi"])
+def test_operator_matches_assembled_matrix(oracle_mod, bc):
+    import scipy.sparse as sp
+    g, ctx, en, es = _problem(oracle_mod, bc=bc)
+    num = g.numbering()
+    A, B, Cm = oracle_mod.assemble_picard(g, num, en, es)
+    K = sp.bmat([[A, B], [Cm, None]]).tocsr()
+    assert ctx.eqn_count() == (num[5], num[3], num[4])
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        xv = rng.standard_normal(num[5])
+        y = ctx.stokes_apply(2 * en, 2 * en, es, xv)
+        ref = K @ xv
+        assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def test_strain_rates_body_force_and_residual(oracle_mod):
+    from oracle.driver import initial_fields
+    g, ctx, en, es = _problem(oracle_mod, bc="solvi")
+    num = g.numbering()
+    rng = np.random.default_rng(1)
+    u, v, p = initial_fields(g)
+    u += 0.1 * rng.standard_normal(u.size) * (g.BCu_t == -1)
+    v += 0.1 * rng.standard_normal(v.size) * (g.BCv_t == -1)
+    p += rng.standard_normal(p.size)
+    ug, vg = u.copy(), v.copy()
+    oracle_mod.apply_bc(g, u, v)
+    ctx.apply_bc(ug, vg)
+    assert np.array_equal(u, ug) and np.array_equal(v, vg)
+    sr = oracle_mod.strain_rates(g, u, v)
+    gs = ctx.strain_rates(u, v)
+    for k in ("div", "exxd", "ezzd", "exz"):
+        np.testing.assert_allclose(gs[k], sr[k], rtol=1e-14, atol=1e-12)
+    rho_n = 3000 + 100 * rng.standard_normal(en.size)
+    rho_s = 3000 + 100 * rng.standard_normal(es.size)
+    rx, rz = oracle_mod.eval_rhs(g, 0.3, -9.81, rho_n, rho_s)
+    grx, grz = ctx.eval_rhs(0.3, -9.81, rho_n, rho_s)
+    np.testing.assert_allclose(grx, rx, rtol=1e-15)
+    np.testing.assert_allclose(grz, rz, rtol=1e-15)
+    sxx, szz, sxz = 2 * en * sr["exxd"], 2 * en * sr["ezzd"], 2 * es * sr["exz"]
+    d = 1.0 + rng.random(num[5])
+    F, nrm = oracle_mod.residual(g, num, sxx, szz, sxz, p, sr["div"], rx, rz, d)
+    Fg, ng = ctx.stokes_residual(sxx, szz, sxz, p, sr["div"], rx, rz, d)
+    np.testing.assert_allclose(Fg, F, rtol=1e-13, atol=1e-13 * np.abs(F).max())
+    np.testing.assert_allclose(ng, nrm, rtol=1e-12)
+
+
+@pytest.mark.parametrize("precision", [mdgpu.MDG_PREC_FP64, mdgpu.MDG_PREC_MIXED])
+@pytest.mark.parametrize("bc", ["pure_shear", "solvi"])
+def test_killer_solver_matches_oracle(oracle_mod, bc, precision):
+    """Defect-correction solve J dx = F: GPU solve vs the oracle KillerSolver
+    (both to tight tolerances) within 1e-8 relative L2."""
+    import scipy.sparse as sp
+    from oracle.driver import evaluate_residual, initial_fields
+    g, ctx, en, es = _problem(oracle_mod, n=51, bc=bc)
+    num = g.numbering()
+    nm = num[3]
+    u, v, p = initial_fields(g)
+    rx, rz = oracle_mod.eval_rhs(g, 0.0, 0.0, np.zeros(en.size), np.zeros(es.size))
+    F, n0 = evaluate_residual(g, num, u, v, p, en, es, rx, rz)
+    A, B, Cm = oracle_mod.assemble_picard(g, num, en, es)
+    du, dp, st, rc = oracle_mod.killer_solve(g, num, A, B, Cm, F[:nm], F[nm:], penalty=1e5,
+                                              max_its_PH=30, rel_tol_KSP=1e-12, max_its_KSP=500,
+                                              lin_abs_div=1e-14, lin_rel_div=1e-14,
+                                              lin_abs_mom=1e-14, lin_rel_mom=1e-14)
+    print("oracle PH its", st.its_PH, "KSP", st.its_KSP_tot, flush=True)
+    sol, o = ctx.killer_solver(2 * en, 2 * en, es, F[:nm], F[nm:], rel_tol=1e-12,
+                               precision=precision)
+    print("gpu its", o.its, "inner", o.inner_its, "levels", o.levels, "rel", o.rel_res,
+          "ms", o.time_ms, flush=True)
+    K = sp.bmat([[A, B], [Cm, None]]).tocsr()
+    assert np.linalg.norm(F - K @ sol) <= 1e-11 * np.linalg.norm(F)
+    ug = sol[:nm]
+    assert np.linalg.norm(ug - du) <= 1e-8 * np.linalg.norm(du)
+    pg, pr = sol[nm:] - sol[nm:].mean(), dp - dp.mean()
+    assert np.linalg.norm(pg - pr) <= 1e-8 * np.linalg.norm(pr)
