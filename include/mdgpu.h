@@ -52,6 +52,8 @@ const char *mdg_last_error(const mdg_ctx *ctx);
 int mdg_dev_alloc(mdg_ctx *ctx, unsigned long long bytes, void **ptr);
 int mdg_dev_free(mdg_ctx *ctx, void *ptr);
 int mdg_memcpy(mdg_ctx *ctx, void *dst, const void *src, unsigned long long bytes);
+/* y = a*x on device arrays of n doubles (e.g. D11_n = 2*eta_n). */
+int mdg_dev_scale(mdg_ctx *ctx, long long n, double a, const double *x, double *y);
 
 /* Coordinates as computed by the library (host copies), lengths Nx, Nz,
  * Nx-1, Nz-1, Nx+1, Nz+1.  Mirrors mesh->xg_coord ... (GridRoutines.c:246-330). */
@@ -160,23 +162,38 @@ int mdg_stokes_residual(mdg_ctx *ctx, const double *sxxd, const double *szzd, co
 #define MDG_PREC_FP64  0
 #define MDG_PREC_MIXED 1
 typedef struct {
-    double penalty;        /* gamma of the viscosity-weighted penalty (1)      */
+    double penalty;        /* gamma of the viscosity-weighted penalty (10)     */
     double rel_tol;        /* ||r||/||r0|| target (1e-8)                       */
     int    max_its;        /* outer iterations (500)                           */
-    int    restart;        /* GCR restart length (20)                          */
-    double inner_tol;      /* relative tolerance of the velocity CG (1e-3)     */
-    int    inner_max_its;  /* (60)                                             */
-    int    cheb_degree;    /* Chebyshev smoothing degree per sweep (3)         */
+    int    restart;        /* GCR restart length (100)                         */
+    double inner_tol;      /* relative tolerance of the velocity CG (1e-6)     */
+    int    inner_max_its;  /* (80)                                             */
+    int    cheb_degree;    /* Chebyshev smoothing degree per sweep (8)         */
     int    coarse_cells;   /* coarsen until <= this many cells per side (8)    */
     int    precision;      /* MDG_PREC_FP64 or MDG_PREC_MIXED                  */
     int    precond;        /* reserved: 0 = multigrid                          */
+    int    cycle;          /* 1 V-cycle, 2 W-cycle (1)                         */
+    int    eta_avg;        /* coarse viscosity: 0 geometric, 1 arithmetic,
+                              2 harmonic, 3 maximum (0)                        */
+    int    check_every;    /* inner CG convergence test period (1)             */
+    int    schur;          /* Schur complement approximation: 0 viscosity-
+                              weighted mass, 1 scaled BFBt (0)                 */
+    double poisson_tol;    /* BFBt pressure-Poisson CG tolerance (1e-3)        */
+    int    poisson_max_its;/* (30)                                             */
     /* outputs */
-    int    its, inner_its, levels, status;
+    int    its, inner_its, poisson_its, levels, status;
     double rel_res, time_ms;
 } mdg_solver_opts;
 int mdg_killer_solver(mdg_ctx *ctx, mdg_solver_opts *opts, const double *D11_n,
                       const double *D22_n, const double *D33_s, const double *rhs_mom,
                       const double *rhs_cont, double *sol);
+
+/* Times the dominant kernel of the solve: the fine-level multigrid smoother
+ * operator (r -= A_g d, FP64) of the last FP64 solve, averaged over reps
+ * launches with HIP events on the context stream.  Returns ms per launch and
+ * the algorithmic (compulsory) bytes per launch.  No counterpart in the
+ * reference; it feeds the roofline figure of the benchmark. */
+int mdg_time_smoother(mdg_ctx *ctx, int reps, double *ms_per_launch, double *bytes_per_launch);
 
 /* Time of the last P2G kernel (ms, HIP events on the context stream) and
  * the number of markers it processed. */

@@ -227,3 +227,18 @@ extern "C" int mdg_memcpy(mdg_ctx *c, void *dst, const void *src, unsigned long 
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDG_OK;
 }
+
+__global__ void k_dev_scale(long n, double a, const double *__restrict__ x, double *__restrict__ y) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i];
+}
+
+extern "C" int mdg_dev_scale(mdg_ctx *c, long long n, double a, const double *x, double *y) {
+    if (!c || n < 0 || (n && (!x || !y))) return MDG_ERR_ARG;
+    if (n == 0) return MDG_OK;
+    if (!mdg_is_device_ptr(x) || !mdg_is_device_ptr(y)) return mdg_fail(c, MDG_ERR_ARG, "device arrays expected");
+    hipLaunchKernelGGL(k_dev_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (long)n, a, x, y);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDG_OK;
+}

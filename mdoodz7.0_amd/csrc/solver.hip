@@ -73,6 +73,73 @@ __global__ void k_axpby(int n, T a, const T *__restrict__ x, T b, T *__restrict_
     if (i < n) y[i] = a * x[i] + b * y[i];
 }
 
+// Pseudo-random +-1 start vector on the free unknowns (power iteration):
+// a smooth start would converge slowly to the oscillatory top eigenvector.
+template <typename T>
+__global__ void k_rand_free(int n, const T *__restrict__ dinv, T *__restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned h = (unsigned)i * 2654435761u;
+    h ^= h >> 15; h *= 2246822519u; h ^= h >> 13;
+    x[i] = (dinv[i] != T(0)) ? ((h & 1u) ? T(1) : T(-1)) * (T(0.5) + T((h >> 8) & 255u) / T(512)) : T(0);
+}
+
+template <typename T>
+__global__ void k_copy(int n, const T *__restrict__ a, T *__restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+// sums nb partials into scal[dst]; optionally scal[to] = scal[from] afterwards
+__global__ void __launch_bounds__(RB) k_sum_store(int nb, const double *__restrict__ part, double *__restrict__ scal,
+                                                  int dst, int from, int to) {
+    __shared__ double s[RB];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += RB) acc += part[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = RB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        scal[dst] = s[0];
+        if (to >= 0) scal[to] = scal[from];
+    }
+}
+// CG direction: p = z + (rz_new / rz_old) p   (beta = 0 when rz_old == 0)
+template <typename T>
+__global__ void k_pcg_dir(int n, const T *__restrict__ z, T *__restrict__ p, const double *__restrict__ scal) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double rzo = scal[0];
+    const T beta = (rzo > 0.0) ? (T)(scal[1] / rzo) : T(0);
+    p[i] = z[i] + beta * p[i];
+}
+// x += a p, r -= a q with a = rz/pq (0 when pq <= 0), partials of r.r
+template <typename T>
+__global__ void __launch_bounds__(RB) k_pcg_step(int n, const T *__restrict__ p, const T *__restrict__ q,
+                                                 T *__restrict__ x, T *__restrict__ r,
+                                                 const double *__restrict__ scal, double *__restrict__ part) {
+    __shared__ double s[RB];
+    const int i = blockIdx.x * RB + threadIdx.x;
+    const double pq = scal[2];
+    const T a = (pq > 0.0) ? (T)(scal[1] / pq) : T(0);
+    double acc = 0.0;
+    if (i < n) {
+        x[i] += a * p[i];
+        const T ri = r[i] - a * q[i];
+        r[i] = ri;
+        acc = (double)ri * (double)ri;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = RB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
 // --------------------------------------------------- multigrid level kernels
 // r = b - A_g x   (or y = A_g x when b == nullptr, negated); velocity only.
 template <typename T>
@@ -293,8 +360,15 @@ __global__ void k_restrict(sgrid gf, sgrid gc, comp_maps mu, comp_maps mv, const
 
 // Coarse viscosity: geometric mean of the fine values whose positions fall
 // in the coarse control volume (cells for eta_n, dual cells for eta_s).
+__device__ inline double avg_in(int mode, double v) {
+    return mode == 0 ? log(v) : (mode == 2 ? 1.0 / v : v);
+}
+__device__ inline double avg_out(int mode, double s, int n) {
+    return mode == 0 ? exp(s / n) : (mode == 2 ? n / s : (mode == 3 ? s : s / n));
+}
+// mode 0 geometric, 1 arithmetic, 2 harmonic, 3 maximum
 template <typename T>
-__global__ void k_coarsen_eta(sgrid gf, sgrid gc, const T *__restrict__ enf, const T *__restrict__ esf,
+__global__ void k_coarsen_eta(sgrid gf, sgrid gc, int mode, const T *__restrict__ enf, const T *__restrict__ esf,
                               T *__restrict__ enc, T *__restrict__ esc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncc = ncell(gc), nvc = gc.Nx * gc.Nz;
@@ -309,8 +383,12 @@ __global__ void k_coarsen_eta(sgrid gf, sgrid gc, const T *__restrict__ enf, con
         double s = 0.0;
         int n = 0;
         for (int l = l0; l <= l1; l++)
-            for (int k = k0; k <= k1; k++) { s += log((double)enf[k + l * (gf.Nx - 1)]); n++; }
-        enc[i] = (T)exp(s / n);
+            for (int k = k0; k <= k1; k++) {
+                const double e = (double)enf[k + l * (gf.Nx - 1)];
+                s = (mode == 3) ? fmax(s, e) : s + avg_in(mode, e);
+                n++;
+            }
+        enc[i] = (T)avg_out(mode, s, n);
     } else if (i < ncc + nvc) {
         const int j = i - ncc, K = j % gc.Nx, L = j / gc.Nx;
         int k0 = (int)ceil((K - 0.5) * Hx / hx), k1 = (int)ceil((K + 0.5) * Hx / hx) - 1;
@@ -320,8 +398,12 @@ __global__ void k_coarsen_eta(sgrid gf, sgrid gc, const T *__restrict__ enf, con
         double s = 0.0;
         int n = 0;
         for (int l = l0; l <= l1; l++)
-            for (int k = k0; k <= k1; k++) { s += log((double)esf[k + l * gf.Nx]); n++; }
-        esc[j] = (T)exp(s / n);
+            for (int k = k0; k <= k1; k++) {
+                const double e = (double)esf[k + l * gf.Nx];
+                s = (mode == 3) ? fmax(s, e) : s + avg_in(mode, e);
+                n++;
+            }
+        esc[j] = (T)avg_out(mode, s, n);
     }
 }
 
@@ -437,6 +519,168 @@ __global__ void k_aug_rhs(sgrid g, double gamma, const double *__restrict__ b, c
     }
 }
 
+
+// ------------------------------------------------ pressure Poisson operator
+// L = C diag(A_g)^-1 C' on the cells of a level, from the velocity Jacobi
+// inverse diagonal dinv (zero on non-free faces, so wall faces drop out):
+//   (L y)_c = sum over faces f of w_f (y_c - y_neighbour),
+//   w = dz^2 dinv on Vx faces, dx^2 dinv on Vz faces.
+// It is the operator of the scaled BFBt Schur-complement approximation.
+template <typename T>
+__device__ inline T lap_row(const sgrid &g, int k, int l, const T *__restrict__ dinv,
+                            const T *__restrict__ y, T *diag) {
+    const int Nx = g.Nx, Ncx = Nx - 1, Ncz = g.Nz - 1, nu = nvx(g);
+    const T dx2 = T(g.dx * g.dx), dz2 = T(g.dz * g.dz);
+    const int c = k + l * Ncx;
+    const T wW = dz2 * dinv[k + (l + 1) * Nx], wE = dz2 * dinv[k + 1 + (l + 1) * Nx];
+    const T wS = dx2 * dinv[nu + (k + 1) + l * (Nx + 1)], wN = dx2 * dinv[nu + (k + 1) + (l + 1) * (Nx + 1)];
+    const T d = wW + wE + wS + wN;
+    if (diag) *diag = d;
+    if (!y) return T(0);
+    T s = d * y[c];
+    if (k > 0) s -= wW * y[c - 1];
+    if (k < Ncx - 1) s -= wE * y[c + 1];
+    if (l > 0) s -= wS * y[c - Ncx];
+    if (l < Ncz - 1) s -= wN * y[c + Ncx];
+    return s;
+}
+template <typename T>
+__global__ void k_lap_residual(sgrid g, const T *__restrict__ dinv, const T *__restrict__ y,
+                               const T *__restrict__ b, T *__restrict__ r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncell(g)) return;
+    const int k = i % (g.Nx - 1), l = i / (g.Nx - 1);
+    r[i] = (b ? b[i] : T(0)) - lap_row<T>(g, k, l, dinv, y, nullptr);
+}
+template <typename T>
+__global__ void k_lap_cheb_res(sgrid g, const T *__restrict__ dinv, const T *__restrict__ d, T *__restrict__ r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncell(g)) return;
+    const int k = i % (g.Nx - 1), l = i / (g.Nx - 1);
+    r[i] -= lap_row<T>(g, k, l, dinv, d, nullptr);
+}
+template <typename T>
+__global__ void k_lap_diag_inv(sgrid g, const T *__restrict__ dinv, T *__restrict__ pdinv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncell(g)) return;
+    const int k = i % (g.Nx - 1), l = i / (g.Nx - 1);
+    T d;
+    (void)lap_row<T>(g, k, l, dinv, (const T *)nullptr, &d);
+    pdinv[i] = T(1) / d;
+}
+template <typename T>
+__global__ void __launch_bounds__(RB) k_lap_apply_dot(sgrid g, const T *__restrict__ dinv, const T *__restrict__ p,
+                                                      T *__restrict__ q, double *__restrict__ part) {
+    __shared__ double s[RB];
+    const int i = blockIdx.x * RB + threadIdx.x;
+    double acc = 0.0;
+    if (i < ncell(g)) {
+        const int k = i % (g.Nx - 1), l = i / (g.Nx - 1);
+        const T v = lap_row<T>(g, k, l, dinv, p, nullptr);
+        q[i] = v;
+        acc = (double)v * (double)p[i];
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = RB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+// cell-centred transfers (bilinear on cell centres, R = P^T)
+template <typename T>
+__global__ void k_prolong_c(sgrid gf, axis_map xm, axis_map zm, int ncxc, const T *__restrict__ xc, T *__restrict__ xf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncell(gf)) return;
+    const int k = i % (gf.Nx - 1), l = i / (gf.Nx - 1);
+    int K0, L0;
+    double wx, wz;
+    bracket(xm, k, &K0, &wx);
+    bracket(zm, l, &L0, &wz);
+    xf[i] += T((1 - wx) * (1 - wz)) * xc[K0 + L0 * ncxc] + T(wx * (1 - wz)) * xc[K0 + 1 + L0 * ncxc] +
+             T((1 - wx) * wz) * xc[K0 + (L0 + 1) * ncxc] + T(wx * wz) * xc[K0 + 1 + (L0 + 1) * ncxc];
+}
+template <typename T>
+__global__ void k_restrict_c(sgrid gf, sgrid gc, axis_map xm, axis_map zm, const T *__restrict__ rf, T *__restrict__ bc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncell(gc)) return;
+    const int K = i % (gc.Nx - 1), L = i / (gc.Nx - 1), ncxf = gf.Nx - 1, nczf = gf.Nz - 1;
+    const double tx0 = ((K - 1 + xm.off_c) * xm.hc) / xm.hf - xm.off_f, tx1 = ((K + 1 + xm.off_c) * xm.hc) / xm.hf - xm.off_f;
+    const double tz0 = ((L - 1 + zm.off_c) * zm.hc) / zm.hf - zm.off_f, tz1 = ((L + 1 + zm.off_c) * zm.hc) / zm.hf - zm.off_f;
+    int k0 = (int)floor(tx0) - 1, k1 = (int)ceil(tx1) + 1, l0 = (int)floor(tz0) - 1, l1 = (int)ceil(tz1) + 1;
+    k0 = k0 < 0 ? 0 : k0; l0 = l0 < 0 ? 0 : l0;
+    k1 = k1 > ncxf - 1 ? ncxf - 1 : k1; l1 = l1 > nczf - 1 ? nczf - 1 : l1;
+    double s = 0.0;
+    for (int l = l0; l <= l1; l++) {
+        const double wz = pweight(zm, l, L);
+        if (wz == 0.0) continue;
+        for (int k = k0; k <= k1; k++) {
+            const double wx = pweight(xm, k, K);
+            if (wx != 0.0) s += wx * wz * (double)rf[k + l * ncxf];
+        }
+    }
+    bc[i] = (T)s;
+}
+// sum of entries -> scal[dst]; subtract mean: x -= scal[dst]/n
+template <typename T>
+__global__ void __launch_bounds__(RB) k_sum_part_vec(int n, const T *__restrict__ a, double *__restrict__ part) {
+    __shared__ double s[RB];
+    double acc = 0.0;
+    for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) acc += (double)a[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = RB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+template <typename T>
+__global__ void k_sub_mean(int n, T *__restrict__ x, const double *__restrict__ scal, int idx, T sign) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = sign * (x[i] - (T)(scal[idx] / n));
+}
+// t = dinv * (C' y) on free velocity faces ((C'y)_f = dz (y_W - y_E) on Vx, dx (y_S - y_N) on Vz)
+template <typename T>
+__global__ void k_grad_scaled(sgrid g, const T *__restrict__ dinv, const T *__restrict__ y, T *__restrict__ t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu = nvx(g), nv = nvz(g), Ncx = g.Nx - 1;
+    if (i < nu) {
+        const int k = i % g.Nx, l = i / g.Nx;
+        t[i] = vx_free(g, k, l) ? dinv[i] * T(g.dz) * (y[k - 1 + (l - 1) * Ncx] - y[k + (l - 1) * Ncx]) : T(0);
+    } else if (i < nu + nv) {
+        const int j = i - nu, k = j % (g.Nx + 1), l = j / (g.Nx + 1);
+        t[i] = vz_free(g, k, l) ? dinv[i] * T(g.dx) * (y[k - 1 + (l - 1) * Ncx] - y[k - 1 + l * Ncx]) : T(0);
+    }
+}
+// m = C (s * dinv * t) on cells (the A_g t product enters negated, s = -1)
+template <typename T>
+__global__ void k_div_scaled(sgrid g, const T *__restrict__ dinv, const T *__restrict__ t, T s, T *__restrict__ m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncell(g)) return;
+    const int k = i % (g.Nx - 1), l = i / (g.Nx - 1), Nx = g.Nx, nu = nvx(g);
+    const int fe = k + 1 + (l + 1) * Nx, fw = k + (l + 1) * Nx;
+    const int fn = nu + k + 1 + (l + 1) * (Nx + 1), fs = nu + k + 1 + l * (Nx + 1);
+    m[i] = s * (T(g.dz) * (dinv[fe] * t[fe] - dinv[fw] * t[fw]) + T(g.dx) * (dinv[fn] * t[fn] - dinv[fs] * t[fs]));
+}
+// rhs_u = r_u - B zp (zp of type T) and the FP64 pressure part of z
+template <typename T>
+__global__ void k_rhs_from_zp(sgrid g, const double *__restrict__ r, const T *__restrict__ zp,
+                              double *__restrict__ zout, T *__restrict__ ru) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu = nvx(g), nv = nvz(g), np = ncell(g), Ncx = g.Nx - 1;
+    if (i < nu) {
+        const int k = i % g.Nx, l = i / g.Nx;
+        ru[i] = vx_free(g, k, l) ? (T)(r[i] - ((double)zp[k + (l - 1) * Ncx] - (double)zp[k - 1 + (l - 1) * Ncx]) * g.dz) : T(0);
+    } else if (i < nu + nv) {
+        const int j = i - nu, k = j % (g.Nx + 1), l = j / (g.Nx + 1);
+        ru[i] = vz_free(g, k, l) ? (T)(r[i] - ((double)zp[k - 1 + l * Ncx] - (double)zp[k - 1 + (l - 1) * Ncx]) * g.dx) : T(0);
+    } else if (i < nu + nv + np) {
+        zout[i] = (double)zp[i - nu - nv];
+    }
+}
+
 // ----------------------------------------------------------- host structures
 template <typename T>
 struct mg_level {
@@ -449,9 +693,14 @@ struct mg_level {
     int nfree = 0;
     dbuf<int> idx;
     dbuf<T> Ainv;
+    // pressure Poisson (BFBt) on the same cells
+    dbuf<T> px, pb, pr, pd, pdinv, pAinv;
+    double plmax = 1.0;
+    axis_map cxm, czm;     // cell-centre transfer to the next coarser level
     void release() {
         en.release(); es.release(); dinv.release(); x.release(); b.release(); r.release();
         d.release(); idx.release(); Ainv.release();
+        px.release(); pb.release(); pr.release(); pd.release(); pdinv.release(); pAinv.release();
     }
 };
 
@@ -459,10 +708,38 @@ template <typename T>
 struct mg_hier {
     std::vector<mg_level<T>> L;
     dbuf<T> cg_x, cg_r, cg_z, cg_p, cg_q;   // inner CG vectors (fine velocity size)
+    dbuf<double> scal, part;                // CG scalars (rz_old, rz_new, pq, rr) and partials
+    double *h_scal = nullptr;               // pinned host copy of scal
+    hipGraph_t graph = nullptr;             // one CG iteration (V-cycle included)
+    hipGraphExec_t exec = nullptr;
+    // pressure Poisson CG (BFBt): vectors, scalars and graph
+    dbuf<T> pc_x, pc_r, pc_p, pc_q, bf_t;
+    dbuf<double> pscal, ppart;
+    double *h_pscal = nullptr;
+    hipGraph_t pgraph = nullptr;
+    hipGraphExec_t pexec = nullptr;
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (pexec) (void)hipGraphExecDestroy(pexec);
+        if (pgraph) (void)hipGraphDestroy(pgraph);
+        exec = nullptr;
+        graph = nullptr;
+        pexec = nullptr;
+        pgraph = nullptr;
+    }
     void release() {
+        drop_graph();
         for (auto &l : L) l.release();
         L.clear();
         cg_x.release(); cg_r.release(); cg_z.release(); cg_p.release(); cg_q.release();
+        scal.release(); part.release();
+        pc_x.release(); pc_r.release(); pc_p.release(); pc_q.release(); bf_t.release();
+        pscal.release(); ppart.release();
+        if (h_scal) (void)hipHostFree(h_scal);
+        if (h_pscal) (void)hipHostFree(h_pscal);
+        h_scal = nullptr;
+        h_pscal = nullptr;
     }
 };
 
@@ -528,8 +805,13 @@ static void host_gauss_jordan_inverse(std::vector<double> &A, int n, bool *ok) {
 }
 
 template <typename T>
+static int build_poisson(mdg_ctx *c, solver_state *s, mg_hier<T> &H);
+
+template <typename T>
 static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solver_opts *o) {
-    H.release();
+    H.drop_graph();
+    for (auto &l : H.L) l.release();
+    H.L.clear();
     const double gamma = o->penalty;
     sgrid g = c->sg;
     const double Lx = c->xmax - c->xmin, Lz = c->zmax - c->zmin;
@@ -551,7 +833,7 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
         } else {
             mg_level<T> &fl = H.L[H.L.size() - 2];
             hipLaunchKernelGGL(k_coarsen_eta<T>, dim3(nblk(ncell(g) + g.Nx * g.Nz)), dim3(256), 0, c->stream, fl.g, g,
-                               fl.en.p, fl.es.p, lv.en.p, lv.es.p);
+                               o->eta_avg, fl.en.p, fl.es.p, lv.en.p, lv.es.p);
         }
         hipLaunchKernelGGL(k_vel_diag_inv<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, g, (T)gamma, lv.en.p, lv.es.p, lv.dinv.p);
         HIPCHK(c, hipGetLastError());
@@ -563,6 +845,8 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
                           g.Nx, g.Nz + 1, gc.Nx, gc.Nz + 1, 0, g.ghostS, g.ghostN};
         lv.mv = comp_maps{make_axis(g.dx, gc.dx, -0.5, gc.Nx + 1), make_axis(g.dz, gc.dz, 0.0, gc.Nz),
                           g.Nx + 1, g.Nz, gc.Nx + 1, gc.Nz, 1, g.ghostW, g.ghostE};
+        lv.cxm = make_axis(g.dx, gc.dx, 0.5, gc.Nx - 1);
+        lv.czm = make_axis(g.dz, gc.dz, 0.5, gc.Nz - 1);
         g = gc;
         ncx = ncx2;
         ncz = ncz2;
@@ -570,10 +854,9 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
     // Chebyshev bounds: power iteration on Dinv*A_g per level
     for (size_t li = 0; li + 1 < H.L.size(); li++) {
         mg_level<T> &lv = H.L[li];
-        // start vector: the Jacobi diagonal (zero on the non-free entries)
-        HIPCHK(c, hipMemcpyAsync(lv.x.p, lv.dinv.p, lv.n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+        hipLaunchKernelGGL(k_rand_free<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.n, (const T *)lv.dinv.p, lv.x.p);
         double lam = 1.0;
-        for (int it = 0; it < 12; it++) {
+        for (int it = 0; it < 30; it++) {
             hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p,
                                (const T *)nullptr, lv.en.p, lv.es.p, lv.r.p);     // r = -A x
             hipLaunchKernelGGL(k_scale_dinv<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.n, T(-1), lv.dinv.p, lv.r.p, lv.d.p);
@@ -584,7 +867,7 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
             const T sc = (T)(1.0 / sqrt(yy));
             hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.n, sc, lv.d.p, T(0), lv.x.p);
         }
-        lv.lmax = 1.1 * lam;
+        lv.lmax = 1.2 * lam;   // power iteration approaches lambda_max from below
     }
     // coarsest: dense inverse of A_g on its free unknowns, assembled by
     // applying the operator to unit vectors on the device
@@ -616,10 +899,215 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
     for (size_t i = 0; i < A.size(); i++) Ai[i] = (T)A[i];
     HIPCHK(c, cl.Ainv.reserve(Ai.size()));
     HIPCHK(c, hipMemcpyAsync(cl.Ainv.p, Ai.data(), Ai.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    if (o->schur == 1) {
+        int rc = build_poisson(c, s, H);
+        if (rc) return rc;
+    }
     const int n0 = H.L[0].n;
     HIPCHK(c, H.cg_x.reserve(n0)); HIPCHK(c, H.cg_r.reserve(n0)); HIPCHK(c, H.cg_z.reserve(n0));
     HIPCHK(c, H.cg_p.reserve(n0)); HIPCHK(c, H.cg_q.reserve(n0));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDG_OK;
+}
+
+
+// ------------------------------------------------ pressure Poisson hierarchy
+template <typename T>
+__global__ void k_dense_full(int n, const T *__restrict__ Ainv, const T *__restrict__ b, T *__restrict__ x) {
+    __shared__ T sb[1024];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) sb[j] = b[j];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double s = 0.0;
+        for (int j = 0; j < n; j++) s += (double)Ainv[(size_t)i * n + j] * (double)sb[j];
+        x[i] = (T)s;
+    }
+}
+
+template <typename T>
+static int build_poisson(mdg_ctx *c, solver_state *s, mg_hier<T> &H) {
+    for (size_t li = 0; li < H.L.size(); li++) {
+        mg_level<T> &lv = H.L[li];
+        const int nc = ncell(lv.g);
+        HIPCHK(c, lv.px.reserve(nc)); HIPCHK(c, lv.pb.reserve(nc)); HIPCHK(c, lv.pr.reserve(nc));
+        HIPCHK(c, lv.pd.reserve(nc)); HIPCHK(c, lv.pdinv.reserve(nc));
+        hipLaunchKernelGGL(k_lap_diag_inv<T>, dim3(nblk(nc)), dim3(256), 0, c->stream, lv.g, (const T *)lv.dinv.p, lv.pdinv.p);
+        if (li + 1 == H.L.size()) break;
+        hipLaunchKernelGGL(k_rand_free<T>, dim3(nblk(nc)), dim3(256), 0, c->stream, nc, (const T *)lv.pdinv.p, lv.px.p);
+        double lam = 1.0;
+        for (int it = 0; it < 30; it++) {
+            hipLaunchKernelGGL(k_lap_residual<T>, dim3(nblk(nc)), dim3(256), 0, c->stream, lv.g, (const T *)lv.dinv.p,
+                               (const T *)lv.px.p, (const T *)nullptr, lv.pr.p);
+            hipLaunchKernelGGL(k_scale_dinv<T>, dim3(nblk(nc)), dim3(256), 0, c->stream, nc, T(-1), lv.pdinv.p, lv.pr.p, lv.pd.p);
+            double yy, xx;
+            int rc;
+            if ((rc = dot_host<T>(c, s, nc, lv.pd.p, lv.pd.p, &yy)) || (rc = dot_host<T>(c, s, nc, lv.px.p, lv.px.p, &xx))) return rc;
+            lam = sqrt(yy / xx);
+            hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(nc)), dim3(256), 0, c->stream, nc, (T)(1.0 / sqrt(yy)), lv.pd.p, T(0), lv.px.p);
+        }
+        lv.plmax = 1.2 * lam;
+    }
+    // coarsest: dense inverse of L + c 1 1' (L is singular with constants in
+    // its kernel; the rank-one term makes the solve a pseudo-inverse on
+    // mean-free right-hand sides)
+    mg_level<T> &cl = H.L.back();
+    const int nc = ncell(cl.g);
+    if (nc > 1024) return mdg_fail(c, MDG_ERR_ARG, "coarsest pressure level too large");
+    std::vector<double> A((size_t)nc * nc);
+    std::vector<T> e(nc), col(nc), dg(nc);
+    for (int j = 0; j < nc; j++) {
+        std::fill(e.begin(), e.end(), T(0));
+        e[j] = T(1);
+        HIPCHK(c, hipMemcpyAsync(cl.px.p, e.data(), nc * sizeof(T), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_lap_residual<T>, dim3(nblk(nc)), dim3(256), 0, c->stream, cl.g, (const T *)cl.dinv.p,
+                           (const T *)cl.px.p, (const T *)nullptr, cl.pr.p);
+        HIPCHK(c, hipMemcpyAsync(col.data(), cl.pr.p, nc * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < nc; i++) A[(size_t)i * nc + j] = -(double)col[i];
+    }
+    double mean_diag = 0.0;
+    for (int i = 0; i < nc; i++) mean_diag += A[(size_t)i * nc + i] / nc;
+    for (size_t q = 0; q < A.size(); q++) A[q] += mean_diag / nc;
+    bool ok;
+    host_gauss_jordan_inverse(A, nc, &ok);
+    if (!ok) return mdg_fail(c, MDG_ERR_NOT_SPD, "singular coarsest pressure operator");
+    std::vector<T> Ai(A.size());
+    for (size_t q = 0; q < A.size(); q++) Ai[q] = (T)A[q];
+    HIPCHK(c, cl.pAinv.reserve(Ai.size()));
+    HIPCHK(c, hipMemcpyAsync(cl.pAinv.p, Ai.data(), Ai.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    const int n0 = ncell(H.L[0].g), nb = mdg_grid1d(n0, RB);
+    HIPCHK(c, H.pc_x.reserve(n0)); HIPCHK(c, H.pc_r.reserve(n0)); HIPCHK(c, H.pc_p.reserve(n0));
+    HIPCHK(c, H.pc_q.reserve(n0)); HIPCHK(c, H.bf_t.reserve(H.L[0].n));
+    HIPCHK(c, H.ppart.reserve((size_t)nb + 1024)); HIPCHK(c, H.pscal.reserve(8));
+    if (!H.h_pscal) HIPCHK(c, hipHostMalloc((void **)&H.h_pscal, 8 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDG_OK;
+}
+
+template <typename T>
+static void pchebyshev(mdg_ctx *c, mg_level<T> &lv, int deg, bool zero_start) {
+    const double lmax = lv.plmax, lmin = lmax / 30.0;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    const int n = ncell(lv.g);
+    if (zero_start) {
+        hipLaunchKernelGGL(k_copy<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (const T *)lv.pb.p, lv.pr.p);
+        hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, lv.px.p, T(0));
+    } else {
+        hipLaunchKernelGGL(k_lap_residual<T>, dim3(nblk(n)), dim3(256), 0, c->stream, lv.g, (const T *)lv.dinv.p,
+                           (const T *)lv.px.p, (const T *)lv.pb.p, lv.pr.p);
+    }
+    hipLaunchKernelGGL(k_scale_dinv<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)(1.0 / theta), lv.pdinv.p, lv.pr.p, lv.pd.p);
+    for (int k = 0; k < deg - 1; k++) {
+        hipLaunchKernelGGL(k_lap_cheb_res<T>, dim3(nblk(n)), dim3(256), 0, c->stream, lv.g, (const T *)lv.dinv.p, (const T *)lv.pd.p, lv.pr.p);
+        const double rho_new = 1.0 / (2.0 * sigma - rho);
+        hipLaunchKernelGGL(k_cheb_upd<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)(rho_new * rho),
+                           (T)(2.0 * rho_new / delta), lv.pdinv.p, lv.pr.p, lv.pd.p, lv.px.p);
+        rho = rho_new;
+    }
+    hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, T(1), lv.pd.p, T(1), lv.px.p);
+}
+
+template <typename T>
+static void pvcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, int deg) {
+    mg_level<T> &lv = H.L[i];
+    const int n = ncell(lv.g);
+    if (i + 1 == H.L.size()) {
+        hipLaunchKernelGGL(k_dense_full<T>, dim3(1), dim3(256), 0, c->stream, n, (const T *)lv.pAinv.p, (const T *)lv.pb.p, lv.px.p);
+        return;
+    }
+    mg_level<T> &cl = H.L[i + 1];
+    pchebyshev(c, lv, deg, true);
+    hipLaunchKernelGGL(k_lap_residual<T>, dim3(nblk(n)), dim3(256), 0, c->stream, lv.g, (const T *)lv.dinv.p,
+                       (const T *)lv.px.p, (const T *)lv.pb.p, lv.pr.p);
+    hipLaunchKernelGGL(k_restrict_c<T>, dim3(nblk(ncell(cl.g))), dim3(256), 0, c->stream, lv.g, cl.g, lv.cxm, lv.czm,
+                       (const T *)lv.pr.p, cl.pb.p);
+    pvcycle(c, H, i + 1, deg);
+    hipLaunchKernelGGL(k_prolong_c<T>, dim3(nblk(n)), dim3(256), 0, c->stream, lv.g, lv.cxm, lv.czm, cl.g.Nx - 1,
+                       (const T *)cl.px.p, lv.px.p);
+    pchebyshev(c, lv, deg, false);
+}
+
+// one CG iteration on L (recorded for graph capture), scalars in pscal
+template <typename T>
+static void poisson_iteration(mdg_ctx *c, mg_hier<T> &H, int deg) {
+    mg_level<T> &l0 = H.L[0];
+    const int n = ncell(l0.g), nb = mdg_grid1d(n, RB), nd = std::min(1024, nb);
+    hipLaunchKernelGGL(k_copy<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (const T *)H.pc_r.p, l0.pb.p);
+    pvcycle(c, H, 0, deg);
+    const T *z = l0.px.p;
+    hipLaunchKernelGGL(k_dot_part<T>, dim3(nd), dim3(RB), 0, c->stream, n, (const T *)H.pc_r.p, z, H.ppart.p);
+    hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nd, (const double *)H.ppart.p, H.pscal.p, 1, -1, -1);
+    hipLaunchKernelGGL(k_pcg_dir<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, z, H.pc_p.p, (const double *)H.pscal.p);
+    hipLaunchKernelGGL(k_lap_apply_dot<T>, dim3(nb), dim3(RB), 0, c->stream, l0.g, (const T *)l0.dinv.p, (const T *)H.pc_p.p,
+                       H.pc_q.p, H.ppart.p);
+    hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nb, (const double *)H.ppart.p, H.pscal.p, 2, -1, -1);
+    hipLaunchKernelGGL(k_pcg_step<T>, dim3(nb), dim3(RB), 0, c->stream, n, (const T *)H.pc_p.p, (const T *)H.pc_q.p,
+                       H.pc_x.p, H.pc_r.p, (const double *)H.pscal.p, H.ppart.p);
+    hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nb, (const double *)H.ppart.p, H.pscal.p, 3, 1, 0);
+}
+
+// remove the mean of a cell vector (sign -1 also negates)
+template <typename T>
+static void remove_mean(mdg_ctx *c, mg_hier<T> &H, T *x, int n, T sign) {
+    const int nd = std::min(1024, mdg_grid1d(n, RB));
+    hipLaunchKernelGGL(k_sum_part_vec<T>, dim3(nd), dim3(RB), 0, c->stream, n, (const T *)x, H.ppart.p);
+    hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nd, (const double *)H.ppart.p, H.pscal.p, 4, -1, -1);
+    hipLaunchKernelGGL(k_sub_mean<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, x, (const double *)H.pscal.p, 4, sign);
+}
+
+// x = L^-1 b by multigrid-preconditioned CG (b in pc_r, mean-free; x -> pc_x)
+template <typename T>
+static int poisson_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solver_opts *o, int *its) {
+    const int n = ncell(H.L[0].g);
+    if (!H.pexec) {
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        poisson_iteration(c, H, o->cheb_degree);
+        HIPCHK(c, hipStreamEndCapture(c->stream, &H.pgraph));
+        HIPCHK(c, hipGraphInstantiate(&H.pexec, H.pgraph, nullptr, nullptr, 0));
+    }
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, H.pc_x.p, T(0));
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, H.pc_p.p, T(0));
+    hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, c->stream, 4, H.pscal.p, 0.0);
+    double rr0;
+    int rc;
+    if ((rc = dot_host<T>(c, s, n, H.pc_r.p, H.pc_r.p, &rr0))) return rc;
+    if (rr0 == 0.0) return MDG_OK;
+    const double tol2 = o->poisson_tol * o->poisson_tol * rr0;
+    for (int it = 0; it < o->poisson_max_its; it++) {
+        HIPCHK(c, hipGraphLaunch(H.pexec, c->stream));
+        *its += 1;
+        HIPCHK(c, hipMemcpyAsync(H.h_pscal, H.pscal.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (!(H.h_pscal[2] > 0.0) || H.h_pscal[3] <= tol2) break;
+    }
+    return MDG_OK;
+}
+
+// Scaled BFBt: z_p = -L^-1 (C D^-1 A_g D^-1 C') L^-1 r_p, with D = diag(A_g),
+// L = C D^-1 C'; then rhs_u = r_u - B z_p.  r is the FP64 outer residual.
+template <typename T>
+static int bfbt_rhs(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solver_opts *o, const double *r,
+                    double *z, int *pits) {
+    mg_level<T> &l0 = H.L[0];
+    const sgrid g = l0.g;
+    const int nu = nvx(g), nv = nvz(g), np = ncell(g);
+    int rc;
+    hipLaunchKernelGGL((k_convert<double, T>), dim3(nblk(np)), dim3(256), 0, c->stream, np, r + nu + nv, H.pc_r.p);
+    remove_mean(c, H, H.pc_r.p, np, T(1));
+    if ((rc = poisson_solve(c, s, H, o, pits))) return rc;
+    hipLaunchKernelGGL(k_grad_scaled<T>, dim3(nblk(nu + nv)), dim3(256), 0, c->stream, g, (const T *)l0.dinv.p,
+                       (const T *)H.pc_x.p, H.bf_t.p);
+    hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(nu + nv)), dim3(256), 0, c->stream, g, (T)o->penalty,
+                       (const T *)H.bf_t.p, (const T *)nullptr, (const T *)l0.en.p, (const T *)l0.es.p, l0.r.p);
+    hipLaunchKernelGGL(k_div_scaled<T>, dim3(nblk(np)), dim3(256), 0, c->stream, g, (const T *)l0.dinv.p,
+                       (const T *)l0.r.p, T(-1), H.pc_r.p);
+    remove_mean(c, H, H.pc_r.p, np, T(1));
+    if ((rc = poisson_solve(c, s, H, o, pits))) return rc;
+    remove_mean(c, H, H.pc_x.p, np, T(-1));
+    hipLaunchKernelGGL(k_rhs_from_zp<T>, dim3(nblk(nu + nv + np)), dim3(256), 0, c->stream, g, r, (const T *)H.pc_x.p, z,
+                       H.cg_r.p);
+    HIPCHK(c, hipGetLastError());
     return MDG_OK;
 }
 
@@ -632,7 +1120,7 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
     double rho = 1.0 / sigma;
     const int n = lv.n;
     if (zero_start) {
-        hipMemcpyAsync(lv.r.p, lv.b.p, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream);
+        hipLaunchKernelGGL(k_copy<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, lv.b.p, lv.r.p);
         hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, lv.x.p, T(0));
     } else {
         hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p,
@@ -649,9 +1137,10 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
     hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, T(1), lv.d.p, T(1), lv.x.p);
 }
 
-// solves approximately A_g x = b on level i (b, x in lv.b, lv.x)
+// solves approximately A_g x = b on level i (b, x in lv.b, lv.x); cycle = 1
+// gives a V-cycle, 2 a W-cycle.
 template <typename T>
-static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg) {
+static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, int cycle) {
     mg_level<T> &lv = H.L[i];
     if (i + 1 == H.L.size()) {
         hipLaunchKernelGGL(k_fill<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.n, lv.x.p, T(0));
@@ -660,64 +1149,89 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg) {
     }
     mg_level<T> &cl = H.L[i + 1];
     chebyshev(c, lv, gamma, deg, true);
-    hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p, lv.b.p,
-                       lv.en.p, lv.es.p, lv.r.p);
-    hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(cl.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, lv.r.p, cl.b.p);
-    vcycle(c, H, i + 1, gamma, deg);
-    hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, cl.x.p, lv.x.p);
+    const int reps = (i + 2 == H.L.size()) ? 1 : cycle;
+    for (int rep = 0; rep < reps; rep++) {
+        hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p, lv.b.p,
+                           lv.en.p, lv.es.p, lv.r.p);
+        hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(cl.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, lv.r.p, cl.b.p);
+        vcycle(c, H, i + 1, gamma, deg, cycle);
+        hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, cl.x.p, lv.x.p);
+    }
     chebyshev(c, lv, gamma, deg, false);
 }
 
-// Preconditioned CG on A_g x = b (b in H.cg_r on entry, x -> H.cg_x).
+// Records one CG iteration (preconditioning V-cycle, direction, operator
+// apply, update) on the stream; all scalars stay in device memory.
+template <typename T>
+static void pcg_iteration(mdg_ctx *c, mg_hier<T> &H, const mdg_solver_opts *o) {
+    mg_level<T> &l0 = H.L[0];
+    const int n = l0.n, nb = mdg_grid1d(n, RB), nd = std::min(1024, nb);
+    const double gamma = o->penalty;
+    T *x = H.cg_x.p, *r = H.cg_r.p, *p = H.cg_p.p, *q = H.cg_q.p;
+    hipLaunchKernelGGL(k_copy<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (const T *)r, l0.b.p);
+    vcycle(c, H, 0, gamma, o->cheb_degree, o->cycle > 1 ? o->cycle : 1);
+    const T *z = l0.x.p;
+    hipLaunchKernelGGL(k_dot_part<T>, dim3(nd), dim3(RB), 0, c->stream, n, (const T *)r, z, H.part.p);
+    hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nd, (const double *)H.part.p, H.scal.p, 1, -1, -1);
+    hipLaunchKernelGGL(k_pcg_dir<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, z, p, (const double *)H.scal.p);
+    hipLaunchKernelGGL(k_vel_apply_dot<T>, dim3(nb), dim3(RB), 0, c->stream, l0.g, (T)gamma, (const T *)p, (const T *)l0.en.p,
+                       (const T *)l0.es.p, q, H.part.p);
+    hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nb, (const double *)H.part.p, H.scal.p, 2, -1, -1);
+    hipLaunchKernelGGL(k_pcg_step<T>, dim3(nb), dim3(RB), 0, c->stream, n, (const T *)p, (const T *)q, x, r,
+                       (const double *)H.scal.p, H.part.p);
+    hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nb, (const double *)H.part.p, H.scal.p, 3, 1, 0);
+}
+
+// Preconditioned CG on A_g x = b (b in H.cg_r on entry, x -> H.cg_x).  The
+// iteration body is captured once per hierarchy as a HIP graph.
 template <typename T>
 static int inner_pcg(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solver_opts *o, int *its) {
     mg_level<T> &l0 = H.L[0];
-    const int n = l0.n;
-    const double gamma = o->penalty;
-    const int nb = mdg_grid1d(n, RB);
-    T *x = H.cg_x.p, *r = H.cg_r.p, *z = H.cg_z.p, *p = H.cg_p.p, *q = H.cg_q.p;
-    HIPCHK(c, s->part.reserve((size_t)nb + 1024));   // k_vel_apply_dot writes nb partials
-    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, x, T(0));
-    double rr0, rz, rr;
+    const int n = l0.n, nb = mdg_grid1d(n, RB);
+    HIPCHK(c, H.part.reserve((size_t)nb + 1024));
+    HIPCHK(c, H.scal.reserve(8));
+    if (!H.h_scal) HIPCHK(c, hipHostMalloc((void **)&H.h_scal, 8 * sizeof(double), hipHostMallocDefault));
+    if (!H.exec) {
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        pcg_iteration(c, H, o);
+        HIPCHK(c, hipStreamEndCapture(c->stream, &H.graph));
+        HIPCHK(c, hipGraphInstantiate(&H.exec, H.graph, nullptr, nullptr, 0));
+    }
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, H.cg_x.p, T(0));
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, H.cg_p.p, T(0));
+    hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, c->stream, 8, H.scal.p, 0.0);
+    double rr0;
     int rc;
-    if ((rc = dot_host<T>(c, s, n, r, r, &rr0))) return rc;
+    if ((rc = dot_host<T>(c, s, n, H.cg_r.p, H.cg_r.p, &rr0))) return rc;
     *its = 0;
     if (rr0 == 0.0) return MDG_OK;
     const double tol2 = o->inner_tol * o->inner_tol * rr0;
+    const int every = o->check_every > 0 ? o->check_every : 1;
     for (int it = 0; it < o->inner_max_its; it++) {
-        // z = V(r)
-        HIPCHK(c, hipMemcpyAsync(l0.b.p, r, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-        vcycle(c, H, 0, gamma, o->cheb_degree);
-        HIPCHK(c, hipMemcpyAsync(z, l0.x.p, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-        double rz_new;
-        if ((rc = dot_host<T>(c, s, n, r, z, &rz_new))) return rc;
-        if (it == 0) HIPCHK(c, hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-        else hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, T(1), z, (T)(rz_new / rz), p);
-        rz = rz_new;
-        hipLaunchKernelGGL(k_vel_apply_dot<T>, dim3(nb), dim3(RB), 0, c->stream, l0.g, (T)gamma, p, l0.en.p, l0.es.p, q, s->part.p);
-        hipLaunchKernelGGL(k_sum_part, dim3(1), dim3(RB), 0, c->stream, nb, s->part.p, s->scal.p);
-        double pq;
-        HIPCHK(c, hipMemcpyAsync(&pq, s->scal.p, 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        if (!(pq > 0.0)) break;   // loss of positivity: keep the current iterate
-        const double alpha = rz / pq;
-        hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)alpha, p, T(1), x);
-        hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)(-alpha), q, T(1), r);
+        HIPCHK(c, hipGraphLaunch(H.exec, c->stream));
         *its = it + 1;
-        if ((rc = dot_host<T>(c, s, n, r, r, &rr))) return rc;
-        if (rr <= tol2) break;
+        if ((it + 1) % every && it + 1 < o->inner_max_its) continue;
+        HIPCHK(c, hipMemcpyAsync(H.h_scal, H.scal.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (!(H.h_scal[2] > 0.0) || H.h_scal[3] <= tol2) break;
     }
     return MDG_OK;
 }
 
 // z = M r (block preconditioner, FP64 in/out)
 template <typename T>
-static int precondition(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solver_opts *o,
+static int precondition(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opts *o,
                         const double *r, double *z, int *inner) {
     const sgrid g = c->sg;
     const int nu = nvx(g), nv = nvz(g), nfull = nu + nv + ncell(g);
-    hipLaunchKernelGGL(k_schur_rhs<T>, dim3(nblk(nfull)), dim3(256), 0, c->stream, g, o->penalty, r, c->eta_n.p, z, H.cg_r.p);
     int its, rc;
+    if (o->schur == 1) {
+        int pits = 0;
+        if ((rc = bfbt_rhs(c, s, H, o, r, z, &pits))) return rc;
+        o->poisson_its += pits;
+    } else {
+        hipLaunchKernelGGL(k_schur_rhs<T>, dim3(nblk(nfull)), dim3(256), 0, c->stream, g, o->penalty, r, c->eta_n.p, z, H.cg_r.p);
+    }
     if ((rc = inner_pcg(c, s, H, o, &its))) return rc;
     *inner += its;
     hipLaunchKernelGGL((k_convert<T, double>), dim3(nblk(nu + nv)), dim3(256), 0, c->stream, nu + nv, H.cg_x.p, z);
@@ -740,7 +1254,7 @@ static int fgcr_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opt
     int rc;
     if ((rc = dot_host<double>(c, s, n, b, b, &bb))) return rc;
     const double bnorm = sqrt(bb);
-    o->its = 0; o->inner_its = 0; o->rel_res = 0.0;
+    o->its = 0; o->inner_its = 0; o->poisson_its = 0; o->rel_res = 0.0;
     if (bnorm == 0.0) return MDG_OK;
     int slot = 0;
     for (int it = 0; it < o->max_its; it++) {
@@ -777,14 +1291,18 @@ static int fgcr_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opt
 }
 
 static void default_opts(mdg_solver_opts *o) {
-    if (o->penalty <= 0) o->penalty = 1.0;
+    if (o->penalty <= 0) o->penalty = 10.0;
     if (o->rel_tol <= 0) o->rel_tol = 1e-8;
     if (o->max_its <= 0) o->max_its = 500;
-    if (o->restart <= 0) o->restart = 20;
-    if (o->inner_tol <= 0) o->inner_tol = 1e-3;
-    if (o->inner_max_its <= 0) o->inner_max_its = 60;
-    if (o->cheb_degree <= 0) o->cheb_degree = 3;
+    if (o->restart <= 0) o->restart = 100;
+    if (o->inner_tol <= 0) o->inner_tol = 1e-6;
+    if (o->inner_max_its <= 0) o->inner_max_its = 80;
+    if (o->cheb_degree <= 0) o->cheb_degree = 8;
     if (o->coarse_cells <= 0) o->coarse_cells = 8;
+    if (o->cycle <= 0) o->cycle = 1;
+    if (o->check_every <= 0) o->check_every = 1;
+    if (o->poisson_tol <= 0) o->poisson_tol = 1e-3;
+    if (o->poisson_max_its <= 0) o->poisson_max_its = 30;
 }
 
 // Grid-layout solve: b (u|v|p) -> x (u|v|p), device arrays of nvx+nvz+ncell.
@@ -836,4 +1354,28 @@ extern "C" int mdg_killer_solver(mdg_ctx *c, mdg_solver_opts *o, const double *D
     HIPCHK(c, hipMemcpyAsync(sol, c->eqbuf.p, (size_t)c->neq * 8, hipMemcpyDefault, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return rc;
+}
+
+// Times the dominant kernel of the solve (the fine-level smoother operator
+// r -= A_g d, FP64) on the hierarchy of the last FP64 solve, reps launches
+// between HIP events on the context stream.  Returns ms per launch and the
+// algorithmic bytes per launch (d, r read, r written, eta_n, eta_s read).
+extern "C" int mdg_time_smoother(mdg_ctx *c, int reps, double *ms, double *bytes) {
+    if (!c || !ms || !bytes || reps < 1) return mdg_fail(c, MDG_ERR_ARG, "bad argument");
+    if (!c->solver || c->solver->hd.L.empty()) return mdg_fail(c, MDG_ERR_STATE, "no FP64 solve has run");
+    mg_level<double> &l0 = c->solver->hd.L[0];
+    const int n = l0.n;
+    hipLaunchKernelGGL(k_cheb_res<double>, dim3(nblk(n)), dim3(256), 0, c->stream, l0.g, 10.0, (const double *)l0.d.p,
+                       (const double *)l0.en.p, (const double *)l0.es.p, l0.r.p);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < reps; i++)
+        hipLaunchKernelGGL(k_cheb_res<double>, dim3(nblk(n)), dim3(256), 0, c->stream, l0.g, 10.0, (const double *)l0.d.p,
+                           (const double *)l0.en.p, (const double *)l0.es.p, l0.r.p);
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = t / reps;
+    *bytes = 8.0 * (3.0 * n + (double)ncell(l0.g) + (double)l0.g.Nx * l0.g.Nz);
+    return MDG_OK;
 }

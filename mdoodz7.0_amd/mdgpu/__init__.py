@@ -74,6 +74,7 @@ def lib():
         L.mdg_dev_alloc.argtypes = [vp, C.c_ulonglong, C.POINTER(vp)]
         L.mdg_dev_free.argtypes = [vp, vp]
         L.mdg_memcpy.argtypes = [vp, vp, vp, C.c_ulonglong]
+        L.mdg_dev_scale.argtypes = [vp, C.c_longlong, C.c_double, vp, vp]
         _extra_signatures(L)
         _LIB = L
     return _LIB
@@ -84,7 +85,10 @@ class SolverOpts(C.Structure):
     _fields_ = [("penalty", C.c_double), ("rel_tol", C.c_double), ("max_its", C.c_int),
                 ("restart", C.c_int), ("inner_tol", C.c_double), ("inner_max_its", C.c_int),
                 ("cheb_degree", C.c_int), ("coarse_cells", C.c_int), ("precision", C.c_int),
-                ("precond", C.c_int), ("its", C.c_int), ("inner_its", C.c_int),
+                ("precond", C.c_int), ("cycle", C.c_int), ("eta_avg", C.c_int),
+                ("check_every", C.c_int), ("schur", C.c_int), ("poisson_tol", C.c_double),
+                ("poisson_max_its", C.c_int), ("its", C.c_int), ("inner_its", C.c_int),
+                ("poisson_its", C.c_int),
                 ("levels", C.c_int), ("status", C.c_int), ("rel_res", C.c_double),
                 ("time_ms", C.c_double)]
 
@@ -105,6 +109,7 @@ def _extra_signatures(L):
     L.mdg_eval_rhs.argtypes = [vp, C.c_double, C.c_double, vp, vp, vp, vp]
     L.mdg_stokes_residual.argtypes = [vp] * 11
     L.mdg_killer_solver.argtypes = [vp, C.POINTER(SolverOpts), vp, vp, vp, vp, vp, vp]
+    L.mdg_time_smoother.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
 
 
 def ptr(a):
@@ -144,7 +149,17 @@ class DeviceArray:
         self.ctx._check(self.ctx._L.mdg_memcpy(self.ctx.h, out.ctypes.data, self.p, out.nbytes))
         return out
 
+    def view(self, offset, n):
+        """A non-owning sub-array starting at element offset."""
+        v = DeviceArray.__new__(DeviceArray)
+        v.ctx, v.n, v.dtype = self.ctx, int(n), self.dtype
+        v.p = self.p + int(offset) * self.dtype.itemsize
+        v._view = True
+        return v
+
     def free(self):
+        if getattr(self, "_view", False):
+            return
         if self.p and self.ctx.h:
             self.ctx._L.mdg_dev_free(self.ctx.h, self.p)
         self.p = None
@@ -269,14 +284,18 @@ class Context:
                                              ptr(out["exxd"]), ptr(out["ezzd"]), ptr(out["exz"])))
         return out
 
-    def eval_rhs(self, gx, gz, rho_n, rho_s):
-        rx = np.zeros(self.Nx * (self.Nz + 1)); rz = np.zeros((self.Nx + 1) * self.Nz)
+    def eval_rhs(self, gx, gz, rho_n, rho_s, rx=None, rz=None):
+        if rx is None:
+            rx = np.zeros(self.Nx * (self.Nz + 1))
+        if rz is None:
+            rz = np.zeros((self.Nx + 1) * self.Nz)
         self._check(self._L.mdg_eval_rhs(self.h, gx, gz, ptr(rho_n), ptr(rho_s), ptr(rx), ptr(rz)))
         return rx, rz
 
-    def stokes_residual(self, sxxd, szzd, sxz, p_corr, div, roger_x, roger_z, d=None):
+    def stokes_residual(self, sxxd, szzd, sxz, p_corr, div, roger_x, roger_z, d=None, F=None):
         neq = self.eqn_count()[0]
-        F = np.zeros(neq)
+        if F is None:
+            F = np.zeros(neq)
         norms = np.zeros(3)
         self._check(self._L.mdg_stokes_residual(self.h, ptr(sxxd), ptr(szzd), ptr(sxz),
                                                 ptr(p_corr), ptr(div), ptr(roger_x),
@@ -294,6 +313,24 @@ class Context:
                                        ptr(np.ascontiguousarray(rhs_cont)), ptr(sol))
         self._check(rc)
         return sol, o
+
+    def scale_into(self, x, a, y):
+        self._check(self._L.mdg_dev_scale(self.h, len(x), a, ptr(x), ptr(y)))
+
+    def killer_solver_dev(self, D11, D22, D33, F, nm, sol, **opts):
+        """killer_solver on device arrays: F and sol hold [mom; cont]."""
+        o = SolverOpts()
+        for k, v in opts.items():
+            setattr(o, k, v)
+        rc = self._L.mdg_killer_solver(self.h, C.byref(o), ptr(D11), ptr(D22), ptr(D33), ptr(F),
+                                       ptr(F.view(nm, len(F) - nm)), ptr(sol))
+        self._check(rc)
+        return o
+
+    def time_smoother(self, reps=50):
+        ms, b = C.c_double(), C.c_double()
+        self._check(self._L.mdg_time_smoother(self.h, reps, C.byref(ms), C.byref(b)))
+        return ms.value, b.value
 
     def last_kernel_ms(self):
         v = C.c_double()
