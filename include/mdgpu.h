@@ -137,6 +137,72 @@ int mdg_apply_bc(mdg_ctx *ctx, double *u, double *v);
 int mdg_strain_rates(mdg_ctx *ctx, const double *u, const double *v, double *div,
                      double *exxd, double *ezzd, double *exz);
 
+/* ExtractDiagonalScale (StokesRoutines.c:1260-1300): the scaling vector d
+ * (equation numbering) = diagonal of the Picard block A on momentum rows and
+ * 1 on continuity rows.  Residual norms with diag_scaling = 1 are taken on
+ * d .* F (mdg_stokes_residual's d argument). */
+int mdg_stokes_diag(mdg_ctx *ctx, const double *D11_n, const double *D22_n, const double *D33_s,
+                    double *d);
+
+/* InterpCentroidsToVerticesDouble (GridRoutines.c:108-212): vertex value =
+ * 1/4 of the four surrounding centre values on the edge-copied padded array;
+ * tag-30/31 centres contribute zero (divisor stays 4), tag-30 vertices get 0.
+ * periodic_x selects the periodic column copy. */
+int mdg_c2v(mdg_ctx *ctx, int periodic_x, const double *cent, double *vert);
+
+/* The node-averaging tail of StrainRateComponents (RheologyDensity.c:2066-2094):
+ * exz_n = 1/4 of the non-air vertex exz around each non-air cell, and exxd,
+ * ezzd, div carried to the vertices by mdg_c2v.  Any output may be NULL. */
+int mdg_strain_rates_nodes(mdg_ctx *ctx, const double *exxd, const double *ezzd,
+                           const double *exz, const double *div, double *exz_n,
+                           double *exxd_s, double *ezzd_s, double *div_s);
+
+/* ---- rheology (NonNewtonianViscosityGrid subset) ------------------------ */
+
+/* One phase of mat_prop (mdoodz-private.h) for the implemented creep laws:
+ * constant viscosity (cstv), dislocation power law (pwlv), Drucker-Prager
+ * plasticity with viscoplastic regularisation (plast = 1); values in the
+ * model's scaled units.  beta is the compressibility giving K = 1/beta. */
+typedef struct {
+    int    cstv, pwlv, plast;
+    double eta0;
+    double pref_pwl, F_pwl, A_pwl, n_pwl, Q_pwl, V_pwl;
+    double C, fric, dil, Slim, eta_vp, n_vp;
+    double beta;
+} mdg_phase_prop;
+
+/* Fields of params / scale used by ViscosityConcise.  average is
+ * model->eta_average: 0 arithmetic, 1 harmonic, 2 geometric. */
+typedef struct {
+    int    step, elastic, average;
+    double dt, R, zeroC, min_eta, max_eta, eta_tol, Tiimin, scale_E, scale_eta;
+} mdg_rheo_params;
+
+/* Node arrays of one node set (centres or vertices).  Inputs: perc is
+ * nph x n (phase_perc_n / phase_perc_s, phase-major); G (mu), T, P (p_in or
+ * P_s), the strain rates, the old stresses (NULL = 0), and the per-node
+ * fric, C, dil (NULL = the phase's own value) and pref (a multiplier of
+ * pref_pwl, NULL = 1; no counterpart in the reference).  Outputs: eta
+ * (eta_n / eta_s, required), eta_phys, the stresses, p_corr, eII_pl
+ * (each may be NULL). */
+typedef struct {
+    const double *perc, *G, *T, *P, *exx, *ezz, *exz, *sxx0, *szz0, *sxz0;
+    const double *fric, *C, *dil, *pref;
+    double *eta, *eta_phys, *sxx, *szz, *sxz, *p_corr, *eII_pl;
+} mdg_rheo_nodes;
+
+#define MDG_MAX_PHASES 64
+
+/* NonNewtonianViscosityGrid (RheologyDensity.c:1131-1426) on the centres
+ * (centroid = 1, loop :1157-1316) or the vertices (centroid = 0, :1320-1407):
+ * per node the elastic effective strain rate E + tau0/(2 G dt), per active
+ * phase (fraction > 1e-13) ViscosityConcise (:385-1080), the phase average
+ * of the effective and dissipative viscosities, and the stresses 2 eta E.
+ * Air nodes (tags 30, and 31 on centres) get zeros.  Returns
+ * MDG_ERR_DIVERGED where the reference exits on a failed local iteration. */
+int mdg_viscosity_grid(mdg_ctx *ctx, const mdg_phase_prop *phases, int nph,
+                       const mdg_rheo_params *prm, int centroid, const mdg_rheo_nodes *f);
+
 /* EvaluateRHS body force (StokesRoutines.c:961-1160): roger_x = gx * mean of
  * the two vertex densities, roger_z = gz * mean of the two cell densities. */
 int mdg_eval_rhs(mdg_ctx *ctx, double gx, double gz, const double *rho_n, const double *rho_s,
@@ -177,7 +243,8 @@ typedef struct {
                               2 harmonic, 3 maximum (0)                        */
     int    check_every;    /* inner CG convergence test period (1)             */
     int    schur;          /* Schur complement approximation: 0 viscosity-
-                              weighted mass, 1 scaled BFBt (0)                 */
+                              weighted mass, 1 scaled BFBt (0; BFBt needs
+                              precision = MDG_PREC_FP64)                       */
     double poisson_tol;    /* BFBt pressure-Poisson CG tolerance (1e-3)        */
     int    poisson_max_its;/* (30)                                             */
     /* outputs */

@@ -65,6 +65,7 @@ struct mdg_ctx {
     dbuf<double>      gx, gy, eqbuf;         // grid-layout work vectors
     dbuf<double>      part;                  // reduction partials
     struct solver_state *solver = nullptr;
+    struct rheo_scratch *rheo = nullptr;
     // runtime
     hipStream_t  stream = nullptr;
     hipEvent_t   ev0 = nullptr, ev1 = nullptr;
@@ -107,4 +108,5 @@ int mdg_node_set(mdg_ctx *c, int centroid, node_set *ns);
 
 // Frees the solver's device state (defined with the solver).
 void mdg_solver_release(mdg_ctx *c);
+void mdg_rheo_release(mdg_ctx *c);
 int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x);

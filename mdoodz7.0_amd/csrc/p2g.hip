@@ -101,6 +101,101 @@ __global__ void __launch_bounds__(256) k_p2g(p2g_geo g, p2g_set fs, const unsign
     }
 }
 
+// Row-tiled variant for the 1-cell stencil.  A block owns P2G_TILE
+// consecutive nodes of one node row; the markers of the cell rows those nodes
+// draw from are contiguous in sorted order, so the block streams them with
+// coalesced loads, turns each into (home node, w, w*value) in LDS, and each
+// node thread then sums its own quarter-cell ranges out of LDS in the same
+// marker order as k_p2g (so both give bit-identical sums).  Consecutive rows
+// are handed to the same XCD (blocks are dispatched round-robin over the 8
+// XCDs), so a cell row shared by two vertex rows is read from HBM once and
+// from that XCD's L2 the second time.
+#define P2G_TILE 256
+template <int NF> struct tile_cap { static constexpr int v = (NF <= 2) ? 2560 : 1536; };
+
+template <int NF>
+__global__ void __launch_bounds__(P2G_TILE)
+k_p2g_tile(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff, const double *__restrict__ mx,
+           const double *__restrict__ mz, const int *__restrict__ mph, int need_phase, int nbx,
+           int nblocks, int *__restrict__ stray) {
+    constexpr int CAP = tile_cap<NF>::v;
+    __shared__ double lw[CAP];
+    __shared__ double lwv[NF][CAP];
+    __shared__ int lhome[CAP];
+    // XCD-aware order: hardware block b runs on XCD b % 8; XCD x gets the
+    // contiguous logical tiles [start(x), start(x) + count(x)).
+    const int b = blockIdx.x, xcd = b & 7, per = nblocks >> 3, rem = nblocks & 7;
+    const int tile = xcd * per + min(xcd, rem) + (b >> 3);
+    const int l = tile / nbx, k0 = (tile - l * nbx) * P2G_TILE;
+    const int kend = min(k0 + P2G_TILE, g.nx);
+    const int t = threadIdx.x, k = k0 + t;
+    const bool own = k < kend;
+    // this node's marker ranges (at most 4 for the 1-cell stencil)
+    unsigned rs[4], re[4];
+    int nr = 0;
+    if (own)
+        for_node_ranges(qoff, g.Ncx, g.Ncz, g.ax, k, l, [&](unsigned s, unsigned e) {
+            if (nr < 4) { rs[nr] = s; re[nr] = e; nr++; }
+        });
+    // the block's cell rows and column span
+    int cx0, cx1, cz0, cz1;
+    if (g.ax.xkind == CELL_AXIS) { cx0 = k0 + g.ax.xoff; cx1 = kend - 1 + g.ax.xoff; }
+    else { cx0 = k0 - 1; cx1 = kend - 1; }
+    if (g.ax.zkind == CELL_AXIS) { cz0 = cz1 = l + g.ax.zoff; }
+    else { cz0 = l - 1; cz1 = l; }
+    cx0 = max(cx0, 0); cx1 = min(cx1, g.Ncx - 1);
+    cz0 = max(cz0, 0); cz1 = min(cz1, g.Ncz - 1);
+    const int node = l * g.nx + k;
+    double W = 0.0, BW[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) BW[f] = 0.0;
+    bool bad = false;
+    for (int cz = cz0; cz <= cz1 && cx0 <= cx1; cz++) {
+        const unsigned S = qoff[4u * (unsigned)(cx0 + cz * g.Ncx)];
+        const unsigned E = qoff[4u * (unsigned)(cx1 + cz * g.Ncx) + 4u];
+        for (unsigned cs = S; cs < E; cs += CAP) {
+            const unsigned ce = min(E, cs + (unsigned)CAP);
+            __syncthreads();   // previous chunk fully consumed
+            for (unsigned j = cs + t; j < ce; j += P2G_TILE) {
+                const double x = mx[j], z = mz[j];
+                const int ip = near_index(x, g.X0, g.dx, g.nx);
+                const int jp = near_index(z, g.Z0, g.dz, g.nz);
+                const double dxm = fabs(g.X[ip] - x), dzm = fabs(g.Z[jp] - z);
+                const double w = (1.0 - dxm / g.dx_itp) * (1.0 - dzm / g.dz_itp);
+                const int p = need_phase ? mph[j] : 0;
+                const unsigned q = j - cs;
+                lw[q] = w;
+                lhome[q] = ip + jp * g.nx;
+#pragma unroll
+                for (int f = 0; f < NF; f++) lwv[f][q] = mark_val(fs.f[f], p, j) * w;
+            }
+            __syncthreads();
+            for (int r = 0; r < nr; r++) {
+                const unsigned a = max(rs[r], cs), e = min(re[r], ce);
+                for (unsigned i = a; i < e; i++) {
+                    const unsigned q = i - cs;
+                    if (lhome[q] != node) { bad = true; continue; }
+                    W += lw[q];
+#pragma unroll
+                    for (int f = 0; f < NF; f++) BW[f] += lwv[f][q];
+                }
+            }
+        }
+    }
+    if (bad) *stray = 1;
+    if (!own) return;
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        const p2g_fld &F = fs.f[f];
+        if (fabs(W) < 1e-30) continue;
+        if (F.ntype && (F.ntype[node] == 30 || F.ntype[node] == 31)) continue;
+        double v = BW[f] / W;
+        if (F.avg == 1) v = 1.0 / v;
+        if (F.avg == 2) v = exp(v);
+        F.dst[node] = v;
+    }
+}
+
 // Phase fractions (P2Mastah with prop = 1): per-thread phase accumulators in LDS.
 __global__ void __launch_bounds__(128) k_p2g_phase(p2g_geo g, int nph, const signed char *ntype,
                                                    const unsigned *__restrict__ qoff,
@@ -149,6 +244,23 @@ static int setup_geo(mdg_ctx *c, int centroid, int stencil, p2g_geo *g) {
 
 static int launch_fields(mdg_ctx *c, p2g_geo g, const p2g_set &s, int nf) {
     const int nn = g.nx * g.nz;
+    if (g.R == 0 && g.S == 0) {
+        // 1-cell stencil, exact homes: row-tiled kernel
+        int need_phase = 0;
+        for (int f = 0; f < nf; f++) need_phase |= (s.f[f].flag == 0);
+        const int nbx = (g.nx + P2G_TILE - 1) / P2G_TILE, nb = nbx * g.nz;
+        dim3 grid(nb), block(P2G_TILE);
+#define TILE_LAUNCH(NF) hipLaunchKernelGGL(k_p2g_tile<NF>, grid, block, 0, c->stream, g, s, c->qoff.p, c->mx.p, c->mz.p, c->mphase.p, need_phase, nbx, nb, c->flag_dev.p)
+        switch (nf) {
+        case 1: TILE_LAUNCH(1); break;
+        case 2: TILE_LAUNCH(2); break;
+        case 3: TILE_LAUNCH(3); break;
+        default: TILE_LAUNCH(4); break;
+        }
+#undef TILE_LAUNCH
+        HIPCHK(c, hipGetLastError());
+        return MDG_OK;
+    }
     dim3 grid(mdg_grid1d(nn, 256)), block(256);
     switch (nf) {
     case 1: hipLaunchKernelGGL(k_p2g<1>, grid, block, 0, c->stream, g, s, c->qoff.p, c->mx.p, c->mz.p, c->mphase.p, c->flag_dev.p); break;

@@ -1312,6 +1312,8 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     HIPCHK(c, s->part.reserve(4096));
     HIPCHK(c, s->scal.reserve(16));
     default_opts(o);
+    if (o->precision == MDG_PREC_MIXED && o->schur == 1)
+        return mdg_fail(c, MDG_ERR_UNSUPPORTED, "the BFBt Schur approximation runs in FP64 only (precision = MDG_PREC_FP64)");
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int rc;
     if (o->precision == MDG_PREC_MIXED) {
@@ -1320,6 +1322,19 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     } else {
         if ((rc = build_hierarchy<double>(c, s, s->hd, o))) return rc;
         rc = fgcr_solve<double>(c, s, s->hd, o, b, x);
+    }
+    // The closed box leaves the pressure defined up to a constant.  The
+    // reference's Powell-Hestenes updates p += penalty * (rhs_cont - C u)
+    // only add zero-sum increments (Solvers.c:1301-1304), so its correction
+    // has zero mean; remove the mean here to select the same solution.
+    {
+        const sgrid g = c->sg;
+        const int np = ncell(g), nb = 256;
+        double *xp = x + nvx(g) + nvz(g);
+        hipLaunchKernelGGL(k_sum_part_vec<double>, dim3(nb), dim3(RB), 0, c->stream, np, (const double *)xp, s->part.p);
+        hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nb, (const double *)s->part.p, s->scal.p, 15, -1, -1);
+        hipLaunchKernelGGL(k_sub_mean<double>, dim3(nblk(np)), dim3(256), 0, c->stream, np, xp, (const double *)s->scal.p, 15, 1.0);
+        HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));

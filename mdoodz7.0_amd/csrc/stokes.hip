@@ -462,3 +462,40 @@ extern "C" int mdg_stokes_residual(mdg_ctx *c, const double *sxxd, const double 
         return mdg_fail(c, MDG_ERR_NAN, "NaN residual");
     return MDG_OK;
 }
+
+// ExtractDiagonalScale (StokesRoutines.c:1260-1300): d = the diagonal of the
+// assembled Picard block A on momentum rows, and 1 on continuity rows (the D
+// block is empty for incompressible cells, so its d keeps SAlloc's 1.0).
+__global__ void k_stokes_diag(sgrid g, const double *__restrict__ en, const double *__restrict__ es,
+                              const int *__restrict__ eu, const int *__restrict__ ev,
+                              const int *__restrict__ ep, double *__restrict__ d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu = nvx(g), nw = nvz(g), nc = ncell(g);
+    if (i < nu) {
+        const int e = eu[i];
+        if (e >= 0) d[e] = diag_vx<double>(g, i % g.Nx, i / g.Nx, en, es, 0.0);
+    } else if (i < nu + nw) {
+        const int j = i - nu, e = ev[j];
+        if (e >= 0) d[e] = diag_vz<double>(g, j % (g.Nx + 1), j / (g.Nx + 1), en, es, 0.0);
+    } else if (i < nu + nw + nc) {
+        const int e = ep[i - nu - nw];
+        if (e >= 0) d[e] = 1.0;
+    }
+}
+
+extern "C" int mdg_stokes_diag(mdg_ctx *c, const double *D11_n, const double *D22_n,
+                               const double *D33_s, double *d) {
+    if (!c || !d) return mdg_fail(c, MDG_ERR_ARG, "null vector");
+    if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
+    int rc;
+    if ((rc = mdg_stage_coeffs(c, D11_n, D22_n, D33_s)) || (rc = mdg_reserve_grid_vectors(c))) return rc;
+    const sgrid g = c->sg;
+    double *dd = mdg_is_device_ptr(d) ? d : c->eqbuf.p;
+    const int n = nvx(g) + nvz(g) + ncell(g);
+    hipLaunchKernelGGL(k_stokes_diag, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, g, c->eta_n.p,
+                       c->eta_s.p, c->eqn_u.p, c->eqn_v.p, c->eqn_p.p, dd);
+    HIPCHK(c, hipGetLastError());
+    if (dd != d) HIPCHK(c, hipMemcpyAsync(d, dd, (size_t)c->neq * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDG_OK;
+}

@@ -99,6 +99,33 @@ class SolverOpts(C.Structure):
 MDG_PREC_FP64, MDG_PREC_MIXED = 0, 1
 
 
+class PhaseProp(C.Structure):
+    """mdg_phase_prop: one phase of mat_prop for the implemented creep laws."""
+    _fields_ = [("cstv", C.c_int), ("pwlv", C.c_int), ("plast", C.c_int),
+                ("eta0", C.c_double), ("pref_pwl", C.c_double), ("F_pwl", C.c_double),
+                ("A_pwl", C.c_double), ("n_pwl", C.c_double), ("Q_pwl", C.c_double),
+                ("V_pwl", C.c_double), ("C", C.c_double), ("fric", C.c_double),
+                ("dil", C.c_double), ("Slim", C.c_double), ("eta_vp", C.c_double),
+                ("n_vp", C.c_double), ("beta", C.c_double)]
+
+
+class RheoParams(C.Structure):
+    """mdg_rheo_params: the params/scale fields ViscosityConcise reads."""
+    _fields_ = [("step", C.c_int), ("elastic", C.c_int), ("average", C.c_int),
+                ("dt", C.c_double), ("R", C.c_double), ("zeroC", C.c_double),
+                ("min_eta", C.c_double), ("max_eta", C.c_double), ("eta_tol", C.c_double),
+                ("Tiimin", C.c_double), ("scale_E", C.c_double), ("scale_eta", C.c_double)]
+
+
+RHEO_IN = ("perc", "G", "T", "P", "exx", "ezz", "exz", "sxx0", "szz0", "sxz0", "fric", "C",
+           "dil", "pref")
+RHEO_OUT = ("eta", "eta_phys", "sxx", "szz", "sxz", "p_corr", "eII_pl")
+
+
+class RheoNodes(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in RHEO_IN + RHEO_OUT]
+
+
 def _extra_signatures(L):
     """Signatures of the Stokes and solver entry points."""
     vp = C.c_void_p
@@ -110,6 +137,11 @@ def _extra_signatures(L):
     L.mdg_stokes_residual.argtypes = [vp] * 11
     L.mdg_killer_solver.argtypes = [vp, C.POINTER(SolverOpts), vp, vp, vp, vp, vp, vp]
     L.mdg_time_smoother.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.mdg_c2v.argtypes = [vp, C.c_int, vp, vp]
+    L.mdg_stokes_diag.argtypes = [vp] * 5
+    L.mdg_strain_rates_nodes.argtypes = [vp] * 9
+    L.mdg_viscosity_grid.argtypes = [vp, C.POINTER(PhaseProp), C.c_int, C.POINTER(RheoParams),
+                                     C.c_int, C.POINTER(RheoNodes)]
 
 
 def ptr(a):
@@ -282,6 +314,58 @@ class Context:
                    exz=np.zeros(self.Nx * self.Nz))
         self._check(self._L.mdg_strain_rates(self.h, ptr(u), ptr(v), ptr(out["div"]),
                                              ptr(out["exxd"]), ptr(out["ezzd"]), ptr(out["exz"])))
+        return out
+
+    def stokes_diag(self, D11, D22, D33, d=None):
+        """ExtractDiagonalScale (StokesRoutines.c:1260-1300) scaling vector."""
+        if d is None:
+            d = np.zeros(self.eqn_count()[0])
+        self._check(self._L.mdg_stokes_diag(self.h, ptr(D11), ptr(D22), ptr(D33), ptr(d)))
+        return d
+
+    def c2v(self, cent, vert=None, periodic_x=0):
+        """InterpCentroidsToVerticesDouble (GridRoutines.c:108-212)."""
+        if vert is None:
+            vert = np.zeros(self.Nx * self.Nz)
+        self._check(self._L.mdg_c2v(self.h, int(periodic_x), ptr(cent), ptr(vert)))
+        return vert
+
+    def strain_rates_nodes(self, exxd, ezzd, exz, div):
+        """exz_n, exxd_s, ezzd_s, div_s of StrainRateComponents
+        (RheologyDensity.c:2066-2094)."""
+        nc, nv = (self.Nx - 1) * (self.Nz - 1), self.Nx * self.Nz
+        out = dict(exz_n=np.zeros(nc), exxd_s=np.zeros(nv), ezzd_s=np.zeros(nv),
+                   div_s=np.zeros(nv))
+        self._check(self._L.mdg_strain_rates_nodes(
+            self.h, ptr(exxd), ptr(ezzd), ptr(exz), ptr(div), ptr(out["exz_n"]),
+            ptr(out["exxd_s"]), ptr(out["ezzd_s"]), ptr(out["div_s"])))
+        return out
+
+    def viscosity_grid(self, phases, prm, centroid, outputs=RHEO_OUT, **arrays):
+        """NonNewtonianViscosityGrid (RheologyDensity.c:1131-1426) on the
+        centres (centroid=1) or vertices (centroid=0).  arrays holds the
+        inputs named in RHEO_IN (perc has shape (nph, n)); outputs named in
+        RHEO_OUT are allocated on the host and returned in a dict, or may be
+        passed in arrays (e.g. as DeviceArray)."""
+        n = self.node_count(centroid)
+        out, keep = {}, []
+        f = RheoNodes()
+        for k in RHEO_IN:
+            a = arrays.get(k)
+            if isinstance(a, np.ndarray):
+                a = np.ascontiguousarray(a, np.float64)
+                keep.append(a)
+            setattr(f, k, ptr(a))
+        for k in RHEO_OUT:
+            a = arrays.get(k)
+            if a is None and k in outputs:
+                a = np.zeros(n)
+            if a is not None:
+                out[k] = a
+            setattr(f, k, ptr(a))
+        ph = (PhaseProp * len(phases))(*phases)
+        self._check(self._L.mdg_viscosity_grid(self.h, ph, len(phases), C.byref(prm),
+                                               int(centroid), C.byref(f)))
         return out
 
     def eval_rhs(self, gx, gz, rho_n, rho_s, rx=None, rz=None):

@@ -112,3 +112,37 @@ def solvi_analytic(x, z, rc=0.2, mm=1.0, mc=1e3, er=-1.0):
     vx[~inside], vz[~inside] = V.real, V.imag
     p[~inside] = (-2.0 * mm * (mc - mm) / (mc + mm) * (rc ** 2 / Zo ** 2 * 2.0 * er)).real
     return vx, vz, p
+
+
+def weak_seeds(seed, n, radius, xmin, xmax, zmin, zmax):
+    """n seeded circle centres (uniform in the box) of the given radius."""
+    k = np.arange(n, dtype=np.uint64)
+    cx = xmin + (xmax - xmin) * uniform(seed + 101, 2 * k)
+    cz = zmin + (zmax - zmin) * uniform(seed + 101, 2 * k + np.uint64(1))
+    return np.stack([cx, cz, np.full(n, radius)], axis=1)
+
+
+def seed_phase(x, z, seeds, chunk=1 << 22):
+    """Phase 1 inside any seed circle, 0 elsewhere."""
+    ph = np.zeros(x.size, np.int32)
+    for s in range(0, x.size, chunk):
+        xs, zs = x[s:s + chunk], z[s:s + chunk]
+        inside = np.zeros(xs.size, bool)
+        for cx, cz, r in seeds:
+            inside |= (xs - cx) ** 2 + (zs - cz) ** 2 < r * r
+        ph[s:s + chunk][inside] = 1
+    return ph
+
+
+def vep_phases(n_pwl=3.0, C=1.0, C_weak_ratio=0.5, fric_deg=30.0, eta_pwl=1.0):
+    """Nondimensional visco-elasto-plastic shear-box phases: a power-law
+    matrix (B = eta_pwl at unit strain rate) with Drucker-Prager yield, and
+    weak seeds with a reduced cohesion.  Returned as keyword dicts of
+    mdg_phase_prop fields."""
+    import math
+    base = dict(cstv=0, pwlv=1, plast=1, eta0=1.0, pref_pwl=1.0, F_pwl=1.0,
+                A_pwl=eta_pwl ** (-n_pwl), n_pwl=n_pwl, Q_pwl=0.0, V_pwl=0.0, C=C,
+                fric=math.radians(fric_deg), dil=0.0, Slim=1e50, eta_vp=0.0, n_vp=1.0,
+                beta=1e-50)
+    weak = dict(base, C=C * C_weak_ratio)
+    return [base, weak]

@@ -36,6 +36,39 @@ class KillerOpts(C.Structure):
                 ("maxru", C.c_double), ("maxru0", C.c_double), ("status", C.c_int)]
 
 
+class OrcPhase(C.Structure):
+    """One phase of mat_prop (mdoodz-private.h) restricted to the rheology subset."""
+    _fields_ = [("cstv", C.c_int), ("pwlv", C.c_int), ("plast", C.c_int),
+                ("eta0", C.c_double), ("pref_pwl", C.c_double), ("F_pwl", C.c_double),
+                ("A_pwl", C.c_double), ("n_pwl", C.c_double), ("Q_pwl", C.c_double),
+                ("V_pwl", C.c_double), ("C", C.c_double), ("fric", C.c_double),
+                ("dil", C.c_double), ("Slim", C.c_double), ("eta_vp", C.c_double),
+                ("n_vp", C.c_double), ("beta_for_K", C.c_double)]
+
+
+class OrcRheoModel(C.Structure):
+    _fields_ = [("step", C.c_int), ("elastic", C.c_int), ("dt", C.c_double), ("R", C.c_double),
+                ("zeroC", C.c_double), ("min_eta", C.c_double), ("max_eta", C.c_double),
+                ("eta_tol", C.c_double), ("Tiimin", C.c_double), ("scale_E", C.c_double),
+                ("scale_eta", C.c_double)]
+
+
+def phase(**kw):
+    """OrcPhase with reference-like defaults (constant viscosity 1, no plasticity)."""
+    d = dict(cstv=1, pwlv=0, plast=0, eta0=1.0, pref_pwl=1.0, F_pwl=1.0, A_pwl=1.0, n_pwl=1.0,
+             Q_pwl=0.0, V_pwl=0.0, C=1e50, fric=0.0, dil=0.0, Slim=1e50, eta_vp=0.0, n_vp=1.0,
+             beta_for_K=1e-50)
+    d.update(kw)
+    return OrcPhase(**d)
+
+
+def rheo_model(**kw):
+    d = dict(step=1, elastic=0, dt=1.0, R=8.314, zeroC=0.0, min_eta=1e-50, max_eta=1e50,
+             eta_tol=1e-13, Tiimin=1e-50, scale_E=1.0, scale_eta=1.0)
+    d.update(kw)
+    return OrcRheoModel(**d)
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -56,6 +89,7 @@ def lib():
                                    C.c_int, C.c_void_p, C.c_int, C.c_int]
         L.orc_c2v.argtypes = [gp, C.c_int, f64p, f64p]
         L.orc_apply_bc.argtypes = [gp, f64p, f64p]
+        L.orc_exz_n.argtypes = [gp, f64p, f64p]
         L.orc_strain_rates.argtypes = [gp, C.c_int, f64p, f64p, f64p, f64p, f64p, f64p,
                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_eval_rhs.argtypes = [gp, C.c_double, C.c_double, f64p, f64p, f64p, f64p]
@@ -69,6 +103,13 @@ def lib():
                                        C.POINTER(KillerOpts)]
         L.orc_killer_solve.restype = C.c_int
         L.orc_set_band_order.argtypes = [gp, i32p, i32p, C.c_int]
+        L.orc_viscosity_concise.argtypes = [C.POINTER(OrcPhase), C.POINTER(OrcRheoModel)] + \
+            [C.c_double] * 10 + [_dp, _dp, _dp, C.POINTER(C.c_int)]
+        L.orc_viscosity_concise.restype = C.c_double
+        L.orc_rheology_nodes.argtypes = [C.POINTER(OrcRheoModel), C.POINTER(OrcPhase), C.c_int,
+                                         C.c_int] + [C.c_void_p] * 14 + [C.c_void_p, C.c_int,
+                                         C.c_int] + [C.c_void_p] * 7
+        L.orc_rheology_nodes.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -168,6 +209,12 @@ def strain_rates(g, u, v, compressible=0):
     return out
 
 
+def exz_n(g, exz):
+    out = np.zeros((g.Nx - 1) * (g.Nz - 1))
+    lib().orc_exz_n(g.c, np.ascontiguousarray(exz), out)
+    return out
+
+
 def eval_rhs(g, gx, gz, rho_n, rho_s):
     rx = np.zeros(g.Nx * (g.Nz + 1)); rz = np.zeros((g.Nx + 1) * g.Nz)
     lib().orc_eval_rhs(g.c, gx, gz, rho_n, rho_s, rx, rz)
@@ -216,3 +263,46 @@ def killer_solve(g, num, A, B, Cm, bu, bp, penalty=1e5, max_its_PH=10, rel_tol_K
     st = lib().orc_killer_solve(nm, nc, *a, *b, *c, np.ascontiguousarray(bu),
                                 np.ascontiguousarray(bp), du, dp, C.byref(o))
     return du, dp, o, st
+
+
+def viscosity_concise(ph, md, G, T, P, Exx, Ezz, Exz, fric=None, Coh=None, dil=None,
+                      pref_mult=1.0):
+    """RheologyDensity.c:385-1080 on one (node, phase): returns
+    (eta_dissipative, etaVE, Pcorr, Eii_pl, status)."""
+    e = C.c_double(); pc = C.c_double(); epl = C.c_double(); st = C.c_int(0)
+    eta = lib().orc_viscosity_concise(C.byref(ph), C.byref(md), G, T, P, Exx, Ezz, Exz,
+                                      ph.fric if fric is None else fric,
+                                      ph.C if Coh is None else Coh,
+                                      ph.dil if dil is None else dil, pref_mult,
+                                      C.byref(e), C.byref(pc), C.byref(epl), C.byref(st))
+    return eta, e.value, pc.value, epl.value, st.value
+
+
+def rheology_nodes(md, phases, perc, G, T, P, exx, ezz, exz, sxx0=None, szz0=None, sxz0=None,
+                   fric=None, Coh=None, dil=None, pref=None, tag=None, centroid=1, average=0):
+    """NonNewtonianViscosityGrid (RheologyDensity.c:1131-1426) on one node set.
+    perc has shape (nph, n).  Returns dict of eta, eta_phys, sxx, szz, sxz, p_corr,
+    eII_pl and the status (-1 when a local iteration failed)."""
+    perc = np.ascontiguousarray(perc, np.float64)
+    nph, n = perc.shape
+    arr = (C.c_double * 1)
+    keep = []
+
+    def p(a, dt=np.float64):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dt), (n,)))
+        keep.append(a)
+        return a.ctypes.data
+    z = np.zeros(n)
+    outs = {k: np.zeros(n) for k in ("eta", "eta_phys", "sxx", "szz", "sxz", "p_corr", "eII_pl")}
+    ph = (OrcPhase * nph)(*phases)
+    st = lib().orc_rheology_nodes(C.byref(md), ph, nph, n, perc.ctypes.data, p(G), p(T), p(P),
+                                  p(exx), p(ezz), p(exz), p(z if sxx0 is None else sxx0),
+                                  p(z if szz0 is None else szz0), p(z if sxz0 is None else sxz0),
+                                  p(fric), p(Coh), p(dil), p(pref), p(tag, np.int8), centroid,
+                                  average, *[outs[k].ctypes.data for k in
+                                             ("eta", "eta_phys", "sxx", "szz", "sxz", "p_corr",
+                                              "eII_pl")])
+    outs["status"] = st
+    return outs

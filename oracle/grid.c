@@ -395,3 +395,21 @@ void orc_eval_rhs(const orc_grid *g, double gx, double gz, const double *rho_n,
             roger_z[c] = -r;
         }
 }
+
+/* RheologyDensity.c:2066-2085: exz_n = 1/4 of the non-air vertex values
+ * around each non-air cell. */
+void orc_exz_n(const orc_grid *g, const double *exz, double *exz_n) {
+    const int Nx = g->Nx, Ncx = Nx - 1, Ncz = g->Nz - 1;
+    for (int l = 0; l < Ncz; l++)
+        for (int k = 0; k < Ncx; k++) {
+            const int c0 = k + l * Ncx, c1 = k + l * Nx;
+            double s = 0.0;
+            if (g->BCp_t[c0] != 30 && g->BCp_t[c0] != 31) {
+                if (g->BCg_t[c1] != 30) s += 0.25 * exz[c1];
+                if (g->BCg_t[c1 + 1] != 30) s += 0.25 * exz[c1 + 1];
+                if (g->BCg_t[c1 + Nx] != 30) s += 0.25 * exz[c1 + Nx];
+                if (g->BCg_t[c1 + Nx + 1] != 30) s += 0.25 * exz[c1 + Nx + 1];
+            }
+            exz_n[c0] = s;
+        }
+}

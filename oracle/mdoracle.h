@@ -67,6 +67,9 @@ void orc_strain_rates(const orc_grid *g, int compressible, const double *u, cons
                       double *div, double *exxd, double *ezzd, double *exz, double *wxz,
                       double *exxd_s, double *ezzd_s, double *div_s);
 
+/* RheologyDensity.c:2066-2085 (exz averaged to the cells) */
+void orc_exz_n(const orc_grid *g, const double *exz, double *exz_n);
+
 /* StokesRoutines.c:961-1160 */
 void orc_eval_rhs(const orc_grid *g, double gx, double gz, const double *rho_n,
                   const double *rho_s, double *roger_x, double *roger_z);
@@ -111,6 +114,40 @@ int orc_killer_solve(int neq_mom, int neq_cont,
                      const int *Cp, const int *Cj, const double *Cx,
                      const double *rhs_mom, const double *rhs_cont, double *du, double *dp,
                      orc_killer_opts *opts);
+
+/* Material parameters of one phase (mat_prop of mdoodz.h, the subset used
+ * by ViscosityConcise below; units are the model's scaled units). */
+typedef struct {
+    int    cstv, pwlv, plast;
+    double eta0;                                     /* constant viscosity   */
+    double pref_pwl, F_pwl, A_pwl, n_pwl, Q_pwl, V_pwl; /* power-law creep   */
+    double C, fric, dil, Slim, eta_vp, n_vp;         /* Drucker-Prager       */
+    double beta_for_K;                               /* compressibility beta */
+} orc_phase;
+
+typedef struct {
+    int    step, elastic;
+    double dt, R, zeroC, min_eta, max_eta, eta_tol, Tiimin, scale_E, scale_eta;
+} orc_rheo_model;
+
+/* RheologyDensity.c:385-1080 (subset: constant, power law, elasticity,
+ * Drucker-Prager plast = 1).  pref_mult multiplies the power-law prefactor
+ * (1 in the reference). */
+double orc_viscosity_concise(const orc_phase *m, const orc_rheo_model *md, double G, double T,
+                             double P, double Exx, double Ezz, double Exz, double fric,
+                             double C, double dil, double pref_mult, double *etaVE,
+                             double *Pcorr, double *Eii_pl, int *status);
+
+/* RheologyDensity.c:1157-1316 / 1320-1407 on one node set.  Nullable
+ * arrays: fric, C, dil (phase values), pref (1), outputs except eta_out. */
+int orc_rheology_nodes(const orc_rheo_model *md, const orc_phase *phases, int nph, int n,
+                       const double *perc, const double *G, const double *T, const double *P,
+                       const double *exx, const double *ezz, const double *exz,
+                       const double *sxx0, const double *szz0, const double *sxz0,
+                       const double *fric, const double *C, const double *dil,
+                       const double *pref, const signed char *tag, int centroid, int average,
+                       double *eta_out, double *eta_phys, double *sxx, double *szz, double *sxz,
+                       double *p_corr, double *eII_pl);
 
 /* Row-interleaved momentum DOF order (Vx row l, then Vz row l) used to
  * give the exact factor of orc_killer_solve a narrow band. */

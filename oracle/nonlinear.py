@@ -1,0 +1,93 @@
+"""Oracle non-linear (Picard) loop for visco-elasto-plastic Stokes.
+TEST INFRASTRUCTURE ONLY.
+
+Restates the order of operations of the non-linear loop of
+Main_DOODZ.c:878-1062 with Newton = 0 and line_search = 0:
+  UpdateNonLinearity (StokesRoutines.c:55):
+    ApplyBC -> StrainRateComponents (RheologyDensity.c:2001-2097, with exz_n
+    and the vertex copies of exxd/ezzd) -> NonNewtonianViscosityGrid
+    (RheologyDensity.c:1131-1426; T and p_in carried to the vertices by
+    InterpCentroidsToVerticesDouble at :1150-1151)
+  RheologicalOperators (Picard: D11 = 2 eta_n, D33 = eta_s)
+  ExtractDiagonalScale when diag_scaling (StokesRoutines.c:1260-1300)
+  EvaluateStokesResidualDecoupled (norms of d .* F) and the convergence test
+  of Main_DOODZ.c:1003
+  SolveStokesDefectDecoupled with alpha = -1 (StokesRoutines.c:836-861).
+"""
+import numpy as np
+
+from . import (apply_bc, assemble_picard, c2v, eval_rhs, exz_n, killer_solve, lib,
+               residual, rheology_nodes)
+from .driver import scatter_solution
+
+RHEO_KEYS = ("eta", "eta_phys", "sxx", "szz", "sxz", "p_corr", "eII_pl")
+
+
+def strain_rate_components(g, u, v):
+    nc, nv = (g.Nx - 1) * (g.Nz - 1), g.Nx * g.Nz
+    out = {k: np.zeros(nc) for k in ("div", "exxd", "ezzd")}
+    out.update({k: np.zeros(nv) for k in ("exz", "wxz", "exxd_s", "ezzd_s", "div_s")})
+    lib().orc_strain_rates(g.c, 0, u, v, out["div"], out["exxd"], out["ezzd"], out["exz"],
+                           out["wxz"].ctypes.data, out["exxd_s"].ctypes.data,
+                           out["ezzd_s"].ctypes.data, out["div_s"].ctypes.data)
+    out["exz_n"] = exz_n(g, out["exz"])
+    return out
+
+
+def update_nonlinearity(g, st, md, phases, average):
+    """Strain rates then the grid rheology on centres and vertices; returns
+    (strain rates, centre outputs, vertex outputs)."""
+    apply_bc(g, st["u"], st["v"])
+    sr = strain_rate_components(g, st["u"], st["v"])
+    T_s = c2v(g, st["T"])
+    P_s = c2v(g, st["p"])
+    cen = rheology_nodes(md, phases, st["perc_n"], st["mu_n"], st["T"], st["p"], sr["exxd"],
+                         sr["ezzd"], sr["exz_n"], st["sxxd0"], st["szzd0"], st["sxz0_n"],
+                         tag=g.BCp_t, centroid=1, average=average)
+    ver = rheology_nodes(md, phases, st["perc_s"], st["mu_s"], T_s, P_s, sr["exxd_s"],
+                         sr["ezzd_s"], sr["exz"], st["sxxd0_s"], st["szzd0_s"], st["sxz0"],
+                         tag=g.BCg_t, centroid=0, average=average)
+    if cen["status"] or ver["status"]:
+        raise RuntimeError("visco-elastic local iterations failed")
+    return sr, cen, ver
+
+
+def picard_diag(g, num, eta_n, eta_s):
+    """ExtractDiagonalScale: diag(A) on momentum rows, 1 on continuity rows."""
+    A, _, _ = assemble_picard(g, num, eta_n, eta_s)
+    return np.concatenate([A.diagonal(), np.ones(num[4])])
+
+
+def picard_loop(g, st, md, phases, nit_max, tol, average=0, diag_scaling=1, gx=0.0, gz=0.0,
+                **killer_kw):
+    """Runs the loop in place on st (u, v, p updated).  tol = (abs_mom,
+    rel_mom, abs_div, rel_div).  Returns the list of residual norm triples
+    (one per evaluated iteration) and the number of linear solves."""
+    num = g.numbering()
+    nm = num[3]
+    rx, rz = eval_rhs(g, gx, gz, st["rho_n"], st["rho_s"])
+    abs_mom, rel_mom, abs_div, rel_div = tol
+    hist, nsolve, res0 = [], 0, None
+    for nit in range(nit_max + 1):
+        sr, cen, ver = update_nonlinearity(g, st, md, phases, average)
+        d = picard_diag(g, num, cen["eta"], ver["eta"]) if diag_scaling else None
+        F, norms = residual(g, num, cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"],
+                            sr["div"], rx, rz, d)
+        if res0 is None:
+            res0 = norms.copy()
+        hist.append(norms.copy())
+        r = norms / res0
+        if ((norms[0] < abs_mom or r[0] < rel_mom) and (norms[1] < abs_mom or r[1] < rel_mom)
+                and (norms[2] < abs_div or r[2] < rel_div)):
+            break
+        Fu, _ = residual(g, num, cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"], sr["div"],
+                         rx, rz)
+        A, B, Cm = assemble_picard(g, num, cen["eta"], ver["eta"])
+        du, dp, stats, rc = killer_solve(g, num, A, B, Cm, Fu[:nm], Fu[nm:], **killer_kw)
+        if rc != 0:
+            raise RuntimeError("KillerSolver status %d" % rc)
+        scatter_solution(num, np.concatenate([du, dp]), st["u"], st["v"], st["p"])
+        apply_bc(g, st["u"], st["v"])
+        nsolve += 1
+    st["eta_n"], st["eta_s"] = cen["eta"], ver["eta"]
+    return hist, nsolve

@@ -1,0 +1,195 @@
+/*
+ * Oracle: per-node visco-elasto-plastic rheology (subset used by the
+ * measured workloads).  TEST INFRASTRUCTURE ONLY.
+ *
+ * ViscosityConcise (RheologyDensity.c:385-1080) for constant viscosity,
+ * dislocation (power-law) creep, Maxwell elasticity and Drucker-Prager
+ * plasticity (plast = 1, viscoplastic regularisation eta_vp), with
+ * LocalIterationViscoElastic (RheologyDensity.c:152-197), and
+ * NonNewtonianViscosityGrid (RheologyDensity.c:1131-1426) for the
+ * phase-weighted grid update.  Grain-size, Peierls, diffusion, GBS, Popov
+ * (plast = 2), kinetics and melting branches are not part of this subset.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include "mdoracle.h"
+
+#define MINV(a, b) ((a) < (b) ? (a) : (b))
+#define MAXV(a, b) ((a) > (b) ? (a) : (b))
+
+/* RheologyDensity.c:152-197: Newton iterations on eta_ve so that
+ * E = elastic*T/(2 eta_el) + T/(2 eta_cst) + C_pwl T^n with T = 2 eta_ve E. */
+static int local_ve(double Eii, double *eta_ve, int elastic, double eta_el, int constant,
+                    double eta_cst, int dislocation, double C_pwl, double n_pwl, double *Eii_pwl_out) {
+    const int nitmax = 20;
+    const double tol = 1.0e-11;
+    double eta = *eta_ve;
+    for (int it = 0; it < nitmax; it++) {
+        const double Tii = 2.0 * eta * Eii;
+        double Eii_cst = 0.0, Eii_pwl = 0.0;
+        if (constant) Eii_cst = Tii / 2.0 / eta_cst;
+        if (dislocation) Eii_pwl = C_pwl * pow(Tii, n_pwl);
+        const double r = Eii - elastic * Tii / (2.0 * eta_el) - (Eii_pwl + Eii_cst);
+        const double res = fabs(r / Eii);
+        *Eii_pwl_out = Eii_pwl;
+        if (res < tol / 100) break;
+        if (it == nitmax - 1 && res > tol) return -1;   /* reference exits */
+        double dfdeta = 0.0;
+        if (elastic) dfdeta += -Eii / eta_el;
+        if (dislocation) dfdeta += -Eii_pwl * n_pwl / eta;
+        if (constant) dfdeta += -Eii / eta_cst;
+        eta -= r / dfdeta;
+    }
+    *eta_ve = eta;
+    return 0;
+}
+
+/* One phase at one node: returns the effective (VE or VEP) viscosity in
+ * *etaVE, the dissipative viscosity as return value, and Pcorr. */
+double orc_viscosity_concise(const orc_phase *m, const orc_rheo_model *md, double G, double T,
+                             double P, double Exx, double Ezz, double Exz, double fric,
+                             double C, double dil, double pref_mult, double *etaVE,
+                             double *Pcorr, double *Eii_pl, int *status) {
+    double dt = (md->step == 0) ? 1e100 : md->dt;
+    int elastic = md->elastic, plastic = (m->plast == 1);
+    if (md->step == 0) { elastic = 0; plastic = 0; }
+    const int constant = m->cstv != 0, dislocation = m->pwlv != 0;
+    double K = 1.0 / m->beta_for_K;
+    const double Eyy = -(Exx + Ezz);
+    double Eii = sqrt(0.5 * (Exx * Exx + Ezz * Ezz + Eyy * Eyy) + Exz * Exz);
+    if (Eii * md->scale_E < 1e-30) Eii = 1e-30 / md->scale_E;
+    *Pcorr = P;
+    if (!elastic) { G = 1e1; dil = 0.0; K = 1e10; }
+    if (T < md->zeroC) T = md->zeroC;
+    double B_pwl = 0.0, C_pwl = 0.0;
+    if (dislocation) {
+        B_pwl = m->pref_pwl * pref_mult * m->F_pwl * pow(m->A_pwl, -1.0 / m->n_pwl) *
+                exp((m->Q_pwl + P * m->V_pwl) / md->R / m->n_pwl / T);
+        C_pwl = pow(2.0 * B_pwl, -m->n_pwl);
+    }
+    const double cos_fric = cos(fric), sin_fric = sin(fric), sin_dil = sin(dil);
+    double Tyield = 0.0;
+    if (plastic) {
+        Tyield = C * cos_fric + P * sin_fric;
+        if (m->Slim < Tyield) { C = m->Slim; Tyield = m->Slim; }
+    }
+    const double eta_el = G * dt;
+    double eta_cst = 0.0, eta_pwl = 0.0;
+    if (constant) eta_cst = m->eta0;
+    if (dislocation) eta_pwl = B_pwl * pow(Eii, 1.0 / m->n_pwl - 1.0);
+    double eta_up = 1.0e100 / md->scale_eta;
+    if (constant) eta_up = MINV(eta_up, eta_cst);
+    if (dislocation) eta_up = MINV(eta_up, eta_pwl);
+    if (elastic) eta_up = MINV(eta_up, eta_el);
+    double eta_ve = eta_up, Eii_pwl = 0.0;
+    if (local_ve(Eii, &eta_ve, elastic, eta_el, constant, eta_cst, dislocation, C_pwl, m->n_pwl,
+                 &Eii_pwl))
+        *status = -1;
+    double Tii = 2.0 * eta_ve * Eii;
+    int is_pl = 0;
+    double eta_vep = eta_ve, gdot = 0.0, eta_vp = m->eta_vp;
+    if (plastic) {
+        /* RheologyDensity.c:655-707 */
+        double F_trial = Tii - Tyield;
+        if (F_trial > 1e-17) {
+            is_pl = 1;
+            const double F_trial0 = F_trial, tol = md->eta_tol;
+            eta_vp = m->eta_vp * pow(Eii, 1.0 / m->n_vp - 1);
+            gdot = F_trial / (eta_ve + eta_vp + K * dt * sin_fric * sin_dil);
+            double Pc = P, Tiic = Tii;
+            for (int it = 0; it < 20; it++) {
+                const double A_P = -sin_dil;
+                const double divp = -gdot * A_P;
+                Pc = P + K * dt * divp;
+                eta_vp = m->eta_vp * pow(fabs(gdot), 1.0 / m->n_vp - 1.0);
+                Tyield = Tyield + gdot * eta_vp;
+                Tiic = Tii - eta_ve * gdot;
+                F_trial = Tiic - Tyield;
+                const double res = fabs(F_trial);
+                if (res < tol || res / F_trial0 < tol) break;
+                const double dFdgdot = -eta_ve - eta_vp / m->n_vp - K * dt * sin_fric * sin_dil;
+                gdot -= F_trial / dFdgdot;
+            }
+            if (Tiic < 0.0) {   /* tension: von Mises minimum stress Tiimin */
+                gdot = (Tii - md->Tiimin) / eta_ve;
+                Tiic = md->Tiimin;
+                Pc = P;
+            }
+            *Pcorr = Pc;
+            eta_vep = Tiic / (2.0 * Eii);
+            Tii = Tiic;
+        }
+    }
+    *etaVE = is_pl ? eta_vep : eta_ve;
+    *Eii_pl = is_pl ? gdot / 2.0 : 0.0;
+    /* dissipative viscosity (post-processing branch, RheologyDensity.c:1011-1017) */
+    double inv = 0.0;
+    if (dislocation) inv += 1.0 / (B_pwl * pow(Eii_pwl, 1.0 / m->n_pwl - 1.0));
+    if (constant) inv += 1.0 / eta_cst;
+    if (is_pl) inv += 1.0 / (Tii / gdot);
+    double eta = 1.0 / inv;
+    if (*etaVE > md->max_eta) *etaVE = md->max_eta;
+    if (*etaVE < md->min_eta) *etaVE = md->min_eta;
+    if (eta > md->max_eta) eta = md->max_eta;
+    if (eta < md->min_eta) eta = md->min_eta;
+    return eta;
+}
+
+/* NonNewtonianViscosityGrid (RheologyDensity.c:1131-1426) on one node set:
+ * n nodes, nph phases with fractions perc[p*n + i].  Centres skip tags 30
+ * and 31 (:1188), vertices tag 30 (:1333). */
+int orc_rheology_nodes(const orc_rheo_model *md, const orc_phase *phases, int nph, int n,
+                       const double *perc, const double *G, const double *T, const double *P,
+                       const double *exx, const double *ezz, const double *exz,
+                       const double *sxx0, const double *szz0, const double *sxz0,
+                       const double *fric, const double *C, const double *dil,
+                       const double *pref, const signed char *tag, int centroid, int average,
+                       double *eta_out, double *eta_phys, double *sxx, double *szz, double *sxz,
+                       double *p_corr, double *eII_pl) {
+    const double el = md->elastic ? 1.0 : 0.0;
+    int status = 0;
+    for (int i = 0; i < n; i++) {
+        if (tag && (tag[i] == 30 || (centroid && tag[i] == 31))) {
+            /* air cells / vertices: every output left at zero */
+            eta_out[i] = 0.0;
+            if (eta_phys) eta_phys[i] = 0.0;
+            if (sxx) sxx[i] = 0.0;
+            if (szz) szz[i] = 0.0;
+            if (sxz) sxz[i] = 0.0;
+            if (p_corr) p_corr[i] = 0.0;
+            if (eII_pl) eII_pl[i] = 0.0;
+            continue;
+        }
+        const double eta_e = md->elastic ? md->dt * G[i] : 1.0;
+        const double Exx = exx[i] + el * sxx0[i] / eta_e / 2.0;
+        const double Ezz = ezz[i] + el * szz0[i] / eta_e / 2.0;
+        const double Exz = exz[i] + el * sxz0[i] / eta_e / 2.0;
+        double e = 0.0, ep = 0.0, pc = 0.0, epl = 0.0;
+        for (int p = 0; p < nph; p++) {
+            const double f = perc[(size_t)p * n + i];
+            if (!(fabs(f) > 1e-13)) continue;
+            double etaVE, Pcorr, Eii_pl;
+            const double eta = orc_viscosity_concise(&phases[p], md, G[i], T[i], P[i], Exx, Ezz, Exz,
+                                                     fric ? fric[i] : phases[p].fric,
+                                                     C ? C[i] : phases[p].C,
+                                                     dil ? dil[i] : phases[p].dil,
+                                                     pref ? pref[i] : 1.0, &etaVE, &Pcorr, &Eii_pl,
+                                                     &status);
+            if (average == 0) { e += f * etaVE; ep += f * eta; }
+            if (average == 1) { e += f / etaVE; ep += f / eta; }
+            if (average == 2) { e += f * log(etaVE); ep += f * log(eta); }
+            pc += f * Pcorr;
+            epl += f * Eii_pl;
+        }
+        if (average == 1) { e = 1.0 / e; ep = 1.0 / ep; }
+        if (average == 2) { e = exp(e); ep = exp(ep); }
+        eta_out[i] = e;
+        if (eta_phys) eta_phys[i] = ep;
+        if (sxx) sxx[i] = 2.0 * e * Exx;
+        if (szz) szz[i] = 2.0 * e * Ezz;
+        if (sxz) sxz[i] = 2.0 * e * Exz;
+        if (p_corr) p_corr[i] = pc;
+        if (eII_pl) eII_pl[i] = epl;
+    }
+    return status;
+}
