@@ -42,13 +42,6 @@ def cpu_model():
     return platform.processor()
 
 
-def free_slip_tags(Nx):
-    import oracle
-    g = oracle.Grid(Nx, Nx, 0.0, 1.0, 0.0, 1.0)
-    g.pure_shear(0.0)
-    return g
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +59,7 @@ def main():
         sys.exit(2)
     import numpy as np
     import mdgpu
+    from mdgpu import learned
     from mdgpu import workloads as W
 
     t0 = time.time()
@@ -75,8 +69,7 @@ def main():
     nc, nv = (Nx - 1) ** 2, Nx * Nx
     log("workload: %d^2 cells, %d markers, built in %.1fs" % (a.cells, n, time.time() - t0))
     ctx = mdgpu.Context(Nx, Nx, 0.0, 1.0, 0.0, 1.0)
-    g = free_slip_tags(Nx)
-    ctx.set_bc(g.BCu_t, g.BCu_v, g.BCv_t, g.BCv_v, g.BCp_t, g.BCg_t)
+    ctx.set_bc(*learned.free_slip_bc(Nx))
     dx_ = ctx.device(n, host=wl["x"])
     dz_ = ctx.device(n, host=wl["z"])
     dph = ctx.device(n, np.int32, wl["phase"])

@@ -128,23 +128,27 @@ int mdg_eqn_count(const mdg_ctx *ctx, int *neq, int *neq_mom, int *neq_cont);
 int mdg_stokes_apply(mdg_ctx *ctx, const double *D11_n, const double *D22_n,
                      const double *D33_s, const double *x, double *y);
 
-/* Ghost velocities from the tags (ApplyBC, StokesRoutines.c:415-500), in place
+/* Ghost velocities from the tags (ApplyBC, StokesRoutines.c:415-500; prototype
+ * mdoodz-private.h:540), in place
  * on the Vx (Nx*(Nz+1)) and Vz ((Nx+1)*Nz) arrays. */
 int mdg_apply_bc(mdg_ctx *ctx, double *u, double *v);
 
-/* StrainRateComponents (RheologyDensity.c:2001-2097, incompressible): div,
- * exxd, ezzd on cells and exz on vertices from u_in, v_in. */
+/* StrainRateComponents (RheologyDensity.c:2001-2097, incompressible;
+ * prototype mdoodz-private.h:426): div, exxd, ezzd on cells and exz on
+ * vertices from u_in, v_in. */
 int mdg_strain_rates(mdg_ctx *ctx, const double *u, const double *v, double *div,
                      double *exxd, double *ezzd, double *exz);
 
-/* ExtractDiagonalScale (StokesRoutines.c:1260-1300): the scaling vector d
+/* ExtractDiagonalScale (StokesRoutines.c:1260-1300; prototype
+ * mdoodz-private.h:604): the scaling vector d
  * (equation numbering) = diagonal of the Picard block A on momentum rows and
  * 1 on continuity rows.  Residual norms with diag_scaling = 1 are taken on
  * d .* F (mdg_stokes_residual's d argument). */
 int mdg_stokes_diag(mdg_ctx *ctx, const double *D11_n, const double *D22_n, const double *D33_s,
                     double *d);
 
-/* InterpCentroidsToVerticesDouble (GridRoutines.c:108-212): vertex value =
+/* InterpCentroidsToVerticesDouble (GridRoutines.c:108-212; prototype
+ * mdoodz-private.h:294): vertex value =
  * 1/4 of the four surrounding centre values on the edge-copied padded array;
  * tag-30/31 centres contribute zero (divisor stays 4), tag-30 vertices get 0.
  * periodic_x selects the periodic column copy. */
@@ -193,7 +197,8 @@ typedef struct {
 
 #define MDG_MAX_PHASES 64
 
-/* NonNewtonianViscosityGrid (RheologyDensity.c:1131-1426) on the centres
+/* NonNewtonianViscosityGrid (RheologyDensity.c:1131-1426; prototype
+ * mdoodz-private.h:423) on the centres
  * (centroid = 1, loop :1157-1316) or the vertices (centroid = 0, :1320-1407):
  * per node the elastic effective strain rate E + tau0/(2 G dt), per active
  * phase (fraction > 1e-13) ViscosityConcise (:385-1080), the phase average
@@ -203,12 +208,14 @@ typedef struct {
 int mdg_viscosity_grid(mdg_ctx *ctx, const mdg_phase_prop *phases, int nph,
                        const mdg_rheo_params *prm, int centroid, const mdg_rheo_nodes *f);
 
-/* EvaluateRHS body force (StokesRoutines.c:961-1160): roger_x = gx * mean of
+/* EvaluateRHS body force (StokesRoutines.c:961-1160; prototype
+ * mdoodz-private.h:357): roger_x = gx * mean of
  * the two vertex densities, roger_z = gz * mean of the two cell densities. */
 int mdg_eval_rhs(mdg_ctx *ctx, double gx, double gz, const double *rho_n, const double *rho_s,
                  double *roger_x, double *roger_z);
 
-/* Residual of EvaluateStokesResidualDecoupled (StokesRoutines.c:867-955)
+/* Residual of EvaluateStokesResidualDecoupled (StokesRoutines.c:867-955;
+ * prototype mdoodz-private.h:497)
  * from the grid stresses: F (equation numbering) and the interior RMS norms
  * norms[0..2] = resx, resz, resp.  d (equation numbering) is the diagonal
  * scaling of ExtractDiagonalScale, or NULL for none. */
@@ -217,8 +224,8 @@ int mdg_stokes_residual(mdg_ctx *ctx, const double *sxxd, const double *szzd, co
                         const double *roger_z, const double *d, double *F, double norms[3]);
 
 /* --------------------------------------------------------------- Stokes solve
- * Replaces KillerSolver (Solvers.c:1054-1454; called from
- * SolveStokesDefectDecoupled, StokesRoutines.c:845-848) for the Picard
+ * Replaces KillerSolver (Solvers.c:1054-1454; prototype mdoodz-private.h:494;
+ * called from SolveStokesDefectDecoupled, StokesRoutines.c:845-848) for the Picard
  * operator: solves K sol = [rhs_mom; rhs_cont] (equation numbering) to a
  * relative 2-norm residual rel_tol.  Outer flexible GCR (FP64) on the
  * augmented system with a viscosity-weighted penalty gamma*eta_n/dV
@@ -237,7 +244,8 @@ typedef struct {
     int    cheb_degree;    /* Chebyshev smoothing degree per sweep (8)         */
     int    coarse_cells;   /* coarsen until <= this many cells per side (8)    */
     int    precision;      /* MDG_PREC_FP64 or MDG_PREC_MIXED                  */
-    int    precond;        /* reserved: 0 = multigrid                          */
+    int    precond;        /* 0 multigrid; 1 multigrid CG started from the
+                              learned network's guess (mdg_pcnet_*)            */
     int    cycle;          /* 1 V-cycle, 2 W-cycle (1)                         */
     int    eta_avg;        /* coarse viscosity: 0 geometric, 1 arithmetic,
                               2 harmonic, 3 maximum (0)                        */
@@ -250,10 +258,45 @@ typedef struct {
     /* outputs */
     int    its, inner_its, poisson_its, levels, status;
     double rel_res, time_ms;
+    int    guess_used;     /* inner solves started from the network's guess    */
+    double guess_gain;     /* mean 1 - ||r0||^2/||b||^2 of those guesses        */
 } mdg_solver_opts;
 int mdg_killer_solver(mdg_ctx *ctx, mdg_solver_opts *opts, const double *D11_n,
                       const double *D22_n, const double *D33_s, const double *rhs_mom,
                       const double *rhs_cont, double *sol);
+
+/* ------------------------------------------------ learned preconditioner
+ * A 4-level U-Net of 3x3 convolutions (4 -> 32 -> 32 -> 64 -> 64 channels
+ * down, 64 -> 32 -> 32 -> 3 up, 2x2 mean pooling, nearest upsampling with
+ * additive skips, ReLU) computed with BF16 MFMA and FP32 accumulation.  No
+ * counterpart in the reference: it is the learned part of the solve.
+ * Images are H x W x 4 FP32 in NHWC order (H, W multiples of 8); outputs use
+ * channels 0..2 of an H x W x 4 array.  Parameters are one FP32 vector:
+ * per layer W[tap][ci][co] (tap = 3*(dy+1) + (dx+1)) then b[co]. */
+int mdg_pcnet_param_count(mdg_ctx *ctx, long long *n);
+int mdg_pcnet_set_params(mdg_ctx *ctx, const float *params, long long n);
+/* params and/or grads (the last mdg_pcnet_grad result) may be NULL. */
+int mdg_pcnet_get_params(mdg_ctx *ctx, float *params, float *grads);
+int mdg_pcnet_forward(mdg_ctx *ctx, int H, int W, const float *x, float *y);
+/* Loss 0.5 * sum(mask (y - t)^2) / sum(mask) over channels 0..2 and its
+ * gradient (added to the previous one when accumulate != 0). */
+int mdg_pcnet_grad(mdg_ctx *ctx, int H, int W, const float *x, const float *t, const float *mask,
+                   int accumulate, double *loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) with the stored gradient. */
+int mdg_pcnet_adam(mdg_ctx *ctx, float lr);
+/* Training pairs generated by the solver itself: while max_pairs > 0,
+ * every inner multigrid-CG solve of the velocity block A_gamma y = b in
+ * mdg_killer_solver (precond = 0) stores the input image (log10(eta_n)/3 and
+ * D^-1 b / s on each cell's W and S faces, s = RMS of D^-1 b) and the target
+ * y / s with its mask, in a ring of max_pairs device slots (one grid size).
+ * mdg_pcnet_capture(ctx, 0) stops capturing; a new call clears the ring. */
+int mdg_pcnet_capture(mdg_ctx *ctx, int max_pairs);
+int mdg_pcnet_captured(mdg_ctx *ctx, int *n, int *H, int *W);
+int mdg_pcnet_get_pair(mdg_ctx *ctx, int i, float *x, float *t, float *mask);
+/* steps Adam updates on the captured pairs, batch pairs per step (chosen
+ * by a counter hash of seed), learning rate lr; mean loss per step. */
+int mdg_pcnet_train(mdg_ctx *ctx, int steps, int batch, float lr, unsigned long long seed,
+                    double *losses);
 
 /* Times the dominant kernel of the solve: the fine-level multigrid smoother
  * operator (r -= A_g d, FP64) of the last FP64 solve, averaged over reps

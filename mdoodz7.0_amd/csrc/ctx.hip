@@ -88,6 +88,7 @@ extern "C" void mdg_destroy(mdg_ctx *c) {
     c->tmp_d.release(); c->gx.release(); c->gy.release(); c->eqbuf.release(); c->part.release();
     mdg_solver_release(c);
     mdg_rheo_release(c);
+    mdg_pcnet_release(c);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

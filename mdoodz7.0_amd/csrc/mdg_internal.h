@@ -66,6 +66,7 @@ struct mdg_ctx {
     dbuf<double>      part;                  // reduction partials
     struct solver_state *solver = nullptr;
     struct rheo_scratch *rheo = nullptr;
+    struct pcnet_state *pcnet = nullptr;
     // runtime
     hipStream_t  stream = nullptr;
     hipEvent_t   ev0 = nullptr, ev1 = nullptr;
@@ -109,4 +110,5 @@ int mdg_node_set(mdg_ctx *c, int centroid, node_set *ns);
 // Frees the solver's device state (defined with the solver).
 void mdg_solver_release(mdg_ctx *c);
 void mdg_rheo_release(mdg_ctx *c);
+void mdg_pcnet_release(mdg_ctx *c);
 int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x);

@@ -23,6 +23,7 @@
 #include <memory>
 
 #include "stokes_internal.h"
+#include "pcnet_internal.h"
 
 #define RB 256   // reduction block size
 
@@ -750,6 +751,8 @@ struct solver_state {
     dbuf<double> part, scal;
     int m = 0;
     size_t nfull = 0;
+    int guess_used = 0;       // inner solves started from the network's guess
+    double guess_gain = 0.0;  // sum of the residual reductions 1 - ||r0||^2/||b||^2
 };
 
 void mdg_solver_release(mdg_ctx *c) {
@@ -899,6 +902,7 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
     for (size_t i = 0; i < A.size(); i++) Ai[i] = (T)A[i];
     HIPCHK(c, cl.Ainv.reserve(Ai.size()));
     HIPCHK(c, hipMemcpyAsync(cl.Ainv.p, Ai.data(), Ai.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // Ai is a local host vector
     if (o->schur == 1) {
         int rc = build_poisson(c, s, H);
         if (rc) return rc;
@@ -975,6 +979,7 @@ static int build_poisson(mdg_ctx *c, solver_state *s, mg_hier<T> &H) {
     for (size_t q = 0; q < A.size(); q++) Ai[q] = (T)A[q];
     HIPCHK(c, cl.pAinv.reserve(Ai.size()));
     HIPCHK(c, hipMemcpyAsync(cl.pAinv.p, Ai.data(), Ai.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // Ai is a local host vector
     const int n0 = ncell(H.L[0].g), nb = mdg_grid1d(n0, RB);
     HIPCHK(c, H.pc_x.reserve(n0)); HIPCHK(c, H.pc_r.reserve(n0)); HIPCHK(c, H.pc_p.reserve(n0));
     HIPCHK(c, H.pc_q.reserve(n0)); HIPCHK(c, H.bf_t.reserve(H.L[0].n));
@@ -1205,7 +1210,31 @@ static int inner_pcg(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solve
     if ((rc = dot_host<T>(c, s, n, H.cg_r.p, H.cg_r.p, &rr0))) return rc;
     *its = 0;
     if (rr0 == 0.0) return MDG_OK;
-    const double tol2 = o->inner_tol * o->inner_tol * rr0;
+    const double tol2 = o->inner_tol * o->inner_tol * rr0;   // relative to ||b||
+    if (o->precond == 1 && mdg_pcnet_ready(c)) {
+        // learned initial guess: the network gives a direction d, scaled by
+        // the residual-minimising alpha = (b, A d) / (A d, A d), so that
+        // ||b - alpha A d|| <= ||b||; multigrid CG finishes the solve
+        if ((rc = mdg_pcnet_guess<T>(c, l0.g, (const T *)H.cg_r.p, (const T *)l0.dinv.p, (const T *)l0.en.p, H.cg_x.p)))
+            return rc;
+        hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(n)), dim3(256), 0, c->stream, l0.g, (T)o->penalty,
+                           (const T *)H.cg_x.p, (const T *)nullptr, (const T *)l0.en.p, (const T *)l0.es.p, H.cg_q.p);
+        double bq, qq;   // q = -A d
+        if ((rc = dot_host<T>(c, s, n, H.cg_r.p, H.cg_q.p, &bq)) || (rc = dot_host<T>(c, s, n, H.cg_q.p, H.cg_q.p, &qq)))
+            return rc;
+        const double alpha = (qq > 0.0) ? -bq / qq : 0.0;
+        if (alpha > 0.0 && std::isfinite(alpha)) {
+            hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)0, (const T *)H.cg_q.p, (T)alpha, H.cg_x.p);
+            hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)alpha, (const T *)H.cg_q.p, (T)1, H.cg_r.p);
+            s->guess_gain += bq * bq / (qq * rr0);
+            s->guess_used++;
+            if (rr0 - bq * bq / qq <= tol2) return MDG_OK;
+        } else {
+            hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, H.cg_x.p, T(0));
+        }
+    } else if ((rc = mdg_pcnet_capture_in<T>(c, l0.g, (const T *)H.cg_r.p, (const T *)l0.dinv.p, (const T *)l0.en.p))) {
+        return rc;
+    }
     const int every = o->check_every > 0 ? o->check_every : 1;
     for (int it = 0; it < o->inner_max_its; it++) {
         HIPCHK(c, hipGraphLaunch(H.exec, c->stream));
@@ -1215,7 +1244,7 @@ static int inner_pcg(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solve
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (!(H.h_scal[2] > 0.0) || H.h_scal[3] <= tol2) break;
     }
-    return MDG_OK;
+    return mdg_pcnet_capture_out<T>(c, l0.g, (const T *)H.cg_x.p);
 }
 
 // z = M r (block preconditioner, FP64 in/out)
@@ -1312,6 +1341,8 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     HIPCHK(c, s->part.reserve(4096));
     HIPCHK(c, s->scal.reserve(16));
     default_opts(o);
+    s->guess_used = 0;
+    s->guess_gain = 0.0;
     if (o->precision == MDG_PREC_MIXED && o->schur == 1)
         return mdg_fail(c, MDG_ERR_UNSUPPORTED, "the BFBt Schur approximation runs in FP64 only (precision = MDG_PREC_FP64)");
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -1343,6 +1374,8 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     o->time_ms = ms;
     o->levels = c->solver ? (int)(o->precision == MDG_PREC_MIXED ? s->hf.L.size() : s->hd.L.size()) : 0;
     o->status = rc;
+    o->guess_used = s->guess_used;
+    o->guess_gain = s->guess_used ? s->guess_gain / s->guess_used : 0.0;
     return rc;
 }
 

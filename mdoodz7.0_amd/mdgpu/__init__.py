@@ -90,7 +90,7 @@ class SolverOpts(C.Structure):
                 ("poisson_max_its", C.c_int), ("its", C.c_int), ("inner_its", C.c_int),
                 ("poisson_its", C.c_int),
                 ("levels", C.c_int), ("status", C.c_int), ("rel_res", C.c_double),
-                ("time_ms", C.c_double)]
+                ("time_ms", C.c_double), ("guess_used", C.c_int), ("guess_gain", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -139,6 +139,16 @@ def _extra_signatures(L):
     L.mdg_time_smoother.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mdg_c2v.argtypes = [vp, C.c_int, vp, vp]
     L.mdg_stokes_diag.argtypes = [vp] * 5
+    L.mdg_pcnet_param_count.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.mdg_pcnet_set_params.argtypes = [vp, vp, C.c_longlong]
+    L.mdg_pcnet_get_params.argtypes = [vp, vp, vp]
+    L.mdg_pcnet_forward.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+    L.mdg_pcnet_grad.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.c_int, C.POINTER(C.c_double)]
+    L.mdg_pcnet_adam.argtypes = [vp, C.c_float]
+    L.mdg_pcnet_capture.argtypes = [vp, C.c_int]
+    L.mdg_pcnet_captured.argtypes = [vp] + [C.POINTER(C.c_int)] * 3
+    L.mdg_pcnet_get_pair.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.mdg_pcnet_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, vp]
     L.mdg_strain_rates_nodes.argtypes = [vp] * 9
     L.mdg_viscosity_grid.argtypes = [vp, C.POINTER(PhaseProp), C.c_int, C.POINTER(RheoParams),
                                      C.c_int, C.POINTER(RheoNodes)]
@@ -322,6 +332,60 @@ class Context:
             d = np.zeros(self.eqn_count()[0])
         self._check(self._L.mdg_stokes_diag(self.h, ptr(D11), ptr(D22), ptr(D33), ptr(d)))
         return d
+
+    # ---------------------------------------------------------- learned PC
+    def pcnet_param_count(self):
+        n = C.c_longlong()
+        self._check(self._L.mdg_pcnet_param_count(self.h, C.byref(n)))
+        return n.value
+
+    def pcnet_set_params(self, flat):
+        flat = np.ascontiguousarray(flat, np.float32)
+        self._check(self._L.mdg_pcnet_set_params(self.h, ptr(flat), flat.size))
+
+    def pcnet_get_params(self, grads=False):
+        n = self.pcnet_param_count()
+        p, g = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        self._check(self._L.mdg_pcnet_get_params(self.h, ptr(p), ptr(g)))
+        return (p, g) if grads else p
+
+    def pcnet_forward(self, x):
+        H, W = x.shape[:2]
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((H, W, 4), np.float32)
+        self._check(self._L.mdg_pcnet_forward(self.h, H, W, ptr(x), ptr(y)))
+        return y
+
+    def pcnet_grad(self, x, t, mask, accumulate=False):
+        H, W = x.shape[:2]
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (x, t, mask)]
+        loss = C.c_double()
+        self._check(self._L.mdg_pcnet_grad(self.h, H, W, *[ptr(a) for a in arrs],
+                                           int(accumulate), C.byref(loss)))
+        return loss.value
+
+    def pcnet_adam(self, lr):
+        self._check(self._L.mdg_pcnet_adam(self.h, C.c_float(lr)))
+
+    def pcnet_capture(self, max_pairs):
+        self._check(self._L.mdg_pcnet_capture(self.h, int(max_pairs)))
+
+    def pcnet_captured(self):
+        n, H, W = C.c_int(), C.c_int(), C.c_int()
+        self._check(self._L.mdg_pcnet_captured(self.h, C.byref(n), C.byref(H), C.byref(W)))
+        return n.value, H.value, W.value
+
+    def pcnet_get_pair(self, i):
+        _, H, W = self.pcnet_captured()
+        x, t, m = (np.zeros((H, W, 4), np.float32) for _ in range(3))
+        self._check(self._L.mdg_pcnet_get_pair(self.h, int(i), ptr(x), ptr(t), ptr(m)))
+        return x, t, m
+
+    def pcnet_train(self, steps, batch=1, lr=1e-3, seed=0):
+        losses = np.zeros(max(steps, 1))
+        self._check(self._L.mdg_pcnet_train(self.h, int(steps), int(batch), C.c_float(lr),
+                                            C.c_ulonglong(seed), ptr(losses)))
+        return losses[:steps]
 
     def c2v(self, cent, vert=None, periodic_x=0):
         """InterpCentroidsToVerticesDouble (GridRoutines.c:108-212)."""

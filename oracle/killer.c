@@ -24,7 +24,7 @@ static void csr_mv(int n, const int *p, const int *j, const double *x, const dou
     for (int i = 0; i < n; i++) {
         double s = 0.0;
         for (int q = p[i]; q < p[i + 1]; q++) s += x[q] * v[j[q]];
-        y[i] = beta * y[i] + alpha * s;
+        y[i] = (beta == 0.0) ? alpha * s : beta * y[i] + alpha * s;   /* y may be uninitialised */
     }
 }
 
@@ -212,10 +212,10 @@ int orc_killer_solve(int nu, int np, const int *Ap, const int *Aj, const double 
         free(L.perm); free(L.L); free(ident);
         return -1;
     }
-    double *work = (double *)malloc(sizeof(double) * (size_t)nu * 16);  /* kspgcr */
-    double *tp = (double *)malloc(sizeof(double) * np);
-    double *ud = (double *)malloc(sizeof(double) * nu), *fu = (double *)malloc(sizeof(double) * nu);
-    double *pd = (double *)malloc(sizeof(double) * np), *fpv = (double *)malloc(sizeof(double) * np);
+    double *work = (double *)calloc((size_t)nu * 16, sizeof(double));  /* kspgcr */
+    double *tp = (double *)calloc(np, sizeof(double));
+    double *ud = (double *)calloc(nu, sizeof(double)), *fu = (double *)calloc(nu, sizeof(double));
+    double *pd = (double *)calloc(np, sizeof(double)), *fpv = (double *)calloc(np, sizeof(double));
     js_op M = {nu, np, Ap, Aj, Bp, Bj, Cp, Cj, Ax, Bx, Cx, pen, tp};
     memset(du, 0, sizeof(double) * nu);
     memset(dp, 0, sizeof(double) * np);
@@ -248,6 +248,8 @@ int orc_killer_solve(int nu, int np, const int *Ap, const int *Aj, const double 
     o->maxdiv = maxdiv; o->maxdiv0 = maxdiv0; o->maxru = maxru; o->maxru0 = maxru0;
     /* Solvers.c:1333-1336 */
     if (fabs(maxdiv) > 10.0 * o->lin_abs_div && maxdiv / maxdiv0 > o->lin_rel_div) o->status = -2;
+    for (int i = 0; i < nu; i++)
+        if (isnan(du[i])) { o->status = -3; break; }
     free(work); free(tp); free(ud); free(fu); free(pd); free(fpv);
     free(L.perm); free(L.L); free(ident);
     return o->status;

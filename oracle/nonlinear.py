@@ -85,7 +85,9 @@ def picard_loop(g, st, md, phases, nit_max, tol, average=0, diag_scaling=1, gx=0
         A, B, Cm = assemble_picard(g, num, cen["eta"], ver["eta"])
         du, dp, stats, rc = killer_solve(g, num, A, B, Cm, Fu[:nm], Fu[nm:], **killer_kw)
         if rc != 0:
-            raise RuntimeError("KillerSolver status %d" % rc)
+            raise RuntimeError("KillerSolver status %d at iteration %d (eta_n in [%g, %g], "
+                               "eta_s in [%g, %g])" % (rc, nit, cen["eta"].min(), cen["eta"].max(),
+                                                       ver["eta"].min(), ver["eta"].max()))
         scatter_solution(num, np.concatenate([du, dp]), st["u"], st["v"], st["p"])
         apply_bc(g, st["u"], st["v"])
         nsolve += 1

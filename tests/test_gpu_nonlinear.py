@@ -54,7 +54,7 @@ def test_picard_loop_matches_oracle(oracle_mod, diag_scaling):
         b = d.pop("beta")
         o_ph.append(O.OrcPhase(beta_for_K=b, **d))
     hist, nsolve = picard_loop(g, st, O.OrcRheoModel(**common), o_ph, 10, TOL, average=0,
-                               diag_scaling=diag_scaling, gz=-1.0, max_its_PH=30,
+                               diag_scaling=diag_scaling, gz=-1.0, penalty=1e3, max_its_PH=50,
                                rel_tol_KSP=1e-12, max_its_KSP=200, lin_abs_div=1e-14,
                                lin_rel_div=1e-12, lin_abs_mom=1e-14, lin_rel_mom=1e-12)
     ctx = mdgpu.Context(n + 1, n + 1, 0.0, 1.0, -1.0, 0.0)
