@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "mdoodz7.0_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the fine-level smoother step on 2047^2 cells,
+# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a copy kernel of
+# known size): profiles/hbm_traffic_p2g_smoother_2047.txt
+SMOOTHER_TRAFFIC_2047 = None
 METRIC = "Stokes solve time @1e-8 resid, 2047², plain vs learned PC; P2G Mmarkers/s"
 
 
@@ -52,6 +56,9 @@ def main():
     ap.add_argument("--ellipses", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels", type=int, default=0, metavar="REPS",
+                    help="profiling mode: only the P2G calls, the fine-level smoother and a "
+                         "copy kernel (calibration of the HBM byte counters), REPS times each")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding is "
@@ -106,6 +113,9 @@ def main():
         stats.update(p2g_ms=[k1, k2], its=o.its, inner=o.inner_its, solve_ms=o.time_ms,
                      rel=o.rel_res, levels=o.levels)
 
+    if a.kernels:
+        kernels_only(ctx, a.kernels, step, f_cent, f_vert, D11, en, es, F, nm, sol, eta_s, rho_s, n)
+        return
     for _ in range(a.warmup):
         step()
     t0 = time.perf_counter()
@@ -116,7 +126,8 @@ def main():
     k_ms, k_bytes = ctx.time_smoother(50)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     p2g_ms = stats["p2g_ms"]
-    p2g_bytes = (16.0 + 16.0) * n + 16.0 * nc + 4.0 * nc      # 2 fields: x, z + 2 values
+    # 2 fields: x, z + 2 values per marker, 2 values per node, 4 uint32 quarter-cell offsets per cell
+    p2g_bytes = 32.0 * n + 16.0 * nc + 16.0 * nc
     p2g_gbs = p2g_bytes / (p2g_ms[0] * 1e-3) / 1e9
     res = {
         "metric": METRIC, "value": 1e3 / ms_step, "unit": "Stokes solves/s", "n_gpus": 1,
@@ -129,8 +140,9 @@ def main():
                    "solver": "FGCR(100) + viscosity-weighted augmented Lagrangian (gamma 10) "
                              "+ multigrid-preconditioned CG, FP64, plain (no learned PC)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_cheb_res<double> (fine-level smoother operator)",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": SMOOTHER_TRAFFIC_2047 if a.cells == 2047 else None,
+                     "kernel": "k_cheb3<double, false> (fine-level fused Chebyshev smoother step)",
                      "kernel_ms": k_ms, "alg_bytes": k_bytes},
         "solve": {"outer_its": stats["its"], "inner_its": stats["inner"],
                   "solve_ms": stats["solve_ms"], "rel_res": stats["rel"], "levels": stats["levels"]},
@@ -141,6 +153,37 @@ def main():
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(wl, a, n)
     print(json.dumps(res), flush=True)
+
+
+def kernels_only(ctx, reps, step, f_cent, f_vert, D11, en, es, F, nm, sol, eta_s, rho_s, n):
+    """The kernels whose rocprofv3 summaries are kept: P2G to centres and
+    vertices, the fine-level smoother (after a one-iteration solve builds
+    the hierarchy) and a plain copy y = 1.0 * x over the marker array (n
+    doubles read, n written) that calibrates FETCH_SIZE / WRITE_SIZE."""
+    import mdgpu
+    out = {}
+    t = []
+    for _ in range(reps):
+        ctx.p2mastah_fused(1, 1, f_cent)
+        t.append(ctx.last_kernel_ms())
+    out["p2g_centres_ms"] = sorted(t)[len(t) // 2]
+    t = []
+    for _ in range(reps):
+        ctx.p2mastah_fused(0, 1, f_vert)
+        t.append(ctx.last_kernel_ms())
+    out["p2g_vertices_ms"] = sorted(t)[len(t) // 2]
+    ctx.scale_into(en, 2.0, D11)
+    try:
+        ctx.killer_solver_dev(D11, D11, es, F, nm, sol, max_its=1)
+    except mdgpu.MdgError as e:
+        if e.code != -6:
+            raise
+    out["smoother_ms"], out["smoother_bytes"] = ctx.time_smoother(reps)
+    for _ in range(reps):
+        ctx.scale_into(eta_s, 1.0, rho_s)
+    out["copy_bytes"] = 16.0 * n
+    out["markers"] = n
+    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(wl, a, n):

@@ -197,6 +197,25 @@ typedef struct {
 
 #define MDG_MAX_PHASES 64
 
+/* Per-phase shear modulus, compressibility and softening parameters
+ * (mat_prop G, bet, C, C_end, phi, phi_end, psi, psi_end, pls_start,
+ * pls_end, H_c, phi_soft, psi_soft, coh_soft). */
+typedef struct {
+    double G, beta, C, C_end, fric, fric_end, dil, dil_end, pls_start, pls_end, H_c;
+    int    fric_soft, dil_soft, coh_soft;
+} mdg_phase_soft;
+
+/* ShearModCompExpGrid (RheologyDensity.c:1634-1730) and
+ * CohesionFrictionDilationGrid (:1539-1616, Softening :1432-1535) on the
+ * centres (centroid = 1) or vertices (0): mu and bet are harmonic phase
+ * averages of G and beta; with strain_pl (the accumulated plastic strain on
+ * the nodes, P2G of the markers' strain_pl) C, fric and dil are arithmetic
+ * averages of the softened phase values (style 0 linear, 1 erfc, 2 linear
+ * hardening).  Air nodes get zeros.  Outputs may be NULL. */
+int mdg_material_grid(mdg_ctx *ctx, const mdg_phase_soft *phases, int nph, int centroid,
+                      int style, const double *perc, const double *strain_pl, double *mu,
+                      double *bet, double *C, double *fric, double *dil);
+
 /* NonNewtonianViscosityGrid (RheologyDensity.c:1131-1426; prototype
  * mdoodz-private.h:423) on the centres
  * (centroid = 1, loop :1157-1316) or the vertices (centroid = 0, :1320-1407):
@@ -230,8 +249,9 @@ int mdg_stokes_residual(mdg_ctx *ctx, const double *sxxd, const double *szzd, co
  * relative 2-norm residual rel_tol.  Outer flexible GCR (FP64) on the
  * augmented system with a viscosity-weighted penalty gamma*eta_n/dV
  * (Powell-Hestenes term of KillerSolver with Dcm = penalty/dV), inner CG on the
- * augmented velocity block preconditioned by a staggered geometric multigrid
- * V-cycle, in FP64 or FP32 (precision).  Zero fields take defaults. */
+ * augmented velocity block (FP64) preconditioned by a staggered geometric
+ * multigrid V-cycle computed in FP64 or FP32 (precision).  Zero fields take
+ * defaults. */
 #define MDG_PREC_FP64  0
 #define MDG_PREC_MIXED 1
 typedef struct {
@@ -255,14 +275,41 @@ typedef struct {
                               precision = MDG_PREC_FP64)                       */
     double poisson_tol;    /* BFBt pressure-Poisson CG tolerance (1e-3)        */
     int    poisson_max_its;/* (30)                                             */
+    int    smoother;       /* 0 fused three-term Chebyshev kernels, 1 the
+                              residual/update two-kernel form (0)              */
     /* outputs */
     int    its, inner_its, poisson_its, levels, status;
     double rel_res, time_ms;
     int    guess_used;     /* inner solves started from the network's guess    */
-    double guess_gain;     /* mean 1 - ||r0||^2/||b||^2 of those guesses        */
+    double guess_gain;     /* mean ||r0||^2/||b||^2 after those guesses        */
 } mdg_solver_opts;
 int mdg_killer_solver(mdg_ctx *ctx, mdg_solver_opts *opts, const double *D11_n,
                       const double *D22_n, const double *D33_s, const double *rhs_mom,
+                      const double *rhs_cont, double *sol);
+
+/* ---------------------------------------------------------------- Newton
+ * RheologicalOperators with Jacobian = 1 (StokesRoutines.c:250-375;
+ * prototype mdoodz-private.h:607), isotropic and incompressible: the
+ * tangent coefficients from central finite differences of the phase-averaged
+ * effective viscosity (DerivativesOnTheFly_n/_s, FD_Jacobian.c:48-323).
+ * D (nrow x n, row-major) receives D11..D14, D21..D24 on the centres
+ * (centroid = 1, nrow = 8) or D31..D34 on the vertices (centroid = 0,
+ * nrow = 4).  Inputs as for mdg_viscosity_grid (outputs unused). */
+int mdg_tangent_grid(mdg_ctx *ctx, const mdg_phase_prop *phases, int nph,
+                     const mdg_rheo_params *prm, int centroid, const mdg_rheo_nodes *f, double *D);
+
+/* y = J x for the Newton Jacobian (BuildJacobianOperatorDecoupled,
+ * StokesAssemblyDecoupled.c:1786-2304, applied matrix-free) in equation
+ * numbering; Dn = 8 x cells, Ds = 4 x vertices from mdg_tangent_grid. */
+int mdg_jacobian_apply(mdg_ctx *ctx, const double *Dn, const double *Ds, const double *x,
+                       double *y);
+
+/* KillerSolver on the Newton system J dx = F (KillerSolver with the
+ * Jacobian blocks, Solvers.c:1054-1454): FGCR on J_gamma, preconditioned by
+ * the Picard block preconditioner built from D11_n = 2 eta_n and D33_s =
+ * eta_s (as the reference preconditions with the Picard factor). */
+int mdg_newton_solver(mdg_ctx *ctx, mdg_solver_opts *opts, const double *Dn, const double *Ds,
+                      const double *D11_n, const double *D33_s, const double *rhs_mom,
                       const double *rhs_cont, double *sol);
 
 /* ------------------------------------------------ learned preconditioner
@@ -298,8 +345,8 @@ int mdg_pcnet_get_pair(mdg_ctx *ctx, int i, float *x, float *t, float *mask);
 int mdg_pcnet_train(mdg_ctx *ctx, int steps, int batch, float lr, unsigned long long seed,
                     double *losses);
 
-/* Times the dominant kernel of the solve: the fine-level multigrid smoother
- * operator (r -= A_g d, FP64) of the last FP64 solve, averaged over reps
+/* Times the dominant kernel of the solve: the fine-level fused Chebyshev
+ * smoother step (FP64) of the last FP64 solve, averaged over reps
  * launches with HIP events on the context stream.  Returns ms per launch and
  * the algorithmic (compulsory) bytes per launch.  No counterpart in the
  * reference; it feeds the roofline figure of the benchmark. */

@@ -67,6 +67,7 @@ struct mdg_ctx {
     struct solver_state *solver = nullptr;
     struct rheo_scratch *rheo = nullptr;
     struct pcnet_state *pcnet = nullptr;
+    struct newton_state *newton = nullptr;
     // runtime
     hipStream_t  stream = nullptr;
     hipEvent_t   ev0 = nullptr, ev1 = nullptr;
@@ -111,4 +112,8 @@ int mdg_node_set(mdg_ctx *c, int centroid, node_set *ns);
 void mdg_solver_release(mdg_ctx *c);
 void mdg_rheo_release(mdg_ctx *c);
 void mdg_pcnet_release(mdg_ctx *c);
+void mdg_newton_release(mdg_ctx *c);
+// Newton operator staged by mdg_newton_solver: y = J_gamma x (grid layout).
+bool mdg_newton_active(const mdg_ctx *c);
+int mdg_newton_apply_grid(mdg_ctx *c, double gamma, const double *x, double *y);
 int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x);

@@ -31,6 +31,7 @@ struct p2g_geo {
     p2g_axes ax;
     const double *X, *Z;
     double X0, Z0, dx, dz, dx_itp, dz_itp;
+    double idx, idz, ix_itp, iz_itp;   // reciprocals of dx, dz, dx_itp, dz_itp
     int R, S;   // stencil radius (0: 1-cell, 1: 9-cell) and search radius
 };
 
@@ -48,8 +49,8 @@ __device__ __forceinline__ void p2g_visit(const p2g_geo &g, const unsigned *__re
             for_node_ranges(qoff, g.Ncx, g.Ncz, g.ax, hk, hl, [&](unsigned s, unsigned e) {
                 for (unsigned i = s; i < e; i++) {
                     const double x = mx[i], z = mz[i];
-                    const int ip = near_index(x, g.X0, g.dx, g.nx);
-                    const int jp = near_index(z, g.Z0, g.dz, g.nz);
+                    const int ip = near_index_fast(x, g.X0, g.dx, g.idx, g.nx);
+                    const int jp = near_index_fast(z, g.Z0, g.dz, g.idz, g.nz);
                     if (g.S == g.R && (ip != hk || jp != hl)) { *stray = 1; continue; }
                     const int ox = k - ip, oz = l - jp;
                     if (ox < -g.R || ox > g.R || oz < -g.R || oz > g.R) continue;
@@ -57,7 +58,7 @@ __device__ __forceinline__ void p2g_visit(const p2g_geo &g, const unsigned *__re
                     // |X[ip] - x| for the home node, |X[ip] +- dx - x| otherwise
                     const double dxm = (ox == 0) ? fabs(g.X[ip] - x) : fabs(g.X[ip] + ox * g.dx - x);
                     const double dzm = (oz == 0) ? fabs(g.Z[jp] - z) : fabs(g.Z[jp] + oz * g.dz - z);
-                    acc(i, (1.0 - dxm / g.dx_itp) * (1.0 - dzm / g.dz_itp));
+                    acc(i, (1.0 - dxm * g.ix_itp) * (1.0 - dzm * g.iz_itp));
                 }
             });
         }
@@ -156,18 +157,47 @@ k_p2g_tile(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff, const doubl
         for (unsigned cs = S; cs < E; cs += CAP) {
             const unsigned ce = min(E, cs + (unsigned)CAP);
             __syncthreads();   // previous chunk fully consumed
-            for (unsigned j = cs + t; j < ce; j += P2G_TILE) {
-                const double x = mx[j], z = mz[j];
-                const int ip = near_index(x, g.X0, g.dx, g.nx);
-                const int jp = near_index(z, g.Z0, g.dz, g.nz);
+            // Every thread first issues all loads of its (up to PER) markers so
+            // that the whole chunk is in flight at once, then computes.
+            constexpr int PER = CAP / P2G_TILE;
+            double xs[PER], zs[PER], vs[NF][PER];
+            int ps[PER];
+#pragma unroll
+            for (int m = 0; m < PER; m++) {
+                const unsigned j = cs + t + (unsigned)m * P2G_TILE;
+                xs[m] = 0.0; zs[m] = 0.0; ps[m] = 0;
+#pragma unroll
+                for (int f = 0; f < NF; f++) vs[f][m] = 0.0;
+                if (j < ce) {
+                    xs[m] = mx[j];
+                    zs[m] = mz[j];
+                    if (need_phase) ps[m] = mph[j];
+#pragma unroll
+                    for (int f = 0; f < NF; f++)
+                        if (fs.f[f].flag == 1) vs[f][m] = fs.f[f].src[j];
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < PER; m++) {
+                const unsigned j = cs + t + (unsigned)m * P2G_TILE;
+                if (j >= ce) continue;
+                const double x = xs[m], z = zs[m];
+                const int ip = near_index_fast(x, g.X0, g.dx, g.idx, g.nx);
+                const int jp = near_index_fast(z, g.Z0, g.dz, g.idz, g.nz);
                 const double dxm = fabs(g.X[ip] - x), dzm = fabs(g.Z[jp] - z);
-                const double w = (1.0 - dxm / g.dx_itp) * (1.0 - dzm / g.dz_itp);
-                const int p = need_phase ? mph[j] : 0;
+                const double w = (1.0 - dxm * g.ix_itp) * (1.0 - dzm * g.iz_itp);
                 const unsigned q = j - cs;
                 lw[q] = w;
                 lhome[q] = ip + jp * g.nx;
 #pragma unroll
-                for (int f = 0; f < NF; f++) lwv[f][q] = mark_val(fs.f[f], p, j) * w;
+                for (int f = 0; f < NF; f++) {
+                    const p2g_fld &F = fs.f[f];
+                    const int p = ps[m];
+                    double v = (F.flag == 0) ? ((p < F.n_src) ? F.src[p] : __builtin_nan("")) : vs[f][m];
+                    if (F.avg == 1) v = 1.0 / v;
+                    if (F.avg == 2) v = log(v);
+                    lwv[f][q] = v * w;
+                }
             }
             __syncthreads();
             for (int r = 0; r < nr; r++) {
@@ -237,6 +267,10 @@ static int setup_geo(mdg_ctx *c, int centroid, int stencil, p2g_geo *g) {
     g->X = ns.X; g->Z = ns.Z; g->X0 = ns.X0; g->Z0 = ns.Z0; g->dx = c->dx; g->dz = c->dz;
     g->dx_itp = (stencil == 9) ? 3.0 * c->dx / 2.0 : c->dx / 2.0;   // ParticleRoutines.c:2218-2221
     g->dz_itp = (stencil == 9) ? 3.0 * c->dz / 2.0 : c->dz / 2.0;
+    // weights multiply by the reciprocal (the reference divides; the two
+    // differ by at most an ulp of each factor)
+    g->idx = 1.0 / c->dx; g->idz = 1.0 / c->dz;
+    g->ix_itp = 1.0 / g->dx_itp; g->iz_itp = 1.0 / g->dz_itp;
     g->R = (stencil == 9) ? 1 : 0;
     g->S = g->R;
     return MDG_OK;

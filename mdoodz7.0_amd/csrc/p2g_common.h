@@ -14,6 +14,22 @@ __device__ __forceinline__ int near_index(double pos, double origin, double h, i
     return i > n - 1 ? n - 1 : i;
 }
 
+// The same index without the division on almost every call: t = d*(1/h) +
+// 0.5 differs from the reference's RN(RN(d/h) + 0.5) by a few ulps at most,
+// so ceil(t) is the reference's result whenever t is farther than a margin
+// (1e-12 relative, far above those ulps) from an integer; only inside that
+// margin (markers within ~1e-12 cells of a mid-cell tie) is the exact
+// division taken.
+__device__ __forceinline__ int near_index_fast(double pos, double origin, double h, double inv_h, int n) {
+    const double distance = pos - origin;
+    const double t = __fma_rn(distance, inv_h, 0.5);
+    const double f = ceil(t);
+    const double margin = 1e-12 * (fabs(t) + 1.0);
+    int i = (f - t > margin && t - (f - 1.0) > margin) ? (int)f - 1 : (int)ceil(distance / h + 0.5) - 1;
+    i = i < 0 ? 0 : i;
+    return i > n - 1 ? n - 1 : i;
+}
+
 // How a node set lines up with the cells along one axis:
 //   CELL_AXIS: node index k covers cell k + off (both halves);
 //   VERT_AXIS: node index k covers the east half of cell k-1 and the west

@@ -694,9 +694,9 @@ extern "C" int mdg_pcnet_adam(mdg_ctx *c, float lr) {
 namespace {
 
 // sum over free velocity faces of (dinv*b)^2 -> part[block]
-template <typename T>
+template <typename V, typename T>
 __global__ void __launch_bounds__(256)
-k_scaled_sumsq(sgrid g, const T *__restrict__ b, const T *__restrict__ dinv, double *__restrict__ part) {
+k_scaled_sumsq(sgrid g, const V *__restrict__ b, const T *__restrict__ dinv, double *__restrict__ part) {
     __shared__ double sh[256];
     const int nu = nvx(g), n = nu + nvz(g);
     double acc = 0.0;
@@ -723,8 +723,8 @@ __global__ void k_scale_from(int nb, const double *__restrict__ part, double nfr
 
 // Input image (H x W x 4) of cell (x = k, y = l): log10(eta_n)/3, then
 // dinv*b/s on the cell's W face Vx(k, l+1) and S face Vz(k+1, l) where free.
-template <typename T, bool F32>
-__global__ void k_pack_input(sgrid g, int H, int W, const T *__restrict__ b, const T *__restrict__ dinv,
+template <typename V, typename T, bool F32>
+__global__ void k_pack_input(sgrid g, int H, int W, const V *__restrict__ b, const T *__restrict__ dinv,
                              const T *__restrict__ en, const double *__restrict__ scale, void *__restrict__ out) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)H * W) return;
@@ -796,11 +796,11 @@ double free_faces(const sgrid &g) {
     return (double)(g.Nx - 2) * (g.Nz - 1) + (double)(g.Nx - 1) * (g.Nz - 2);
 }
 
-template <typename T>
-int compute_scale(mdg_ctx *c, pcnet_state *s, const sgrid &g, const T *b, const T *dinv, double *scale) {
+template <typename V, typename T>
+int compute_scale(mdg_ctx *c, pcnet_state *s, const sgrid &g, const V *b, const T *dinv, double *scale) {
     const int nb = 256;
     HIPCHK(c, s->lpart.reserve(1024));
-    hipLaunchKernelGGL(k_scaled_sumsq<T>, dim3(nb), dim3(256), 0, c->stream, g, b, dinv, s->lpart.p);
+    hipLaunchKernelGGL((k_scaled_sumsq<V, T>), dim3(nb), dim3(256), 0, c->stream, g, b, dinv, s->lpart.p);
     hipLaunchKernelGGL(k_scale_from, dim3(1), dim3(1), 0, c->stream, nb, (const double *)s->lpart.p, free_faces(g), scale);
     HIPCHK(c, hipGetLastError());
     return MDG_OK;
@@ -808,25 +808,25 @@ int compute_scale(mdg_ctx *c, pcnet_state *s, const sgrid &g, const T *b, const 
 
 }  // namespace
 
-template <typename T>
-int mdg_pcnet_guess(mdg_ctx *c, const sgrid &g, const T *b, const T *dinv, const T *en, T *x0) {
+template <typename V, typename T>
+int mdg_pcnet_guess(mdg_ctx *c, const sgrid &g, const V *b, const T *dinv, const T *en, V *x0) {
     pcnet_state *s = net_of(c);
     int H, W, rc;
     image_dims(g, &H, &W);
     if ((rc = check_dims(c, H, W)) || (rc = ensure_buffers(c, s, H, W))) return rc;
     HIPCHK(c, s->guess_scale.reserve(1));
-    if ((rc = compute_scale<T>(c, s, g, b, dinv, s->guess_scale.p))) return rc;
-    hipLaunchKernelGGL((k_pack_input<T, false>), dim3(nblk256((long)H * W)), dim3(256), 0, c->stream, g, H, W, b, dinv,
+    if ((rc = compute_scale<V, T>(c, s, g, b, dinv, s->guess_scale.p))) return rc;
+    hipLaunchKernelGGL((k_pack_input<V, T, false>), dim3(nblk256((long)H * W)), dim3(256), 0, c->stream, g, H, W, b, dinv,
                        en, (const double *)s->guess_scale.p, (void *)A(s, A_X));
     if ((rc = forward(c, s))) return rc;
-    hipLaunchKernelGGL(k_unpack_guess<T>, dim3(nblk256(nvx(g) + nvz(g))), dim3(256), 0, c->stream, g, H, W,
+    hipLaunchKernelGGL(k_unpack_guess<V>, dim3(nblk256(nvx(g) + nvz(g))), dim3(256), 0, c->stream, g, H, W,
                        (const float *)s->y.p, (const double *)s->guess_scale.p, x0);
     HIPCHK(c, hipGetLastError());
     return MDG_OK;
 }
 
-template <typename T>
-int mdg_pcnet_capture_in(mdg_ctx *c, const sgrid &g, const T *b, const T *dinv, const T *en) {
+template <typename V, typename T>
+int mdg_pcnet_capture_in(mdg_ctx *c, const sgrid &g, const V *b, const T *dinv, const T *en) {
     pcnet_state *s = c->pcnet;
     if (!s || !s->cap_on || s->cap_pending) return MDG_OK;
     int H, W;
@@ -844,8 +844,8 @@ int mdg_pcnet_capture_in(mdg_ctx *c, const sgrid &g, const T *b, const T *dinv, 
     }
     const int slot = s->cap_next;
     int rc;
-    if ((rc = compute_scale<T>(c, s, g, b, dinv, s->cap_scale.p + slot))) return rc;
-    hipLaunchKernelGGL((k_pack_input<T, true>), dim3(nblk256((long)H * W)), dim3(256), 0, c->stream, g, H, W, b, dinv,
+    if ((rc = compute_scale<V, T>(c, s, g, b, dinv, s->cap_scale.p + slot))) return rc;
+    hipLaunchKernelGGL((k_pack_input<V, T, true>), dim3(nblk256((long)H * W)), dim3(256), 0, c->stream, g, H, W, b, dinv,
                        en, (const double *)(s->cap_scale.p + slot), (void *)(s->cap_x.p + img * slot));
     HIPCHK(c, hipGetLastError());
     s->cap_pending = 1;
@@ -861,7 +861,7 @@ int mdg_pcnet_capture_out(mdg_ctx *c, const sgrid &g, const T *x) {
     // the target is the solution's direction: x scaled to unit RMS over the
     // free faces (the solver rescales the network's guess optimally)
     HIPCHK(c, s->guess_scale.reserve(1));
-    int rc = compute_scale<T>(c, s, g, x, (const T *)nullptr, s->guess_scale.p);
+    int rc = compute_scale<T, T>(c, s, g, x, (const T *)nullptr, s->guess_scale.p);
     if (rc) return rc;
     hipLaunchKernelGGL(k_pack_target<T>, dim3(nblk256((long)H * W)), dim3(256), 0, c->stream, g, H, W, x,
                        (const double *)s->guess_scale.p, s->cap_t.p + img * slot, s->cap_m.p + img * slot);
@@ -872,12 +872,14 @@ int mdg_pcnet_capture_out(mdg_ctx *c, const sgrid &g, const T *x) {
     return MDG_OK;
 }
 
-template int mdg_pcnet_guess<double>(mdg_ctx *, const sgrid &, const double *, const double *, const double *, double *);
-template int mdg_pcnet_guess<float>(mdg_ctx *, const sgrid &, const float *, const float *, const float *, float *);
-template int mdg_pcnet_capture_in<double>(mdg_ctx *, const sgrid &, const double *, const double *, const double *);
-template int mdg_pcnet_capture_in<float>(mdg_ctx *, const sgrid &, const float *, const float *, const float *);
+template int mdg_pcnet_guess<double, double>(mdg_ctx *, const sgrid &, const double *, const double *, const double *,
+                                             double *);
+template int mdg_pcnet_guess<double, float>(mdg_ctx *, const sgrid &, const double *, const float *, const float *,
+                                            double *);
+template int mdg_pcnet_capture_in<double, double>(mdg_ctx *, const sgrid &, const double *, const double *,
+                                                  const double *);
+template int mdg_pcnet_capture_in<double, float>(mdg_ctx *, const sgrid &, const double *, const float *, const float *);
 template int mdg_pcnet_capture_out<double>(mdg_ctx *, const sgrid &, const double *);
-template int mdg_pcnet_capture_out<float>(mdg_ctx *, const sgrid &, const float *);
 
 extern "C" int mdg_pcnet_capture(mdg_ctx *c, int max_pairs) {
     if (!c || max_pairs < 0) return mdg_fail(c, MDG_ERR_ARG, "max_pairs must be >= 0");

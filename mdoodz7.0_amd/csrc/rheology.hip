@@ -9,121 +9,12 @@
 // bounded by FP64 issue rather than by its ~20 doubles of traffic per node.
 #include "mdg_internal.h"
 
+#include "rheo_device.h"
+
+using rheo::rheo_ctl;
+using rheo::viscosity_concise;
+
 namespace {
-
-struct rheo_ctl {
-    int    step, elastic, average, nph;
-    double dt, R, zeroC, min_eta, max_eta, eta_tol, Tiimin, scale_E, scale_eta;
-};
-
-// LocalIterationViscoElastic (RheologyDensity.c:152-197): Newton iterations on
-// eta_ve so that E = el*T/(2 eta_el) + T/(2 eta_cst) + C_pwl T^n, T = 2 eta_ve E.
-// Returns false when the reference would exit (20 iterations, res > tol).
-__device__ bool local_ve(double Eii, double &eta, bool elastic, double eta_el, bool constant,
-                         double eta_cst, bool dislocation, double C_pwl, double n_pwl,
-                         double &Eii_pwl) {
-    const double tol = 1.0e-11;
-    for (int it = 0; it < 20; it++) {
-        const double Tii = 2.0 * eta * Eii;
-        const double e_cst = constant ? Tii / 2.0 / eta_cst : 0.0;
-        const double e_pwl = dislocation ? C_pwl * pow(Tii, n_pwl) : 0.0;
-        Eii_pwl = e_pwl;
-        const double r = Eii - (elastic ? 1.0 : 0.0) * Tii / (2.0 * eta_el) - (e_pwl + e_cst);
-        const double res = fabs(r / Eii);
-        if (res < tol / 100) return true;
-        if (it == 19 && res > tol) return false;
-        double dfdeta = 0.0;
-        if (elastic) dfdeta += -Eii / eta_el;
-        if (dislocation) dfdeta += -e_pwl * n_pwl / eta;
-        if (constant) dfdeta += -Eii / eta_cst;
-        eta -= r / dfdeta;
-    }
-    return true;
-}
-
-// ViscosityConcise (RheologyDensity.c:385-1080) for one (node, phase):
-// returns the dissipative viscosity; etaVE, Pcorr, Eii_pl by reference.
-__device__ double viscosity_concise(const mdg_phase_prop &m, const rheo_ctl &md, double G, double T,
-                                    double P, double Exx, double Ezz, double Exz, double fric,
-                                    double C, double dil, double pref_mult, double &etaVE,
-                                    double &Pcorr, double &Eii_pl, bool &ok) {
-    const double dt = (md.step == 0) ? 1e100 : md.dt;
-    bool elastic = md.elastic != 0, plastic = (m.plast == 1);
-    if (md.step == 0) { elastic = false; plastic = false; }          // :483
-    const bool constant = m.cstv != 0, dislocation = m.pwlv != 0;
-    double K = 1.0 / m.beta;
-    const double Eyy = -(Exx + Ezz);
-    double Eii = sqrt(0.5 * (Exx * Exx + Ezz * Ezz + Eyy * Eyy) + Exz * Exz);   // :457-459
-    if (Eii * md.scale_E < 1e-30) Eii = 1e-30 / md.scale_E;
-    Pcorr = P;
-    if (!elastic) { G = 1e1; dil = 0.0; K = 1e10; }                   // :493
-    if (T < md.zeroC) T = md.zeroC;
-    double B_pwl = 0.0, C_pwl = 0.0;
-    if (dislocation) {                                                 // :501-504
-        B_pwl = m.pref_pwl * pref_mult * m.F_pwl * pow(m.A_pwl, -1.0 / m.n_pwl) *
-                exp((m.Q_pwl + P * m.V_pwl) / md.R / m.n_pwl / T);
-        C_pwl = pow(2.0 * B_pwl, -m.n_pwl);
-    }
-    const double sin_fric = sin(fric), cos_fric = cos(fric), sin_dil = sin(dil);
-    double Tyield = 0.0;
-    if (plastic) {                                                     // :530-539
-        Tyield = C * cos_fric + P * sin_fric;
-        if (m.Slim < Tyield) Tyield = m.Slim;
-    }
-    const double eta_el = G * dt;
-    const double eta_cst = constant ? m.eta0 : 0.0;
-    double eta_up = 1.0e100 / md.scale_eta;                            // :585-592
-    if (constant) eta_up = fmin(eta_up, eta_cst);
-    if (dislocation) eta_up = fmin(eta_up, B_pwl * pow(Eii, 1.0 / m.n_pwl - 1.0));
-    if (elastic) eta_up = fmin(eta_up, eta_el);
-    double eta_ve = eta_up, Eii_pwl = 0.0;
-    if (!local_ve(Eii, eta_ve, elastic, eta_el, constant, eta_cst, dislocation, C_pwl, m.n_pwl,
-                  Eii_pwl))
-        ok = false;
-    double Tii = 2.0 * eta_ve * Eii;
-    bool is_pl = false;
-    double eta_vep = eta_ve, gdot = 0.0;
-    if (plastic) {                                                     // :655-707
-        double F_trial = Tii - Tyield;
-        if (F_trial > 1e-17) {
-            is_pl = true;
-            const double F0 = F_trial, tol = md.eta_tol;
-            const double Kdt_ss = K * dt * sin_fric * sin_dil;
-            double eta_vp = m.eta_vp * pow(Eii, 1.0 / m.n_vp - 1);
-            gdot = F_trial / (eta_ve + eta_vp + Kdt_ss);
-            double Pc = P, Tiic = Tii;
-            for (int it = 0; it < 20; it++) {
-                Pc = P + K * dt * (gdot * sin_dil);
-                eta_vp = m.eta_vp * pow(fabs(gdot), 1.0 / m.n_vp - 1.0);
-                Tyield = Tyield + gdot * eta_vp;    // accumulates, as in the reference
-                Tiic = Tii - eta_ve * gdot;
-                F_trial = Tiic - Tyield;
-                const double res = fabs(F_trial);
-                if (res < tol || res / F0 < tol) break;
-                gdot -= F_trial / (-eta_ve - eta_vp / m.n_vp - Kdt_ss);
-            }
-            if (Tiic < 0.0) {                                          // :694-701
-                gdot = (Tii - md.Tiimin) / eta_ve;
-                Tiic = md.Tiimin;
-                Pc = P;
-            }
-            Pcorr = Pc;
-            eta_vep = Tiic / (2.0 * Eii);
-            Tii = Tiic;
-        }
-    }
-    etaVE = is_pl ? eta_vep : eta_ve;                                  // :1293-1301
-    Eii_pl = is_pl ? gdot / 2.0 : 0.0;
-    double inv = 0.0;                                                  // :1024-1030
-    if (dislocation) inv += 1.0 / (B_pwl * pow(Eii_pwl, 1.0 / m.n_pwl - 1.0));
-    if (constant) inv += 1.0 / eta_cst;
-    if (is_pl) inv += 1.0 / (Tii / gdot);
-    double eta = 1.0 / inv;
-    etaVE = fmax(fmin(etaVE, md.max_eta), md.min_eta);                 // :1058-1073
-    if (eta > md.max_eta) eta = md.max_eta;
-    if (eta < md.min_eta) eta = md.min_eta;
-    return eta;
-}
 
 __global__ void __launch_bounds__(256)
 k_viscosity_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
@@ -227,6 +118,87 @@ k_v2c_exz(int Nx, int Nz, const double *__restrict__ exz, const signed char *__r
     exz_n[c] = s;
 }
 
+// Softening (RheologyDensity.c:1432-1535) of one phase at accumulated
+// plastic strain e: style 0 piecewise linear (slope times e, as the
+// reference), 1 erfc, 2 linear hardening of the cohesion.
+__device__ void soften(const mdg_phase_soft &m, double e, int style, double &fric, double &dil, double &C) {
+    fric = m.fric; dil = m.dil; C = m.C;
+    const double ds = m.pls_end - m.pls_start;
+    if (style == 1) {
+        const double mu = 0.5 * (m.pls_end + m.pls_start);
+        if (m.fric_soft == 1) {
+            const double d = m.fric - m.fric_end;
+            const double f = m.fric - d / 2.0 * erfc(-(e - mu) / ds) - m.fric_end;
+            const double f0 = m.fric - d / 2.0 * erfc(-(0.0 - mu) / ds) - m.fric_end;
+            fric = f * d / f0 + m.fric_end;
+        }
+        if (m.dil_soft == 1) {
+            const double d = m.dil - m.dil_end;
+            const double f = m.dil - d / 2.0 * erfc(-(e - mu) / ds) - m.dil_end;
+            const double f0 = m.dil - d / 2.0 * erfc(-(0.0 - mu) / ds) - m.dil_end;
+            dil = f * d / f0 + m.dil_end;
+        }
+        if (m.coh_soft == 1) {
+            const double d = m.C - m.C_end;
+            const double f = m.C - d / 2.0 * erfc(-(e - mu) / ds) - m.C_end;
+            const double f0 = m.C - d / 2.0 * erfc(-(0.0 - mu) / ds) - m.C_end;
+            C = f * d / f0 + m.C_end;
+        }
+    } else if (style == 2) {
+        C = fmax(m.C + m.H_c * e, m.C_end);
+    } else {
+        if (e < m.pls_start) {
+            // initial values (already set)
+        } else if (e >= m.pls_end) {
+            if (m.fric_soft == 1) fric = m.fric_end;
+            if (m.dil_soft == 1) dil = m.dil_end;
+            if (m.coh_soft == 1) C = m.C_end;
+        } else {
+            if (m.fric_soft == 1) fric = m.fric + (m.fric_end - m.fric) / ds * e;
+            if (m.dil_soft == 1) dil = m.dil + (m.dil_end - m.dil) / ds * e;
+            if (m.coh_soft == 1) C = m.C + (m.C_end - m.C) / ds * e;
+        }
+    }
+}
+
+// ShearModCompExpGrid (RheologyDensity.c:1634-1730: harmonic G and beta)
+// and CohesionFrictionDilationGrid (:1539-1616: arithmetic average of the
+// softened friction, dilation and cohesion) on one node set.
+__global__ void __launch_bounds__(256)
+k_material_grid(int n, int nph, int style, int centroid, const mdg_phase_soft *__restrict__ phases,
+                const signed char *__restrict__ tag, const double *__restrict__ perc,
+                const double *__restrict__ epl, double *__restrict__ mu, double *__restrict__ bet,
+                double *__restrict__ C, double *__restrict__ fric, double *__restrict__ dil) {
+    __shared__ mdg_phase_soft sph[MDG_MAX_PHASES];
+    for (int p = threadIdx.x; p < nph; p += blockDim.x) sph[p] = phases[p];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const signed char t = tag[i];
+    double a = 0.0, b = 0.0, cc = 0.0, ff = 0.0, dd = 0.0;
+    if (!(t == 30 || (centroid && t == 31))) {
+        for (int p = 0; p < nph; p++) {
+            const double w = perc[(size_t)p * n + i];
+            a += w * 1.0 / sph[p].G;
+            b += w * 1.0 / sph[p].beta;
+            if (epl) {
+                double f, d, c;
+                soften(sph[p], epl[i], style, f, d, c);
+                ff += w * f; dd += w * d; cc += w * c;
+            }
+        }
+        a = 1.0 / a;
+        b = 1.0 / b;
+    }
+    if (mu) mu[i] = a;
+    if (bet) bet[i] = b;
+    if (epl) {
+        if (C) C[i] = cc;
+        if (fric) fric[i] = ff;
+        if (dil) dil[i] = dd;
+    }
+}
+
 // Output buffer: the caller's device array, or a scratch buffer copied back.
 int out_dev(mdg_ctx *c, double *user, size_t n, dbuf<double> &tmp, double **o) {
     if (!user) { *o = nullptr; return MDG_OK; }
@@ -241,6 +213,7 @@ int out_dev(mdg_ctx *c, double *user, size_t n, dbuf<double> &tmp, double **o) {
 struct rheo_scratch {
     dbuf<double> in[14], out[7];
     dbuf<mdg_phase_prop> ph;
+    dbuf<mdg_phase_soft> soft;
 };
 
 namespace {
@@ -258,6 +231,7 @@ void mdg_rheo_release(mdg_ctx *c) {
     for (auto &b : c->rheo->in) b.release();
     for (auto &b : c->rheo->out) b.release();
     c->rheo->ph.release();
+    c->rheo->soft.release();
     delete c->rheo;
     c->rheo = nullptr;
 }
@@ -362,5 +336,36 @@ extern "C" int mdg_strain_rates_nodes(mdg_ctx *c, const double *exxd, const doub
         if (!srcs[q]) return mdg_fail(c, MDG_ERR_ARG, "vertex output %d needs its centre input", q);
         if ((rc = mdg_c2v(c, 0, srcs[q], dsts[q]))) return rc;
     }
+    return MDG_OK;
+}
+
+extern "C" int mdg_material_grid(mdg_ctx *c, const mdg_phase_soft *phases, int nph, int centroid,
+                                 int style, const double *perc, const double *strain_pl,
+                                 double *mu, double *bet, double *C, double *fric, double *dil) {
+    if (!c || !phases || !perc) return mdg_fail(c, MDG_ERR_ARG, "null argument");
+    if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
+    if (nph < 1 || nph > MDG_MAX_PHASES) return mdg_fail(c, MDG_ERR_ARG, "nph = %d outside [1, %d]", nph, MDG_MAX_PHASES);
+    if (style < 0 || style > 2) return mdg_fail(c, MDG_ERR_ARG, "softening style must be 0, 1 or 2");
+    const int n = centroid ? (c->Nx - 1) * (c->Nz - 1) : c->Nx * c->Nz;
+    rheo_scratch *s = scratch_of(c);
+    const double *dperc, *depl;
+    int rc;
+    if ((rc = mdg_stage_in(c, perc, (size_t)n * nph, s->in[0], &dperc)) ||
+        (rc = mdg_stage_in(c, strain_pl, (size_t)n, s->in[1], &depl)))
+        return rc;
+    double *const outs[5] = {mu, bet, C, fric, dil};
+    double *d[5];
+    for (int q = 0; q < 5; q++)
+        if ((rc = out_dev(c, outs[q], (size_t)n, s->out[q], &d[q]))) return rc;
+    HIPCHK(c, s->soft.reserve(nph));
+    HIPCHK(c, hipMemcpyAsync(s->soft.p, phases, nph * sizeof(mdg_phase_soft), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_material_grid, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, n, nph, style,
+                       centroid ? 1 : 0, (const mdg_phase_soft *)s->soft.p, centroid ? c->BCp_t.p : c->BCg_t.p,
+                       dperc, depl, d[0], d[1], d[2], d[3], d[4]);
+    HIPCHK(c, hipGetLastError());
+    for (int q = 0; q < 5; q++)
+        if (outs[q] && d[q] != outs[q])
+            HIPCHK(c, hipMemcpyAsync(outs[q], d[q], (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDG_OK;
 }

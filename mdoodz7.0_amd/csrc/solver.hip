@@ -16,11 +16,18 @@
 //     staggered geometric multigrid V-cycle (Chebyshev-Jacobi smoothing,
 //     coarse grids with ceil(n/2) cells and viscosity averaged geometrically,
 //     linear interpolation between non-nested grids, dense coarsest solve).
-// The outer GCR, the operator and the residual are FP64; the inner CG and
-// multigrid run in FP64 ("plain") or FP32 ("mixed", the FP64 outer
-// iteration recovering the accuracy).
+// The outer GCR, the operator, the residual and the inner CG are FP64; the
+// multigrid V-cycle that preconditions the CG runs in FP64 ("plain") or FP32
+// ("mixed").  An FP32 inner CG cannot work at the sizes of interest: the
+// viscosity contrast puts the condition number of the scaled velocity block
+// beyond 1/eps_FP32, so FP32 iterates stop improving before the inner
+// tolerance; an FP32 preconditioner only needs to be a good approximate
+// inverse, and the FP64 CG around it keeps its attainable accuracy.
+// Smoothing uses the three-term Chebyshev recurrence on the iterates, one
+// fused stencil kernel per step (k_cheb3).
 #include <cmath>
 #include <memory>
+#include <type_traits>
 
 #include "stokes_internal.h"
 #include "pcnet_internal.h"
@@ -219,6 +226,84 @@ __global__ void __launch_bounds__(256) k_cheb_res(sgrid g, T gamma, const T *__r
         if (vz_free(g, k, l)) r[i] -= apply_vz_row<T>(g, k, l, u, v, en, es, gamma, nullptr);
     }
 }
+// ------------------------------------------- fused three-term Chebyshev step
+// Tiles of CHEB_TX columns x CHEB_TZ rows; one thread owns column k and
+// updates, row by row, both Vx(k, l) and Vz(k, l) (the thread of column
+// Nx-1 also zeroes the never-free Vz column Nx).  Hardware block b runs on
+// XCD b % 8; the tile order gives XCD x a contiguous band of rows, so the
+// rows above and below a tile (stencil halo, both velocity components and
+// the viscosities) are read from the same XCD's L2.
+#define CHEB_TX 256
+#define CHEB_TZ 4
+
+struct tile_grid {
+    int nbx, ntiles;
+};
+static inline tile_grid make_tiles(const sgrid &g) {
+    const int nbx = (g.Nx + CHEB_TX - 1) / CHEB_TX, nbz = (g.Nz + 1 + CHEB_TZ - 1) / CHEB_TZ;
+    return tile_grid{nbx, nbx * nbz};
+}
+__device__ inline int xcd_tile(int b, int ntiles) {
+    const int xcd = b & 7, per = ntiles >> 3, rem = ntiles & 7;
+    return xcd * per + min(xcd, rem) + (b >> 3);
+}
+
+// One step of the Chebyshev recurrence written on the iterates,
+//   x_{k+1} = x_k + a (x_k - x_{k-1}) + c D^-1 (b - A_g x_k),
+// D = diag(A_g) taken from the stencil itself.  FIRST: a = 0 and x_{k-1}
+// is not read.  xo holds x_{k-1} on entry and x_{k+1} on exit (each entry
+// is read and written by its own thread only).
+template <typename T, bool FIRST>
+__global__ void __launch_bounds__(CHEB_TX) k_cheb3(sgrid g, tile_grid tg, T gamma, T a, T cc,
+                                                   const T *__restrict__ xc, T *__restrict__ xo,
+                                                   const T *__restrict__ b, const T *__restrict__ en,
+                                                   const T *__restrict__ es) {
+    const int tile = xcd_tile(blockIdx.x, tg.ntiles);
+    const int tz = tile / tg.nbx, k = (tile - tz * tg.nbx) * CHEB_TX + threadIdx.x;
+    if (k >= g.Nx) return;
+    const int nu = nvx(g), NxV = g.Nx + 1;
+    const T *u = xc, *v = xc + nu;
+    T *uo = xo, *vo = xo + nu;
+    const T *bu = b, *bv = b + nu;
+    const int l0 = tz * CHEB_TZ, l1 = min(l0 + CHEB_TZ, g.Nz + 1);
+    for (int l = l0; l < l1; l++) {
+        {
+            const int i = k + l * g.Nx;
+            T y = T(0);
+            if (vx_free(g, k, l)) {
+                T dg;
+                const T ax = apply_vx_row<T>(g, k, l, u, v, en, es, gamma, &dg);
+                const T xi = u[i];
+                y = xi + cc * (bu[i] - ax) / dg;
+                if (!FIRST) y += a * (xi - uo[i]);
+            }
+            uo[i] = y;
+        }
+        if (l < g.Nz) {
+            const int j = k + l * NxV;
+            T y = T(0);
+            if (vz_free(g, k, l)) {
+                T dg;
+                const T ax = apply_vz_row<T>(g, k, l, u, v, en, es, gamma, &dg);
+                const T xi = v[j];
+                y = xi + cc * (bv[j] - ax) / dg;
+                if (!FIRST) y += a * (xi - vo[j]);
+            }
+            vo[j] = y;
+            if (k == g.Nx - 1) vo[j + 1] = T(0);
+        }
+    }
+}
+// First step from a zero start: x1 = c D^-1 b, and x0 = 0 written out.
+template <typename T>
+__global__ void k_cheb_first(int n, T c, const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ x1,
+                             T *__restrict__ x0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x1[i] = c * dinv[i] * b[i];
+    x0[i] = T(0);
+}
+
 // x += d ; d = a*d + b*Dinv*r
 template <typename T>
 __global__ void k_cheb_upd(int n, T a, T b, const T *__restrict__ dinv, const T *__restrict__ r,
@@ -410,14 +495,14 @@ __global__ void k_coarsen_eta(sgrid gf, sgrid gc, int mode, const T *__restrict_
 
 // Dense coarsest solve: x[idx[i]] = sum_j Ainv[i][j] b[idx[j]]
 template <typename T>
-__global__ void __launch_bounds__(256) k_dense_solve(int n, const int *__restrict__ idx, const T *__restrict__ Ainv,
+__global__ void __launch_bounds__(256) k_dense_solve(int n, const int *__restrict__ idx, const double *__restrict__ Ainv,
                                                      const T *__restrict__ b, T *__restrict__ x) {
     __shared__ T sb[1024];
     for (int j = threadIdx.x; j < n; j += blockDim.x) sb[j] = b[idx[j]];
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         double s = 0.0;
-        for (int j = 0; j < n; j++) s += (double)Ainv[(size_t)i * n + j] * (double)sb[j];
+        for (int j = 0; j < n; j++) s += Ainv[(size_t)i * n + j] * (double)sb[j];
         x[idx[i]] = (T)s;
     }
 }
@@ -668,15 +753,15 @@ __global__ void k_div_scaled(sgrid g, const T *__restrict__ dinv, const T *__res
 // rhs_u = r_u - B zp (zp of type T) and the FP64 pressure part of z
 template <typename T>
 __global__ void k_rhs_from_zp(sgrid g, const double *__restrict__ r, const T *__restrict__ zp,
-                              double *__restrict__ zout, T *__restrict__ ru) {
+                              double *__restrict__ zout, double *__restrict__ ru) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int nu = nvx(g), nv = nvz(g), np = ncell(g), Ncx = g.Nx - 1;
     if (i < nu) {
         const int k = i % g.Nx, l = i / g.Nx;
-        ru[i] = vx_free(g, k, l) ? (T)(r[i] - ((double)zp[k + (l - 1) * Ncx] - (double)zp[k - 1 + (l - 1) * Ncx]) * g.dz) : T(0);
+        ru[i] = vx_free(g, k, l) ? r[i] - ((double)zp[k + (l - 1) * Ncx] - (double)zp[k - 1 + (l - 1) * Ncx]) * g.dz : 0.0;
     } else if (i < nu + nv) {
         const int j = i - nu, k = j % (g.Nx + 1), l = j / (g.Nx + 1);
-        ru[i] = vz_free(g, k, l) ? (T)(r[i] - ((double)zp[k - 1 + l * Ncx] - (double)zp[k - 1 + (l - 1) * Ncx]) * g.dx) : T(0);
+        ru[i] = vz_free(g, k, l) ? r[i] - ((double)zp[k - 1 + l * Ncx] - (double)zp[k - 1 + (l - 1) * Ncx]) * g.dx : 0.0;
     } else if (i < nu + nv + np) {
         zout[i] = (double)zp[i - nu - nv];
     }
@@ -693,7 +778,7 @@ struct mg_level {
     // coarsest level
     int nfree = 0;
     dbuf<int> idx;
-    dbuf<T> Ainv;
+    dbuf<double> Ainv;     // coarsest dense inverse, FP64 at every precision
     // pressure Poisson (BFBt) on the same cells
     dbuf<T> px, pb, pr, pd, pdinv, pAinv;
     double plmax = 1.0;
@@ -708,9 +793,12 @@ struct mg_level {
 template <typename T>
 struct mg_hier {
     std::vector<mg_level<T>> L;
-    dbuf<T> cg_x, cg_r, cg_z, cg_p, cg_q;   // inner CG vectors (fine velocity size)
+    // inner CG vectors (fine velocity size): FP64 at every precision, the
+    // level type T only sets the precision of the multigrid V-cycle
+    dbuf<double> cg_x, cg_r, cg_z, cg_p, cg_q;
     dbuf<double> scal, part;                // CG scalars (rz_old, rz_new, pq, rr) and partials
     double *h_scal = nullptr;               // pinned host copy of scal
+    bool fused = true;                      // three-term Chebyshev kernels
     hipGraph_t graph = nullptr;             // one CG iteration (V-cycle included)
     hipGraphExec_t exec = nullptr;
     // pressure Poisson CG (BFBt): vectors, scalars and graph
@@ -752,7 +840,7 @@ struct solver_state {
     int m = 0;
     size_t nfull = 0;
     int guess_used = 0;       // inner solves started from the network's guess
-    double guess_gain = 0.0;  // sum of the residual reductions 1 - ||r0||^2/||b||^2
+    double guess_gain = 0.0;  // sum of ||r0||^2 / ||b||^2 after the guesses
 };
 
 void mdg_solver_release(mdg_ctx *c) {
@@ -815,6 +903,7 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
     H.drop_graph();
     for (auto &l : H.L) l.release();
     H.L.clear();
+    H.fused = (o->smoother == 0);
     const double gamma = o->penalty;
     sgrid g = c->sg;
     const double Lx = c->xmax - c->xmin, Lz = c->zmax - c->zmin;
@@ -898,11 +987,9 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
     bool ok;
     host_gauss_jordan_inverse(A, nf, &ok);
     if (!ok) return mdg_fail(c, MDG_ERR_NOT_SPD, "singular coarsest operator");
-    std::vector<T> Ai(A.size());
-    for (size_t i = 0; i < A.size(); i++) Ai[i] = (T)A[i];
-    HIPCHK(c, cl.Ainv.reserve(Ai.size()));
-    HIPCHK(c, hipMemcpyAsync(cl.Ainv.p, Ai.data(), Ai.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));   // Ai is a local host vector
+    HIPCHK(c, cl.Ainv.reserve(A.size()));
+    HIPCHK(c, hipMemcpyAsync(cl.Ainv.p, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // A is a local host vector
     if (o->schur == 1) {
         int rc = build_poisson(c, s, H);
         if (rc) return rc;
@@ -1118,12 +1205,35 @@ static int bfbt_rhs(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solver
 
 // ----------------------------------------------------------------- V-cycle
 template <typename T>
-static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool zero_start) {
-    // x (+)= Chebyshev(deg) correction for A_g x = b, using lv.r, lv.d
+static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool zero_start, bool fused) {
+    // x (+)= Chebyshev(deg) correction for A_g x = b
     const double lmax = lv.lmax, lmin = lmax / 30.0;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
     double rho = 1.0 / sigma;
     const int n = lv.n;
+    if (fused) {
+        // three-term form on two iterate buffers: lv.x and lv.d
+        const tile_grid tg = make_tiles(lv.g);
+        T *cur = lv.d.p, *prev = lv.x.p;
+        if (zero_start)
+            hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)(1.0 / theta),
+                               (const T *)lv.dinv.p, (const T *)lv.b.p, cur, prev);
+        else
+            hipLaunchKernelGGL((k_cheb3<T, true>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, lv.g, tg, (T)gamma, T(0),
+                               (T)(1.0 / theta), (const T *)prev, cur, (const T *)lv.b.p, (const T *)lv.en.p,
+                               (const T *)lv.es.p);
+        for (int k = 0; k < deg - 1; k++) {
+            const double rho_new = 1.0 / (2.0 * sigma - rho);
+            hipLaunchKernelGGL((k_cheb3<T, false>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, lv.g, tg, (T)gamma,
+                               (T)(rho_new * rho), (T)(2.0 * rho_new / delta), (const T *)cur, prev,
+                               (const T *)lv.b.p, (const T *)lv.en.p, (const T *)lv.es.p);
+            std::swap(cur, prev);
+            rho = rho_new;
+        }
+        if (cur != lv.x.p)
+            hipLaunchKernelGGL(k_copy<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (const T *)cur, lv.x.p);
+        return;
+    }
     if (zero_start) {
         hipLaunchKernelGGL(k_copy<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, lv.b.p, lv.r.p);
         hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, lv.x.p, T(0));
@@ -1153,7 +1263,7 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, i
         return;
     }
     mg_level<T> &cl = H.L[i + 1];
-    chebyshev(c, lv, gamma, deg, true);
+    chebyshev(c, lv, gamma, deg, true, H.fused);
     const int reps = (i + 2 == H.L.size()) ? 1 : cycle;
     for (int rep = 0; rep < reps; rep++) {
         hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p, lv.b.p,
@@ -1162,7 +1272,7 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, i
         vcycle(c, H, i + 1, gamma, deg, cycle);
         hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, cl.x.p, lv.x.p);
     }
-    chebyshev(c, lv, gamma, deg, false);
+    chebyshev(c, lv, gamma, deg, false, H.fused);
 }
 
 // Records one CG iteration (preconditioning V-cycle, direction, operator
@@ -1172,17 +1282,24 @@ static void pcg_iteration(mdg_ctx *c, mg_hier<T> &H, const mdg_solver_opts *o) {
     mg_level<T> &l0 = H.L[0];
     const int n = l0.n, nb = mdg_grid1d(n, RB), nd = std::min(1024, nb);
     const double gamma = o->penalty;
-    T *x = H.cg_x.p, *r = H.cg_r.p, *p = H.cg_p.p, *q = H.cg_q.p;
-    hipLaunchKernelGGL(k_copy<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (const T *)r, l0.b.p);
+    double *x = H.cg_x.p, *r = H.cg_r.p, *p = H.cg_p.p, *q = H.cg_q.p;
+    hipLaunchKernelGGL((k_convert<double, T>), dim3(nblk(n)), dim3(256), 0, c->stream, n, (const double *)r, l0.b.p);
     vcycle(c, H, 0, gamma, o->cheb_degree, o->cycle > 1 ? o->cycle : 1);
-    const T *z = l0.x.p;
-    hipLaunchKernelGGL(k_dot_part<T>, dim3(nd), dim3(RB), 0, c->stream, n, (const T *)r, z, H.part.p);
+    const double *z;
+    if constexpr (std::is_same<T, double>::value) {
+        z = l0.x.p;
+    } else {
+        hipLaunchKernelGGL((k_convert<T, double>), dim3(nblk(n)), dim3(256), 0, c->stream, n, (const T *)l0.x.p, H.cg_z.p);
+        z = H.cg_z.p;
+    }
+    // the CG itself runs on the FP64 operator (c->eta_n, c->eta_s)
+    hipLaunchKernelGGL(k_dot_part<double>, dim3(nd), dim3(RB), 0, c->stream, n, (const double *)r, z, H.part.p);
     hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nd, (const double *)H.part.p, H.scal.p, 1, -1, -1);
-    hipLaunchKernelGGL(k_pcg_dir<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, z, p, (const double *)H.scal.p);
-    hipLaunchKernelGGL(k_vel_apply_dot<T>, dim3(nb), dim3(RB), 0, c->stream, l0.g, (T)gamma, (const T *)p, (const T *)l0.en.p,
-                       (const T *)l0.es.p, q, H.part.p);
+    hipLaunchKernelGGL(k_pcg_dir<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, z, p, (const double *)H.scal.p);
+    hipLaunchKernelGGL(k_vel_apply_dot<double>, dim3(nb), dim3(RB), 0, c->stream, l0.g, gamma, (const double *)p,
+                       (const double *)c->eta_n.p, (const double *)c->eta_s.p, q, H.part.p);
     hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nb, (const double *)H.part.p, H.scal.p, 2, -1, -1);
-    hipLaunchKernelGGL(k_pcg_step<T>, dim3(nb), dim3(RB), 0, c->stream, n, (const T *)p, (const T *)q, x, r,
+    hipLaunchKernelGGL(k_pcg_step<double>, dim3(nb), dim3(RB), 0, c->stream, n, (const double *)p, (const double *)q, x, r,
                        (const double *)H.scal.p, H.part.p);
     hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nb, (const double *)H.part.p, H.scal.p, 3, 1, 0);
 }
@@ -1202,37 +1319,46 @@ static int inner_pcg(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solve
         HIPCHK(c, hipStreamEndCapture(c->stream, &H.graph));
         HIPCHK(c, hipGraphInstantiate(&H.exec, H.graph, nullptr, nullptr, 0));
     }
-    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, H.cg_x.p, T(0));
-    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, H.cg_p.p, T(0));
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, H.cg_x.p, 0.0);
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, H.cg_p.p, 0.0);
     hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, c->stream, 8, H.scal.p, 0.0);
     double rr0;
     int rc;
-    if ((rc = dot_host<T>(c, s, n, H.cg_r.p, H.cg_r.p, &rr0))) return rc;
+    if ((rc = dot_host<double>(c, s, n, H.cg_r.p, H.cg_r.p, &rr0))) return rc;
     *its = 0;
     if (rr0 == 0.0) return MDG_OK;
     const double tol2 = o->inner_tol * o->inner_tol * rr0;   // relative to ||b||
     if (o->precond == 1 && mdg_pcnet_ready(c)) {
         // learned initial guess: the network gives a direction d, scaled by
-        // the residual-minimising alpha = (b, A d) / (A d, A d), so that
-        // ||b - alpha A d|| <= ||b||; multigrid CG finishes the solve
-        if ((rc = mdg_pcnet_guess<T>(c, l0.g, (const T *)H.cg_r.p, (const T *)l0.dinv.p, (const T *)l0.en.p, H.cg_x.p)))
+        // the energy-optimal alpha = (b, d) / (d, A d), the Galerkin
+        // projection of A^-1 b on d, so the A-norm error of the start can
+        // only shrink; multigrid CG finishes the solve from there
+        if ((rc = mdg_pcnet_guess<double, T>(c, l0.g, (const double *)H.cg_r.p, (const T *)l0.dinv.p, (const T *)l0.en.p,
+                                             H.cg_x.p)))
             return rc;
-        hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(n)), dim3(256), 0, c->stream, l0.g, (T)o->penalty,
-                           (const T *)H.cg_x.p, (const T *)nullptr, (const T *)l0.en.p, (const T *)l0.es.p, H.cg_q.p);
-        double bq, qq;   // q = -A d
-        if ((rc = dot_host<T>(c, s, n, H.cg_r.p, H.cg_q.p, &bq)) || (rc = dot_host<T>(c, s, n, H.cg_q.p, H.cg_q.p, &qq)))
+        hipLaunchKernelGGL(k_vel_residual<double>, dim3(nblk(n)), dim3(256), 0, c->stream, l0.g, o->penalty,
+                           (const double *)H.cg_x.p, (const double *)nullptr, (const double *)c->eta_n.p,
+                           (const double *)c->eta_s.p, H.cg_q.p);
+        double bd, dq;   // q = -A d
+        if ((rc = dot_host<double>(c, s, n, H.cg_r.p, H.cg_x.p, &bd)) ||
+            (rc = dot_host<double>(c, s, n, H.cg_x.p, H.cg_q.p, &dq)))
             return rc;
-        const double alpha = (qq > 0.0) ? -bq / qq : 0.0;
+        const double alpha = (dq < 0.0) ? -bd / dq : 0.0;
         if (alpha > 0.0 && std::isfinite(alpha)) {
-            hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)0, (const T *)H.cg_q.p, (T)alpha, H.cg_x.p);
-            hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)alpha, (const T *)H.cg_q.p, (T)1, H.cg_r.p);
-            s->guess_gain += bq * bq / (qq * rr0);
+            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, 0.0, (const double *)H.cg_q.p, alpha,
+                               H.cg_x.p);
+            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, alpha, (const double *)H.cg_q.p, 1.0,
+                               H.cg_r.p);
+            double rr;
+            if ((rc = dot_host<double>(c, s, n, H.cg_r.p, H.cg_r.p, &rr))) return rc;
+            s->guess_gain += rr / rr0;
             s->guess_used++;
-            if (rr0 - bq * bq / qq <= tol2) return MDG_OK;
+            if (rr <= tol2) return MDG_OK;
         } else {
-            hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, H.cg_x.p, T(0));
+            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, H.cg_x.p, 0.0);
         }
-    } else if ((rc = mdg_pcnet_capture_in<T>(c, l0.g, (const T *)H.cg_r.p, (const T *)l0.dinv.p, (const T *)l0.en.p))) {
+    } else if ((rc = mdg_pcnet_capture_in<double, T>(c, l0.g, (const double *)H.cg_r.p, (const T *)l0.dinv.p,
+                                                      (const T *)l0.en.p))) {
         return rc;
     }
     const int every = o->check_every > 0 ? o->check_every : 1;
@@ -1244,7 +1370,7 @@ static int inner_pcg(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solve
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (!(H.h_scal[2] > 0.0) || H.h_scal[3] <= tol2) break;
     }
-    return mdg_pcnet_capture_out<T>(c, l0.g, (const T *)H.cg_x.p);
+    return mdg_pcnet_capture_out<double>(c, l0.g, (const double *)H.cg_x.p);
 }
 
 // z = M r (block preconditioner, FP64 in/out)
@@ -1259,11 +1385,12 @@ static int precondition(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_o
         if ((rc = bfbt_rhs(c, s, H, o, r, z, &pits))) return rc;
         o->poisson_its += pits;
     } else {
-        hipLaunchKernelGGL(k_schur_rhs<T>, dim3(nblk(nfull)), dim3(256), 0, c->stream, g, o->penalty, r, c->eta_n.p, z, H.cg_r.p);
+        hipLaunchKernelGGL(k_schur_rhs<double>, dim3(nblk(nfull)), dim3(256), 0, c->stream, g, o->penalty, r, c->eta_n.p, z,
+                           H.cg_r.p);
     }
     if ((rc = inner_pcg(c, s, H, o, &its))) return rc;
     *inner += its;
-    hipLaunchKernelGGL((k_convert<T, double>), dim3(nblk(nu + nv)), dim3(256), 0, c->stream, nu + nv, H.cg_x.p, z);
+    hipLaunchKernelGGL(k_copy<double>, dim3(nblk(nu + nv)), dim3(256), 0, c->stream, nu + nv, (const double *)H.cg_x.p, z);
     HIPCHK(c, hipGetLastError());
     return MDG_OK;
 }
@@ -1289,7 +1416,22 @@ static int fgcr_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opt
     for (int it = 0; it < o->max_its; it++) {
         double *v = s->V.p + (size_t)slot * n, *z = s->Z.p + (size_t)slot * n;
         if ((rc = precondition<T>(c, s, H, o, R, z, &o->inner_its))) return rc;
-        hipLaunchKernelGGL(k_apply_Kg, dim3(nblk(n)), dim3(256), 0, c->stream, g, o->penalty, z, c->eta_n.p, c->eta_s.p, v);
+        // Search directions keep a zero-mean pressure: the closed box leaves
+        // one pressure (Picard) or pressure-velocity (Newton) mode free, and
+        // the reference's Powell-Hestenes increments are zero-sum
+        // (Solvers.c:1301-1304), so the solution is sought in that subspace.
+        {
+            double *zp = z + nvx(g) + nvz(g);
+            const int np = ncell(g), nb2 = 256;
+            hipLaunchKernelGGL(k_sum_part_vec<double>, dim3(nb2), dim3(RB), 0, c->stream, np, (const double *)zp, s->part.p);
+            hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nb2, (const double *)s->part.p, s->scal.p, 14, -1, -1);
+            hipLaunchKernelGGL(k_sub_mean<double>, dim3(nblk(np)), dim3(256), 0, c->stream, np, zp, (const double *)s->scal.p, 14, 1.0);
+        }
+        if (mdg_newton_active(c)) {
+            if ((rc = mdg_newton_apply_grid(c, o->penalty, z, v))) return rc;
+        } else {
+            hipLaunchKernelGGL(k_apply_Kg, dim3(nblk(n)), dim3(256), 0, c->stream, g, o->penalty, z, c->eta_n.p, c->eta_s.p, v);
+        }
         for (int j = 0; j < slot; j++) {
             double a;
             if ((rc = dot_host<double>(c, s, n, s->V.p + (size_t)j * n, v, &a))) return rc;
@@ -1404,26 +1546,31 @@ extern "C" int mdg_killer_solver(mdg_ctx *c, mdg_solver_opts *o, const double *D
     return rc;
 }
 
-// Times the dominant kernel of the solve (the fine-level smoother operator
-// r -= A_g d, FP64) on the hierarchy of the last FP64 solve, reps launches
-// between HIP events on the context stream.  Returns ms per launch and the
-// algorithmic bytes per launch (d, r read, r written, eta_n, eta_s read).
+// Times the dominant kernel of the solve (the fine-level fused Chebyshev
+// smoother step x_{k+1} = x_k + a (x_k - x_{k-1}) + c D^-1 (b - A_g x_k),
+// FP64) on the hierarchy of the last FP64 solve, reps launches between HIP
+// events on the context stream.  Returns ms per launch and the algorithmic
+// bytes per launch: x_k, x_{k-1} and b read, x_{k+1} written (8 B each per
+// velocity face), eta_n and eta_s read once (8 B per cell and per vertex).
 extern "C" int mdg_time_smoother(mdg_ctx *c, int reps, double *ms, double *bytes) {
     if (!c || !ms || !bytes || reps < 1) return mdg_fail(c, MDG_ERR_ARG, "bad argument");
     if (!c->solver || c->solver->hd.L.empty()) return mdg_fail(c, MDG_ERR_STATE, "no FP64 solve has run");
     mg_level<double> &l0 = c->solver->hd.L[0];
     const int n = l0.n;
-    hipLaunchKernelGGL(k_cheb_res<double>, dim3(nblk(n)), dim3(256), 0, c->stream, l0.g, 10.0, (const double *)l0.d.p,
-                       (const double *)l0.en.p, (const double *)l0.es.p, l0.r.p);
+    const tile_grid tg = make_tiles(l0.g);
+    auto launch = [&]() {
+        hipLaunchKernelGGL((k_cheb3<double, false>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, l0.g, tg, 10.0, 0.5,
+                           0.1, (const double *)l0.x.p, l0.d.p, (const double *)l0.b.p, (const double *)l0.en.p,
+                           (const double *)l0.es.p);
+    };
+    launch();
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    for (int i = 0; i < reps; i++)
-        hipLaunchKernelGGL(k_cheb_res<double>, dim3(nblk(n)), dim3(256), 0, c->stream, l0.g, 10.0, (const double *)l0.d.p,
-                           (const double *)l0.en.p, (const double *)l0.es.p, l0.r.p);
+    for (int i = 0; i < reps; i++) launch();
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float t = 0.f;
     HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
     *ms = t / reps;
-    *bytes = 8.0 * (3.0 * n + (double)ncell(l0.g) + (double)l0.g.Nx * l0.g.Nz);
+    *bytes = 8.0 * (4.0 * n + (double)ncell(l0.g) + (double)l0.g.Nx * l0.g.Nz);
     return MDG_OK;
 }

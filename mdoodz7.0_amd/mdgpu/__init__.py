@@ -87,7 +87,8 @@ class SolverOpts(C.Structure):
                 ("cheb_degree", C.c_int), ("coarse_cells", C.c_int), ("precision", C.c_int),
                 ("precond", C.c_int), ("cycle", C.c_int), ("eta_avg", C.c_int),
                 ("check_every", C.c_int), ("schur", C.c_int), ("poisson_tol", C.c_double),
-                ("poisson_max_its", C.c_int), ("its", C.c_int), ("inner_its", C.c_int),
+                ("poisson_max_its", C.c_int), ("smoother", C.c_int), ("its", C.c_int),
+                ("inner_its", C.c_int),
                 ("poisson_its", C.c_int),
                 ("levels", C.c_int), ("status", C.c_int), ("rel_res", C.c_double),
                 ("time_ms", C.c_double), ("guess_used", C.c_int), ("guess_gain", C.c_double)]
@@ -122,6 +123,13 @@ RHEO_IN = ("perc", "G", "T", "P", "exx", "ezz", "exz", "sxx0", "szz0", "sxz0", "
 RHEO_OUT = ("eta", "eta_phys", "sxx", "szz", "sxz", "p_corr", "eII_pl")
 
 
+class PhaseSoft(C.Structure):
+    """mdg_phase_soft: moduli and softening parameters of one phase."""
+    _fields_ = [(k, C.c_double) for k in ("G", "beta", "C", "C_end", "fric", "fric_end", "dil",
+                                           "dil_end", "pls_start", "pls_end", "H_c")] + \
+        [(k, C.c_int) for k in ("fric_soft", "dil_soft", "coh_soft")]
+
+
 class RheoNodes(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in RHEO_IN + RHEO_OUT]
 
@@ -139,6 +147,11 @@ def _extra_signatures(L):
     L.mdg_time_smoother.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mdg_c2v.argtypes = [vp, C.c_int, vp, vp]
     L.mdg_stokes_diag.argtypes = [vp] * 5
+    L.mdg_tangent_grid.argtypes = [vp, C.POINTER(PhaseProp), C.c_int, C.POINTER(RheoParams),
+                                   C.c_int, C.POINTER(RheoNodes), vp]
+    L.mdg_jacobian_apply.argtypes = [vp] * 5
+    L.mdg_newton_solver.argtypes = [vp, C.POINTER(SolverOpts)] + [vp] * 7
+    L.mdg_material_grid.argtypes = [vp, C.POINTER(PhaseSoft), C.c_int, C.c_int, C.c_int] + [vp] * 7
     L.mdg_pcnet_param_count.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.mdg_pcnet_set_params.argtypes = [vp, vp, C.c_longlong]
     L.mdg_pcnet_get_params.argtypes = [vp, vp, vp]
@@ -404,6 +417,54 @@ class Context:
             self.h, ptr(exxd), ptr(ezzd), ptr(exz), ptr(div), ptr(out["exz_n"]),
             ptr(out["exxd_s"]), ptr(out["ezzd_s"]), ptr(out["div_s"])))
         return out
+
+    def material_grid(self, phases, centroid, perc, strain_pl=None, style=0):
+        """ShearModCompExpGrid + CohesionFrictionDilationGrid on one node set."""
+        n = self.node_count(centroid)
+        perc = np.ascontiguousarray(perc, np.float64)
+        out = {k: np.zeros(n) for k in ("mu", "bet", "C", "fric", "dil")}
+        arr = (PhaseSoft * len(phases))(*phases)
+        sp = None if strain_pl is None else np.ascontiguousarray(strain_pl, np.float64)
+        self._check(self._L.mdg_material_grid(self.h, arr, len(phases), int(centroid), int(style),
+                                              ptr(perc), ptr(sp), *[ptr(out[k]) for k in
+                                                                    ("mu", "bet", "C", "fric",
+                                                                     "dil")]))
+        return out
+
+    def tangent_grid(self, phases, prm, centroid, **arrays):
+        """Newton tangent coefficients: (8, cells) or (4, vertices)."""
+        n = self.node_count(centroid)
+        D = np.zeros(((8 if centroid else 4), n))
+        f, keep = RheoNodes(), []
+        for k in RHEO_IN:
+            a = arrays.get(k)
+            if isinstance(a, np.ndarray):
+                a = np.ascontiguousarray(a, np.float64)
+                keep.append(a)
+            setattr(f, k, ptr(a))
+        ph = (PhaseProp * len(phases))(*phases)
+        self._check(self._L.mdg_tangent_grid(self.h, ph, len(phases), C.byref(prm), int(centroid),
+                                             C.byref(f), ptr(D)))
+        return D
+
+    def jacobian_apply(self, Dn, Ds, x):
+        y = np.zeros(len(x))
+        self._check(self._L.mdg_jacobian_apply(self.h, ptr(np.ascontiguousarray(Dn)),
+                                               ptr(np.ascontiguousarray(Ds)),
+                                               ptr(np.ascontiguousarray(x)), ptr(y)))
+        return y
+
+    def newton_solver(self, Dn, Ds, D11, D33, rhs_mom, rhs_cont, **opts):
+        o = SolverOpts()
+        for k, v in opts.items():
+            setattr(o, k, v)
+        sol = np.zeros(len(rhs_mom) + len(rhs_cont))
+        rc = self._L.mdg_newton_solver(self.h, C.byref(o), ptr(np.ascontiguousarray(Dn)),
+                                       ptr(np.ascontiguousarray(Ds)), ptr(D11), ptr(D33),
+                                       ptr(np.ascontiguousarray(rhs_mom)),
+                                       ptr(np.ascontiguousarray(rhs_cont)), ptr(sol))
+        self._check(rc)
+        return sol, o
 
     def viscosity_grid(self, phases, prm, centroid, outputs=RHEO_OUT, **arrays):
         """NonNewtonianViscosityGrid (RheologyDensity.c:1131-1426) on the

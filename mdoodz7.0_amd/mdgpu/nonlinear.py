@@ -13,6 +13,7 @@ routines:
   EvaluateStokesResidualDecoupled      -> Context.stokes_residual
   convergence test                     Main_DOODZ.c:1003
   SolveStokesDefectDecoupled           -> Context.killer_solver, alpha = -1
+                                          or LineSearchDecoupled's step
   ExtractSolutions2 + ApplyBC          scatter below + Context.apply_bc
 """
 import numpy as np
@@ -41,30 +42,99 @@ class PicardLoop:
         self.ctx, self.phases, self.prm = ctx, phases, prm
         self.eqn = (eqn_u, eqn_v, eqn_p)
 
+    def node_inputs(self, st, sr):
+        """Rheology inputs on centres and vertices (NonNewtonianViscosityGrid
+        arguments; T and p_in carried to the vertices, RheologyDensity.c:1150-1151)."""
+        c = self.ctx
+        T_s, P_s = c.c2v(st["T"]), c.c2v(st["p"])
+        cin = dict(perc=st["perc_n"], G=st["mu_n"], T=st["T"], P=st["p"], exx=sr["exxd"],
+                   ezz=sr["ezzd"], exz=sr["exz_n"], sxx0=st["sxxd0"], szz0=st["szzd0"],
+                   sxz0=st["sxz0_n"])
+        vin = dict(perc=st["perc_s"], G=st["mu_s"], T=T_s, P=P_s, exx=sr["exxd_s"],
+                   ezz=sr["ezzd_s"], exz=sr["exz"], sxx0=st["sxxd0_s"], szz0=st["szzd0_s"],
+                   sxz0=st["sxz0"])
+        return cin, vin
+
     def update_nonlinearity(self, st):
         c = self.ctx
         c.apply_bc(st["u"], st["v"])
         sr = c.strain_rates(st["u"], st["v"])
         sr.update(c.strain_rates_nodes(sr["exxd"], sr["ezzd"], sr["exz"], sr["div"]))
-        T_s, P_s = c.c2v(st["T"]), c.c2v(st["p"])
-        cen = c.viscosity_grid(self.phases, self.prm, 1, perc=st["perc_n"], G=st["mu_n"],
-                               T=st["T"], P=st["p"], exx=sr["exxd"], ezz=sr["ezzd"],
-                               exz=sr["exz_n"], sxx0=st["sxxd0"], szz0=st["szzd0"],
-                               sxz0=st["sxz0_n"])
-        ver = c.viscosity_grid(self.phases, self.prm, 0, perc=st["perc_s"], G=st["mu_s"],
-                               T=T_s, P=P_s, exx=sr["exxd_s"], ezz=sr["ezzd_s"], exz=sr["exz"],
-                               sxx0=st["sxxd0_s"], szz0=st["szzd0_s"], sxz0=st["sxz0"])
+        cin, vin = self.node_inputs(st, sr)
+        cen = c.viscosity_grid(self.phases, self.prm, 1, **cin)
+        ver = c.viscosity_grid(self.phases, self.prm, 0, **vin)
+        self.last_inputs = (cin, vin)
         return sr, cen, ver
 
-    def run(self, st, nit_max, tol, diag_scaling=1, gx=0.0, gz=0.0, **solver_opts):
+    def residual_norms(self, st, d, rx, rz):
+        sr, cen, ver = self.update_nonlinearity(st)
+        _, norms = self.ctx.stokes_residual(cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"],
+                                            sr["div"], rx, rz, d=d)
+        return np.asarray(norms, dtype=float)
+
+    def line_search(self, st, dx, d, rx, rz, res_f, min_step=0.0, let_res_grow=0, newton=False):
+        """LineSearchDecoupled (StokesRoutines.c:582-807), Picard branch: six
+        trial steps from alpha = -0 down to -2.5, each followed by
+        UpdateNonLinearity and the residual norms, then the acceptance cascade
+        on the combined, xz, x and z norms.  Returns (alpha, stagnated) and
+        leaves u, v, p as they were."""
+        # Newton steps search [-1, -min_step], Picard steps [-2.5, 0] (:603-613)
+        ntry = 6
+        minalpha, maxalpha = (-1.0, -min_step) if newton else (-2.5, -0.0)
+        saved = [st[k].copy() for k in ("u", "v", "p")]
+        dalpha = abs(maxalpha - minalpha) / (ntry - 1)
+        trials, rows = [], []
+        alpha = maxalpha
+        for kk in range(ntry):
+            if kk > 0:
+                alpha -= dalpha
+            trials.append(alpha)
+            st["u"][:], st["v"][:], st["p"][:] = saved
+            scatter(*self.eqn, dx, st["u"], st["v"], st["p"], alpha)
+            self.ctx.apply_bc(st["u"], st["v"])
+            rows.append(self.residual_norms(st, d, rx, rz))
+        st["u"][:], st["v"][:], st["p"][:] = saved
+        R = np.array(rows)
+        comb = np.sqrt((R ** 2).sum(1))
+        xz = np.hypot(R[:, 0], R[:, 1])
+        best = [int(np.argmin(a)) for a in (comb, xz, R[:, 0], R[:, 1])]
+        f0 = np.asarray(res_f, dtype=float)
+        choice = None
+        if comb[best[0]] < np.sqrt((f0 ** 2).sum()):
+            choice = best[0]
+        elif R[best[1], 0] < f0[0] and R[best[1], 1] < f0[1]:
+            choice = best[1]
+        elif R[best[2], 0] < f0[0]:
+            choice = best[2]
+        elif R[best[3], 1] < f0[1]:
+            choice = best[3]
+        alpha = trials[choice] if choice is not None else 0.0
+        if abs(alpha) < 1e-13 or choice is None:
+            return (min_step, False) if let_res_grow else (0.0, True)
+        return alpha, False
+
+    def run(self, st, nit_max, tol, diag_scaling=1, gx=0.0, gz=0.0, line_search=False,
+            newton=False, picard2newton_tol=1e-1, max_its_pic=10, **solver_opts):
         """Iterates in place on st; tol = (abs_mom, rel_mom, abs_div, rel_div).
-        Returns (residual norm history, number of linear solves, solver stats)."""
+        Returns (residual norm history, number of linear solves, solver stats);
+        with line_search the chosen steps are in st["alphas"].  With newton,
+        the loop switches from Picard to Newton once a relative residual of
+        the previous iteration drops below picard2newton_tol or after
+        max_its_pic iterations (Main_DOODZ.c:884-896); Newton steps use the
+        tangent coefficients and the line search (InputOutput.c:1334).
+        st["newton_steps"] lists the iteration numbers solved with Newton."""
         c = self.ctx
         rx, rz = c.eval_rhs(gx, gz, st["rho_n"], st["rho_s"])
         abs_mom, rel_mom, abs_div, rel_div = tol
         hist, stats, res0 = [], [], None
         nm = c.eqn_count()[1]
+        is_newton = False
+        st["newton_steps"] = []
         for nit in range(nit_max + 1):
+            if newton and nit > 0 and not is_newton:
+                rr = hist[-1] / res0
+                if (rr < picard2newton_tol).any() or nit >= max_its_pic:
+                    is_newton = True
             sr, cen, ver = self.update_nonlinearity(st)
             D11, D33 = 2.0 * cen["eta"], ver["eta"]
             d = c.stokes_diag(D11, D11, D33) if diag_scaling else None
@@ -80,9 +150,22 @@ class PicardLoop:
                 break
             F, _ = c.stokes_residual(cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"],
                                      sr["div"], rx, rz)
-            dx, o = c.killer_solver(D11, D11, D33, F[:nm], F[nm:], **solver_opts)
+            if is_newton:
+                cin, vin = self.last_inputs
+                Dn = c.tangent_grid(self.phases, self.prm, 1, **cin)
+                Ds = c.tangent_grid(self.phases, self.prm, 0, **vin)
+                dx, o = c.newton_solver(Dn, Ds, D11, D33, F[:nm], F[nm:], **solver_opts)
+                st["newton_steps"].append(nit)
+            else:
+                dx, o = c.killer_solver(D11, D11, D33, F[:nm], F[nm:], **solver_opts)
             stats.append(o)
-            scatter(*self.eqn, dx, st["u"], st["v"], st["p"])
+            alpha, stagnated = -1.0, False
+            if line_search or is_newton:
+                alpha, stagnated = self.line_search(st, dx, d, rx, rz, norms, newton=is_newton)
+                st.setdefault("alphas", []).append(alpha)
+            scatter(*self.eqn, dx, st["u"], st["v"], st["p"], alpha)
             c.apply_bc(st["u"], st["v"])
+            if stagnated:
+                break
         st["eta_n"], st["eta_s"] = cen["eta"], ver["eta"]
         return hist, len(stats), stats

@@ -53,6 +53,12 @@ class OrcRheoModel(C.Structure):
                 ("scale_eta", C.c_double)]
 
 
+class OrcPhaseSoft(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("G", "bet", "C", "C_end", "phi", "phi_end", "psi",
+                                           "psi_end", "pls_start", "pls_end", "H_c")] + \
+        [(k, C.c_int) for k in ("phi_soft", "psi_soft", "coh_soft")]
+
+
 def phase(**kw):
     """OrcPhase with reference-like defaults (constant viscosity 1, no plasticity)."""
     d = dict(cstv=1, pwlv=0, plast=0, eta0=1.0, pref_pwl=1.0, F_pwl=1.0, A_pwl=1.0, n_pwl=1.0,
@@ -110,6 +116,10 @@ def lib():
                                          C.c_int] + [C.c_void_p] * 14 + [C.c_void_p, C.c_int,
                                          C.c_int] + [C.c_void_p] * 7
         L.orc_rheology_nodes.restype = C.c_int
+        L.orc_tangent_nodes.argtypes = L.orc_rheology_nodes.argtypes[:21] + [C.c_void_p]
+        L.orc_tangent_nodes.restype = C.c_int
+        L.orc_material_grid.argtypes = [C.POINTER(OrcPhaseSoft), C.c_int, C.c_int] + \
+            [C.c_void_p] * 3 + [C.c_int, C.c_int] + [C.c_void_p] * 5
         _LIB = L
     return _LIB
 
@@ -306,3 +316,47 @@ def rheology_nodes(md, phases, perc, G, T, P, exx, ezz, exz, sxx0=None, szz0=Non
                                               "eII_pl")])
     outs["status"] = st
     return outs
+
+
+def material_grid(phases, perc, strain_pl=None, tag=None, centroid=1, style=0):
+    """ShearModCompExpGrid + CohesionFrictionDilationGrid (RheologyDensity.c:
+    1432-1730) on one node set; phases are OrcPhaseSoft, perc (nph, n)."""
+    perc = np.ascontiguousarray(perc, np.float64)
+    nph, n = perc.shape
+    out = {k: np.zeros(n) for k in ("mu", "bet", "C", "fric", "dil")}
+    sp = None if strain_pl is None else np.ascontiguousarray(strain_pl, np.float64)
+    tg = None if tag is None else np.ascontiguousarray(tag, np.int8)
+    arr = (OrcPhaseSoft * nph)(*phases)
+    lib().orc_material_grid(arr, nph, n, perc.ctypes.data,
+                            None if sp is None else sp.ctypes.data,
+                            None if tg is None else tg.ctypes.data, centroid, style,
+                            *[out[k].ctypes.data for k in ("mu", "bet", "C", "fric", "dil")])
+    return out
+
+
+def tangent_nodes(md, phases, perc, G, T, P, exx, ezz, exz, sxx0=None, szz0=None, sxz0=None,
+                  fric=None, Coh=None, dil=None, pref=None, tag=None, centroid=1, average=0):
+    """Newton tangent coefficients (StokesRoutines.c:250-375 with
+    FD_Jacobian.c:48-323): array (8, n) on centres (D11..D14, D21..D24) or
+    (4, n) on vertices (D31..D34)."""
+    perc = np.ascontiguousarray(perc, np.float64)
+    nph, n = perc.shape
+    keep = []
+
+    def p(a, dt=np.float64):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dt), (n,)))
+        keep.append(a)
+        return a.ctypes.data
+    z = np.zeros(n)
+    D = np.zeros(((8 if centroid else 4), n))
+    ph = (OrcPhase * nph)(*phases)
+    st = lib().orc_tangent_nodes(C.byref(md), ph, nph, n, perc.ctypes.data, p(G), p(T), p(P),
+                                 p(exx), p(ezz), p(exz), p(z if sxx0 is None else sxx0),
+                                 p(z if szz0 is None else szz0), p(z if sxz0 is None else sxz0),
+                                 p(fric), p(Coh), p(dil), p(pref), p(tag, np.int8), centroid,
+                                 average, D.ctypes.data)
+    if st:
+        raise RuntimeError("local iterations failed in the tangent")
+    return D

@@ -149,6 +149,28 @@ int orc_rheology_nodes(const orc_rheo_model *md, const orc_phase *phases, int np
                        double *eta_out, double *eta_phys, double *sxx, double *szz, double *sxz,
                        double *p_corr, double *eII_pl);
 
+/* Newton tangent coefficients by finite differences of the averaged
+ * viscosity (StokesRoutines.c:250-375, FD_Jacobian.c:48-323). */
+int orc_tangent_nodes(const orc_rheo_model *md, const orc_phase *phases, int nph, int n,
+                      const double *perc, const double *G, const double *T, const double *P,
+                      const double *exx, const double *ezz, const double *exz,
+                      const double *sxx0, const double *szz0, const double *sxz0,
+                      const double *fric, const double *C, const double *dil,
+                      const double *pref, const signed char *tag, int centroid, int average,
+                      double *D);
+
+/* Per-phase elastic moduli and softening parameters (mat_prop: G, bet, C,
+ * C_end, phi, phi_end, psi, psi_end, pls_start, pls_end, H_c, *_soft). */
+typedef struct {
+    double G, bet, C, C_end, phi, phi_end, psi, psi_end, pls_start, pls_end, H_c;
+    int    phi_soft, psi_soft, coh_soft;
+} orc_phase_soft;
+
+/* ShearModCompExpGrid + CohesionFrictionDilationGrid on one node set. */
+void orc_material_grid(const orc_phase_soft *ph, int nph, int n, const double *perc,
+                       const double *strain_pl, const signed char *tag, int centroid, int style,
+                       double *mu, double *bet, double *C, double *fric, double *dil);
+
 /* Row-interleaved momentum DOF order (Vx row l, then Vz row l) used to
  * give the exact factor of orc_killer_solve a narrow band. */
 void orc_set_band_order(const orc_grid *g, const int *eqn_u, const int *eqn_v, int neq_mom);

@@ -58,11 +58,60 @@ def picard_diag(g, num, eta_n, eta_s):
     return np.concatenate([A.diagonal(), np.ones(num[4])])
 
 
+def line_search(g, num, st, dx, md, phases, average, d, rx, rz, res_f, min_step=0.0,
+                let_res_grow=0):
+    """LineSearchDecoupled (StokesRoutines.c:582-807) for Picard steps: six
+    trial steps alpha in [-2.5, 0] (evenly spaced from the top), each
+    followed by UpdateNonLinearity and the residual, then the acceptance
+    cascade on the combined, xz, x and z norms.  Returns (alpha, stagnated);
+    u, v, p are left unchanged."""
+    ntry, minalpha, maxalpha = 6, -2.5, -0.0
+    saved = [st[k].copy() for k in ("u", "v", "p")]
+    dalpha = abs(maxalpha - minalpha) / (ntry - 1)
+    alphav, R = [], []
+    alpha = maxalpha
+    for kk in range(ntry):
+        if kk > 0:
+            alpha -= dalpha
+        alphav.append(alpha)
+        st["u"][:], st["v"][:], st["p"][:] = saved
+        scatter_solution(num, dx, st["u"], st["v"], st["p"], alpha)
+        apply_bc(g, st["u"], st["v"])
+        sr, cen, ver = update_nonlinearity(g, st, md, phases, average)
+        _, norms = residual(g, num, cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"],
+                            sr["div"], rx, rz, d)
+        R.append(norms.copy())
+    st["u"][:], st["v"][:], st["p"][:] = saved
+    R = np.array(R)
+    fxzp = np.sqrt((R ** 2).sum(1))
+    fxz = np.sqrt(R[:, 0] ** 2 + R[:, 1] ** 2)
+    # first index of the minimum, as the strict "<" scan of the reference
+    ixzp, ixz, ix, iz = (int(np.argmin(a)) for a in (fxzp, fxz, R[:, 0], R[:, 1]))
+    frac, success = 1.0, False
+    fxzp0 = np.sqrt((np.asarray(res_f) ** 2).sum())
+    if fxzp[ixzp] < frac * fxzp0:
+        alpha, success = alphav[ixzp], True
+    if not success and R[ixz, 0] < frac * res_f[0] and R[ixz, 1] < frac * res_f[1]:
+        alpha, success = alphav[ixz], True
+    if not success and R[ix, 0] < frac * res_f[0]:
+        alpha, success = alphav[ix], True
+    if not success and R[iz, 1] < frac * res_f[1]:
+        alpha, success = alphav[iz], True
+    stagnated = False
+    if abs(alpha) < 1e-13 or not success:
+        if let_res_grow:
+            alpha = min_step
+        else:
+            alpha, stagnated = 0.0, True
+    return alpha, stagnated
+
+
 def picard_loop(g, st, md, phases, nit_max, tol, average=0, diag_scaling=1, gx=0.0, gz=0.0,
-                **killer_kw):
+                line_search_on=False, **killer_kw):
     """Runs the loop in place on st (u, v, p updated).  tol = (abs_mom,
     rel_mom, abs_div, rel_div).  Returns the list of residual norm triples
-    (one per evaluated iteration) and the number of linear solves."""
+    (one per evaluated iteration) and the number of linear solves; with
+    line_search_on the chosen step lengths are in st["alphas"]."""
     num = g.numbering()
     nm = num[3]
     rx, rz = eval_rhs(g, gx, gz, st["rho_n"], st["rho_s"])
@@ -88,8 +137,36 @@ def picard_loop(g, st, md, phases, nit_max, tol, average=0, diag_scaling=1, gx=0
             raise RuntimeError("KillerSolver status %d at iteration %d (eta_n in [%g, %g], "
                                "eta_s in [%g, %g])" % (rc, nit, cen["eta"].min(), cen["eta"].max(),
                                                        ver["eta"].min(), ver["eta"].max()))
-        scatter_solution(num, np.concatenate([du, dp]), st["u"], st["v"], st["p"])
+        dx = np.concatenate([du, dp])
+        alpha, stagnated = -1.0, False
+        if line_search_on:
+            alpha, stagnated = line_search(g, num, st, dx, md, phases, average, d, rx, rz,
+                                           norms)
+            st.setdefault("alphas", []).append(alpha)
+        scatter_solution(num, dx, st["u"], st["v"], st["p"], alpha)
         apply_bc(g, st["u"], st["v"])
         nsolve += 1
+        if stagnated:
+            break
     st["eta_n"], st["eta_s"] = cen["eta"], ver["eta"]
     return hist, nsolve
+
+
+def residual_vector(g, num, st, md, phases, average, rx, rz):
+    """Unscaled residual F (equation numbering) of the state st."""
+    sr, cen, ver = update_nonlinearity(g, st, md, phases, average)
+    F, _ = residual(g, num, cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"], sr["div"], rx, rz)
+    return F
+
+
+def jacobian_fd_apply(g, num, st, md, phases, average, rx, rz, x, eps):
+    """(F(s + eps x) - F(s - eps x)) / (2 eps): the directional derivative of
+    the discrete residual, the operator the Newton Jacobian linearises."""
+    out = []
+    for sgn in (1.0, -1.0):
+        s2 = dict(st)
+        s2["u"], s2["v"], s2["p"] = st["u"].copy(), st["v"].copy(), st["p"].copy()
+        scatter_solution(num, x, s2["u"], s2["v"], s2["p"], alpha=sgn * eps)
+        apply_bc(g, s2["u"], s2["v"])
+        out.append(residual_vector(g, num, s2, md, phases, average, rx, rz))
+    return (out[0] - out[1]) / (2.0 * eps)

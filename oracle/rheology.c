@@ -193,3 +193,171 @@ int orc_rheology_nodes(const orc_rheo_model *md, const orc_phase *phases, int np
     }
     return status;
 }
+
+/* Softening (RheologyDensity.c:1432-1535) of one phase at accumulated plastic
+ * strain e: returns fric, dil, C through the pointers. */
+static void soften(const orc_phase_soft *m, double e, int style, double *fric, double *dil, double *C) {
+    *fric = m->phi; *dil = m->psi; *C = m->C;
+    const double dstrain = m->pls_end - m->pls_start;
+    if (style == 1) {
+        const double mu = 0.5 * (m->pls_end + m->pls_start);
+        if (m->phi_soft == 1) {
+            const double d = m->phi - m->phi_end;
+            const double f = m->phi - d / 2.0 * erfc(-(e - mu) / dstrain) - m->phi_end;
+            const double f0 = m->phi - d / 2.0 * erfc(-(0.0 - mu) / dstrain) - m->phi_end;
+            *fric = f * d / f0 + m->phi_end;
+        }
+        if (m->psi_soft == 1) {
+            const double d = m->psi - m->psi_end;
+            const double f = m->psi - d / 2.0 * erfc(-(e - mu) / dstrain) - m->psi_end;
+            const double f0 = m->psi - d / 2.0 * erfc(-(0.0 - mu) / dstrain) - m->psi_end;
+            *dil = f * d / f0 + m->psi_end;
+        }
+        if (m->coh_soft == 1) {
+            const double d = m->C - m->C_end;
+            const double f = m->C - d / 2.0 * erfc(-(e - mu) / dstrain) - m->C_end;
+            const double f0 = m->C - d / 2.0 * erfc(-(0.0 - mu) / dstrain) - m->C_end;
+            *C = f * d / f0 + m->C_end;
+        }
+    } else if (style == 2) {
+        *C = fmax(m->C + m->H_c * e, m->C_end);
+    } else {
+        if (e < m->pls_start) {
+            if (m->phi_soft == 1) *fric = m->phi;
+            if (m->psi_soft == 1) *dil = m->psi;
+            if (m->coh_soft == 1) *C = m->C;
+        }
+        if (e >= m->pls_end) {
+            if (m->phi_soft == 1) *fric = m->phi_end;
+            if (m->psi_soft == 1) *dil = m->psi_end;
+            if (m->coh_soft == 1) *C = m->C_end;
+        }
+        if (e >= m->pls_start && e < m->pls_end) {
+            /* the reference scales the slope by e itself, not e - pls_start */
+            if (m->phi_soft == 1) *fric = m->phi + (m->phi_end - m->phi) / dstrain * e;
+            if (m->psi_soft == 1) *dil = m->psi + (m->psi_end - m->psi) / dstrain * e;
+            if (m->coh_soft == 1) *C = m->C + (m->C_end - m->C) / dstrain * e;
+        }
+    }
+}
+
+/* ShearModCompExpGrid (RheologyDensity.c:1634-1730, harmonic G and beta)
+ * and CohesionFrictionDilationGrid (:1539-1616, arithmetic phase average of
+ * the softened fric, dil, C) on one node set.  strain_pl may be NULL (no
+ * softening outputs). */
+void orc_material_grid(const orc_phase_soft *ph, int nph, int n, const double *perc,
+                       const double *strain_pl, const signed char *tag, int centroid, int style,
+                       double *mu, double *bet, double *C, double *fric, double *dil) {
+    for (int i = 0; i < n; i++) {
+        mu[i] = bet[i] = 0.0;
+        if (strain_pl) C[i] = fric[i] = dil[i] = 0.0;
+        if (tag && (tag[i] == 30 || (centroid && tag[i] == 31))) continue;
+        double a = 0.0, b = 0.0, cc = 0.0, ff = 0.0, dd = 0.0;
+        for (int p = 0; p < nph; p++) {
+            const double w = perc[(size_t)p * n + i];
+            a += w * 1.0 / ph[p].G;
+            b += w * 1.0 / ph[p].bet;
+            if (strain_pl) {
+                double f, d, c;
+                soften(&ph[p], strain_pl[i], style, &f, &d, &c);
+                ff += w * f; dd += w * d; cc += w * c;
+            }
+        }
+        mu[i] = 1.0 / a;
+        bet[i] = 1.0 / b;
+        if (strain_pl) { C[i] = cc; fric[i] = ff; dil[i] = dd; }
+    }
+}
+
+/* Newton tangent coefficients (RheologicalOperators with Jacobian = 1,
+ * StokesRoutines.c:250-375, isotropic and incompressible) with the
+ * derivatives of the phase-averaged effective viscosity by central finite
+ * differences (DerivativesOnTheFly_n/_s, PhaseRheologyLoop_v1,
+ * FD_Jacobian.c:48-323): perturbation tol*|E| + tol/1e10 with tol = 1e-5
+ * on centres and 1e-7 on vertices; the shear derivative is taken w.r.t.
+ * gxz = 2 Exz (denominator 4 pert).  D has 8 rows on centres (D11..D14,
+ * D21..D24) and 4 on vertices (D31..D34), each of n nodes. */
+int orc_tangent_nodes(const orc_rheo_model *md, const orc_phase *phases, int nph, int n,
+                      const double *perc, const double *G, const double *T, const double *P,
+                      const double *exx, const double *ezz, const double *exz,
+                      const double *sxx0, const double *szz0, const double *sxz0,
+                      const double *fric, const double *C, const double *dil,
+                      const double *pref, const signed char *tag, int centroid, int average,
+                      double *D) {
+    const double el = md->elastic ? 1.0 : 0.0;
+    const double tol = centroid ? 1e-5 : 1e-7;
+    const int nrow = centroid ? 8 : 4;
+    int status = 0;
+    for (int i = 0; i < n; i++) {
+        for (int r = 0; r < nrow; r++) D[(size_t)r * n + i] = 0.0;
+        if (tag && (tag[i] == 30 || (centroid && tag[i] == 31))) continue;
+        const double eta_e = md->elastic ? md->dt * G[i] : 1.0;
+        /* EffectiveStrainRate (RheologyDensity.c:116-146), isotropic */
+        const double Exx = exx[i] + el * sxx0[i] / eta_e / 2.0;
+        const double Ezz = ezz[i] + el * szz0[i] / eta_e / 2.0;
+        const double Exz = exz[i] + el * sxz0[i] / eta_e / 2.0;
+        const double px = tol * fabs(Exx) + tol / 1e10, pz = tol * fabs(Ezz) + tol / 1e10;
+        const double pg = tol * fabs(Exz) + tol / 1e10, pp = tol * fabs(P[i]) + tol / 1e10;
+        /* perturbed states: (dExx, dEzz, dExz, dP, sign, denominator) */
+        const double st[8][6] = {{px, 0, 0, 0, 1, 2 * px}, {-px, 0, 0, 0, -1, 2 * px},
+                                 {0, pz, 0, 0, 1, 2 * pz}, {0, -pz, 0, 0, -1, 2 * pz},
+                                 {0, 0, pg, 0, 1, 4 * pg}, {0, 0, -pg, 0, -1, 4 * pg},
+                                 {0, 0, 0, pp, 1, 2 * pp}, {0, 0, 0, -pp, -1, 2 * pp}};
+        double der[4] = {0, 0, 0, 0}, e = 0.0;
+        /* phase-averaged effective viscosity and per-phase values at the base state */
+        double eta_ph[64];
+        for (int p = 0; p < nph; p++) {
+            const double f = perc[(size_t)p * n + i];
+            eta_ph[p] = 0.0;
+            if (!(fabs(f) > 1e-13)) continue;
+            double etaVE, Pc, Epl;
+            orc_viscosity_concise(&phases[p], md, G[i], T[i], P[i], Exx, Ezz, Exz,
+                                  fric ? fric[i] : phases[p].fric, C ? C[i] : phases[p].C,
+                                  dil ? dil[i] : phases[p].dil, pref ? pref[i] : 1.0, &etaVE,
+                                  &Pc, &Epl, &status);
+            eta_ph[p] = etaVE;
+            if (average == 0) e += f * etaVE;
+            if (average == 1) e += f / etaVE;
+            if (average == 2) e += f * log(etaVE);
+        }
+        if (average == 1) e = 1.0 / e;
+        if (average == 2) e = exp(e);
+        for (int s = 0; s < 8; s++) {
+            for (int p = 0; p < nph; p++) {
+                const double f = perc[(size_t)p * n + i];
+                if (!(fabs(f) > 1e-13)) continue;
+                double etaVE, Pc, Epl;
+                orc_viscosity_concise(&phases[p], md, G[i], T[i], P[i] + st[s][3], Exx + st[s][0],
+                                      Ezz + st[s][1], Exz + st[s][2],
+                                      fric ? fric[i] : phases[p].fric, C ? C[i] : phases[p].C,
+                                      dil ? dil[i] : phases[p].dil, pref ? pref[i] : 1.0,
+                                      &etaVE, &Pc, &Epl, &status);
+                double v = st[s][4] * f * etaVE / st[s][5];
+                if (average == 1) v /= eta_ph[p] * eta_ph[p];
+                if (average == 2) v /= eta_ph[p];
+                der[s / 2] += v;
+            }
+        }
+        for (int q = 0; q < 4; q++) {
+            if (average == 1) der[q] *= e * e;
+            if (average == 2) der[q] *= e;
+        }
+        if (centroid) {
+            D[0 * (size_t)n + i] = 2.0 * e + 2.0 * der[0] * Exx;
+            D[1 * (size_t)n + i] = 2.0 * der[1] * Exx;
+            D[2 * (size_t)n + i] = 2.0 * der[2] * Exx;
+            D[3 * (size_t)n + i] = 2.0 * der[3] * Exx;
+            D[4 * (size_t)n + i] = 2.0 * der[0] * Ezz;
+            D[5 * (size_t)n + i] = 2.0 * e + 2.0 * der[1] * Ezz;
+            D[6 * (size_t)n + i] = 2.0 * der[2] * Ezz;
+            D[7 * (size_t)n + i] = 2.0 * der[3] * Ezz;
+        } else {
+            /* Dxz = Gxz/2 = Exz for the isotropic case */
+            D[0 * (size_t)n + i] = 2.0 * der[0] * Exz;
+            D[1 * (size_t)n + i] = 2.0 * der[1] * Exz;
+            D[2 * (size_t)n + i] = e + 2.0 * der[2] * Exz;
+            D[3 * (size_t)n + i] = 2.0 * der[3] * Exz;
+        }
+    }
+    return status;
+}

@@ -1,0 +1,141 @@
+"""Newton linearisation on the GPU: tangent coefficients against the oracle
+(same finite-difference algorithm, FD_Jacobian.c:48-323) and the
+matrix-free Jacobian-vector product against central differences of the
+oracle's own discrete residual (the operator the Jacobian linearises).
+
+Tolerances: tangent 1e-6 relative to each coefficient row's largest value
+(the coefficients are differences of viscosities perturbed by 1e-5 or 1e-7
+relative, so last-bit differences of device pow/exp are amplified ~1e5-1e7);
+J v 1e-5 relative L2 (central difference of the residual with eps = 1e-6 of
+the state's scale, smooth rheology)."""
+import numpy as np
+import pytest
+
+import mdgpu
+from mdgpu import setups as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(O, n, C, seed=4):
+    from test_gpu_nonlinear import _problem
+    g, st = _problem(O, n, seed=seed)
+    phs = S.vep_phases(C=C)
+    common = dict(step=1, elastic=1, dt=1.0, R=1.0, zeroC=0.0, min_eta=1e-4, max_eta=1e4,
+                  eta_tol=1e-13, Tiimin=1e-5, scale_E=1.0, scale_eta=1.0)
+    o_ph = []
+    for d in phs:
+        d = dict(d)
+        b = d.pop("beta")
+        o_ph.append(O.OrcPhase(beta_for_K=b, **d))
+    ctx = mdgpu.Context(n + 1, n + 1, 0.0, 1.0, -1.0, 0.0)
+    ctx.set_bc(g.BCu_t, g.BCu_v, g.BCv_t, g.BCv_v, g.BCp_t, g.BCg_t)
+    return g, st, ctx, o_ph, [mdgpu.PhaseProp(**d) for d in phs], common
+
+
+def _perturbed_state(O, g, st, rng, amp=0.05):
+    u, v = st["u"], st["v"]
+    u += amp * rng.standard_normal(u.size) * (g.BCu_t == -1)
+    v += amp * rng.standard_normal(v.size) * (g.BCv_t == -1)
+    st["p"] += 0.2 * rng.standard_normal(st["p"].size)
+    O.apply_bc(g, u, v)
+
+
+def _tangents(O, g, st, o_ph, g_ph, common, ctx, average=0):
+    from oracle.nonlinear import strain_rate_components
+    sr = strain_rate_components(g, st["u"], st["v"])
+    P_s = O.c2v(g, st["p"])
+    T_s = O.c2v(g, st["T"])
+    md = O.OrcRheoModel(**common)
+    prm = mdgpu.RheoParams(average=average, **common)
+    cin = dict(perc=st["perc_n"], G=st["mu_n"], T=st["T"], P=st["p"], exx=sr["exxd"],
+               ezz=sr["ezzd"], exz=sr["exz_n"], sxx0=st["sxxd0"], szz0=st["szzd0"],
+               sxz0=st["sxz0_n"])
+    vin = dict(perc=st["perc_s"], G=st["mu_s"], T=T_s, P=P_s, exx=sr["exxd_s"],
+               ezz=sr["ezzd_s"], exz=sr["exz"], sxx0=st["sxxd0_s"], szz0=st["szzd0_s"],
+               sxz0=st["sxz0"])
+    Dn_o = O.tangent_nodes(md, o_ph, *[cin[k] for k in ("perc", "G", "T", "P", "exx", "ezz", "exz",
+                                                       "sxx0", "szz0", "sxz0")],
+                           tag=g.BCp_t, centroid=1, average=average)
+    Ds_o = O.tangent_nodes(md, o_ph, *[vin[k] for k in ("perc", "G", "T", "P", "exx", "ezz", "exz",
+                                                       "sxx0", "szz0", "sxz0")],
+                           tag=g.BCg_t, centroid=0, average=average)
+    Dn_g = ctx.tangent_grid(g_ph, prm, 1, **cin)
+    Ds_g = ctx.tangent_grid(g_ph, prm, 0, **vin)
+    return Dn_o, Ds_o, Dn_g, Ds_g
+
+
+@pytest.mark.parametrize("average", [0, 1, 2])
+@pytest.mark.parametrize("C", [1.0, 50.0])
+def test_tangent_matches_oracle(oracle_mod, average, C):
+    O = oracle_mod
+    g, st, ctx, o_ph, g_ph, common = _setup(O, 23, C)
+    _perturbed_state(O, g, st, np.random.default_rng(1))
+    Dn_o, Ds_o, Dn_g, Ds_g = _tangents(O, g, st, o_ph, g_ph, common, ctx, average)
+    for D_o, D_g, eta_row in ((Dn_o, Dn_g, 0), (Ds_o, Ds_g, 2)):
+        eta_scale = np.abs(D_o[eta_row]).max()     # D11 ~ 2 eta (centres), D33 ~ eta (vertices)
+        for r in range(D_o.shape[0]):
+            scale = np.abs(D_o[r]).max() + 1e-300
+            # rows 3 and 7 are the pressure derivatives D14, D24, D34: without
+            # yielding the viscosity does not depend on P and they are pure
+            # finite-difference round-off, a few ulps of eta divided by the
+            # perturbation 2e-5 |P| (centres) or 2e-7 |P| (vertices), i.e. up
+            # to ~1e-7 of eta here; 1e-6 of the viscosity row bounds them
+            noise = 1e-6 * eta_scale if r % 4 == 3 else 0.0
+            np.testing.assert_allclose(D_g[r], D_o[r], rtol=1e-6, atol=max(1e-6 * scale, noise),
+                                       err_msg="row %d" % r)
+
+
+def test_jacobian_is_derivative_of_residual(oracle_mod):
+    """Smooth rheology (no yielding): J v equals the directional derivative
+    of the discrete residual F(u, v, p) of the Picard loop."""
+    from oracle.nonlinear import jacobian_fd_apply
+    O = oracle_mod
+    g, st, ctx, o_ph, g_ph, common = _setup(O, 23, C=50.0)
+    rng = np.random.default_rng(2)
+    _perturbed_state(O, g, st, rng)
+    Dn_o, Ds_o, Dn_g, Ds_g = _tangents(O, g, st, o_ph, g_ph, common, ctx)
+    num = g.numbering()
+    rx, rz = O.eval_rhs(g, 0.0, -1.0, st["rho_n"], st["rho_s"])
+    md = O.OrcRheoModel(**common)
+    for trial in range(3):
+        x = rng.standard_normal(num[5])
+        x[num[3]:] -= x[num[3]:].mean()
+        jv = ctx.jacobian_apply(Dn_g, Ds_g, x)
+        fd = jacobian_fd_apply(g, num, st, md, o_ph, 0, rx, rz, x, 1e-6)
+        err = np.linalg.norm(jv - fd) / np.linalg.norm(fd)
+        print("trial", trial, "rel err", err)
+        assert err < 1e-5
+
+
+def test_newton_loop_converges_to_picard_solution(oracle_mod):
+    """Picard then Newton (with the forced line search) reaches the same
+    non-linear solution as the oracle's Picard iterations driven to a tight
+    tolerance, in fewer iterations."""
+    from mdgpu.nonlinear import PicardLoop
+    from oracle.nonlinear import picard_loop
+    O = oracle_mod
+    n = 23
+    g, st, ctx, o_ph, g_ph, common = _setup(O, n, C=50.0)
+    st_gpu = {k: v.copy() for k, v in st.items()}
+    tol = (1e-14, 1e-11, 1e-14, 1e-11)
+    hist_o, ns_o = picard_loop(g, st, O.OrcRheoModel(**common), o_ph, 60, tol, diag_scaling=1,
+                               gz=-1.0, penalty=1e3, max_its_PH=50, rel_tol_KSP=1e-12,
+                               max_its_KSP=200, lin_abs_div=1e-14, lin_rel_div=1e-13,
+                               lin_abs_mom=1e-14, lin_rel_mom=1e-13)
+    eu, ev, ep = g.numbering()[:3]
+    loop = PicardLoop(ctx, eu, ev, ep, g_ph, mdgpu.RheoParams(average=0, **common))
+    # inexact Newton: each linear solve to 1e-10 (the loop's own test is on
+    # the non-linear residual, 1e-11 relative)
+    hist_g, ns_g, stats = loop.run(st_gpu, 30, tol, diag_scaling=1, gz=-1.0, newton=True,
+                                   picard2newton_tol=1e-2, max_its_pic=5, rel_tol=1e-10,
+                                   max_its=400)
+    print("oracle Picard solves", ns_o, "GPU Picard+Newton solves", ns_g, "Newton at",
+          st_gpu["newton_steps"], "alphas", st_gpu.get("alphas"))
+    print("GPU history", np.array(hist_g)[:, 0])
+    assert st_gpu["newton_steps"], "the loop never switched to Newton"
+    assert ns_g < ns_o
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(st_gpu["u"], st["u"]) < 1e-8
+    assert rel(st_gpu["v"], st["v"]) < 1e-8
+    assert rel(st_gpu["p"], st["p"]) < 1e-8

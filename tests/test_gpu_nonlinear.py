@@ -2,8 +2,9 @@
 
 Both loops start from the same markers-derived phase fractions, the same
 old stresses and the pure-shear boundary velocities, run the non-linear
-iterations of Main_DOODZ.c:878-1062 (Picard, alpha = -1, diagonal-scaled
-residual norms) and must agree on the number of iterations and linear
+iterations of Main_DOODZ.c:878-1062 (Picard, alpha = -1 or the line search
+of StokesRoutines.c:582-807, diagonal-scaled or plain residual norms) and
+must agree on the number of iterations, the chosen step lengths and linear
 solves, the residual history (1e-6 relative; the norms of late iterations
 are differences of nearly equal stresses), and the final velocity and
 pressure to 1e-8 relative L2.
@@ -38,14 +39,15 @@ def _problem(O, n, seed):
     return g, st
 
 
-@pytest.mark.parametrize("diag_scaling", [1, 0])
-def test_picard_loop_matches_oracle(oracle_mod, diag_scaling):
+@pytest.mark.parametrize("diag_scaling,line_search,C", [(1, False, 1.0), (0, False, 1.0),
+                                                        (1, True, 2.0)])
+def test_picard_loop_matches_oracle(oracle_mod, diag_scaling, line_search, C):
     O = oracle_mod
     from oracle.nonlinear import picard_loop
     n = 31
     g, st = _problem(O, n, seed=4)
     st_gpu = {k: v.copy() for k, v in st.items()}
-    phs = S.vep_phases(C=1.0)
+    phs = S.vep_phases(C=C)
     common = dict(step=1, elastic=1, dt=1.0, R=1.0, zeroC=0.0, min_eta=1e-4, max_eta=1e4,
                   eta_tol=1e-13, Tiimin=1e-5, scale_E=1.0, scale_eta=1.0)
     o_ph = []
@@ -54,7 +56,8 @@ def test_picard_loop_matches_oracle(oracle_mod, diag_scaling):
         b = d.pop("beta")
         o_ph.append(O.OrcPhase(beta_for_K=b, **d))
     hist, nsolve = picard_loop(g, st, O.OrcRheoModel(**common), o_ph, 10, TOL, average=0,
-                               diag_scaling=diag_scaling, gz=-1.0, penalty=1e3, max_its_PH=50,
+                               diag_scaling=diag_scaling, gz=-1.0, line_search_on=line_search,
+                               penalty=1e3, max_its_PH=50,
                                rel_tol_KSP=1e-12, max_its_KSP=200, lin_abs_div=1e-14,
                                lin_rel_div=1e-12, lin_abs_mom=1e-14, lin_rel_mom=1e-12)
     ctx = mdgpu.Context(n + 1, n + 1, 0.0, 1.0, -1.0, 0.0)
@@ -63,7 +66,9 @@ def test_picard_loop_matches_oracle(oracle_mod, diag_scaling):
     loop = PicardLoop(ctx, eu, ev, ep, [mdgpu.PhaseProp(**d) for d in phs],
                       mdgpu.RheoParams(average=0, **common))
     ghist, gsolve, stats = loop.run(st_gpu, 10, TOL, diag_scaling=diag_scaling, gz=-1.0,
-                                    rel_tol=1e-11, max_its=400)
+                                    line_search=line_search, rel_tol=1e-11, max_its=400)
+    if line_search:   # LineSearchDecoupled picks the same step lengths
+        assert st_gpu["alphas"] == st["alphas"], (st_gpu["alphas"], st["alphas"])
     print("oracle history", np.array(hist)[:, 0], "gpu", np.array(ghist)[:, 0])
     assert gsolve == nsolve and len(ghist) == len(hist)
     np.testing.assert_allclose(np.array(ghist), np.array(hist), rtol=1e-6, atol=1e-15)

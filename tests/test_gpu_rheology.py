@@ -136,3 +136,29 @@ def test_strain_rate_node_averages(oracle_mod):
     np.testing.assert_allclose(out["exz_n"], oracle_mod.exz_n(g, sr["exz"]), rtol=0, atol=1e-13)
     for k, r in (("exxd_s", exs), ("ezzd_s", ezs), ("div_s", dvs)):
         np.testing.assert_allclose(out[k], r, rtol=0, atol=1e-13, err_msg=k)
+
+
+@pytest.mark.parametrize("centroid", [1, 0])
+@pytest.mark.parametrize("style", [0, 1, 2])
+def test_material_grid_matches_oracle(oracle_mod, centroid, style):
+    g, ctx = _ctx(oracle_mod)
+    n = ctx.node_count(centroid)
+    rng = np.random.default_rng(20 + style + 3 * centroid)
+    vals = [dict(G=1.0, bet=2.0, C=1.0, C_end=0.2, phi=0.5, phi_end=0.1, psi=0.2, psi_end=0.0,
+                 pls_start=0.1, pls_end=1.1, H_c=-0.3, phi_soft=1, psi_soft=1, coh_soft=1),
+            dict(G=3.0, bet=0.5, C=2.0, C_end=1.0, phi=0.6, phi_end=0.3, psi=0.1, psi_end=0.05,
+                 pls_start=0.0, pls_end=0.5, H_c=0.2, phi_soft=0, psi_soft=1, coh_soft=1)]
+    o_ph = [oracle_mod.OrcPhaseSoft(**v) for v in vals]
+    g_ph = [mdgpu.PhaseSoft(G=v["G"], beta=v["bet"], C=v["C"], C_end=v["C_end"], fric=v["phi"],
+                            fric_end=v["phi_end"], dil=v["psi"], dil_end=v["psi_end"],
+                            pls_start=v["pls_start"], pls_end=v["pls_end"], H_c=v["H_c"],
+                            fric_soft=v["phi_soft"], dil_soft=v["psi_soft"],
+                            coh_soft=v["coh_soft"]) for v in vals]
+    f = rng.uniform(0, 1, n)
+    perc = np.stack([f, 1 - f])
+    epl = rng.uniform(0, 2, n)
+    ref = oracle_mod.material_grid(o_ph, perc, epl, tag=(g.BCp_t if centroid else g.BCg_t),
+                                   centroid=centroid, style=style)
+    out = ctx.material_grid(g_ph, centroid, perc, epl, style=style)
+    for k in ("mu", "bet", "C", "fric", "dil"):
+        np.testing.assert_allclose(out[k], ref[k], rtol=1e-14, atol=0, err_msg=k)

@@ -203,3 +203,85 @@ def test_exz_n_reproduces_linear_field():
     exz = (0.3 * X - 1.2 * Z + 0.1).ravel()
     xc, zc = np.meshgrid(g.xc, g.zc)
     np.testing.assert_allclose(O.exz_n(g, exz), (0.3 * xc - 1.2 * zc + 0.1).ravel(), atol=1e-14)
+
+
+def _soft(**kw):
+    d = dict(G=1.0, bet=1.0, C=1.0, C_end=0.2, phi=0.5, phi_end=0.1, psi=0.2, psi_end=0.0,
+             pls_start=0.1, pls_end=1.1, H_c=0.0, phi_soft=1, psi_soft=1, coh_soft=1)
+    d.update(kw)
+    return O.OrcPhaseSoft(**d)
+
+
+def test_material_grid_harmonic_moduli():
+    perc = np.array([[0.25, 1.0, 0.0], [0.75, 0.0, 1.0]])
+    out = O.material_grid([_soft(G=2.0, bet=4.0), _soft(G=8.0, bet=1.0)], perc)
+    np.testing.assert_allclose(out["mu"], [1 / (0.25 / 2 + 0.75 / 8), 2.0, 8.0], rtol=1e-15)
+    np.testing.assert_allclose(out["bet"], [1 / (0.25 / 4 + 0.75), 4.0, 1.0], rtol=1e-15)
+
+
+def test_linear_softening_closed_form():
+    """style 0: piecewise linear in the accumulated strain, with the
+    reference's slope times e (RheologyDensity.c:1478-1497)."""
+    e = np.array([0.0, 0.05, 0.1, 0.6, 1.1, 3.0])
+    out = O.material_grid([_soft()], np.ones((1, e.size)), strain_pl=e, style=0)
+    slope = (0.2 - 1.0) / 1.0
+    want = np.where(e < 0.1, 1.0, np.where(e >= 1.1, 0.2, 1.0 + slope * e))
+    np.testing.assert_allclose(out["C"], want, rtol=1e-15)
+    np.testing.assert_allclose(out["fric"], np.where(e < 0.1, 0.5, np.where(e >= 1.1, 0.1, 0.5 - 0.4 * e)),
+                               rtol=1e-15)
+    # no softening flag keeps the initial value
+    out = O.material_grid([_soft(coh_soft=0)], np.ones((1, e.size)), strain_pl=e, style=0)
+    np.testing.assert_array_equal(out["C"], 1.0)
+
+
+def test_erfc_softening_limits_and_hardening():
+    from math import erfc
+    e = np.array([0.0, 0.6, 50.0])
+    out = O.material_grid([_soft()], np.ones((1, 3)), strain_pl=e, style=1)
+    # exactly the initial value at zero strain, the end value far beyond pls_end
+    assert out["C"][0] == pytest.approx(1.0, rel=1e-14)
+    assert out["C"][2] == pytest.approx(0.2, rel=1e-12)
+    mu, ds, d = 0.6, 1.0, 0.8
+    f = lambda x: 1.0 - d / 2 * erfc(-(x - mu) / ds) - 0.2
+    assert out["C"][1] == pytest.approx(f(0.6) * d / f(0.0) + 0.2, rel=1e-14)
+    out = O.material_grid([_soft(H_c=-0.5)], np.ones((1, 3)), strain_pl=e, style=2)
+    np.testing.assert_allclose(out["C"], np.maximum(1.0 - 0.5 * e, 0.2), rtol=1e-15)
+
+
+def test_tangent_constant_viscosity():
+    n = 4
+    D = O.tangent_nodes(O.rheo_model(), [O.phase(eta0=3.0)], np.ones((1, n)), np.ones(n),
+                        np.ones(n), np.zeros(n), np.full(n, 0.3), np.full(n, -0.2),
+                        np.full(n, 0.1))
+    np.testing.assert_allclose(D[0], 6.0, rtol=1e-12)
+    np.testing.assert_allclose(D[5], 6.0, rtol=1e-12)
+    np.testing.assert_allclose(D[[1, 2, 3, 4, 6, 7]], 0.0, atol=1e-9)
+    Dv = O.tangent_nodes(O.rheo_model(), [O.phase(eta0=3.0)], np.ones((1, n)), np.ones(n),
+                         np.ones(n), np.zeros(n), np.full(n, 0.3), np.full(n, -0.2),
+                         np.full(n, 0.1), centroid=0)
+    np.testing.assert_allclose(Dv[2], 3.0, rtol=1e-12)
+    np.testing.assert_allclose(Dv[[0, 1, 3]], 0.0, atol=1e-9)
+
+
+@pytest.mark.parametrize("centroid", [1, 0])
+def test_tangent_power_law_analytic(centroid):
+    """eta = B E^(1/n-1) exp(P V/(n R T)): derivatives in closed form."""
+    nn, A, V, R, T = 3.0, 2.0, 0.4, 1.0, 1.0
+    ph = O.phase(cstv=0, pwlv=1, n_pwl=nn, A_pwl=A, V_pwl=V)
+    Exx, Ezz, Exz, P = 0.7, -0.3, 0.25, 0.5
+    D = O.tangent_nodes(O.rheo_model(R=R), [ph], np.ones((1, 1)), [1.0], [T], [P], [Exx], [Ezz],
+                        [Exz], centroid=centroid)
+    E = math.sqrt(0.5 * (Exx ** 2 + Ezz ** 2 + (Exx + Ezz) ** 2) + Exz ** 2)
+    B = A ** (-1 / nn) * math.exp(P * V / (nn * R * T))
+    eta = B * E ** (1 / nn - 1)
+    dEta = B * (1 / nn - 1) * E ** (1 / nn - 2)
+    de_xx = dEta * (2 * Exx + Ezz) / (2 * E)
+    de_zz = dEta * (Exx + 2 * Ezz) / (2 * E)
+    de_g = dEta * Exz / (2 * E)               # w.r.t. gxz = 2 Exz
+    de_p = eta * V / (nn * R * T)
+    if centroid:
+        want = [2 * eta + 2 * de_xx * Exx, 2 * de_zz * Exx, 2 * de_g * Exx, 2 * de_p * Exx,
+                2 * de_xx * Ezz, 2 * eta + 2 * de_zz * Ezz, 2 * de_g * Ezz, 2 * de_p * Ezz]
+    else:
+        want = [2 * de_xx * Exz, 2 * de_zz * Exz, eta + 2 * de_g * Exz, 2 * de_p * Exz]
+    np.testing.assert_allclose(D[:, 0], want, rtol=1e-7)
