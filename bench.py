@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--kernels", type=int, default=0, metavar="REPS",
                     help="profiling mode: only the P2G calls, the fine-level smoother and a "
                          "copy kernel (calibration of the HBM byte counters), REPS times each")
+    ap.add_argument("--trace-solve", type=int, default=0, metavar="ITS",
+                    help="profiling mode: P2G then ITS outer iterations of the solve with the "
+                         "inner CG launched kernel by kernel (no HIP graphs), for kernel tracers")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding is "
@@ -115,6 +118,21 @@ def main():
 
     if a.kernels:
         kernels_only(ctx, a.kernels, step, f_cent, f_vert, D11, en, es, F, nm, sol, eta_s, rho_s, n)
+        return
+    if a.trace_solve:
+        ctx.p2mastah_fused(1, 1, f_cent)
+        ctx.p2mastah_fused(0, 1, f_vert)
+        ctx.eval_rhs(0.0, -9.81, rn, rs, rgx, rgz)
+        ctx.stokes_residual(zc, zc, zv, zc, zc, rgx, rgz, F=F)
+        ctx.scale_into(en, 2.0, D11)
+        try:
+            o = ctx.killer_solver_dev(D11, D11, es, F, nm, sol, max_its=a.trace_solve, graphs=1)
+        except mdgpu.MdgError as e:
+            if e.code != -6:
+                raise
+            o = None
+        print(json.dumps({"trace_solve_its": a.trace_solve,
+                          "inner_its": o.inner_its if o else None}), flush=True)
         return
     for _ in range(a.warmup):
         step()

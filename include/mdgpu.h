@@ -265,7 +265,11 @@ typedef struct {
     int    coarse_cells;   /* coarsen until <= this many cells per side (8)    */
     int    precision;      /* MDG_PREC_FP64 or MDG_PREC_MIXED                  */
     int    precond;        /* 0 multigrid; 1 multigrid CG started from the
-                              learned network's guess (mdg_pcnet_*)            */
+                              learned network's guess; 2 two more outer
+                              directions per iteration from the network
+                              applied to the residual; 3 the network's
+                              correction first, multigrid applied to what it
+                              leaves (mdg_pcnet_*)                             */
     int    cycle;          /* 1 V-cycle, 2 W-cycle (1)                         */
     int    eta_avg;        /* coarse viscosity: 0 geometric, 1 arithmetic,
                               2 harmonic, 3 maximum (0)                        */
@@ -277,11 +281,18 @@ typedef struct {
     int    poisson_max_its;/* (30)                                             */
     int    smoother;       /* 0 fused three-term Chebyshev kernels, 1 the
                               residual/update two-kernel form (0)              */
+    int    graphs;         /* 0 replay each inner CG iteration as a captured
+                              HIP graph, 1 launch its kernels one by one (for
+                              profilers that do not trace graph nodes) (0)     */
     /* outputs */
     int    its, inner_its, poisson_its, levels, status;
     double rel_res, time_ms;
-    int    guess_used;     /* inner solves started from the network's guess    */
-    double guess_gain;     /* mean ||r0||^2/||b||^2 after those guesses        */
+    int    guess_used;     /* precond 1: inner solves started from the
+                              network's guess; 2: network directions used      */
+    double guess_gain;     /* precond 1: mean ||r0||^2/||b||^2 after the
+                              guesses; 2: mean ||R||^2 after/before each
+                              iteration's network directions; 3: mean
+                              ||R - K z0||^2/||R||^2 of the network part       */
 } mdg_solver_opts;
 int mdg_killer_solver(mdg_ctx *ctx, mdg_solver_opts *opts, const double *D11_n,
                       const double *D22_n, const double *D33_s, const double *rhs_mom,
@@ -340,6 +351,12 @@ int mdg_pcnet_adam(mdg_ctx *ctx, float lr);
 int mdg_pcnet_capture(mdg_ctx *ctx, int max_pairs);
 int mdg_pcnet_captured(mdg_ctx *ctx, int *n, int *H, int *W);
 int mdg_pcnet_get_pair(mdg_ctx *ctx, int i, float *x, float *t, float *mask);
+/* As mdg_pcnet_capture, for the outer level: the next solves record, at
+ * every outer iteration k, the network input built from the residual R_k and,
+ * once the solve has converged to x*, the target x* - x_k (velocity and
+ * pressure parts each over its RMS).  Pairs of a solve that does not
+ * converge are dropped.  These pairs train the network for precond = 2. */
+int mdg_pcnet_capture_outer(mdg_ctx *ctx, int max_pairs);
 /* steps Adam updates on the captured pairs, batch pairs per step (chosen
  * by a counter hash of seed), learning rate lr; mean loss per step. */
 int mdg_pcnet_train(mdg_ctx *ctx, int steps, int batch, float lr, unsigned long long seed,

@@ -103,18 +103,20 @@ __global__ void __launch_bounds__(256) k_p2g(p2g_geo g, p2g_set fs, const unsign
 }
 
 // Row-tiled variant for the 1-cell stencil.  A block owns P2G_TILE
-// consecutive nodes of one node row; the markers of the cell rows those nodes
-// draw from are contiguous in sorted order, so the block streams them with
-// coalesced loads, turns each into (home node, w, w*value) in LDS, and each
-// node thread then sums its own quarter-cell ranges out of LDS in the same
-// marker order as k_p2g (so both give bit-identical sums).  Consecutive rows
-// are handed to the same XCD (blocks are dispatched round-robin over the 8
-// XCDs), so a cell row shared by two vertex rows is read from HBM once and
-// from that XCD's L2 the second time.
+// consecutive nodes of ROWS node rows; the markers of the cell rows those
+// nodes draw from are contiguous in sorted order, so the block streams them
+// with coalesced loads, turns each into (home node, w, w*value) in LDS once,
+// and each node thread then sums its own quarter-cell ranges out of LDS in
+// the same marker order as k_p2g.  Node sets whose rows sit on cell edges
+// (vertices, Vz nodes) take half of each of two cell rows; with ROWS = 4
+// rows per block a cell row is computed (K+1)/K = 1.25 times instead of
+// twice.  Consecutive tiles are handed to the same XCD (blocks are
+// dispatched round-robin over the 8 XCDs), so a cell row shared by two
+// blocks is read from HBM once and from that XCD's L2 the second time.
 #define P2G_TILE 256
 template <int NF> struct tile_cap { static constexpr int v = (NF <= 2) ? 2560 : 1536; };
 
-template <int NF>
+template <int NF, int ROWS>
 __global__ void __launch_bounds__(P2G_TILE)
 k_p2g_tile(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff, const double *__restrict__ mx,
            const double *__restrict__ mz, const int *__restrict__ mph, int need_phase, int nbx,
@@ -127,29 +129,44 @@ k_p2g_tile(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff, const doubl
     // contiguous logical tiles [start(x), start(x) + count(x)).
     const int b = blockIdx.x, xcd = b & 7, per = nblocks >> 3, rem = nblocks & 7;
     const int tile = xcd * per + min(xcd, rem) + (b >> 3);
-    const int l = tile / nbx, k0 = (tile - l * nbx) * P2G_TILE;
+    const int lt = tile / nbx, k0 = (tile - lt * nbx) * P2G_TILE;
+    const int l0 = lt * ROWS, l1 = min(l0 + ROWS, g.nz);     // node rows [l0, l1)
     const int kend = min(k0 + P2G_TILE, g.nx);
     const int t = threadIdx.x, k = k0 + t;
     const bool own = k < kend;
-    // this node's marker ranges (at most 4 for the 1-cell stencil)
-    unsigned rs[4], re[4];
-    int nr = 0;
-    if (own)
-        for_node_ranges(qoff, g.Ncx, g.Ncz, g.ax, k, l, [&](unsigned s, unsigned e) {
-            if (nr < 4) { rs[nr] = s; re[nr] = e; nr++; }
-        });
+    // each node's marker ranges (at most 4 for the 1-cell stencil)
+    unsigned rs[ROWS][4], re[ROWS][4];
+    int nr[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+        nr[r] = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) { rs[r][q] = 0u; re[r][q] = 0u; }
+        if (own && l0 + r < l1)
+            for_node_ranges(qoff, g.Ncx, g.Ncz, g.ax, k, l0 + r, [&](unsigned s0, unsigned e0) {
+                if (nr[r] < 4) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        if (q == nr[r]) { rs[r][q] = s0; re[r][q] = e0; }
+                    nr[r]++;
+                }
+            });
+    }
     // the block's cell rows and column span
     int cx0, cx1, cz0, cz1;
     if (g.ax.xkind == CELL_AXIS) { cx0 = k0 + g.ax.xoff; cx1 = kend - 1 + g.ax.xoff; }
     else { cx0 = k0 - 1; cx1 = kend - 1; }
-    if (g.ax.zkind == CELL_AXIS) { cz0 = cz1 = l + g.ax.zoff; }
-    else { cz0 = l - 1; cz1 = l; }
+    if (g.ax.zkind == CELL_AXIS) { cz0 = l0 + g.ax.zoff; cz1 = l1 - 1 + g.ax.zoff; }
+    else { cz0 = l0 - 1; cz1 = l1 - 1; }
     cx0 = max(cx0, 0); cx1 = min(cx1, g.Ncx - 1);
     cz0 = max(cz0, 0); cz1 = min(cz1, g.Ncz - 1);
-    const int node = l * g.nx + k;
-    double W = 0.0, BW[NF];
+    double W[ROWS], BW[ROWS][NF];
 #pragma unroll
-    for (int f = 0; f < NF; f++) BW[f] = 0.0;
+    for (int r = 0; r < ROWS; r++) {
+        W[r] = 0.0;
+#pragma unroll
+        for (int f = 0; f < NF; f++) BW[r][f] = 0.0;
+    }
     bool bad = false;
     for (int cz = cz0; cz <= cz1 && cx0 <= cx1; cz++) {
         const unsigned S = qoff[4u * (unsigned)(cx0 + cz * g.Ncx)];
@@ -200,14 +217,20 @@ k_p2g_tile(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff, const doubl
                 }
             }
             __syncthreads();
-            for (int r = 0; r < nr; r++) {
-                const unsigned a = max(rs[r], cs), e = min(re[r], ce);
-                for (unsigned i = a; i < e; i++) {
-                    const unsigned q = i - cs;
-                    if (lhome[q] != node) { bad = true; continue; }
-                    W += lw[q];
 #pragma unroll
-                    for (int f = 0; f < NF; f++) BW[f] += lwv[f][q];
+            for (int r = 0; r < ROWS; r++) {
+                const int node = (l0 + r) * g.nx + k;
+#pragma unroll
+                for (int qr = 0; qr < 4; qr++) {
+                    if (qr >= nr[r]) continue;
+                    const unsigned a = max(rs[r][qr], cs), e = min(re[r][qr], ce);
+                    for (unsigned i = a; i < e; i++) {
+                        const unsigned q = i - cs;
+                        if (lhome[q] != node) { bad = true; continue; }
+                        W[r] += lw[q];
+#pragma unroll
+                        for (int f = 0; f < NF; f++) BW[r][f] += lwv[f][q];
+                    }
                 }
             }
         }
@@ -215,14 +238,19 @@ k_p2g_tile(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff, const doubl
     if (bad) *stray = 1;
     if (!own) return;
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-        const p2g_fld &F = fs.f[f];
-        if (fabs(W) < 1e-30) continue;
-        if (F.ntype && (F.ntype[node] == 30 || F.ntype[node] == 31)) continue;
-        double v = BW[f] / W;
-        if (F.avg == 1) v = 1.0 / v;
-        if (F.avg == 2) v = exp(v);
-        F.dst[node] = v;
+    for (int r = 0; r < ROWS; r++) {
+        if (l0 + r >= l1) break;
+        const int node = (l0 + r) * g.nx + k;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const p2g_fld &F = fs.f[f];
+            if (fabs(W[r]) < 1e-30) continue;
+            if (F.ntype && (F.ntype[node] == 30 || F.ntype[node] == 31)) continue;
+            double v = BW[r][f] / W[r];
+            if (F.avg == 1) v = 1.0 / v;
+            if (F.avg == 2) v = exp(v);
+            F.dst[node] = v;
+        }
     }
 }
 
@@ -282,15 +310,19 @@ static int launch_fields(mdg_ctx *c, p2g_geo g, const p2g_set &s, int nf) {
         // 1-cell stencil, exact homes: row-tiled kernel
         int need_phase = 0;
         for (int f = 0; f < nf; f++) need_phase |= (s.f[f].flag == 0);
-        const int nbx = (g.nx + P2G_TILE - 1) / P2G_TILE, nb = nbx * g.nz;
+        // node rows on cell edges in z draw from two cell rows: 4 rows a block
+        const int rows = (g.ax.zkind == VERT_AXIS) ? 4 : 1;
+        const int nbx = (g.nx + P2G_TILE - 1) / P2G_TILE, nb = nbx * ((g.nz + rows - 1) / rows);
         dim3 grid(nb), block(P2G_TILE);
-#define TILE_LAUNCH(NF) hipLaunchKernelGGL(k_p2g_tile<NF>, grid, block, 0, c->stream, g, s, c->qoff.p, c->mx.p, c->mz.p, c->mphase.p, need_phase, nbx, nb, c->flag_dev.p)
+#define TILE_LAUNCH(NF, R) hipLaunchKernelGGL((k_p2g_tile<NF, R>), grid, block, 0, c->stream, g, s, c->qoff.p, c->mx.p, c->mz.p, c->mphase.p, need_phase, nbx, nb, c->flag_dev.p)
+#define TILE_ROWS(NF) do { if (rows == 4) TILE_LAUNCH(NF, 4); else TILE_LAUNCH(NF, 1); } while (0)
         switch (nf) {
-        case 1: TILE_LAUNCH(1); break;
-        case 2: TILE_LAUNCH(2); break;
-        case 3: TILE_LAUNCH(3); break;
-        default: TILE_LAUNCH(4); break;
+        case 1: TILE_ROWS(1); break;
+        case 2: TILE_ROWS(2); break;
+        case 3: TILE_ROWS(3); break;
+        default: TILE_ROWS(4); break;
         }
+#undef TILE_ROWS
 #undef TILE_LAUNCH
         HIPCHK(c, hipGetLastError());
         return MDG_OK;

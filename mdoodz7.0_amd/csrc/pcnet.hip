@@ -293,6 +293,8 @@ struct pc_layer {
 }  // namespace
 
 // Parameters, optimiser state and activation buffers of the network.
+#define OUTER_PEND_MAX 256   // outer pairs kept per solve
+
 struct pcnet_state {
     pc_layer L[NL];
     long nparam = 0;
@@ -310,6 +312,12 @@ struct pcnet_state {
     int cap_on = 0, cap_max = 0, cap_n = 0, cap_next = 0, cap_pending = 0, cap_H = 0, cap_W = 0;
     dbuf<float> cap_x, cap_t, cap_m;
     dbuf<double> cap_scale;
+    // outer-level capture (kind 1): iterates x_k of the running solve, kept
+    // until the converged x* gives the targets x* - x_k
+    int cap_kind = 0;
+    int pend_n = 0;
+    int pend_slot[OUTER_PEND_MAX];
+    dbuf<double> pend_x, oscale;
 };
 
 namespace {
@@ -565,6 +573,7 @@ void mdg_pcnet_release(mdg_ctx *c) {
     s->act.release(); s->y.release(); s->gbuf.release(); s->tbuf.release(); s->mbuf.release();
     s->part.release(); s->lpart.release(); s->loss.release(); s->guess_scale.release();
     s->cap_x.release(); s->cap_t.release(); s->cap_m.release(); s->cap_scale.release();
+    s->pend_x.release(); s->oscale.release();
     delete s;
     c->pcnet = nullptr;
 }
@@ -828,7 +837,7 @@ int mdg_pcnet_guess(mdg_ctx *c, const sgrid &g, const V *b, const T *dinv, const
 template <typename V, typename T>
 int mdg_pcnet_capture_in(mdg_ctx *c, const sgrid &g, const V *b, const T *dinv, const T *en) {
     pcnet_state *s = c->pcnet;
-    if (!s || !s->cap_on || s->cap_pending) return MDG_OK;
+    if (!s || !s->cap_on || s->cap_kind != 0 || s->cap_pending) return MDG_OK;
     int H, W;
     image_dims(g, &H, &W);
     if (s->cap_n > 0 && (H != s->cap_H || W != s->cap_W))
@@ -855,7 +864,7 @@ int mdg_pcnet_capture_in(mdg_ctx *c, const sgrid &g, const V *b, const T *dinv, 
 template <typename T>
 int mdg_pcnet_capture_out(mdg_ctx *c, const sgrid &g, const T *x) {
     pcnet_state *s = c->pcnet;
-    if (!s || !s->cap_on || !s->cap_pending) return MDG_OK;
+    if (!s || !s->cap_on || s->cap_kind != 0 || !s->cap_pending) return MDG_OK;
     const int H = s->cap_H, W = s->cap_W, slot = s->cap_next;
     const long img = (long)H * W * 4;
     // the target is the solution's direction: x scaled to unit RMS over the
@@ -881,6 +890,244 @@ template int mdg_pcnet_capture_in<double, double>(mdg_ctx *, const sgrid &, cons
 template int mdg_pcnet_capture_in<double, float>(mdg_ctx *, const sgrid &, const double *, const float *, const float *);
 template int mdg_pcnet_capture_out<double>(mdg_ctx *, const sgrid &, const double *);
 
+
+// ------------------------------------------------- outer (FGCR) directions
+namespace {
+
+// Sums of squares of the velocity part (free faces, optionally times dinv)
+// and of the pressure part (cells, optionally times pfac * eta_n) of a
+// grid-layout vector: part[b] and part[nb + b].
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_sumsq2(sgrid g, const double *__restrict__ v, const T *__restrict__ dinv, const double *__restrict__ en,
+         double pfac, double *__restrict__ part) {
+    __shared__ double sh[2][256];
+    const int nu = nvx(g), nvel = nu + nvz(g), n = nvel + ncell(g);
+    double au = 0.0, ap = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        if (i < nvel) {
+            const double a = (dinv ? (double)dinv[i] : 1.0) * v[i];
+            au += a * a;
+        } else {
+            const double a = (en ? pfac * en[i - nvel] : 1.0) * v[i];
+            ap += a * a;
+        }
+    }
+    sh[0][threadIdx.x] = au;
+    sh[1][threadIdx.x] = ap;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sh[0][0];
+        part[gridDim.x + blockIdx.x] = sh[1][0];
+    }
+}
+// scale[0] = RMS of the velocity part over the free faces, scale[1] = RMS of
+// the pressure part over the cells (1 where zero)
+__global__ void k_scale2_from(int nb, const double *__restrict__ part, double nfree, double ncells,
+                              double *__restrict__ scale) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < nb; i++) { a += part[i]; b += part[nb + i]; }
+    a = sqrt(a / nfree);
+    b = sqrt(b / ncells);
+    scale[0] = (a > 0.0) ? a : 1.0;
+    scale[1] = (b > 0.0) ? b : 1.0;
+}
+
+// Input image of cell (x, y) from the augmented residual R: log10(eta_n)/3;
+// dinv*R_u on the W face and dinv*R_v on the S face over scale[0] (the
+// Jacobi-preconditioned momentum residual); pfac*eta_n*R_p over scale[1]
+// (the Schur-approximated continuity residual).
+template <typename T, bool F32>
+__global__ void k_pack_outer(sgrid g, int H, int W, double pfac, const double *__restrict__ R,
+                             const T *__restrict__ dinv, const double *__restrict__ en,
+                             const double *__restrict__ scale, void *__restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)H * W) return;
+    const int y = (int)(i / W), x = (int)(i - (long)y * W);
+    const int Ncx = g.Nx - 1, Ncz = g.Nz - 1, nu = nvx(g), nvel = nu + nvz(g);
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f;
+    if (x < Ncx && y < Ncz) {
+        const int cc = x + y * Ncx;
+        const double iu = 1.0 / scale[0];
+        v0 = (float)(log10(en[cc]) / 3.0);
+        const int ju = x + (y + 1) * g.Nx;
+        if (vx_free(g, x, y + 1)) v1 = (float)((double)dinv[ju] * R[ju] * iu);
+        const int jv = nu + (x + 1) + y * (g.Nx + 1);
+        if (vz_free(g, x + 1, y)) v2 = (float)((double)dinv[jv] * R[jv] * iu);
+        v3 = (float)(pfac * en[cc] * R[nvel + cc] / scale[1]);
+    }
+    if (F32) {
+        float *o = (float *)out + i * 4;
+        o[0] = v0; o[1] = v1; o[2] = v2; o[3] = v3;
+    } else {
+        bf16_t *o = (bf16_t *)out + i * 4;
+        o[0] = f2bf(v0); o[1] = f2bf(v1); o[2] = f2bf(v2); o[3] = f2bf(v3);
+    }
+}
+// Target image and mask of an outer pair from the error e = x* - x_k:
+// e_u (W face), e_v (S face) over scale[0], e_p over scale[1].
+__global__ void k_pack_outer_target(sgrid g, int H, int W, const double *__restrict__ e,
+                                    const double *__restrict__ scale, float *__restrict__ t,
+                                    float *__restrict__ m) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)H * W) return;
+    const int y = (int)(i / W), x = (int)(i - (long)y * W);
+    const int Ncx = g.Nx - 1, Ncz = g.Nz - 1, nu = nvx(g), nvel = nu + nvz(g);
+    float t0 = 0.f, t1 = 0.f, t2 = 0.f, m0 = 0.f, m1 = 0.f, m2 = 0.f;
+    if (x < Ncx && y < Ncz) {
+        const int ju = x + (y + 1) * g.Nx;
+        if (vx_free(g, x, y + 1)) { t0 = (float)(e[ju] / scale[0]); m0 = 1.f; }
+        const int jv = nu + (x + 1) + y * (g.Nx + 1);
+        if (vz_free(g, x + 1, y)) { t1 = (float)(e[jv] / scale[0]); m1 = 1.f; }
+        t2 = (float)(e[nvel + x + y * Ncx] / scale[1]);
+        m2 = 1.f;
+    }
+    float *to = t + i * 4, *mo = m + i * 4;
+    to[0] = t0; to[1] = t1; to[2] = t2; to[3] = 0.f;
+    mo[0] = m0; mo[1] = m1; mo[2] = m2; mo[3] = 0.f;
+}
+// Network output -> two grid-layout directions: zu (velocity channels on
+// the free faces, zero pressure) and zp (pressure channel, zero velocity).
+__global__ void k_unpack_outer(sgrid g, int H, int W, const float *__restrict__ y, double *__restrict__ zu,
+                               double *__restrict__ zp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu = nvx(g), nvel = nu + nvz(g), n = nvel + ncell(g), Ncx = g.Nx - 1;
+    if (i >= n) return;
+    double a = 0.0, b = 0.0;
+    if (i < nu) {
+        const int k = i % g.Nx, l = i / g.Nx;
+        if (vx_free(g, k, l) && l - 1 < H && k < W) a = y[((long)(l - 1) * W + k) * 4 + 0];
+    } else if (i < nvel) {
+        const int j = i - nu, k = j % (g.Nx + 1), l = j / (g.Nx + 1);
+        if (vz_free(g, k, l) && l < H && k - 1 < W) a = y[((long)l * W + (k - 1)) * 4 + 1];
+    } else {
+        const int j = i - nvel, x = j % Ncx, yy = j / Ncx;
+        b = y[((long)yy * W + x) * 4 + 2];
+    }
+    zu[i] = a;
+    zp[i] = b;
+}
+__global__ void k_err(long n, const double *__restrict__ xs, double *__restrict__ xk) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) xk[i] = xs[i] - xk[i];
+}
+
+template <typename T>
+int outer_scales(mdg_ctx *c, pcnet_state *s, const sgrid &g, const double *v, const T *dinv, const double *en,
+                 double pfac, double *scale) {
+    const int nb = 256;
+    HIPCHK(c, s->lpart.reserve(2 * nb));
+    hipLaunchKernelGGL(k_sumsq2<T>, dim3(nb), dim3(256), 0, c->stream, g, v, dinv, en, pfac, s->lpart.p);
+    hipLaunchKernelGGL(k_scale2_from, dim3(1), dim3(1), 0, c->stream, nb, (const double *)s->lpart.p, free_faces(g),
+                       (double)ncell(g), scale);
+    HIPCHK(c, hipGetLastError());
+    return MDG_OK;
+}
+
+}  // namespace
+
+int mdg_pcnet_outer_capturing(mdg_ctx *c) {
+    return c->pcnet && c->pcnet->cap_on && c->pcnet->cap_kind == 1;
+}
+
+template <typename T>
+int mdg_pcnet_outer_dirs(mdg_ctx *c, const sgrid &g, double gamma, const double *R, const T *dinv,
+                         const double *en, double *zu, double *zp) {
+    pcnet_state *s = net_of(c);
+    int H, W, rc;
+    image_dims(g, &H, &W);
+    if ((rc = check_dims(c, H, W)) || (rc = ensure_buffers(c, s, H, W))) return rc;
+    HIPCHK(c, s->oscale.reserve(2));
+    const double pfac = (1.0 + gamma) / (g.dx * g.dz);
+    if ((rc = outer_scales<T>(c, s, g, R, dinv, en, pfac, s->oscale.p))) return rc;
+    hipLaunchKernelGGL((k_pack_outer<T, false>), dim3(nblk256((long)H * W)), dim3(256), 0, c->stream, g, H, W, pfac, R,
+                       dinv, en, (const double *)s->oscale.p, (void *)A(s, A_X));
+    if ((rc = forward(c, s))) return rc;
+    const int n = nvx(g) + nvz(g) + ncell(g);
+    hipLaunchKernelGGL(k_unpack_outer, dim3(nblk256(n)), dim3(256), 0, c->stream, g, H, W, (const float *)s->y.p, zu, zp);
+    HIPCHK(c, hipGetLastError());
+    return MDG_OK;
+}
+
+template <typename T>
+int mdg_pcnet_outer_capture_in(mdg_ctx *c, const sgrid &g, double gamma, const double *R, const T *dinv,
+                               const double *en, const double *x) {
+    pcnet_state *s = c->pcnet;
+    if (!s || !s->cap_on || s->cap_kind != 1 || s->pend_n >= OUTER_PEND_MAX) return MDG_OK;
+    int H, W;
+    image_dims(g, &H, &W);
+    if (s->cap_n > 0 && (H != s->cap_H || W != s->cap_W))
+        return mdg_fail(c, MDG_ERR_ARG, "captured pairs must share one grid size (%dx%d vs %dx%d)", H, W, s->cap_H, s->cap_W);
+    const long img = (long)H * W * 4;
+    const long n = (long)nvx(g) + nvz(g) + ncell(g);
+    if (s->cap_n == 0 && s->pend_n == 0) {
+        s->cap_H = H;
+        s->cap_W = W;
+        HIPCHK(c, s->cap_x.reserve(img * s->cap_max));
+        HIPCHK(c, s->cap_t.reserve(img * s->cap_max));
+        HIPCHK(c, s->cap_m.reserve(img * s->cap_max));
+        HIPCHK(c, s->cap_scale.reserve(2L * s->cap_max));
+    }
+    HIPCHK(c, s->pend_x.reserve(n * OUTER_PEND_MAX));
+    // the ring slot this pair will take once the solve has converged
+    const int slot = (s->cap_next + s->pend_n) % s->cap_max;
+    if (s->pend_n >= s->cap_max) return MDG_OK;
+    const double pfac = (1.0 + gamma) / (g.dx * g.dz);
+    int rc;
+    if ((rc = outer_scales<T>(c, s, g, R, dinv, en, pfac, s->cap_scale.p + 2L * slot))) return rc;
+    hipLaunchKernelGGL((k_pack_outer<T, true>), dim3(nblk256((long)H * W)), dim3(256), 0, c->stream, g, H, W, pfac, R,
+                       dinv, en, (const double *)(s->cap_scale.p + 2L * slot), (void *)(s->cap_x.p + img * slot));
+    HIPCHK(c, hipMemcpyAsync(s->pend_x.p + n * s->pend_n, x, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipGetLastError());
+    s->pend_slot[s->pend_n++] = slot;
+    return MDG_OK;
+}
+
+int mdg_pcnet_outer_capture_done(mdg_ctx *c, const sgrid &g, const double *xstar, int converged) {
+    pcnet_state *s = c->pcnet;
+    if (!s || !s->cap_on || s->cap_kind != 1) return MDG_OK;
+    if (!converged) {       // no reference solution: drop the pending pairs
+        s->pend_n = 0;
+        return MDG_OK;
+    }
+    const long img = (long)s->cap_H * s->cap_W * 4;
+    const long n = (long)nvx(g) + nvz(g) + ncell(g);
+    HIPCHK(c, s->oscale.reserve(2));
+    for (int q = 0; q < s->pend_n; q++) {
+        double *e = s->pend_x.p + n * q;
+        const int slot = s->pend_slot[q];
+        hipLaunchKernelGGL(k_err, dim3(nblk256(n)), dim3(256), 0, c->stream, n, xstar, e);
+        int rc = outer_scales<double>(c, s, g, e, (const double *)nullptr, (const double *)nullptr, 1.0, s->oscale.p);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_pack_outer_target, dim3(nblk256(img / 4)), dim3(256), 0, c->stream, g, s->cap_H, s->cap_W,
+                           (const double *)e, (const double *)s->oscale.p, s->cap_t.p + img * slot,
+                           s->cap_m.p + img * slot);
+        HIPCHK(c, hipGetLastError());
+    }
+    s->cap_next = (s->cap_next + s->pend_n) % s->cap_max;
+    s->cap_n = std::min(s->cap_max, s->cap_n + s->pend_n);
+    s->pend_n = 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDG_OK;
+}
+
+template int mdg_pcnet_outer_dirs<double>(mdg_ctx *, const sgrid &, double, const double *, const double *,
+                                          const double *, double *, double *);
+template int mdg_pcnet_outer_dirs<float>(mdg_ctx *, const sgrid &, double, const double *, const float *,
+                                         const double *, double *, double *);
+template int mdg_pcnet_outer_capture_in<double>(mdg_ctx *, const sgrid &, double, const double *, const double *,
+                                                const double *, const double *);
+template int mdg_pcnet_outer_capture_in<float>(mdg_ctx *, const sgrid &, double, const double *, const float *,
+                                               const double *, const double *);
+
 extern "C" int mdg_pcnet_capture(mdg_ctx *c, int max_pairs) {
     if (!c || max_pairs < 0) return mdg_fail(c, MDG_ERR_ARG, "max_pairs must be >= 0");
     pcnet_state *s = net_of(c);
@@ -896,6 +1143,15 @@ extern "C" int mdg_pcnet_capture(mdg_ctx *c, int max_pairs) {
     s->cap_on = 1;
     s->cap_n = 0;
     s->cap_next = 0;
+    s->cap_kind = 0;
+    s->pend_n = 0;
+    return MDG_OK;
+}
+
+extern "C" int mdg_pcnet_capture_outer(mdg_ctx *c, int max_pairs) {
+    int rc = mdg_pcnet_capture(c, max_pairs);
+    if (rc) return rc;
+    if (max_pairs > 0) c->pcnet->cap_kind = 1;
     return MDG_OK;
 }
 

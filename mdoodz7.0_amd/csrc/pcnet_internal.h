@@ -17,3 +17,19 @@ template <typename V, typename T>
 int mdg_pcnet_capture_in(mdg_ctx *c, const sgrid &g, const V *b, const T *dinv, const T *en);
 template <typename T>
 int mdg_pcnet_capture_out(mdg_ctx *c, const sgrid &g, const T *x);
+
+// Outer (FGCR) level.  From the augmented residual R (grid layout, FP64) the
+// network gives two search directions: zu (velocity, zero pressure) and zp
+// (pressure, zero velocity).  Input channels: log10(eta_n)/3, the
+// Jacobi-scaled momentum residual on the W and S faces, the Schur-scaled
+// continuity residual (1+gamma) eta_n/dV R_p, each part over its RMS.
+template <typename T>
+int mdg_pcnet_outer_dirs(mdg_ctx *c, const sgrid &g, double gamma, const double *R, const T *dinv,
+                         const double *en, double *zu, double *zp);
+// Outer training pairs: input from R_k with the iterate x_k kept at each
+// iteration; once the solve has converged to x*, targets x* - x_k.
+int mdg_pcnet_outer_capturing(mdg_ctx *c);
+template <typename T>
+int mdg_pcnet_outer_capture_in(mdg_ctx *c, const sgrid &g, double gamma, const double *R, const T *dinv,
+                               const double *en, const double *x);
+int mdg_pcnet_outer_capture_done(mdg_ctx *c, const sgrid &g, const double *xstar, int converged);

@@ -129,15 +129,14 @@ __global__ void __launch_bounds__(RB) k_pcg_step(int n, const T *__restrict__ p,
                                                  T *__restrict__ x, T *__restrict__ r,
                                                  const double *__restrict__ scal, double *__restrict__ part) {
     __shared__ double s[RB];
-    const int i = blockIdx.x * RB + threadIdx.x;
     const double pq = scal[2];
     const T a = (pq > 0.0) ? (T)(scal[1] / pq) : T(0);
     double acc = 0.0;
-    if (i < n) {
+    for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
         x[i] += a * p[i];
         const T ri = r[i] - a * q[i];
         r[i] = ri;
-        acc = (double)ri * (double)ri;
+        acc += (double)ri * (double)ri;
     }
     s[threadIdx.x] = acc;
     __syncthreads();
@@ -172,20 +171,21 @@ __global__ void __launch_bounds__(RB) k_vel_apply_dot(sgrid g, T gamma, const T 
                                                       const T *__restrict__ en, const T *__restrict__ es,
                                                       T *__restrict__ y, double *__restrict__ part) {
     __shared__ double s[RB];
-    const int i = blockIdx.x * RB + threadIdx.x;
     const int nu = nvx(g), nv = nvz(g);
     const T *u = p, *v = p + nu;
     double acc = 0.0;
-    if (i < nu) {
-        const int k = i % g.Nx, l = i / g.Nx;
-        const T q = vx_free(g, k, l) ? apply_vx_row<T>(g, k, l, u, v, en, es, gamma, nullptr) : T(0);
-        y[i] = q;
-        acc = (double)q * (double)p[i];
-    } else if (i < nu + nv) {
-        const int j = i - nu, k = j % (g.Nx + 1), l = j / (g.Nx + 1);
-        const T q = vz_free(g, k, l) ? apply_vz_row<T>(g, k, l, u, v, en, es, gamma, nullptr) : T(0);
-        y[i] = q;
-        acc = (double)q * (double)p[i];
+    for (int i = blockIdx.x * RB + threadIdx.x; i < nu + nv; i += gridDim.x * RB) {
+        if (i < nu) {
+            const int k = i % g.Nx, l = i / g.Nx;
+            const T q = vx_free(g, k, l) ? apply_vx_row<T>(g, k, l, u, v, en, es, gamma, nullptr) : T(0);
+            y[i] = q;
+            acc += (double)q * (double)p[i];
+        } else {
+            const int j = i - nu, k = j % (g.Nx + 1), l = j / (g.Nx + 1);
+            const T q = vz_free(g, k, l) ? apply_vz_row<T>(g, k, l, u, v, en, es, gamma, nullptr) : T(0);
+            y[i] = q;
+            acc += (double)q * (double)p[i];
+        }
     }
     s[threadIdx.x] = acc;
     __syncthreads();
@@ -233,8 +233,8 @@ __global__ void __launch_bounds__(256) k_cheb_res(sgrid g, T gamma, const T *__r
 // XCD b % 8; the tile order gives XCD x a contiguous band of rows, so the
 // rows above and below a tile (stencil halo, both velocity components and
 // the viscosities) are read from the same XCD's L2.
-#define CHEB_TX 256
-#define CHEB_TZ 4
+#define CHEB_TX 128
+#define CHEB_TZ 1
 
 struct tile_grid {
     int nbx, ntiles;
@@ -254,7 +254,7 @@ __device__ inline int xcd_tile(int b, int ntiles) {
 // is not read.  xo holds x_{k-1} on entry and x_{k+1} on exit (each entry
 // is read and written by its own thread only).
 template <typename T, bool FIRST>
-__global__ void __launch_bounds__(CHEB_TX) k_cheb3(sgrid g, tile_grid tg, T gamma, T a, T cc,
+__global__ void __launch_bounds__(CHEB_TX) k_cheb3(sgrid g, stc<T> cf, tile_grid tg, T gamma, T a, T cc,
                                                    const T *__restrict__ xc, T *__restrict__ xo,
                                                    const T *__restrict__ b, const T *__restrict__ en,
                                                    const T *__restrict__ es) {
@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(CHEB_TX) k_cheb3(sgrid g, tile_grid tg, T gamm
             T y = T(0);
             if (vx_free(g, k, l)) {
                 T dg;
-                const T ax = apply_vx_row<T>(g, k, l, u, v, en, es, gamma, &dg);
+                const T ax = apply_vx_rcp<T>(g, cf, k, l, u, v, en, es, gamma, &dg);
                 const T xi = u[i];
                 y = xi + cc * (bu[i] - ax) / dg;
                 if (!FIRST) y += a * (xi - uo[i]);
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(CHEB_TX) k_cheb3(sgrid g, tile_grid tg, T gamm
             T y = T(0);
             if (vz_free(g, k, l)) {
                 T dg;
-                const T ax = apply_vz_row<T>(g, k, l, u, v, en, es, gamma, &dg);
+                const T ax = apply_vz_rcp<T>(g, cf, k, l, u, v, en, es, gamma, &dg);
                 const T xi = v[j];
                 y = xi + cc * (bv[j] - ax) / dg;
                 if (!FIRST) y += a * (xi - vo[j]);
@@ -327,10 +327,11 @@ __global__ void k_scale_dinv(int n, T s, const T *__restrict__ dinv, const T *__
 // coarse bracket clamped to [0, nc - 2].  Shared by P and R = P^T.
 struct axis_map {
     double hf, hc, off_f, off_c;
-    int nc;   // number of coarse coordinates
+    int nc;       // number of coarse coordinates
+    double rat;   // hf / hc
 };
 __host__ __device__ inline void bracket(const axis_map &m, int f, int *K0, double *w1) {
-    double t = ((f + m.off_f) * m.hf) / m.hc - m.off_c;
+    double t = (f + m.off_f) * m.rat - m.off_c;
     int k0 = (int)floor(t);
     if (k0 < 0) k0 = 0;
     if (k0 > m.nc - 2) k0 = m.nc - 2;
@@ -373,25 +374,32 @@ __device__ inline T coarse_val(const comp_maps &m, const sgrid &gc, const T *__r
     }
 }
 
-// x_f += P x_c for both components
+// (P x_c) at fine node (k, l) of one component
+template <typename T>
+__device__ __forceinline__ T prolong_at(const comp_maps &m, const sgrid &gc, const T *__restrict__ src, int k, int l) {
+    int K0, L0;
+    double wx, wz;
+    bracket(m.xm, k, &K0, &wx);
+    bracket(m.zm, l, &L0, &wz);
+    const T a = coarse_val<T>(m, gc, src, K0, L0), b = coarse_val<T>(m, gc, src, K0 + 1, L0);
+    const T c = coarse_val<T>(m, gc, src, K0, L0 + 1), d = coarse_val<T>(m, gc, src, K0 + 1, L0 + 1);
+    return T((1 - wx) * (1 - wz)) * a + T(wx * (1 - wz)) * b + T((1 - wx) * wz) * c + T(wx * wz) * d;
+}
+// x_f += P x_c for both components (each branch reads its own parameter
+// struct directly: selecting one through a pointer would copy both kernel
+// arguments to scratch memory)
 template <typename T>
 __global__ void k_prolong(sgrid gf, sgrid gc, comp_maps mu, comp_maps mv, const T *__restrict__ xc,
                           T *__restrict__ xf) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int nuf = nvx(gf), nvf = nvz(gf), nuc = nvx(gc);
-    const comp_maps *m;
-    int k, l;
-    const T *src;
-    if (i < nuf) { k = i % gf.Nx; l = i / gf.Nx; if (!vx_free(gf, k, l)) return; m = &mu; src = xc; }
-    else if (i < nuf + nvf) { const int j = i - nuf; k = j % (gf.Nx + 1); l = j / (gf.Nx + 1); if (!vz_free(gf, k, l)) return; m = &mv; src = xc + nuc; }
-    else return;
-    int K0, L0;
-    double wx, wz;
-    bracket(m->xm, k, &K0, &wx);
-    bracket(m->zm, l, &L0, &wz);
-    const T a = coarse_val<T>(*m, gc, src, K0, L0), b = coarse_val<T>(*m, gc, src, K0 + 1, L0);
-    const T c = coarse_val<T>(*m, gc, src, K0, L0 + 1), d = coarse_val<T>(*m, gc, src, K0 + 1, L0 + 1);
-    xf[i] += T((1 - wx) * (1 - wz)) * a + T(wx * (1 - wz)) * b + T((1 - wx) * wz) * c + T(wx * wz) * d;
+    if (i < nuf) {
+        const int k = i % gf.Nx, l = i / gf.Nx;
+        if (vx_free(gf, k, l)) xf[i] += prolong_at<T>(mu, gc, xc, k, l);
+    } else if (i < nuf + nvf) {
+        const int j = i - nuf, k = j % (gf.Nx + 1), l = j / (gf.Nx + 1);
+        if (vz_free(gf, k, l)) xf[i] += prolong_at<T>(mv, gc, xc + nuc, k, l);
+    }
 }
 
 // Sum of P weights times the fine residual onto coarse node (K, L) itself
@@ -407,6 +415,26 @@ __device__ inline double restrict_node(const comp_maps &m, const sgrid &gf, cons
     k0 = k0 < 0 ? 0 : k0; l0 = l0 < 0 ? 0 : l0;
     k1 = k1 > m.nxf - 1 ? m.nxf - 1 : k1; l1 = l1 > m.nzf - 1 ? m.nzf - 1 : l1;
     double s = 0.0;
+    // the x weights of the window, once per node (the window spans at most
+    // 2 hc/hf + 4 <= 8 fine columns; wider windows take the direct loop)
+    constexpr int RW = 10;
+    if (k1 - k0 + 1 <= RW) {
+        double wxs[RW];
+#pragma unroll
+        for (int q = 0; q < RW; q++) wxs[q] = (k0 + q <= k1) ? pweight(m.xm, k0 + q, K) : 0.0;
+        for (int l = l0; l <= l1; l++) {
+            const double wz = pweight(m.zm, l, L);
+            if (wz == 0.0) continue;
+#pragma unroll
+            for (int q = 0; q < RW; q++) {
+                const int k = k0 + q;
+                if (k > k1 || wxs[q] == 0.0) continue;
+                const bool fr = (m.comp == 0) ? vx_free(gf, k, l) : vz_free(gf, k, l);
+                if (fr) s += wxs[q] * wz * (double)rf[k + l * m.nxf];
+            }
+        }
+        return s;
+    }
     for (int l = l0; l <= l1; l++) {
         const double wz = pweight(m.zm, l, L);
         if (wz == 0.0) continue;
@@ -836,6 +864,7 @@ struct solver_state {
     mg_hier<double> hd;
     mg_hier<float> hf;
     dbuf<double> X, R, Rt, B, Zp, V, Z;   // outer FGCR (V, Z: restart slots)
+    dbuf<double> Nu, Np, Ku, Kp;          // learned outer directions and K times them
     dbuf<double> part, scal;
     int m = 0;
     size_t nfull = 0;
@@ -849,6 +878,7 @@ void mdg_solver_release(mdg_ctx *c) {
     s->hd.release(); s->hf.release();
     s->X.release(); s->R.release(); s->Rt.release(); s->B.release(); s->Zp.release();
     s->V.release(); s->Z.release(); s->part.release(); s->scal.release();
+    s->Nu.release(); s->Np.release(); s->Ku.release(); s->Kp.release();
     delete s;
     c->solver = nullptr;
 }
@@ -866,7 +896,7 @@ static int dot_host(mdg_ctx *c, solver_state *s, int n, const T *a, const T *b, 
     return MDG_OK;
 }
 
-static axis_map make_axis(double hf, double hc, double off, int nc) { return axis_map{hf, hc, off, off, nc}; }
+static axis_map make_axis(double hf, double hc, double off, int nc) { return axis_map{hf, hc, off, off, nc, hf / hc}; }
 
 // ------------------------------------------------------------------ setup
 static void host_gauss_jordan_inverse(std::vector<double> &A, int n, bool *ok) {
@@ -1152,7 +1182,8 @@ static void remove_mean(mdg_ctx *c, mg_hier<T> &H, T *x, int n, T sign) {
 template <typename T>
 static int poisson_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solver_opts *o, int *its) {
     const int n = ncell(H.L[0].g);
-    if (!H.pexec) {
+    const bool graphs = (o->graphs == 0);
+    if (graphs && !H.pexec) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         poisson_iteration(c, H, o->cheb_degree);
         HIPCHK(c, hipStreamEndCapture(c->stream, &H.pgraph));
@@ -1167,7 +1198,12 @@ static int poisson_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_s
     if (rr0 == 0.0) return MDG_OK;
     const double tol2 = o->poisson_tol * o->poisson_tol * rr0;
     for (int it = 0; it < o->poisson_max_its; it++) {
-        HIPCHK(c, hipGraphLaunch(H.pexec, c->stream));
+        if (graphs) {
+            HIPCHK(c, hipGraphLaunch(H.pexec, c->stream));
+        } else {
+            poisson_iteration(c, H, o->cheb_degree);
+            HIPCHK(c, hipGetLastError());
+        }
         *its += 1;
         HIPCHK(c, hipMemcpyAsync(H.h_pscal, H.pscal.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1214,17 +1250,18 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
     if (fused) {
         // three-term form on two iterate buffers: lv.x and lv.d
         const tile_grid tg = make_tiles(lv.g);
+        const stc<T> cf = make_stc<T>(lv.g);
         T *cur = lv.d.p, *prev = lv.x.p;
         if (zero_start)
             hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)(1.0 / theta),
                                (const T *)lv.dinv.p, (const T *)lv.b.p, cur, prev);
         else
-            hipLaunchKernelGGL((k_cheb3<T, true>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, lv.g, tg, (T)gamma, T(0),
+            hipLaunchKernelGGL((k_cheb3<T, true>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, lv.g, cf, tg, (T)gamma, T(0),
                                (T)(1.0 / theta), (const T *)prev, cur, (const T *)lv.b.p, (const T *)lv.en.p,
                                (const T *)lv.es.p);
         for (int k = 0; k < deg - 1; k++) {
             const double rho_new = 1.0 / (2.0 * sigma - rho);
-            hipLaunchKernelGGL((k_cheb3<T, false>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, lv.g, tg, (T)gamma,
+            hipLaunchKernelGGL((k_cheb3<T, false>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, lv.g, cf, tg, (T)gamma,
                                (T)(rho_new * rho), (T)(2.0 * rho_new / delta), (const T *)cur, prev,
                                (const T *)lv.b.p, (const T *)lv.en.p, (const T *)lv.es.p);
             std::swap(cur, prev);
@@ -1280,7 +1317,8 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, i
 template <typename T>
 static void pcg_iteration(mdg_ctx *c, mg_hier<T> &H, const mdg_solver_opts *o) {
     mg_level<T> &l0 = H.L[0];
-    const int n = l0.n, nb = mdg_grid1d(n, RB), nd = std::min(1024, nb);
+    // reduction kernels run at most 8192 blocks (fewer partials to sum)
+    const int n = l0.n, nb = std::min(8192, mdg_grid1d(n, RB)), nd = std::min(1024, nb);
     const double gamma = o->penalty;
     double *x = H.cg_x.p, *r = H.cg_r.p, *p = H.cg_p.p, *q = H.cg_q.p;
     hipLaunchKernelGGL((k_convert<double, T>), dim3(nblk(n)), dim3(256), 0, c->stream, n, (const double *)r, l0.b.p);
@@ -1313,7 +1351,8 @@ static int inner_pcg(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solve
     HIPCHK(c, H.part.reserve((size_t)nb + 1024));
     HIPCHK(c, H.scal.reserve(8));
     if (!H.h_scal) HIPCHK(c, hipHostMalloc((void **)&H.h_scal, 8 * sizeof(double), hipHostMallocDefault));
-    if (!H.exec) {
+    const bool graphs = (o->graphs == 0);
+    if (graphs && !H.exec) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         pcg_iteration(c, H, o);
         HIPCHK(c, hipStreamEndCapture(c->stream, &H.graph));
@@ -1363,7 +1402,12 @@ static int inner_pcg(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solve
     }
     const int every = o->check_every > 0 ? o->check_every : 1;
     for (int it = 0; it < o->inner_max_its; it++) {
-        HIPCHK(c, hipGraphLaunch(H.exec, c->stream));
+        if (graphs) {
+            HIPCHK(c, hipGraphLaunch(H.exec, c->stream));
+        } else {
+            pcg_iteration(c, H, o);
+            HIPCHK(c, hipGetLastError());
+        }
         *its = it + 1;
         if ((it + 1) % every && it + 1 < o->inner_max_its) continue;
         HIPCHK(c, hipMemcpyAsync(H.h_scal, H.scal.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1412,10 +1456,23 @@ static int fgcr_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opt
     const double bnorm = sqrt(bb);
     o->its = 0; o->inner_its = 0; o->poisson_its = 0; o->rel_res = 0.0;
     if (bnorm == 0.0) return MDG_OK;
+    const bool learned = (o->precond == 2) && mdg_pcnet_ready(c);
+    const bool learned_mult = (o->precond == 3) && mdg_pcnet_ready(c);
+    if (learned || learned_mult) { HIPCHK(c, s->Nu.reserve(n)); HIPCHK(c, s->Np.reserve(n)); }
+    if (learned_mult) { HIPCHK(c, s->Ku.reserve(n)); HIPCHK(c, s->Kp.reserve(n)); }
+    auto apply_K = [&](const double *z, double *y) -> int {
+        if (mdg_newton_active(c)) return mdg_newton_apply_grid(c, o->penalty, z, y);
+        hipLaunchKernelGGL(k_apply_Kg, dim3(nblk(n)), dim3(256), 0, c->stream, g, o->penalty, z, c->eta_n.p, c->eta_s.p, y);
+        return MDG_OK;
+    };
     int slot = 0;
-    for (int it = 0; it < o->max_its; it++) {
+    bool done = false;
+    // One GCR step with the direction already stored in Z slot `slot`:
+    // orthogonalise K z against the stored directions, then x += (v.R) z and
+    // R -= (v.R) v; sets done when the original system's residual meets the
+    // tolerance.
+    auto gcr_step = [&]() -> int {
         double *v = s->V.p + (size_t)slot * n, *z = s->Z.p + (size_t)slot * n;
-        if ((rc = precondition<T>(c, s, H, o, R, z, &o->inner_its))) return rc;
         // Search directions keep a zero-mean pressure: the closed box leaves
         // one pressure (Picard) or pressure-velocity (Newton) mode free, and
         // the reference's Powell-Hestenes increments are zero-sum
@@ -1427,37 +1484,98 @@ static int fgcr_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opt
             hipLaunchKernelGGL(k_sum_store, dim3(1), dim3(RB), 0, c->stream, nb2, (const double *)s->part.p, s->scal.p, 14, -1, -1);
             hipLaunchKernelGGL(k_sub_mean<double>, dim3(nblk(np)), dim3(256), 0, c->stream, np, zp, (const double *)s->scal.p, 14, 1.0);
         }
+        int rc2;
         if (mdg_newton_active(c)) {
-            if ((rc = mdg_newton_apply_grid(c, o->penalty, z, v))) return rc;
+            if ((rc2 = mdg_newton_apply_grid(c, o->penalty, z, v))) return rc2;
         } else {
             hipLaunchKernelGGL(k_apply_Kg, dim3(nblk(n)), dim3(256), 0, c->stream, g, o->penalty, z, c->eta_n.p, c->eta_s.p, v);
         }
         for (int j = 0; j < slot; j++) {
             double a;
-            if ((rc = dot_host<double>(c, s, n, s->V.p + (size_t)j * n, v, &a))) return rc;
+            if ((rc2 = dot_host<double>(c, s, n, s->V.p + (size_t)j * n, v, &a))) return rc2;
             hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, -a, s->V.p + (size_t)j * n, 1.0, v);
             hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, -a, s->Z.p + (size_t)j * n, 1.0, z);
         }
         double vv, vr;
-        if ((rc = dot_host<double>(c, s, n, v, v, &vv))) return rc;
+        if ((rc2 = dot_host<double>(c, s, n, v, v, &vv))) return rc2;
         if (!(vv > 0.0)) return mdg_fail(c, MDG_ERR_DIVERGED, "GCR breakdown");
         const double nv = sqrt(vv);
         hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, 0.0, v, 1.0 / nv, v);
         hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, 0.0, z, 1.0 / nv, z);
-        if ((rc = dot_host<double>(c, s, n, v, R, &vr))) return rc;
+        if ((rc2 = dot_host<double>(c, s, n, v, R, &vr))) return rc2;
         hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, vr, z, 1.0, x);
         hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, -vr, v, 1.0, R);
         // residual of the original (non-augmented) system
         hipLaunchKernelGGL(k_true_residual, dim3(nblk(n)), dim3(256), 0, c->stream, g, o->penalty, R, c->eta_n.p, Rt);
         double rr;
-        if ((rc = dot_host<double>(c, s, n, Rt, Rt, &rr))) return rc;
-        o->its = it + 1;
+        if ((rc2 = dot_host<double>(c, s, n, Rt, Rt, &rr))) return rc2;
         o->rel_res = sqrt(rr) / bnorm;
         if (std::isnan(o->rel_res)) return mdg_fail(c, MDG_ERR_NAN, "NaN in the Stokes solve");
-        if (o->rel_res <= o->rel_tol) return MDG_OK;
-        slot = (slot + 1 == m) ? 0 : slot + 1;
-        if (slot == 0) { /* restart: discard the stored directions */ }
+        done = o->rel_res <= o->rel_tol;
+        slot = (slot + 1 == m) ? 0 : slot + 1;   // wrap = restart
+        return MDG_OK;
+    };
+    const T *dinv0 = H.L[0].dinv.p;
+    for (int it = 0; it < o->max_its && !done; it++) {
+        if (mdg_pcnet_outer_capturing(c) &&
+            (rc = mdg_pcnet_outer_capture_in<T>(c, g, o->penalty, R, dinv0, c->eta_n.p, x)))
+            return rc;
+        if (learned_mult) {
+            // network guess first, multigrid on what it leaves:
+            //   z0 = a N_u(R) + b N_p(R)  (a, b: least squares on R - K z0),
+            //   z  = z0 + M (R - K z0)
+            double *z = s->Z.p + (size_t)slot * n, *r0 = s->B.p;
+            if ((rc = mdg_pcnet_outer_dirs<T>(c, g, o->penalty, R, dinv0, c->eta_n.p, s->Nu.p, s->Np.p)) ||
+                (rc = apply_K(s->Nu.p, s->Ku.p)) || (rc = apply_K(s->Np.p, s->Kp.p)))
+                return rc;
+            double a11, a12, a22, b1, b2, rr;
+            if ((rc = dot_host<double>(c, s, n, s->Ku.p, s->Ku.p, &a11)) ||
+                (rc = dot_host<double>(c, s, n, s->Ku.p, s->Kp.p, &a12)) ||
+                (rc = dot_host<double>(c, s, n, s->Kp.p, s->Kp.p, &a22)) ||
+                (rc = dot_host<double>(c, s, n, s->Ku.p, R, &b1)) || (rc = dot_host<double>(c, s, n, s->Kp.p, R, &b2)) ||
+                (rc = dot_host<double>(c, s, n, R, R, &rr)))
+                return rc;
+            const double det = a11 * a22 - a12 * a12;
+            double a = 0.0, b2c = 0.0;
+            if (det > 1e-14 * a11 * a22 && det > 0.0) {
+                a = (b1 * a22 - b2 * a12) / det;
+                b2c = (a11 * b2 - a12 * b1) / det;
+            }
+            // z0 and r0 = R - K z0
+            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, a, (const double *)s->Nu.p, 0.0, z);
+            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, b2c, (const double *)s->Np.p, 1.0, z);
+            HIPCHK(c, hipMemcpyAsync(r0, R, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, -a, (const double *)s->Ku.p, 1.0, r0);
+            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, -b2c, (const double *)s->Kp.p, 1.0, r0);
+            double r0r0;
+            if ((rc = dot_host<double>(c, s, n, r0, r0, &r0r0))) return rc;
+            s->guess_used += 2;
+            s->guess_gain += (rr > 0.0) ? r0r0 / rr : 1.0;
+            if ((rc = precondition<T>(c, s, H, o, r0, s->Nu.p, &o->inner_its))) return rc;
+            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, 1.0, (const double *)s->Nu.p, 1.0, z);
+        } else if ((rc = precondition<T>(c, s, H, o, R, s->Z.p + (size_t)slot * n, &o->inner_its))) {
+            return rc;
+        }
+        if ((rc = gcr_step())) return rc;
+        o->its = it + 1;
+        if (done || !learned) continue;
+        // learned part: two more directions from the network applied to the
+        // new residual (velocity and pressure parts, each with its own GCR
+        // coefficient); multigrid keeps driving the iteration
+        double r0, r1;
+        if ((rc = dot_host<double>(c, s, n, R, R, &r0))) return rc;
+        if ((rc = mdg_pcnet_outer_dirs<T>(c, g, o->penalty, R, dinv0, c->eta_n.p, s->Nu.p, s->Np.p))) return rc;
+        for (int q = 0; q < 2 && !done; q++) {
+            HIPCHK(c, hipMemcpyAsync(s->Z.p + (size_t)slot * n, q ? s->Np.p : s->Nu.p, (size_t)n * sizeof(double),
+                                     hipMemcpyDeviceToDevice, c->stream));
+            if ((rc = gcr_step())) return rc;
+        }
+        if ((rc = dot_host<double>(c, s, n, R, R, &r1))) return rc;
+        s->guess_used += 2;
+        s->guess_gain += (r0 > 0.0) ? r1 / r0 : 1.0;
     }
+    if (mdg_pcnet_outer_capturing(c) && (rc = mdg_pcnet_outer_capture_done(c, g, x, done ? 1 : 0))) return rc;
+    if (done) return MDG_OK;
     return mdg_fail(c, MDG_ERR_DIVERGED, "no convergence in %d iterations (rel. residual %.3e)", o->max_its, o->rel_res);
 }
 
@@ -1485,6 +1603,9 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     default_opts(o);
     s->guess_used = 0;
     s->guess_gain = 0.0;
+    if (o->precond < 0 || o->precond > 3) return mdg_fail(c, MDG_ERR_ARG, "precond must be 0, 1, 2 or 3");
+    if (o->precond != 0 && !mdg_pcnet_ready(c))
+        return mdg_fail(c, MDG_ERR_STATE, "precond %d needs the network parameters (mdg_pcnet_set_params)", o->precond);
     if (o->precision == MDG_PREC_MIXED && o->schur == 1)
         return mdg_fail(c, MDG_ERR_UNSUPPORTED, "the BFBt Schur approximation runs in FP64 only (precision = MDG_PREC_FP64)");
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -1517,7 +1638,10 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     o->levels = c->solver ? (int)(o->precision == MDG_PREC_MIXED ? s->hf.L.size() : s->hd.L.size()) : 0;
     o->status = rc;
     o->guess_used = s->guess_used;
-    o->guess_gain = s->guess_used ? s->guess_gain / s->guess_used : 0.0;
+    // precond 1: mean ||r0||^2/||b||^2 over the guessed inner solves;
+    // precond 2: mean ||R||^2 ratio across each iteration's network pair
+    const int ng = (o->precond >= 2) ? s->guess_used / 2 : s->guess_used;
+    o->guess_gain = ng ? s->guess_gain / ng : 0.0;
     return rc;
 }
 
@@ -1558,8 +1682,9 @@ extern "C" int mdg_time_smoother(mdg_ctx *c, int reps, double *ms, double *bytes
     mg_level<double> &l0 = c->solver->hd.L[0];
     const int n = l0.n;
     const tile_grid tg = make_tiles(l0.g);
+    const stc<double> cf = make_stc<double>(l0.g);
     auto launch = [&]() {
-        hipLaunchKernelGGL((k_cheb3<double, false>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, l0.g, tg, 10.0, 0.5,
+        hipLaunchKernelGGL((k_cheb3<double, false>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, l0.g, cf, tg, 10.0, 0.5,
                            0.1, (const double *)l0.x.p, l0.d.p, (const double *)l0.b.p, (const double *)l0.en.p,
                            (const double *)l0.es.p);
     };

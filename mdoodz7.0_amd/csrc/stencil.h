@@ -114,6 +114,69 @@ __device__ inline T apply_vz_row(const sgrid &g, int k, int l, const T *__restri
     return y * dV;
 }
 
+// The same two rows with the grid constants precomputed (reciprocals
+// instead of divisions) and the diagonal returned: used by the multigrid
+// smoother, whose result only has to be a good approximate inverse, so the
+// last-bit difference from the divisions of apply_vx_row does not matter.
+template <typename T>
+struct stc {
+    T dV, idx, idz, idx2, idz2, idxz;
+};
+template <typename T>
+inline stc<T> make_stc(const sgrid &g) {
+    return stc<T>{T(g.dx * g.dz), T(1.0 / g.dx), T(1.0 / g.dz), T(1.0 / (g.dx * g.dx)), T(1.0 / (g.dz * g.dz)),
+                  T(1.0 / (g.dx * g.dz))};
+}
+template <typename T>
+__device__ inline T apply_vx_rcp(const sgrid &g, const stc<T> &cf, int k, int l, const T *__restrict__ u,
+                                 const T *__restrict__ v, const T *__restrict__ en, const T *__restrict__ es,
+                                 T gamma, T *diag_out) {
+    const int Nx = g.Nx, Ncx = Nx - 1, NxV = Nx + 1;
+    const int c = k + l * Nx, cW = (k - 1) + (l - 1) * Ncx, cE = k + (l - 1) * Ncx;
+    const T enW = en[cW], enE = en[cE];
+    const T D33S = es[k + (l - 1) * Nx], D33N = es[k + l * Nx];
+    const bool gS = (l == 1), gN = (l == g.Nz - 1);
+    const T fS = ghost_factor<T>(g.ghostS, gS), fN = ghost_factor<T>(g.ghostN, gN);
+    const T regS = gS ? T(0) : T(1), regN = gN ? T(0) : T(1);
+    const T uc = u[c], uw = u[c - 1], ue = u[c + 1];
+    const int sw = k + (l - 1) * NxV, nw = k + l * NxV;
+    const T vsw = v[sw], vse = v[sw + 1], vnw = v[nw], vne = v[nw + 1];
+    const T uC = T(2) * (enE + enW) * cf.idx2 + (D33N * fN + D33S * fS) * cf.idz2;
+    T y = uC * uc - T(2) * cf.idx2 * (enW * uw + enE * ue);
+    y -= cf.idz2 * (D33S * regS * u[c - Nx] + D33N * regN * u[c + Nx]);
+    y += (D33S * (vse - vsw) + D33N * (vnw - vne)) * cf.idxz;
+    // gamma * (eta_W div_W - eta_E div_E) / dx
+    const T divW = (uc - uw) * cf.idx + (vnw - vsw) * cf.idz;
+    const T divE = (ue - uc) * cf.idx + (vne - vse) * cf.idz;
+    y += gamma * (enW * divW - enE * divE) * cf.idx;
+    *diag_out = (uC + gamma * (enW + enE) * cf.idx2) * cf.dV;
+    return y * cf.dV;
+}
+template <typename T>
+__device__ inline T apply_vz_rcp(const sgrid &g, const stc<T> &cf, int k, int l, const T *__restrict__ u,
+                                 const T *__restrict__ v, const T *__restrict__ en, const T *__restrict__ es,
+                                 T gamma, T *diag_out) {
+    const int Nx = g.Nx, Ncx = Nx - 1, NxV = Nx + 1;
+    const int c = k + l * NxV, cS = (k - 1) + (l - 1) * Ncx, cN = (k - 1) + l * Ncx;
+    const T enS = en[cS], enN = en[cN];
+    const T D33W = es[(k - 1) + l * Nx], D33E = es[k + l * Nx];
+    const bool gW = (k == 1), gE = (k == Nx - 1);
+    const T fW = ghost_factor<T>(g.ghostW, gW), fE = ghost_factor<T>(g.ghostE, gE);
+    const T regW = gW ? T(0) : T(1), regE = gE ? T(0) : T(1);
+    const T vc = v[c], vs = v[c - NxV], vn = v[c + NxV];
+    const int s = k + l * Nx;   // Vx(k, l) is SE, Vx(k-1, l) SW
+    const T usw = u[s - 1], use = u[s], unw = u[s - 1 + Nx], une = u[s + Nx];
+    const T vC = (D33E * fE + D33W * fW) * cf.idx2 + T(2) * (enN + enS) * cf.idz2;
+    T y = vC * vc - cf.idx2 * (D33W * regW * v[c - 1] + D33E * regE * v[c + 1]);
+    y -= T(2) * cf.idz2 * (enS * vs + enN * vn);
+    y += (D33W * (unw - usw) + D33E * (use - une)) * cf.idxz;
+    const T divS = (use - usw) * cf.idx + (vc - vs) * cf.idz;
+    const T divN = (une - unw) * cf.idx + (vn - vc) * cf.idz;
+    y += gamma * (enS * divS - enN * divN) * cf.idz;
+    *diag_out = (vC + gamma * (enS + enN) * cf.idz2) * cf.dV;
+    return y * cf.dV;
+}
+
 // Diagonal of A_gamma (Jacobi scaling of the smoothers)
 template <typename T>
 __device__ inline T diag_vx(const sgrid &g, int k, int l, const T *__restrict__ en,

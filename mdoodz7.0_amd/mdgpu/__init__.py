@@ -87,7 +87,8 @@ class SolverOpts(C.Structure):
                 ("cheb_degree", C.c_int), ("coarse_cells", C.c_int), ("precision", C.c_int),
                 ("precond", C.c_int), ("cycle", C.c_int), ("eta_avg", C.c_int),
                 ("check_every", C.c_int), ("schur", C.c_int), ("poisson_tol", C.c_double),
-                ("poisson_max_its", C.c_int), ("smoother", C.c_int), ("its", C.c_int),
+                ("poisson_max_its", C.c_int), ("smoother", C.c_int), ("graphs", C.c_int),
+                ("its", C.c_int),
                 ("inner_its", C.c_int),
                 ("poisson_its", C.c_int),
                 ("levels", C.c_int), ("status", C.c_int), ("rel_res", C.c_double),
@@ -159,6 +160,7 @@ def _extra_signatures(L):
     L.mdg_pcnet_grad.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.c_int, C.POINTER(C.c_double)]
     L.mdg_pcnet_adam.argtypes = [vp, C.c_float]
     L.mdg_pcnet_capture.argtypes = [vp, C.c_int]
+    L.mdg_pcnet_capture_outer.argtypes = [vp, C.c_int]
     L.mdg_pcnet_captured.argtypes = [vp] + [C.POINTER(C.c_int)] * 3
     L.mdg_pcnet_get_pair.argtypes = [vp, C.c_int, vp, vp, vp]
     L.mdg_pcnet_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, vp]
@@ -382,6 +384,9 @@ class Context:
 
     def pcnet_capture(self, max_pairs):
         self._check(self._L.mdg_pcnet_capture(self.h, int(max_pairs)))
+
+    def pcnet_capture_outer(self, max_pairs):
+        self._check(self._L.mdg_pcnet_capture_outer(self.h, int(max_pairs)))
 
     def pcnet_captured(self):
         n, H, W = C.c_int(), C.c_int(), C.c_int()

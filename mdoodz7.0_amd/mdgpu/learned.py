@@ -12,6 +12,13 @@ module only sequences the calls:
 4. solve with precond = 1: every inner multigrid CG starts from the
    network's guess, and the guess is dropped whenever it does not lower the
    residual, so the plain method remains the fallback and the judge.
+
+The outer variant (collect_outer_pairs, precond = 2) captures instead the
+(residual R_k, error x* - x_k) pairs of the outer FGCR iterations and, once
+trained, adds the network's velocity and pressure directions for the current
+residual to every outer iteration; GCR takes each with its own optimal
+coefficient, so a poor direction costs an operator application and no
+accuracy.
 """
 import numpy as np
 
@@ -98,3 +105,11 @@ def collect_pairs(ctx, problems, max_pairs, **opts):
 def train(ctx, steps, batch=4, lr=1e-3, seed=0, init_seed=1):
     ctx.pcnet_set_params(init_params(init_seed))
     return ctx.pcnet_train(steps, batch=batch, lr=lr, seed=seed)
+
+
+def collect_outer_pairs(ctx, problems, max_pairs, **opts):
+    """Plain solves with outer (residual, error) pair capture on."""
+    ctx.pcnet_capture_outer(max_pairs)
+    stats = [p.solve(ctx, precond=0, **opts)[1] for p in problems]
+    ctx.pcnet_capture(0)
+    return stats, ctx.pcnet_captured()[0]

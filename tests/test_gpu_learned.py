@@ -41,3 +41,46 @@ def test_learned_guess_trains_and_keeps_the_solution():
     assert ol.guess_used > 0
     assert np.linalg.norm(xl - xp) <= 1e-6 * np.linalg.norm(xp)
     assert ol.inner_its <= 1.05 * op.inner_its
+
+
+def test_learned_outer_directions_train_and_keep_the_solution():
+    """Outer variant: the network is trained on the (residual, error) pairs
+    of the plain solver's own outer iterations and then adds its velocity
+    and pressure directions to every outer iteration (precond = 2).  GCR
+    weights each direction optimally, so the learned solve reaches the same
+    solution; the outer iteration counts and times of both are reported and
+    the learned solve may not need more multigrid-preconditioned iterations."""
+    n = 63
+    ctx = mdgpu.Context(n + 1, n + 1, 0.0, 1.0, 0.0, 1.0)
+    ctx.set_bc(*L.free_slip_bc(n + 1))
+    probs = [L.EllipseProblem(ctx, n, seed, n_ell=6) for seed in range(1, 10)]
+    stats, npairs = L.collect_outer_pairs(ctx, probs[:8], 2000, rel_tol=1e-8)
+    assert npairs == sum(s.its for s in stats)
+    losses = L.train(ctx, 1500, batch=8, lr=1e-3, seed=3)
+    first, last = losses[:20].mean(), losses[-50:].mean()
+    print("outer pairs", npairs, "loss", first, "->", last)
+    assert last < 0.7 * first
+    test = probs[8]
+    xp, op = test.solve(ctx, precond=0, rel_tol=1e-8)
+    xl, ol = test.solve(ctx, precond=2, rel_tol=1e-8)
+    print("plain: outer %d inner %d %.1f ms | learned: outer %d inner %d %.1f ms, network "
+          "directions %d, residual ratio per iteration pair %.3f | outer gain %.3f" %
+          (op.its, op.inner_its, op.time_ms, ol.its, ol.inner_its, ol.time_ms, ol.guess_used,
+           ol.guess_gain, op.its / max(ol.its, 1)))
+    assert op.rel_res <= 1e-8 and ol.rel_res <= 1e-8
+    assert ol.guess_used > 0
+    assert np.linalg.norm(xl - xp) <= 1e-6 * np.linalg.norm(xp)
+    assert ol.its <= op.its
+
+
+def test_learned_precond_needs_parameters():
+    """precond 1 or 2 without network parameters is an error, not a silent
+    fall back to the plain method."""
+    n = 31
+    ctx = mdgpu.Context(n + 1, n + 1, 0.0, 1.0, 0.0, 1.0)
+    ctx.set_bc(*L.free_slip_bc(n + 1))
+    p = L.EllipseProblem(ctx, n, 1, n_ell=3)
+    for pc in (1, 2):
+        with pytest.raises(mdgpu.MdgError) as e:
+            p.solve(ctx, precond=pc, rel_tol=1e-8)
+        assert e.value.code == -4
