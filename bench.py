@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the fine-level smoother step on 2047^2 cells,
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a copy kernel of
 # known size): profiles/hbm_traffic_p2g_smoother_2047.txt
-SMOOTHER_TRAFFIC_2047 = None
+SMOOTHER_TRAFFIC_2047 = 336.2e6
 METRIC = "Stokes solve time @1e-8 resid, 2047², plain vs learned PC; P2G Mmarkers/s"
 
 
@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--ellipses", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the untimed mixed-precision solve reported beside the FP64 one")
     ap.add_argument("--kernels", type=int, default=0, metavar="REPS",
                     help="profiling mode: only the P2G calls, the fine-level smoother and a "
                          "copy kernel (calibration of the HBM byte counters), REPS times each")
@@ -168,6 +170,18 @@ def main():
                 "kernel_ms_vertices": p2g_ms[1], "achieved_gbs": p2g_gbs,
                 "frac": p2g_gbs / HBM_PEAK_GBS, "fields": 2, "markers": n, "sort_s": t_sort},
     }
+    if not a.no_variants:
+        # same problem with the FP32 multigrid V-cycle (FP64 CG and FGCR
+        # around it), after the timed steps; its error against the FP64 solve
+        x64 = sol.numpy().copy()
+        sol2 = ctx.device(neq, host=np.zeros(neq))
+        o2 = ctx.killer_solver_dev(D11, D11, es, F, nm, sol2, precision=mdgpu.MDG_PREC_MIXED)
+        x32 = sol2.numpy()
+        res["mixed"] = {"precision": "FP32 multigrid V-cycle inside FP64 CG and FP64 FGCR",
+                        "solve_ms": o2.time_ms, "outer_its": o2.its, "inner_its": o2.inner_its,
+                        "rel_res": o2.rel_res,
+                        "rel_l2_vs_fp64": float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64)),
+                        "speedup_vs_fp64": stats["solve_ms"] / o2.time_ms}
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(wl, a, n)
     print(json.dumps(res), flush=True)

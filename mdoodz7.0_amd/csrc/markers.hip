@@ -61,8 +61,10 @@ __global__ void k_marker_index(int n, const int *__restrict__ perm, const double
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = perm[i];
-    ip[s] = near_index(mx[i], X0, dx, nx);
-    jp[s] = near_index(mz[i], Z0, dz, nz);
+    // the same index routine as the P2G kernels, so the bit-exact index
+    // checks (ties, edges) cover the division-free path
+    ip[s] = near_index_fast(mx[i], X0, dx, 1.0 / dx, nx);
+    jp[s] = near_index_fast(mz[i], Z0, dz, 1.0 / dz, nz);
 }
 
 extern "C" int mdg_markers_set(mdg_ctx *c, int n, const double *x, const double *z,

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) k_p2g(p2g_geo g, p2g_set fs, const unsign
 // dispatched round-robin over the 8 XCDs), so a cell row shared by two
 // blocks is read from HBM once and from that XCD's L2 the second time.
 #define P2G_TILE 256
-template <int NF> struct tile_cap { static constexpr int v = (NF <= 2) ? 2560 : 1536; };
+template <int NF> struct tile_cap { static constexpr int v = (NF <= 2) ? 1280 : 768; };
 
 template <int NF, int ROWS>
 __global__ void __launch_bounds__(P2G_TILE)

@@ -254,6 +254,40 @@ __device__ inline int xcd_tile(int b, int ntiles) {
 // is not read.  xo holds x_{k-1} on entry and x_{k+1} on exit (each entry
 // is read and written by its own thread only).
 template <typename T, bool FIRST>
+__device__ __forceinline__ void cheb3_at(const sgrid &g, const stc<T> &cf, T gamma, T a, T cc, const T *__restrict__ xc,
+                                         T *__restrict__ xo, const T *__restrict__ b, const T *__restrict__ en,
+                                         const T *__restrict__ es, int k, int l) {
+    const int nu = nvx(g), NxV = g.Nx + 1;
+    const T *u = xc, *v = xc + nu;
+    T *uo = xo, *vo = xo + nu;
+    {
+        const int i = k + l * g.Nx;
+        T y = T(0);
+        if (vx_free(g, k, l)) {
+            T dg;
+            const T ax = apply_vx_rcp<T>(g, cf, k, l, u, v, en, es, gamma, &dg);
+            const T xi = u[i];
+            y = xi + cc * (b[i] - ax) / dg;
+            if (!FIRST) y += a * (xi - uo[i]);
+        }
+        uo[i] = y;
+    }
+    if (l < g.Nz) {
+        const int j = k + l * NxV;
+        T y = T(0);
+        if (vz_free(g, k, l)) {
+            T dg;
+            const T ax = apply_vz_rcp<T>(g, cf, k, l, u, v, en, es, gamma, &dg);
+            const T xi = v[j];
+            y = xi + cc * (b[nu + j] - ax) / dg;
+            if (!FIRST) y += a * (xi - vo[j]);
+        }
+        vo[j] = y;
+        if (k == g.Nx - 1) vo[j + 1] = T(0);
+    }
+}
+
+template <typename T, bool FIRST>
 __global__ void __launch_bounds__(CHEB_TX) k_cheb3(sgrid g, stc<T> cf, tile_grid tg, T gamma, T a, T cc,
                                                    const T *__restrict__ xc, T *__restrict__ xo,
                                                    const T *__restrict__ b, const T *__restrict__ en,
@@ -261,39 +295,49 @@ __global__ void __launch_bounds__(CHEB_TX) k_cheb3(sgrid g, stc<T> cf, tile_grid
     const int tile = xcd_tile(blockIdx.x, tg.ntiles);
     const int tz = tile / tg.nbx, k = (tile - tz * tg.nbx) * CHEB_TX + threadIdx.x;
     if (k >= g.Nx) return;
-    const int nu = nvx(g), NxV = g.Nx + 1;
-    const T *u = xc, *v = xc + nu;
-    T *uo = xo, *vo = xo + nu;
-    const T *bu = b, *bv = b + nu;
     const int l0 = tz * CHEB_TZ, l1 = min(l0 + CHEB_TZ, g.Nz + 1);
-    for (int l = l0; l < l1; l++) {
-        {
-            const int i = k + l * g.Nx;
-            T y = T(0);
-            if (vx_free(g, k, l)) {
-                T dg;
-                const T ax = apply_vx_rcp<T>(g, cf, k, l, u, v, en, es, gamma, &dg);
-                const T xi = u[i];
-                y = xi + cc * (bu[i] - ax) / dg;
-                if (!FIRST) y += a * (xi - uo[i]);
-            }
-            uo[i] = y;
-        }
-        if (l < g.Nz) {
-            const int j = k + l * NxV;
-            T y = T(0);
-            if (vz_free(g, k, l)) {
-                T dg;
-                const T ax = apply_vz_rcp<T>(g, cf, k, l, u, v, en, es, gamma, &dg);
-                const T xi = v[j];
-                y = xi + cc * (bv[j] - ax) / dg;
-                if (!FIRST) y += a * (xi - vo[j]);
-            }
-            vo[j] = y;
-            if (k == g.Nx - 1) vo[j + 1] = T(0);
-        }
-    }
+    for (int l = l0; l < l1; l++) cheb3_at<T, FIRST>(g, cf, gamma, a, cc, xc, xo, b, en, es, k, l);
 }
+
+// A whole Chebyshev smoothing call on a small level in one workgroup (one
+// launch instead of deg): the same steps as the host loop of chebyshev(),
+// with a workgroup barrier between steps.  x holds x_0 (unless zero_start)
+// and the result; d is the second iterate buffer.
+#define CHEB_BLOCK 1024
+#define CHEB_BLOCK_MAX 2500    // levels up to this many velocity unknowns
+template <typename T>
+__global__ void __launch_bounds__(CHEB_BLOCK) k_cheb_block(sgrid g, stc<T> cf, T gamma, int deg, double theta,
+                                                           double delta, int zero_start, T *__restrict__ x,
+                                                           T *__restrict__ d, const T *__restrict__ b,
+                                                           const T *__restrict__ en, const T *__restrict__ es,
+                                                           const T *__restrict__ dinv) {
+    const int t = threadIdx.x, n = nvx(g) + nvz(g), npos = g.Nx * (g.Nz + 1);
+    const double sigma = theta / delta;
+    T *cur = d, *prev = x;
+    if (zero_start) {
+        for (int i = t; i < n; i += CHEB_BLOCK) {
+            cur[i] = T(1.0 / theta) * dinv[i] * b[i];
+            prev[i] = T(0);
+        }
+    } else {
+        for (int p = t; p < npos; p += CHEB_BLOCK)
+            cheb3_at<T, true>(g, cf, gamma, T(0), T(1.0 / theta), prev, cur, b, en, es, p % g.Nx, p / g.Nx);
+    }
+    __syncthreads();
+    double rho = 1.0 / sigma;
+    for (int s = 0; s < deg - 1; s++) {
+        const double rho_new = 1.0 / (2.0 * sigma - rho);
+        const T a = T(rho_new * rho), cc = T(2.0 * rho_new / delta);
+        for (int p = t; p < npos; p += CHEB_BLOCK)
+            cheb3_at<T, false>(g, cf, gamma, a, cc, cur, prev, b, en, es, p % g.Nx, p / g.Nx);
+        __syncthreads();
+        T *tmp = cur; cur = prev; prev = tmp;
+        rho = rho_new;
+    }
+    if (cur != x)
+        for (int i = t; i < n; i += CHEB_BLOCK) x[i] = cur[i];
+}
+
 // First step from a zero start: x1 = c D^-1 b, and x0 = 0 written out.
 template <typename T>
 __global__ void k_cheb_first(int n, T c, const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ x1,
@@ -1247,6 +1291,12 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
     double rho = 1.0 / sigma;
     const int n = lv.n;
+    if (fused && n <= CHEB_BLOCK_MAX) {
+        hipLaunchKernelGGL(k_cheb_block<T>, dim3(1), dim3(CHEB_BLOCK), 0, c->stream, lv.g, make_stc<T>(lv.g), (T)gamma, deg,
+                           theta, delta, zero_start ? 1 : 0, lv.x.p, lv.d.p, (const T *)lv.b.p, (const T *)lv.en.p,
+                           (const T *)lv.es.p, (const T *)lv.dinv.p);
+        return;
+    }
     if (fused) {
         // three-term form on two iterate buffers: lv.x and lv.d
         const tile_grid tg = make_tiles(lv.g);
