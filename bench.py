@@ -204,6 +204,15 @@ def kernels_only(ctx, reps, step, f_cent, f_vert, D11, en, es, F, nm, sol, eta_s
         ctx.p2mastah_fused(0, 1, f_vert)
         t.append(ctx.last_kernel_ms())
     out["p2g_vertices_ms"] = sorted(t)[len(t) // 2]
+    # the same two fields both arithmetically averaged (no per-marker log)
+    srt = f_cent[0]["flag"]
+    f_arith = [dict(src=eta_s, dst=f_cent[0]["dst"], flag=srt, avg=0),
+               dict(src=rho_s, dst=f_cent[1]["dst"], flag=srt, avg=0)]
+    t = []
+    for _ in range(reps):
+        ctx.p2mastah_fused(1, 1, f_arith)
+        t.append(ctx.last_kernel_ms())
+    out["p2g_centres_arith_ms"] = sorted(t)[len(t) // 2]
     ctx.scale_into(en, 2.0, D11)
     try:
         ctx.killer_solver_dev(D11, D11, es, F, nm, sol, max_its=1)

@@ -361,6 +361,13 @@ int mdg_pcnet_capture_outer(mdg_ctx *ctx, int max_pairs);
  * by a counter hash of seed), learning rate lr; mean loss per step. */
 int mdg_pcnet_train(mdg_ctx *ctx, int steps, int batch, float lr, unsigned long long seed,
                     double *losses);
+/* Training for precond = 2 on outer pairs with the loss the solver sees:
+ * L = min over a, b of ||R - a K z_u - b K z_p||^2 / ||R||^2, z_u and z_p the
+ * network's velocity and pressure directions for the captured residual R and
+ * K the augmented operator of that pair's viscosities.  Same sampling and
+ * Adam as mdg_pcnet_train; losses (may be NULL) get the mean L per step. */
+int mdg_pcnet_train_residual(mdg_ctx *ctx, int steps, int batch, float lr, unsigned long long seed,
+                             double *losses);
 
 /* Times the dominant kernel of the solve: the fine-level fused Chebyshev
  * smoother step (FP64) of the last FP64 solve, averaged over reps

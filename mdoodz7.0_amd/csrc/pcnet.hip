@@ -318,6 +318,10 @@ struct pcnet_state {
     int pend_n = 0;
     int pend_slot[OUTER_PEND_MAX];
     dbuf<double> pend_x, oscale;
+    // per outer pair, for the residual-loss training: R_k, eta_n, eta_s, gamma
+    dbuf<double> cap_R, cap_en, cap_es, cap_gamma;
+    // training work vectors (grid layout)
+    dbuf<double> t_zu, t_zp, t_ku, t_kp, t_r, t_g, t_part;
 };
 
 namespace {
@@ -574,6 +578,9 @@ void mdg_pcnet_release(mdg_ctx *c) {
     s->part.release(); s->lpart.release(); s->loss.release(); s->guess_scale.release();
     s->cap_x.release(); s->cap_t.release(); s->cap_m.release(); s->cap_scale.release();
     s->pend_x.release(); s->oscale.release();
+    s->cap_R.release(); s->cap_en.release(); s->cap_es.release(); s->cap_gamma.release();
+    s->t_zu.release(); s->t_zp.release(); s->t_ku.release(); s->t_kp.release(); s->t_r.release(); s->t_g.release();
+    s->t_part.release();
     delete s;
     c->pcnet = nullptr;
 }
@@ -1086,6 +1093,17 @@ int mdg_pcnet_outer_capture_in(mdg_ctx *c, const sgrid &g, double gamma, const d
     hipLaunchKernelGGL((k_pack_outer<T, true>), dim3(nblk256((long)H * W)), dim3(256), 0, c->stream, g, H, W, pfac, R,
                        dinv, en, (const double *)(s->cap_scale.p + 2L * slot), (void *)(s->cap_x.p + img * slot));
     HIPCHK(c, hipMemcpyAsync(s->pend_x.p + n * s->pend_n, x, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    // what the residual-loss training needs to re-apply this pair's operator
+    const long nc = ncell(g), nv = (long)g.Nx * g.Nz;
+    HIPCHK(c, s->cap_R.reserve(n * s->cap_max));
+    HIPCHK(c, s->cap_en.reserve(nc * s->cap_max));
+    HIPCHK(c, s->cap_es.reserve(nv * s->cap_max));
+    HIPCHK(c, s->cap_gamma.reserve(s->cap_max));
+    HIPCHK(c, hipMemcpyAsync(s->cap_R.p + n * slot, R, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s->cap_en.p + nc * slot, c->eta_n.p, nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s->cap_es.p + nv * slot, c->eta_s.p, nv * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s->cap_gamma.p + slot, &gamma, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // gamma is a host local
     HIPCHK(c, hipGetLastError());
     s->pend_slot[s->pend_n++] = slot;
     return MDG_OK;
@@ -1127,6 +1145,141 @@ template int mdg_pcnet_outer_capture_in<double>(mdg_ctx *, const sgrid &, double
                                                 const double *, const double *);
 template int mdg_pcnet_outer_capture_in<float>(mdg_ctx *, const sgrid &, double, const double *, const float *,
                                                const double *, const double *);
+
+
+// ---------------------------------------------- residual-loss training
+namespace {
+
+// the six inner products of the two-direction least-squares fit:
+// part[b*6 + j] for (Ku.Ku, Ku.Kp, Kp.Kp, Ku.R, Kp.R, R.R)
+__global__ void __launch_bounds__(256) k_dots6(long n, const double *__restrict__ ku, const double *__restrict__ kp,
+                                               const double *__restrict__ r, double *__restrict__ part) {
+    __shared__ double sh[6][256];
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double u = ku[i], p = kp[i], q = r[i];
+        a[0] += u * u; a[1] += u * p; a[2] += p * p; a[3] += u * q; a[4] += p * q; a[5] += q * q;
+    }
+    for (int j = 0; j < 6; j++) sh[j][threadIdx.x] = a[j];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int j = 0; j < 6; j++) sh[j][threadIdx.x] += sh[j][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 6; j++) part[blockIdx.x * 6 + j] = sh[j][0];
+}
+// y = x - a u - b v
+__global__ void k_res2(long n, const double *__restrict__ x, double a, const double *__restrict__ u, double b,
+                       const double *__restrict__ v, double *__restrict__ y) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[i] - a * u[i] - b * v[i];
+}
+// negate the pressure block of a grid-layout vector (in place)
+__global__ void k_flip_p(long nvel, long n, double *__restrict__ x) {
+    const long i = nvel + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = -x[i];
+}
+// Output-image gradient from the grid gradients: the transpose of
+// k_unpack_outer (channel 0 <- W face of gu, 1 <- S face of gu, 2 <- cell of
+// gp), each times its factor.
+__global__ void k_pack_grad(sgrid g, int H, int W, const double *__restrict__ gvec, double fu, double fp,
+                            float *__restrict__ dy) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)H * W) return;
+    const int y = (int)(i / W), x = (int)(i - (long)y * W);
+    const int Ncx = g.Nx - 1, Ncz = g.Nz - 1, nu = nvx(g), nvel = nu + nvz(g);
+    float d0 = 0.f, d1 = 0.f, d2 = 0.f;
+    if (x < Ncx && y < Ncz) {
+        const int ju = x + (y + 1) * g.Nx;
+        if (vx_free(g, x, y + 1)) d0 = (float)(fu * gvec[ju]);
+        const int jv = nu + (x + 1) + y * (g.Nx + 1);
+        if (vz_free(g, x + 1, y)) d1 = (float)(fu * gvec[jv]);
+        d2 = (float)(fp * gvec[nvel + x + y * Ncx]);
+    }
+    float *o = dy + i * 4;
+    o[0] = d0; o[1] = d1; o[2] = d2; o[3] = 0.f;
+}
+
+}  // namespace
+
+// Trains the network for precond = 2 on the captured outer pairs with the
+// loss the solver actually sees: the residual left after the best
+// combination of the two network directions,
+//   L = min_{a,b} ||R - a K z_u - b K z_p||^2 / ||R||^2,
+// whose gradient (envelope theorem) is dL/dz_u = -2a K^T r / ||R||^2 and
+// dL/dz_p = -2b K^T r / ||R||^2 with r the residual at the optimum;
+// K^T = P K P with P negating the pressure block (B = -C^T).
+extern "C" int mdg_pcnet_train_residual(mdg_ctx *c, int steps, int batch, float lr, unsigned long long seed,
+                                        double *losses) {
+    if (!c || steps < 0 || batch < 1) return mdg_fail(c, MDG_ERR_ARG, "bad training arguments");
+    pcnet_state *s = net_of(c);
+    if (!s->P.p) return mdg_fail(c, MDG_ERR_STATE, "network parameters not set");
+    if (s->cap_n < 1 || s->cap_kind != 1 || !s->cap_R.p)
+        return mdg_fail(c, MDG_ERR_STATE, "no captured outer pairs (mdg_pcnet_capture_outer)");
+    const sgrid g = c->sg;
+    int H, W, rc;
+    image_dims(g, &H, &W);
+    if (H != s->cap_H || W != s->cap_W) return mdg_fail(c, MDG_ERR_ARG, "pairs were captured on another grid");
+    if ((rc = check_dims(c, H, W)) || (rc = ensure_buffers(c, s, H, W))) return rc;
+    const long n = (long)nvx(g) + nvz(g) + ncell(g), nvel = (long)nvx(g) + nvz(g);
+    const long nc = ncell(g), nv = (long)g.Nx * g.Nz, img = (long)H * W * 4;
+    HIPCHK(c, s->t_zu.reserve(n)); HIPCHK(c, s->t_zp.reserve(n)); HIPCHK(c, s->t_ku.reserve(n));
+    HIPCHK(c, s->t_kp.reserve(n)); HIPCHK(c, s->t_r.reserve(n)); HIPCHK(c, s->t_g.reserve(n));
+    const int nbd = 256;
+    HIPCHK(c, s->t_part.reserve(6 * nbd));
+    HIPCHK(c, c->eta_n.reserve(nc));
+    HIPCHK(c, c->eta_s.reserve(nv));
+    std::vector<double> gam(s->cap_n), part(6 * nbd);
+    HIPCHK(c, hipMemcpy(gam.data(), s->cap_gamma.p, s->cap_n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int st = 0; st < steps; st++) {
+        double lsum = 0.0;
+        for (int j = 0; j < batch; j++) {
+            unsigned long long h = seed * 0x9E3779B97F4A7C15ull + (unsigned long long)st * 0xBF58476D1CE4E5B9ull + (unsigned long long)j;
+            h ^= h >> 31; h *= 0x94D049BB133111EBull; h ^= h >> 29;
+            const int i = (int)(h % (unsigned long long)s->cap_n);
+            const double *R = s->cap_R.p + n * i;
+            HIPCHK(c, hipMemcpyAsync(c->eta_n.p, s->cap_en.p + nc * i, nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->eta_s.p, s->cap_es.p + nv * i, nv * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            hipLaunchKernelGGL(k_to_bf16, dim3(nblk256(img)), dim3(256), 0, c->stream, img, (const float *)(s->cap_x.p + img * i), A(s, A_X));
+            if ((rc = forward(c, s))) return rc;
+            hipLaunchKernelGGL(k_unpack_outer, dim3(nblk256(n)), dim3(256), 0, c->stream, g, H, W, (const float *)s->y.p,
+                               s->t_zu.p, s->t_zp.p);
+            if ((rc = mdg_apply_Kg_grid(c, gam[i], s->t_zu.p, s->t_ku.p)) || (rc = mdg_apply_Kg_grid(c, gam[i], s->t_zp.p, s->t_kp.p)))
+                return rc;
+            hipLaunchKernelGGL(k_dots6, dim3(nbd), dim3(256), 0, c->stream, n, (const double *)s->t_ku.p, (const double *)s->t_kp.p, R,
+                               s->t_part.p);
+            HIPCHK(c, hipMemcpyAsync(part.data(), s->t_part.p, 6 * nbd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            double d[6] = {0, 0, 0, 0, 0, 0};
+            for (int b = 0; b < nbd; b++)
+                for (int q = 0; q < 6; q++) d[q] += part[b * 6 + q];
+            const double det = d[0] * d[2] - d[1] * d[1];
+            double a = 0.0, bb = 0.0, loss = 1.0;
+            if (d[5] > 0.0 && det > 1e-12 * d[0] * d[2] && det > 0.0) {
+                a = (d[3] * d[2] - d[4] * d[1]) / det;
+                bb = (d[0] * d[4] - d[1] * d[3]) / det;
+                loss = (d[5] - a * d[3] - bb * d[4]) / d[5];
+            }
+            // r = R - a Ku - b Kp;  g = K^T r = P K P r
+            hipLaunchKernelGGL(k_res2, dim3(nblk256(n)), dim3(256), 0, c->stream, n, R, a, (const double *)s->t_ku.p, bb,
+                               (const double *)s->t_kp.p, s->t_r.p);
+            hipLaunchKernelGGL(k_flip_p, dim3(nblk256(n - nvel)), dim3(256), 0, c->stream, nvel, n, s->t_r.p);
+            if ((rc = mdg_apply_Kg_grid(c, gam[i], s->t_r.p, s->t_g.p))) return rc;
+            hipLaunchKernelGGL(k_flip_p, dim3(nblk256(n - nvel)), dim3(256), 0, c->stream, nvel, n, s->t_g.p);
+            const double inv = d[5] > 0.0 ? 1.0 / d[5] : 0.0;
+            hipLaunchKernelGGL(k_pack_grad, dim3(nblk256((long)H * W)), dim3(256), 0, c->stream, g, H, W,
+                               (const double *)s->t_g.p, -2.0 * a * inv, -2.0 * bb * inv, Gb(s, 0));
+            HIPCHK(c, hipGetLastError());
+            if ((rc = backward(c, s, j > 0))) return rc;
+            lsum += loss;
+        }
+        if ((rc = mdg_pcnet_adam(c, lr / (float)batch))) return rc;
+        if (losses) losses[st] = lsum / batch;
+    }
+    return MDG_OK;
+}
 
 extern "C" int mdg_pcnet_capture(mdg_ctx *c, int max_pairs) {
     if (!c || max_pairs < 0) return mdg_fail(c, MDG_ERR_ARG, "max_pairs must be >= 0");

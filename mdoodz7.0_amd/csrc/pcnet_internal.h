@@ -33,3 +33,7 @@ template <typename T>
 int mdg_pcnet_outer_capture_in(mdg_ctx *c, const sgrid &g, double gamma, const double *R, const T *dinv,
                                const double *en, const double *x);
 int mdg_pcnet_outer_capture_done(mdg_ctx *c, const sgrid &g, const double *xstar, int converged);
+
+// y = K_gamma x for the augmented Picard system on the context's current
+// viscosities (grid layout, FP64); defined with the solver.
+int mdg_apply_Kg_grid(mdg_ctx *c, double gamma, const double *x, double *y);

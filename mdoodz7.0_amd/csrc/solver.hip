@@ -1629,6 +1629,14 @@ static int fgcr_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opt
     return mdg_fail(c, MDG_ERR_DIVERGED, "no convergence in %d iterations (rel. residual %.3e)", o->max_its, o->rel_res);
 }
 
+int mdg_apply_Kg_grid(mdg_ctx *c, double gamma, const double *x, double *y) {
+    const sgrid g = c->sg;
+    const int n = nvx(g) + nvz(g) + ncell(g);
+    hipLaunchKernelGGL(k_apply_Kg, dim3(nblk(n)), dim3(256), 0, c->stream, g, gamma, x, c->eta_n.p, c->eta_s.p, y);
+    HIPCHK(c, hipGetLastError());
+    return MDG_OK;
+}
+
 static void default_opts(mdg_solver_opts *o) {
     if (o->penalty <= 0) o->penalty = 10.0;
     if (o->rel_tol <= 0) o->rel_tol = 1e-8;

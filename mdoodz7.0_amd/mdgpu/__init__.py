@@ -164,6 +164,7 @@ def _extra_signatures(L):
     L.mdg_pcnet_captured.argtypes = [vp] + [C.POINTER(C.c_int)] * 3
     L.mdg_pcnet_get_pair.argtypes = [vp, C.c_int, vp, vp, vp]
     L.mdg_pcnet_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, vp]
+    L.mdg_pcnet_train_residual.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, vp]
     L.mdg_strain_rates_nodes.argtypes = [vp] * 9
     L.mdg_viscosity_grid.argtypes = [vp, C.POINTER(PhaseProp), C.c_int, C.POINTER(RheoParams),
                                      C.c_int, C.POINTER(RheoNodes)]
@@ -403,6 +404,12 @@ class Context:
         losses = np.zeros(max(steps, 1))
         self._check(self._L.mdg_pcnet_train(self.h, int(steps), int(batch), C.c_float(lr),
                                             C.c_ulonglong(seed), ptr(losses)))
+        return losses[:steps]
+
+    def pcnet_train_residual(self, steps, batch=1, lr=1e-3, seed=0):
+        losses = np.zeros(max(steps, 1))
+        self._check(self._L.mdg_pcnet_train_residual(self.h, int(steps), int(batch), C.c_float(lr),
+                                                     C.c_ulonglong(seed), ptr(losses)))
         return losses[:steps]
 
     def c2v(self, cent, vert=None, periodic_x=0):

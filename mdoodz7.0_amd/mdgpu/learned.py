@@ -107,6 +107,15 @@ def train(ctx, steps, batch=4, lr=1e-3, seed=0, init_seed=1):
     return ctx.pcnet_train(steps, batch=batch, lr=lr, seed=seed)
 
 
+def train_residual(ctx, steps, batch=4, lr=1e-3, seed=0, init_seed=1, warm_steps=0):
+    """Residual-loss training on outer pairs (precond = 2): optionally
+    warm-started by warm_steps of the error (L2) loss on the same pairs."""
+    ctx.pcnet_set_params(init_params(init_seed))
+    if warm_steps:
+        ctx.pcnet_train(warm_steps, batch=batch, lr=lr, seed=seed + 1)
+    return ctx.pcnet_train_residual(steps, batch=batch, lr=lr, seed=seed)
+
+
 def collect_outer_pairs(ctx, problems, max_pairs, **opts):
     """Plain solves with outer (residual, error) pair capture on."""
     ctx.pcnet_capture_outer(max_pairs)
