@@ -57,7 +57,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
-                    help="skip the untimed mixed-precision solve reported beside the FP64 one")
+                    help="skip the untimed mixed-precision and learned solves reported beside "
+                         "the FP64 one")
     ap.add_argument("--kernels", type=int, default=0, metavar="REPS",
                     help="profiling mode: only the P2G calls, the fine-level smoother and a "
                          "copy kernel (calibration of the HBM byte counters), REPS times each")
@@ -138,11 +139,13 @@ def main():
         return
     for _ in range(a.warmup):
         step()
+        log("warm-up step: solve %.1f s" % (stats["solve_ms"] / 1e3))
     t0 = time.perf_counter()
     for _ in range(a.steps):
         step()
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / a.steps
+    log("timed steps: %.1f ms per step" % ms_step)
     k_ms, k_bytes = ctx.time_smoother(50)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     p2g_ms = stats["p2g_ms"]
@@ -177,14 +180,55 @@ def main():
         sol2 = ctx.device(neq, host=np.zeros(neq))
         o2 = ctx.killer_solver_dev(D11, D11, es, F, nm, sol2, precision=mdgpu.MDG_PREC_MIXED)
         x32 = sol2.numpy()
+        log("mixed-precision solve: %.1f s" % (o2.time_ms / 1e3))
         res["mixed"] = {"precision": "FP32 multigrid V-cycle inside FP64 CG and FP64 FGCR",
                         "solve_ms": o2.time_ms, "outer_its": o2.its, "inner_its": o2.inner_its,
                         "rel_res": o2.rel_res,
                         "rel_l2_vs_fp64": float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64)),
                         "speedup_vs_fp64": stats["solve_ms"] / o2.time_ms}
+        res["learned"] = learned_variant(a, ctx, D11, es, F, nm, neq, x64, stats["solve_ms"])
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(wl, a, n)
     print(json.dumps(res), flush=True)
+
+
+def learned_variant(a, ctx, D11, es, F, nm, neq, x64, fp64_ms):
+    """The learned outer directions on the same 2047^2 problem: the network
+    (U-Net, BF16 MFMA) is trained on the GPU on residual/error pairs that the
+    plain solver records on 16 seeded 255^2 ellipse fields (the network is
+    resolution-independent), then the 2047^2 problem is solved with precond 3
+    (network correction, multigrid on the rest) in FP64 and with the FP32
+    V-cycle.  Untimed by the step loop; training time reported."""
+    import numpy as np
+    import mdgpu
+    from mdgpu import learned
+    nt = 255
+    tctx = mdgpu.Context(nt + 1, nt + 1, 0.0, 1.0, 0.0, 1.0)
+    tctx.set_bc(*learned.free_slip_bc(nt + 1))
+    n_ell = max(4, int(round(a.ellipses * (nt / a.cells) ** 2)))
+    probs = [learned.EllipseProblem(tctx, nt, a.seed + 100 + i, n_ell) for i in range(16)]
+    t0 = time.perf_counter()
+    _, npairs = learned.collect_outer_pairs(tctx, probs, 3000, rel_tol=1e-8)
+    log("learned: %d training pairs captured" % npairs)
+    losses = learned.train_residual(tctx, 1000, batch=8, lr=1e-3, seed=3)
+    log("learned: 1000 training steps, loss %.3f" % losses[-20:].mean())
+    losses = np.concatenate([losses, tctx.pcnet_train_residual(2000, batch=8, lr=1e-3, seed=4)])
+    log("learned: 3000 training steps, loss %.3f" % losses[-20:].mean())
+    t_train = time.perf_counter() - t0
+    ctx.pcnet_set_params(tctx.pcnet_get_params())
+    out = {"training": {"pairs": npairs, "grid": "%d^2" % nt, "steps": 3000, "batch": 8,
+                        "loss_first": float(losses[:20].mean()),
+                        "loss_last": float(losses[-50:].mean()), "seconds": t_train}}
+    for name, prec in (("fp64", mdgpu.MDG_PREC_FP64), ("mixed", mdgpu.MDG_PREC_MIXED)):
+        sol3 = ctx.device(neq, host=np.zeros(neq))
+        o3 = ctx.killer_solver_dev(D11, D11, es, F, nm, sol3, precond=3, precision=prec)
+        x3 = sol3.numpy()
+        log("learned %s solve: %.1f s" % (name, o3.time_ms / 1e3))
+        out[name] = {"solve_ms": o3.time_ms, "outer_its": o3.its, "inner_its": o3.inner_its,
+                     "rel_res": o3.rel_res, "network_residual_ratio": o3.guess_gain,
+                     "rel_l2_vs_fp64": float(np.linalg.norm(x3 - x64) / np.linalg.norm(x64)),
+                     "speedup_vs_plain_fp64": fp64_ms / o3.time_ms}
+    return out
 
 
 def kernels_only(ctx, reps, step, f_cent, f_vert, D11, en, es, F, nm, sol, eta_s, rho_s, n):

@@ -1664,6 +1664,7 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     if (o->precond < 0 || o->precond > 3) return mdg_fail(c, MDG_ERR_ARG, "precond must be 0, 1, 2 or 3");
     if (o->precond != 0 && !mdg_pcnet_ready(c))
         return mdg_fail(c, MDG_ERR_STATE, "precond %d needs the network parameters (mdg_pcnet_set_params)", o->precond);
+    if (o->schur < 0 || o->schur > 1) return mdg_fail(c, MDG_ERR_ARG, "schur must be 0 or 1");
     if (o->precision == MDG_PREC_MIXED && o->schur == 1)
         return mdg_fail(c, MDG_ERR_UNSUPPORTED, "the BFBt Schur approximation runs in FP64 only (precision = MDG_PREC_FP64)");
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
