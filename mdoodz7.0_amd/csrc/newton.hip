@@ -70,10 +70,13 @@ k_tangent_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
     }
     if (md.average == 1) e = 1.0 / e;
     else if (md.average == 2) e = exp(e);
+    // Central differences per phase: each phase's +h and -h values are
+    // differenced before the 1/(2h) scaling (the reference sums the scaled
+    // +h and -h terms of all phases, FD_Jacobian.c:66-78, leaving round-off
+    // of size ulp(eta)/h in two-phase cells: O(1) at the h = 1e-15 floor of
+    // a zero strain-rate component).  Same central difference, no cancellation.
     double der[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int s = 0; s < 8; s++) {
-        const int q = s >> 1;
-        const double sg = (s & 1) ? -1.0 : 1.0;
+    for (int q = 0; q < 4; q++) {
         const double h = (q == 0) ? px : (q == 1) ? pz : (q == 2) ? pg : pp;
         const double den = (q == 2) ? 4.0 * h : 2.0 * h;
         for (int p = 0; p < md.nph; p++) {
@@ -84,11 +87,14 @@ k_tangent_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
             double eb = 0.0, Pc, Epl;
             if (md.average != 0)   // the phase's own value at the base state
                 viscosity_concise(m, md, G, T, P, Exx, Ezz, Exz, fr, co, di, pref, eb, Pc, Epl, ok);
-            double etaVE;
-            viscosity_concise(m, md, G, T, P + (q == 3 ? sg * h : 0.0), Exx + (q == 0 ? sg * h : 0.0),
-                              Ezz + (q == 1 ? sg * h : 0.0), Exz + (q == 2 ? sg * h : 0.0), fr, co, di,
-                              pref, etaVE, Pc, Epl, ok);
-            double v = sg * w * etaVE / den;
+            double epm[2];
+            for (int sgn = 0; sgn < 2; sgn++) {
+                const double d = sgn ? -h : h;
+                viscosity_concise(m, md, G, T, P + (q == 3 ? d : 0.0), Exx + (q == 0 ? d : 0.0),
+                                  Ezz + (q == 1 ? d : 0.0), Exz + (q == 2 ? d : 0.0), fr, co, di, pref, epm[sgn],
+                                  Pc, Epl, ok);
+            }
+            double v = w * (epm[0] - epm[1]) / den;
             if (md.average == 1) v /= eb * eb;
             else if (md.average == 2) v /= eb;
             der[q] += v;

@@ -114,21 +114,24 @@ class PicardLoop:
         return alpha, False
 
     def run(self, st, nit_max, tol, diag_scaling=1, gx=0.0, gz=0.0, line_search=False,
-            newton=False, picard2newton_tol=1e-1, max_its_pic=10, **solver_opts):
+            newton=False, picard2newton=True, picard2newton_tol=1e-1, max_its_pic=10,
+            **solver_opts):
         """Iterates in place on st; tol = (abs_mom, rel_mom, abs_div, rel_div).
         Returns (residual norm history, number of linear solves, solver stats);
         with line_search the chosen steps are in st["alphas"].  With newton,
         the loop switches from Picard to Newton once a relative residual of
         the previous iteration drops below picard2newton_tol or after
-        max_its_pic iterations (Main_DOODZ.c:884-896); Newton steps use the
-        tangent coefficients and the line search (InputOutput.c:1334).
+        max_its_pic iterations (Main_DOODZ.c:884-896); with picard2newton
+        False (the reference's default Picard2Newton = 0, InputOutput.c:1193)
+        every iteration is a Newton step.  Newton steps use the tangent
+        coefficients and the line search (InputOutput.c:1334).
         st["newton_steps"] lists the iteration numbers solved with Newton."""
         c = self.ctx
         rx, rz = c.eval_rhs(gx, gz, st["rho_n"], st["rho_s"])
         abs_mom, rel_mom, abs_div, rel_div = tol
         hist, stats, res0 = [], [], None
         nm = c.eqn_count()[1]
-        is_newton = False
+        is_newton = bool(newton and not picard2newton)
         st["newton_steps"] = []
         for nit in range(nit_max + 1):
             if newton and nit > 0 and not is_newton:

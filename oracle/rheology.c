@@ -322,20 +322,31 @@ int orc_tangent_nodes(const orc_rheo_model *md, const orc_phase *phases, int nph
         }
         if (average == 1) e = 1.0 / e;
         if (average == 2) e = exp(e);
-        for (int s = 0; s < 8; s++) {
+        /* Central differences per phase.  The reference accumulates the
+         * scaled +h and -h terms of all phases into one sum
+         * (PhaseRheologyLoop_v1, FD_Jacobian.c:66-78), which leaves round-off
+         * of size ulp(eta)/h in cells holding two phases (O(1) with the
+         * h = 1e-15 floor of a zero strain-rate component); here each phase's
+         * +h and -h values are differenced first: the same central difference
+         * without that cancellation. */
+        for (int q = 0; q < 4; q++) {
             for (int p = 0; p < nph; p++) {
                 const double f = perc[(size_t)p * n + i];
                 if (!(fabs(f) > 1e-13)) continue;
-                double etaVE, Pc, Epl;
-                orc_viscosity_concise(&phases[p], md, G[i], T[i], P[i] + st[s][3], Exx + st[s][0],
-                                      Ezz + st[s][1], Exz + st[s][2],
-                                      fric ? fric[i] : phases[p].fric, C ? C[i] : phases[p].C,
-                                      dil ? dil[i] : phases[p].dil, pref ? pref[i] : 1.0,
-                                      &etaVE, &Pc, &Epl, &status);
-                double v = st[s][4] * f * etaVE / st[s][5];
+                double eta_pm[2];
+                for (int sgn = 0; sgn < 2; sgn++) {
+                    const int s = 2 * q + sgn;
+                    double Pc, Epl;
+                    orc_viscosity_concise(&phases[p], md, G[i], T[i], P[i] + st[s][3], Exx + st[s][0],
+                                          Ezz + st[s][1], Exz + st[s][2],
+                                          fric ? fric[i] : phases[p].fric, C ? C[i] : phases[p].C,
+                                          dil ? dil[i] : phases[p].dil, pref ? pref[i] : 1.0,
+                                          &eta_pm[sgn], &Pc, &Epl, &status);
+                }
+                double v = f * (eta_pm[0] - eta_pm[1]) / st[2 * q][5];
                 if (average == 1) v /= eta_ph[p] * eta_ph[p];
                 if (average == 2) v /= eta_ph[p];
-                der[s / 2] += v;
+                der[q] += v;
             }
         }
         for (int q = 0; q < 4; q++) {

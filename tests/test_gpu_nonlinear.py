@@ -77,3 +77,77 @@ def test_picard_loop_matches_oracle(oracle_mod, diag_scaling, line_search, C):
     assert rel(st_gpu["v"], st["v"]) < 1e-8
     assert rel(st_gpu["p"], st["p"]) < 1e-8
     assert rel(st_gpu["eta_n"], st["eta_n"]) < 1e-8
+
+
+def _shear_template(O, linear):
+    """The ShearTemplate cases of the reference's NewtonIterationConvergence
+    test (TESTS/NewtonIterationConvergence.cpp:97-142, parameters
+    TESTS/ShearTemplate/LinearPureshearIsotropic.txt and
+    NonLinearPureshearIsotropic.txt): 21 x 21 vertices on [-0.5, 0.5]^2,
+    4 x 4 markers per cell, pure shear at a background rate of 1, a circular
+    inclusion of radius 0.05 (eta = 10, constant) in a matrix of eta = 1
+    (linear case) or a 'homemade' power law eta = Eii^(1/3 - 1) (pwlv = 1,
+    n = 3, eta0 = 1: A = eta0^-n, FlowLaws.c:58-70); no elasticity, no
+    plasticity (C = 1e90), min_eta 1e-3, max_eta 1e6; initial velocity the
+    background pure shear (GridRoutines.c:355-385)."""
+    box = (-0.5, 0.5, -0.5, 0.5)
+    n = 20
+    g = O.Grid(n + 1, n + 1, *box)
+    g.pure_shear(1.0)
+    x, z = S.put_markers(g.xg, g.zg, g.dx, g.dz, 4, 4, seed=1)
+    ph = ((x * x + z * z) < 0.05 ** 2).astype(np.int32)
+    perc_n = O.p2mastah(g, x, z, ph, np.zeros(2), prop=1, centroid=1, nb_phases=2)
+    perc_s = O.p2mastah(g, x, z, ph, np.zeros(2), prop=1, centroid=0, nb_phases=2)
+    Nx = n + 1
+    u = np.zeros(Nx * (Nx + 1))
+    v = np.zeros((Nx + 1) * Nx)
+    for l in range(Nx + 1):
+        u[l * Nx:(l + 1) * Nx] = -np.asarray(g.xg)
+    for l in range(Nx):
+        v[l * (Nx + 1):(l + 1) * (Nx + 1)] = g.zg[l]
+    for arr, t, val in ((u, g.BCu_t, g.BCu_v), (v, g.BCv_t, g.BCv_v)):
+        m = t == 0
+        arr[m] = val[m]
+    O.apply_bc(g, u, v)
+    nc, nv = n * n, Nx * Nx
+    z0, z0s = np.zeros(nc), np.zeros(nv)
+    st = dict(u=u, v=v, p=np.zeros(nc), T=np.full(nc, 274.15), mu_n=np.ones(nc), mu_s=np.ones(nv),
+              perc_n=perc_n, perc_s=perc_s, sxxd0=z0.copy(), szzd0=z0.copy(), sxz0_n=z0.copy(),
+              sxxd0_s=z0s.copy(), szzd0_s=z0s.copy(), sxz0=z0s.copy(), rho_n=np.ones(nc),
+              rho_s=np.ones(nv))
+    common = dict(cstv=0, pwlv=0, plast=0, eta0=1.0, pref_pwl=1.0, F_pwl=1.0, A_pwl=1.0,
+                  n_pwl=3.0, Q_pwl=0.0, V_pwl=0.0, C=1e90, fric=np.radians(30.0), dil=0.0,
+                  Slim=1e100, eta_vp=0.0, n_vp=1.0, beta=1e-11)
+    matrix = dict(common, cstv=1) if linear else dict(common, pwlv=1)
+    inclusion = dict(common, cstv=1, eta0=10.0)
+    ctx = mdgpu.Context(Nx, Nx, *box)
+    ctx.set_bc(g.BCu_t, g.BCu_v, g.BCv_t, g.BCv_v, g.BCp_t, g.BCg_t)
+    prm = mdgpu.RheoParams(step=1, elastic=0, average=0, dt=0.5, R=8.314, zeroC=273.15,
+                           min_eta=1e-3, max_eta=1e6, eta_tol=1e-11, Tiimin=0.0, scale_E=1.0,
+                           scale_eta=1.0)
+    eu, ev, ep = g.numbering()[:3]
+    loop = PicardLoop(ctx, eu, ev, ep, [mdgpu.PhaseProp(**matrix), mdgpu.PhaseProp(**inclusion)], prm)
+    return st, loop
+
+
+@pytest.mark.parametrize("linear", [True, False])
+def test_shear_template_newton_iteration_counts(oracle_mod, linear):
+    """Known answers of the reference's ShearTemplate tests: Newton with the
+    line search (Newton = 1, Picard2Newton = 0, nit_max = 10, non-linear
+    tolerances 1e-9, diag_scaling = 0) takes exactly 1 step for the linear
+    case and more than 1 and fewer than 10 for the power-law case
+    (NewtonIterationConvergence.cpp:104 and :141-142)."""
+    st, loop = _shear_template(oracle_mod, linear)
+    tol = (1e-9, 1e-9, 1e-9, 1e-9)
+    hist, nsolve, stats = loop.run(st, 10, tol, diag_scaling=0, newton=True, picard2newton=False,
+                                   rel_tol=1e-11, max_its=400)
+    h = np.array(hist)
+    print("linear" if linear else "power law", "steps", nsolve, "history", h[:, 0],
+          "alphas", st.get("alphas"))
+    last, first = h[-1], h[0]
+    converged = all(last[i] < tol[0] or last[i] / first[i] < tol[1] for i in range(3))
+    assert converged
+    if linear:
+        assert nsolve == 1
+    else:
+        assert 1 < nsolve < 10
