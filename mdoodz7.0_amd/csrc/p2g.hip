@@ -186,14 +186,12 @@ k_p2g_tile(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff, const doubl
 #pragma unroll
                 for (int f = 0; f < NF; f++) vs[f][m] = 0.0;
                 if (j < ce) {
-                    // the marker streams are read once: non-temporal loads keep
-                    // them from displacing the node and offset data in cache
-                    xs[m] = __builtin_nontemporal_load(mx + j);
-                    zs[m] = __builtin_nontemporal_load(mz + j);
+                    xs[m] = mx[j];
+                    zs[m] = mz[j];
                     if (need_phase) ps[m] = mph[j];
 #pragma unroll
                     for (int f = 0; f < NF; f++)
-                        if (fs.f[f].flag == 1) vs[f][m] = __builtin_nontemporal_load(fs.f[f].src + j);
+                        if (fs.f[f].flag == 1) vs[f][m] = fs.f[f].src[j];
                 }
             }
 #pragma unroll
