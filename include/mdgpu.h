@@ -376,9 +376,14 @@ int mdg_pcnet_train_residual(mdg_ctx *ctx, int steps, int batch, float lr, unsig
  * reference; it feeds the roofline figure of the benchmark. */
 int mdg_time_smoother(mdg_ctx *ctx, int reps, double *ms_per_launch, double *bytes_per_launch);
 
-/* Time of the last P2G kernel (ms, HIP events on the context stream) and
- * the number of markers it processed. */
+/* Time of the last P2G kernel (ms, HIP events on the context stream). */
 int mdg_last_kernel_ms(const mdg_ctx *ctx, double *ms);
+/* Time of the last computation of the cached per-marker P2G weights (ms):
+ * the first 1-cell-stencil P2G call on a node set after mdg_markers_set
+ * computes each marker's nearest node and weight once; later calls on that
+ * set read them.  No counterpart in the reference (which recomputes them in
+ * every P2Mastah call); reported beside the P2G kernel times. */
+int mdg_last_weights_ms(const mdg_ctx *ctx, double *ms);
 
 #ifdef __cplusplus
 }

@@ -81,6 +81,7 @@ extern "C" void mdg_destroy(mdg_ctx *c) {
     c->BCu_v.release(); c->BCv_v.release();
     c->mx.release(); c->mz.release(); c->mphase.release(); c->perm.release();
     c->qoff.release(); c->keys.release(); c->keys2.release(); c->idx.release();
+    for (auto &b : c->mw) b.release();
     c->sort_tmp.release(); c->field_tmp.release(); c->node_tmp.release();
     c->table_tmp.release(); c->ntype_tmp.release(); c->flag_dev.release();
     c->eqn_u.release(); c->eqn_v.release(); c->eqn_p.release(); c->eta_n.release();
@@ -208,6 +209,12 @@ int mdg_node_set(mdg_ctx *c, int centroid, node_set *ns) {
 extern "C" int mdg_last_kernel_ms(const mdg_ctx *c, double *ms) {
     if (!c || !ms) return MDG_ERR_ARG;
     *ms = c->last_ms;
+    return MDG_OK;
+}
+
+extern "C" int mdg_last_weights_ms(const mdg_ctx *c, double *ms) {
+    if (!c || !ms) return MDG_ERR_ARG;
+    *ms = c->weights_ms;
     return MDG_OK;
 }
 

@@ -115,6 +115,7 @@ extern "C" int mdg_markers_set(mdg_ctx *c, int n, const double *x, const double 
     HIPCHK(c, hipStreamSynchronize(c->stream));
     tx.release(); tz.release(); tp.release();
     c->n_markers = n;
+    for (int q = 0; q < 4; q++) c->mw_state[q] = 0;   // cached P2G weights are stale
     return MDG_OK;
 }
 

@@ -47,6 +47,13 @@ struct mdg_ctx {
     dbuf<int>         mphase;        // sorted phase
     dbuf<int>         perm;          // sorted slot -> caller index
     dbuf<unsigned>    qoff;          // 4*ncells+1 quarter-cell offsets
+    // per-marker P2G weights of the 1-cell stencil for the four node sets
+    // (centres, vertices, Vx, Vz), computed on first use after mdg_markers_set;
+    // state 0 not computed, 1 valid, 2 some marker's nearest node differs from
+    // the one its sort key implies (then the kernels compute weights inline)
+    dbuf<double>      mw[4];
+    int               mw_state[4] = {0, 0, 0, 0};
+    double            weights_ms = 0.0;
     dbuf<unsigned>    keys, keys2;   // sort scratch
     dbuf<int>         idx;           // sort scratch
     dbuf<unsigned char> sort_tmp;

@@ -33,6 +33,7 @@ struct p2g_geo {
     double X0, Z0, dx, dz, dx_itp, dz_itp;
     double idx, idz, ix_itp, iz_itp;   // reciprocals of dx, dz, dx_itp, dz_itp
     int R, S;   // stencil radius (0: 1-cell, 1: 9-cell) and search radius
+    int set;    // node set: 0 centres, 1 vertices, 2 Vx, 3 Vz
 };
 
 template <typename F>
@@ -69,7 +70,7 @@ __device__ __forceinline__ double mark_val(const p2g_fld &f, int p, unsigned i) 
     // a phase outside the table reads as NaN instead of out of bounds
     double v = (f.flag == 0) ? ((p < f.n_src) ? f.src[p] : __builtin_nan("")) : f.src[i];
     if (f.avg == 1) v = 1.0 / v;
-    if (f.avg == 2) v = log(v);
+    if (f.avg == 2) v = p2g_log(v);
     return v;
 }
 
@@ -102,6 +103,31 @@ __global__ void __launch_bounds__(256) k_p2g(p2g_geo g, p2g_set fs, const unsign
     }
 }
 
+// Per-marker weight of the 1-cell stencil for one node set, in sorted
+// order: the same nearest node and weight the kernels compute inline, kept
+// while the markers do not move (MDOODZ interpolates 15-25 fields per step
+// from the same positions).  A marker whose nearest node is not the one its
+// (cell, quarter) key implies raises the flag; the node set then keeps the
+// inline path, which handles it.
+__global__ void __launch_bounds__(256) k_p2g_weights(p2g_geo g, int n, const unsigned *__restrict__ keys,
+                                                     const double *__restrict__ mx, const double *__restrict__ mz,
+                                                     double *__restrict__ w, int *__restrict__ stray) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = mx[i], z = mz[i];
+    const int ip = near_index_fast(x, g.X0, g.dx, g.idx, g.nx);
+    const int jp = near_index_fast(z, g.Z0, g.dz, g.idz, g.nz);
+    const unsigned key = keys[i], cell = key >> 2, qd = key & 3u;
+    if (cell >= (unsigned)g.Ncx * (unsigned)g.Ncz) { w[i] = 0.0; return; }   // phase -1: in no range
+    const int cx = (int)(cell % (unsigned)g.Ncx), cz = (int)(cell / (unsigned)g.Ncx);
+    const int hx = (int)(qd & 1u), hz = (int)(qd >> 1);
+    const int ek = (g.ax.xkind == CELL_AXIS) ? cx - g.ax.xoff : cx + hx;
+    const int el = (g.ax.zkind == CELL_AXIS) ? cz - g.ax.zoff : cz + hz;
+    if (ip != ek || jp != el) *stray = 1;
+    const double dxm = fabs(g.X[ip] - x), dzm = fabs(g.Z[jp] - z);
+    w[i] = (1.0 - dxm * g.ix_itp) * (1.0 - dzm * g.iz_itp);
+}
+
 // Row-tiled variant for the 1-cell stencil.  A block owns P2G_TILE
 // consecutive nodes of ROWS node rows; the markers of the cell rows those
 // nodes draw from are contiguous in sorted order, so the block streams them
@@ -116,15 +142,15 @@ __global__ void __launch_bounds__(256) k_p2g(p2g_geo g, p2g_set fs, const unsign
 #define P2G_TILE 256
 template <int NF> struct tile_cap { static constexpr int v = (NF <= 2) ? 1280 : 768; };
 
-template <int NF, int ROWS>
+template <int NF, int ROWS, bool PRE>
 __global__ void __launch_bounds__(P2G_TILE)
 k_p2g_tile(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff, const double *__restrict__ mx,
-           const double *__restrict__ mz, const int *__restrict__ mph, int need_phase, int nbx,
-           int nblocks, int *__restrict__ stray) {
+           const double *__restrict__ mz, const double *__restrict__ mw, const int *__restrict__ mph,
+           int need_phase, int nbx, int nblocks, int *__restrict__ stray) {
     constexpr int CAP = tile_cap<NF>::v;
     __shared__ double lw[CAP];
     __shared__ double lwv[NF][CAP];
-    __shared__ int lhome[CAP];
+    __shared__ int lhome[PRE ? 1 : CAP];   // homes are checked when the weights are cached
     // XCD-aware order: hardware block b runs on XCD b % 8; XCD x gets the
     // contiguous logical tiles [start(x), start(x) + count(x)).
     const int b = blockIdx.x, xcd = b & 7, per = nblocks >> 3, rem = nblocks & 7;
@@ -186,8 +212,12 @@ k_p2g_tile(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff, const doubl
 #pragma unroll
                 for (int f = 0; f < NF; f++) vs[f][m] = 0.0;
                 if (j < ce) {
-                    xs[m] = mx[j];
-                    zs[m] = mz[j];
+                    if (PRE) {
+                        xs[m] = mw[j];      // the cached weight
+                    } else {
+                        xs[m] = mx[j];
+                        zs[m] = mz[j];
+                    }
                     if (need_phase) ps[m] = mph[j];
 #pragma unroll
                     for (int f = 0; f < NF; f++)
@@ -198,21 +228,26 @@ k_p2g_tile(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff, const doubl
             for (int m = 0; m < PER; m++) {
                 const unsigned j = cs + t + (unsigned)m * P2G_TILE;
                 if (j >= ce) continue;
-                const double x = xs[m], z = zs[m];
-                const int ip = near_index_fast(x, g.X0, g.dx, g.idx, g.nx);
-                const int jp = near_index_fast(z, g.Z0, g.dz, g.idz, g.nz);
-                const double dxm = fabs(g.X[ip] - x), dzm = fabs(g.Z[jp] - z);
-                const double w = (1.0 - dxm * g.ix_itp) * (1.0 - dzm * g.iz_itp);
                 const unsigned q = j - cs;
+                double w;
+                if (PRE) {
+                    w = xs[m];
+                } else {
+                    const double x = xs[m], z = zs[m];
+                    const int ip = near_index_fast(x, g.X0, g.dx, g.idx, g.nx);
+                    const int jp = near_index_fast(z, g.Z0, g.dz, g.idz, g.nz);
+                    const double dxm = fabs(g.X[ip] - x), dzm = fabs(g.Z[jp] - z);
+                    w = (1.0 - dxm * g.ix_itp) * (1.0 - dzm * g.iz_itp);
+                    lhome[q] = ip + jp * g.nx;
+                }
                 lw[q] = w;
-                lhome[q] = ip + jp * g.nx;
 #pragma unroll
                 for (int f = 0; f < NF; f++) {
                     const p2g_fld &F = fs.f[f];
                     const int p = ps[m];
                     double v = (F.flag == 0) ? ((p < F.n_src) ? F.src[p] : __builtin_nan("")) : vs[f][m];
                     if (F.avg == 1) v = 1.0 / v;
-                    if (F.avg == 2) v = log(v);
+                    if (F.avg == 2) v = p2g_log(v);
                     lwv[f][q] = v * w;
                 }
             }
@@ -226,7 +261,7 @@ k_p2g_tile(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff, const doubl
                     const unsigned a = max(rs[r][qr], cs), e = min(re[r][qr], ce);
                     for (unsigned i = a; i < e; i++) {
                         const unsigned q = i - cs;
-                        if (lhome[q] != node) { bad = true; continue; }
+                        if (!PRE && lhome[q] != node) { bad = true; continue; }
                         W[r] += lw[q];
 #pragma unroll
                         for (int f = 0; f < NF; f++) BW[r][f] += lwv[f][q];
@@ -301,6 +336,30 @@ static int setup_geo(mdg_ctx *c, int centroid, int stencil, p2g_geo *g) {
     g->ix_itp = 1.0 / g->dx_itp; g->iz_itp = 1.0 / g->dz_itp;
     g->R = (stencil == 9) ? 1 : 0;
     g->S = g->R;
+    g->set = (centroid == 1) ? 0 : (centroid == 0) ? 1 : (centroid == -1) ? 2 : 3;
+    return MDG_OK;
+}
+
+// Computes the cached 1-cell-stencil weights of g's node set if they are
+// not current (once per node set after each mdg_markers_set).
+static int ensure_weights(mdg_ctx *c, const p2g_geo &g) {
+    if (g.R != 0 || c->mw_state[g.set] != 0) return MDG_OK;
+    const int n = c->n_markers;
+    HIPCHK(c, c->mw[g.set].reserve((size_t)n + 1));
+    HIPCHK(c, hipMemsetAsync(c->flag_dev.p, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (n > 0)
+        hipLaunchKernelGGL(k_p2g_weights, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, g, n, (const unsigned *)c->keys2.p,
+                           (const double *)c->mx.p, (const double *)c->mz.p, c->mw[g.set].p, c->flag_dev.p);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    int stray = 0;
+    HIPCHK(c, hipMemcpyAsync(&stray, c->flag_dev.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->weights_ms = ms;
+    c->mw_state[g.set] = stray ? 2 : 1;
     return MDG_OK;
 }
 
@@ -314,8 +373,11 @@ static int launch_fields(mdg_ctx *c, p2g_geo g, const p2g_set &s, int nf) {
         const int rows = (g.ax.zkind == VERT_AXIS) ? 4 : 1;
         const int nbx = (g.nx + P2G_TILE - 1) / P2G_TILE, nb = nbx * ((g.nz + rows - 1) / rows);
         dim3 grid(nb), block(P2G_TILE);
-#define TILE_LAUNCH(NF, R) hipLaunchKernelGGL((k_p2g_tile<NF, R>), grid, block, 0, c->stream, g, s, c->qoff.p, c->mx.p, c->mz.p, c->mphase.p, need_phase, nbx, nb, c->flag_dev.p)
-#define TILE_ROWS(NF) do { if (rows == 4) TILE_LAUNCH(NF, 4); else TILE_LAUNCH(NF, 1); } while (0)
+        const bool pre = (c->mw_state[g.set] == 1);
+        const double *mw = c->mw[g.set].p;
+#define TILE_LAUNCH(NF, R, P) hipLaunchKernelGGL((k_p2g_tile<NF, R, P>), grid, block, 0, c->stream, g, s, c->qoff.p, c->mx.p, c->mz.p, mw, c->mphase.p, need_phase, nbx, nb, c->flag_dev.p)
+#define TILE_ROWS(NF) do { if (pre) { if (rows == 4) TILE_LAUNCH(NF, 4, true); else TILE_LAUNCH(NF, 1, true); } \
+                           else { if (rows == 4) TILE_LAUNCH(NF, 4, false); else TILE_LAUNCH(NF, 1, false); } } while (0)
         switch (nf) {
         case 1: TILE_ROWS(1); break;
         case 2: TILE_ROWS(2); break;
@@ -424,7 +486,7 @@ extern "C" int mdg_p2mastah_fused(mdg_ctx *c, int centroid, int stencil,
     if (nfields < 1 || nfields > 16 || !fields) return mdg_fail(c, MDG_ERR_ARG, "nfields must be 1..16");
     p2g_geo g;
     int rc = setup_geo(c, centroid, stencil, &g);
-    if (rc) return rc;
+    if (rc || (rc = ensure_weights(c, g))) return rc;
     const size_t nn = (size_t)g.nx * g.nz, N = (size_t)c->n_markers + 1;
     HIPCHK(c, c->field_tmp.reserve(N * P2G_MAXF));
     HIPCHK(c, c->table_tmp.reserve(P2G_MAXTAB * P2G_MAXF));

@@ -30,6 +30,42 @@ __device__ __forceinline__ int near_index_fast(double pos, double origin, double
     return i > n - 1 ? n - 1 : i;
 }
 
+// Natural logarithm for the per-marker term of geometric averaging, the
+// largest remaining instruction cost of the P2G kernels once the weights are
+// cached.  Accurate to ~2 ulp (far below the 1e-12 P2G tolerance) at about a
+// third of the library log's instructions:
+//   v = 2^k m, m in [sqrt(1/2), sqrt(2));  s = (m - 1)/(m + 1);
+//   log m = 2 atanh(s) = 2s + 2s z (1/3 + z/5 + ... + z^9/21),  z = s^2,
+// the series cut where z^10/23 < 2^-54 (|s| <= 0.1716);
+//   log v = k ln2_hi + (2s + (k ln2_lo + 2s z P(z))),
+// ln2_hi = ln 2 with its low 21 mantissa bits cleared (k ln2_hi is exact for
+// any exponent) and ln2_lo = ln 2 - ln2_hi.  Non-positive, infinite, NaN and
+// subnormal arguments go to the library log.
+__device__ __noinline__ double p2g_log_lib(double v) { return log(v); }
+__device__ __forceinline__ double p2g_log(double v) {
+    if (!(v >= 2.2250738585072014e-308) || !(v <= 1.7976931348623157e308)) return p2g_log_lib(v);
+    int e;
+    double m = frexp(v, &e);                      // v = m 2^e, m in [0.5, 1)
+    if (m < 0.70710678118654752440) { m = 2.0 * m; e -= 1; }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double p = 1.0 / 21.0;
+    p = fma(p, z, 1.0 / 19.0);
+    p = fma(p, z, 1.0 / 17.0);
+    p = fma(p, z, 1.0 / 15.0);
+    p = fma(p, z, 1.0 / 13.0);
+    p = fma(p, z, 1.0 / 11.0);
+    p = fma(p, z, 1.0 / 9.0);
+    p = fma(p, z, 1.0 / 7.0);
+    p = fma(p, z, 1.0 / 5.0);
+    p = fma(p, z, 1.0 / 3.0);
+    const double s2 = s + s;
+    const double k = (double)e;
+    const double r = fma(s2 * z, p, k * 1.9082149292705877e-10);
+    return k * 0.6931471803691238 + (s2 + r);
+}
+
 // How a node set lines up with the cells along one axis:
 //   CELL_AXIS: node index k covers cell k + off (both halves);
 //   VERT_AXIS: node index k covers the east half of cell k-1 and the west

@@ -71,6 +71,7 @@ def lib():
                                    C.c_int, C.c_int, vp]
         L.mdg_p2mastah_fused.argtypes = [vp, C.c_int, C.c_int, C.POINTER(P2GField), C.c_int]
         L.mdg_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double)]
+        L.mdg_last_weights_ms.argtypes = [vp, C.POINTER(C.c_double)]
         L.mdg_dev_alloc.argtypes = [vp, C.c_ulonglong, C.POINTER(vp)]
         L.mdg_dev_free.argtypes = [vp, vp]
         L.mdg_memcpy.argtypes = [vp, vp, vp, C.c_ulonglong]
@@ -556,4 +557,9 @@ class Context:
     def last_kernel_ms(self):
         v = C.c_double()
         self._check(self._L.mdg_last_kernel_ms(self.h, C.byref(v)))
+        return v.value
+
+    def last_weights_ms(self):
+        v = C.c_double()
+        self._check(self._L.mdg_last_weights_ms(self.h, C.byref(v)))
         return v.value

@@ -125,3 +125,28 @@ def test_device_resident_sorted_fields(oracle_mod):
     out2 = ctx.device(g.node_count(0), host=np.zeros(g.node_count(0)))
     ctx.p2mastah_fused(0, 1, [dict(src=fs, dst=out2, flag=1 | mdgpu.MDG_SRC_SORTED, avg=0)])
     assert np.array_equal(out.numpy(), out2.numpy())
+
+
+@pytest.mark.parametrize("centroid", [1, 0, -1, -2])
+def test_cached_weights_follow_new_markers(oracle_mod, centroid):
+    """The per-marker weights cached by the first 1-cell P2G call of a node
+    set are recomputed after mdg_markers_set: P2G on a second marker set
+    (moved markers, same context) matches the oracle on that set, and a
+    repeated call on it is bit-identical."""
+    g, ctx, x, z, phase = _case(oracle_mod)
+    f = np.cos(3 * x) * np.sin(2 * z) + 2.0
+    nn = g.node_count(centroid)
+    ctx.p2mastah(f, flag=1, avg=0, centroid=centroid, interp_stencil=1)
+    rng = np.random.default_rng(3)
+    x2 = x + 0.3 * g.dx * rng.standard_normal(x.size)
+    z2 = z + 0.3 * g.dz * rng.standard_normal(z.size)
+    ctx.markers_set(x2, z2, phase)
+    f2 = np.sin(5 * x2) + 3.0
+    for avg in (0, 2):
+        ref = oracle_mod.p2mastah(g, x2, z2, phase, f2, flag=1, avg=avg, centroid=centroid,
+                                  stencil=1)
+        got = ctx.p2mastah(f2, flag=1, avg=avg, centroid=centroid, interp_stencil=1)
+        again = ctx.p2mastah(f2, flag=1, avg=avg, centroid=centroid, interp_stencil=1)
+        assert got.shape == (nn,)
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0)
+        assert np.array_equal(got, again)
