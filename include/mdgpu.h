@@ -270,7 +270,8 @@ typedef struct {
                               applied to the residual; 3 the network's
                               correction first, multigrid applied to what it
                               leaves (mdg_pcnet_*)                             */
-    int    cycle;          /* 1 V-cycle, 2 W-cycle (1)                         */
+    int    cycle;          /* 1 V-cycle (1; other values are refused: the
+                              W-cycle stalled on the ellipse workloads)        */
     int    eta_avg;        /* coarse viscosity: 0 geometric, 1 arithmetic,
                               2 harmonic, 3 maximum (0)                        */
     int    check_every;    /* inner CG convergence test period (1)             */
@@ -281,6 +282,8 @@ typedef struct {
     int    poisson_max_its;/* (30)                                             */
     int    smoother;       /* 0 fused three-term Chebyshev kernels, 1 the
                               residual/update two-kernel form (0)              */
+    int    cheb_degree_coarse; /* smoothing degree on levels of at most 200k
+                              velocity unknowns (0: cheb_degree)               */
     int    graphs;         /* 0 replay each inner CG iteration as a captured
                               HIP graph, 1 launch its kernels one by one (for
                               profilers that do not trace graph nodes) (0)     */

@@ -305,6 +305,7 @@ __global__ void __launch_bounds__(CHEB_TX) k_cheb3(sgrid g, stc<T> cf, tile_grid
 // and the result; d is the second iterate buffer.
 #define CHEB_BLOCK 1024
 #define CHEB_BLOCK_MAX 2500    // levels up to this many velocity unknowns
+#define CHEB_COARSE_N 200000   // levels smoothed with cheb_degree_coarse
 template <typename T>
 __global__ void __launch_bounds__(CHEB_BLOCK) k_cheb_block(sgrid g, stc<T> cf, T gamma, int deg, double theta,
                                                            double delta, int zero_start, T *__restrict__ x,
@@ -871,6 +872,7 @@ struct mg_hier {
     dbuf<double> scal, part;                // CG scalars (rz_old, rz_new, pq, rr) and partials
     double *h_scal = nullptr;               // pinned host copy of scal
     bool fused = true;                      // three-term Chebyshev kernels
+    int deg_coarse = 0;                     // smoothing degree on small levels (0: same)
     hipGraph_t graph = nullptr;             // one CG iteration (V-cycle included)
     hipGraphExec_t exec = nullptr;
     // pressure Poisson CG (BFBt): vectors, scalars and graph
@@ -978,6 +980,7 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
     for (auto &l : H.L) l.release();
     H.L.clear();
     H.fused = (o->smoother == 0);
+    H.deg_coarse = o->cheb_degree_coarse;
     const double gamma = o->penalty;
     sgrid g = c->sg;
     const double Lx = c->xmax - c->xmin, Lz = c->zmax - c->zmin;
@@ -1344,6 +1347,7 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
 template <typename T>
 static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, int cycle) {
     mg_level<T> &lv = H.L[i];
+    if (lv.n <= CHEB_COARSE_N && H.deg_coarse > 0) deg = H.deg_coarse;
     if (i + 1 == H.L.size()) {
         hipLaunchKernelGGL(k_fill<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.n, lv.x.p, T(0));
         hipLaunchKernelGGL(k_dense_solve<T>, dim3(1), dim3(256), 0, c->stream, lv.nfree, lv.idx.p, lv.Ainv.p, lv.b.p, lv.x.p);
@@ -1665,6 +1669,7 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     if (o->precond != 0 && !mdg_pcnet_ready(c))
         return mdg_fail(c, MDG_ERR_STATE, "precond %d needs the network parameters (mdg_pcnet_set_params)", o->precond);
     if (o->schur < 0 || o->schur > 1) return mdg_fail(c, MDG_ERR_ARG, "schur must be 0 or 1");
+    if (o->cycle != 1) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "only the V-cycle (cycle = 1) is supported");
     if (o->precision == MDG_PREC_MIXED && o->schur == 1)
         return mdg_fail(c, MDG_ERR_UNSUPPORTED, "the BFBt Schur approximation runs in FP64 only (precision = MDG_PREC_FP64)");
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
