@@ -282,8 +282,10 @@ typedef struct {
     int    poisson_max_its;/* (30)                                             */
     int    smoother;       /* 0 fused three-term Chebyshev kernels, 1 the
                               residual/update two-kernel form (0)              */
-    int    cheb_degree_coarse; /* smoothing degree on levels of at most 200k
-                              velocity unknowns (0: cheb_degree)               */
+    int    cheb_degree_coarse; /* smoothing degree on every level below the
+                              finest (0: cheb_degree)                          */
+    double cheb_range;     /* the Chebyshev smoother targets eigenvalues of
+                              D^-1 A in [lmax / cheb_range, lmax] (30)         */
     int    graphs;         /* 0 replay each inner CG iteration as a captured
                               HIP graph, 1 launch its kernels one by one (for
                               profilers that do not trace graph nodes) (0)     */

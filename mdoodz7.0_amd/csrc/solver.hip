@@ -305,7 +305,6 @@ __global__ void __launch_bounds__(CHEB_TX) k_cheb3(sgrid g, stc<T> cf, tile_grid
 // and the result; d is the second iterate buffer.
 #define CHEB_BLOCK 1024
 #define CHEB_BLOCK_MAX 2500    // levels up to this many velocity unknowns
-#define CHEB_COARSE_N 200000   // levels smoothed with cheb_degree_coarse
 template <typename T>
 __global__ void __launch_bounds__(CHEB_BLOCK) k_cheb_block(sgrid g, stc<T> cf, T gamma, int deg, double theta,
                                                            double delta, int zero_start, T *__restrict__ x,
@@ -447,47 +446,60 @@ __global__ void k_prolong(sgrid gf, sgrid gc, comp_maps mu, comp_maps mv, const 
     }
 }
 
-// Sum of P weights times the fine residual onto coarse node (K, L) itself
-// (no ghost folding).
-template <typename T>
-__device__ inline double restrict_node(const comp_maps &m, const sgrid &gf, const T *__restrict__ rf, int K, int L) {
-    // fine index ranges whose bracket can include K / L
-    const double tx0 = ((K - 1 + m.xm.off_c) * m.xm.hc) / m.xm.hf - m.xm.off_f;
-    const double tx1 = ((K + 1 + m.xm.off_c) * m.xm.hc) / m.xm.hf - m.xm.off_f;
-    const double tz0 = ((L - 1 + m.zm.off_c) * m.zm.hc) / m.zm.hf - m.zm.off_f;
-    const double tz1 = ((L + 1 + m.zm.off_c) * m.zm.hc) / m.zm.hf - m.zm.off_f;
-    int k0 = (int)floor(tx0) - 1, k1 = (int)ceil(tx1) + 1, l0 = (int)floor(tz0) - 1, l1 = (int)ceil(tz1) + 1;
-    k0 = k0 < 0 ? 0 : k0; l0 = l0 < 0 ? 0 : l0;
-    k1 = k1 > m.nxf - 1 ? m.nxf - 1 : k1; l1 = l1 > m.nzf - 1 ? m.nzf - 1 : l1;
-    double s = 0.0;
-    // the x weights of the window, once per node (the window spans at most
-    // 2 hc/hf + 4 <= 8 fine columns; wider windows take the direct loop)
-    constexpr int RW = 10;
-    if (k1 - k0 + 1 <= RW) {
-        double wxs[RW];
-#pragma unroll
-        for (int q = 0; q < RW; q++) wxs[q] = (k0 + q <= k1) ? pweight(m.xm, k0 + q, K) : 0.0;
-        for (int l = l0; l <= l1; l++) {
-            const double wz = pweight(m.zm, l, L);
-            if (wz == 0.0) continue;
-#pragma unroll
-            for (int q = 0; q < RW; q++) {
-                const int k = k0 + q;
-                if (k > k1 || wxs[q] == 0.0) continue;
-                const bool fr = (m.comp == 0) ? vx_free(gf, k, l) : vz_free(gf, k, l);
-                if (fr) s += wxs[q] * wz * (double)rf[k + l * m.nxf];
-            }
+// Restriction tables. For each coarse index K of one axis: the first fine
+// index whose prolongation weight to K is non-zero, and the weights of RT
+// consecutive fine indices from there (zeros included). The weights come from
+// the same pweight as the prolongation, so restriction stays its exact
+// transpose; the fine window searched is the one the bracket can reach.
+#define RT 6
+struct rtab {
+    const int *fx, *fz;        // first fine index per coarse column / row
+    const double *wx, *wz;     // RT weights per coarse column / row
+};
+__global__ void k_rtab(axis_map m, int nf, int nc, int *__restrict__ f0, double *__restrict__ w, int *__restrict__ bad) {
+    const int K = blockIdx.x * blockDim.x + threadIdx.x;
+    if (K >= nc) return;
+    const double t0 = ((K - 1 + m.off_c) * m.hc) / m.hf - m.off_f;
+    const double t1 = ((K + 1 + m.off_c) * m.hc) / m.hf - m.off_f;
+    int k0 = (int)floor(t0) - 1, k1 = (int)ceil(t1) + 1;
+    k0 = k0 < 0 ? 0 : k0;
+    k1 = k1 > nf - 1 ? nf - 1 : k1;
+    for (int q = 0; q < RT; q++) w[(size_t)K * RT + q] = 0.0;
+    int first = -1;
+    for (int f = k0; f <= k1; f++) {
+        const double v = pweight(m, f, K);
+        if (v == 0.0) continue;
+        if (first < 0) first = f;
+        if (f - first >= RT) {
+            bad[0] = 1;
+            continue;
         }
-        return s;
+        w[(size_t)K * RT + (f - first)] = v;
     }
-    for (int l = l0; l <= l1; l++) {
-        const double wz = pweight(m.zm, l, L);
+    f0[K] = first < 0 ? 0 : first;
+}
+
+// Sum of P weights times the fine residual onto coarse node (K, L) itself
+// (no ghost folding); rows in increasing z, columns in increasing x.
+template <typename T>
+__device__ inline double restrict_node(const comp_maps &m, const rtab &t, const sgrid &gf, const T *__restrict__ rf,
+                                       int K, int L) {
+    const int kx = t.fx[K], lz = t.fz[L];
+    double wxs[RT];
+#pragma unroll
+    for (int q = 0; q < RT; q++) wxs[q] = t.wx[(size_t)K * RT + q];
+    double s = 0.0;
+#pragma unroll
+    for (int p = 0; p < RT; p++) {
+        const double wz = t.wz[(size_t)L * RT + p];
         if (wz == 0.0) continue;
-        for (int k = k0; k <= k1; k++) {
+        const int l = lz + p;
+#pragma unroll
+        for (int q = 0; q < RT; q++) {
+            if (wxs[q] == 0.0) continue;
+            const int k = kx + q;
             const bool fr = (m.comp == 0) ? vx_free(gf, k, l) : vz_free(gf, k, l);
-            if (!fr) continue;
-            const double wx = pweight(m.xm, k, K);
-            if (wx != 0.0) s += wx * wz * (double)rf[k + l * m.nxf];
+            if (fr) s += wxs[q] * wz * (double)rf[k + l * m.nxf];
         }
     }
     return s;
@@ -495,24 +507,24 @@ __device__ inline double restrict_node(const comp_maps &m, const sgrid &gf, cons
 
 // b_c = P^T r_f, with ghost contributions folded back like P maps them.
 template <typename T>
-__global__ void k_restrict(sgrid gf, sgrid gc, comp_maps mu, comp_maps mv, const T *__restrict__ rf,
+__global__ void k_restrict(sgrid gf, sgrid gc, comp_maps mu, comp_maps mv, rtab tu, rtab tv, const T *__restrict__ rf,
                            T *__restrict__ bc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int nuc = nvx(gc), nvc = nvz(gc), nuf = nvx(gf);
     if (i < nuc) {
         const int K = i % gc.Nx, L = i / gc.Nx;
         if (!vx_free(gc, K, L)) { bc[i] = T(0); return; }
-        double s = restrict_node<T>(mu, gf, rf, K, L);
-        if (L == 1) s += (mu.ghost_lo == 1 ? 1.0 : -1.0) * restrict_node<T>(mu, gf, rf, K, 0);
-        if (L == gc.Nz - 1) s += (mu.ghost_hi == 1 ? 1.0 : -1.0) * restrict_node<T>(mu, gf, rf, K, gc.Nz);
+        double s = restrict_node<T>(mu, tu, gf, rf, K, L);
+        if (L == 1) s += (mu.ghost_lo == 1 ? 1.0 : -1.0) * restrict_node<T>(mu, tu, gf, rf, K, 0);
+        if (L == gc.Nz - 1) s += (mu.ghost_hi == 1 ? 1.0 : -1.0) * restrict_node<T>(mu, tu, gf, rf, K, gc.Nz);
         bc[i] = (T)s;
     } else if (i < nuc + nvc) {
         const int j = i - nuc, K = j % (gc.Nx + 1), L = j / (gc.Nx + 1);
         if (!vz_free(gc, K, L)) { bc[i] = T(0); return; }
         const T *r = rf + nuf;
-        double s = restrict_node<T>(mv, gf, r, K, L);
-        if (K == 1) s += (mv.ghost_lo == 1 ? 1.0 : -1.0) * restrict_node<T>(mv, gf, r, 0, L);
-        if (K == gc.Nx - 1) s += (mv.ghost_hi == 1 ? 1.0 : -1.0) * restrict_node<T>(mv, gf, r, gc.Nx, L);
+        double s = restrict_node<T>(mv, tv, gf, r, K, L);
+        if (K == 1) s += (mv.ghost_lo == 1 ? 1.0 : -1.0) * restrict_node<T>(mv, tv, gf, r, 0, L);
+        if (K == gc.Nx - 1) s += (mv.ghost_hi == 1 ? 1.0 : -1.0) * restrict_node<T>(mv, tv, gf, r, gc.Nx, L);
         bc[i] = (T)s;
     }
 }
@@ -847,6 +859,7 @@ struct mg_level {
     int nu = 0, nv = 0, n = 0;
     dbuf<T> en, es, dinv, x, b, r, d;
     double lmax = 1.0;
+    double cheb_div = 30.0;  // smoothed range [lmax / cheb_div, lmax]
     comp_maps mu, mv;      // transfer to the next coarser level
     // coarsest level
     int nfree = 0;
@@ -856,7 +869,11 @@ struct mg_level {
     dbuf<T> px, pb, pr, pd, pdinv, pAinv;
     double plmax = 1.0;
     axis_map cxm, czm;     // cell-centre transfer to the next coarser level
+    dbuf<int> rt_f;        // restriction tables (Vx x, Vx z, Vz x, Vz z)
+    dbuf<double> rt_w;
+    rtab tu{}, tv{};
     void release() {
+        rt_f.release(); rt_w.release();
         en.release(); es.release(); dinv.release(); x.release(); b.release(); r.release();
         d.release(); idx.release(); Ainv.release();
         px.release(); pb.release(); pr.release(); pd.release(); pdinv.release(); pAinv.release();
@@ -872,7 +889,7 @@ struct mg_hier {
     dbuf<double> scal, part;                // CG scalars (rz_old, rz_new, pq, rr) and partials
     double *h_scal = nullptr;               // pinned host copy of scal
     bool fused = true;                      // three-term Chebyshev kernels
-    int deg_coarse = 0;                     // smoothing degree on small levels (0: same)
+    int deg_coarse = 0;                     // smoothing degree below the finest level (0: same)
     hipGraph_t graph = nullptr;             // one CG iteration (V-cycle included)
     hipGraphExec_t exec = nullptr;
     // pressure Poisson CG (BFBt): vectors, scalars and graph
@@ -1016,6 +1033,29 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
                           g.Nx + 1, g.Nz, gc.Nx + 1, gc.Nz, 1, g.ghostW, g.ghostE};
         lv.cxm = make_axis(g.dx, gc.dx, 0.5, gc.Nx - 1);
         lv.czm = make_axis(g.dz, gc.dz, 0.5, gc.Nz - 1);
+        {
+            const axis_map *am[4] = {&lv.mu.xm, &lv.mu.zm, &lv.mv.xm, &lv.mv.zm};
+            const int nf[4] = {lv.mu.nxf, lv.mu.nzf, lv.mv.nxf, lv.mv.nzf};
+            const int nc[4] = {lv.mu.nxc, lv.mu.nzc, lv.mv.nxc, lv.mv.nzc};
+            const int tot = nc[0] + nc[1] + nc[2] + nc[3];
+            HIPCHK(c, lv.rt_f.reserve(tot + 1));
+            HIPCHK(c, lv.rt_w.reserve((size_t)tot * RT));
+            int *bad = lv.rt_f.p + tot;
+            hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, c->stream, 1, bad, 0);
+            int off[4];
+            for (int q = 0, o2 = 0; q < 4; q++) {
+                off[q] = o2;
+                hipLaunchKernelGGL(k_rtab, dim3(nblk(nc[q])), dim3(256), 0, c->stream, *am[q], nf[q], nc[q], lv.rt_f.p + o2,
+                                   lv.rt_w.p + (size_t)o2 * RT, bad);
+                o2 += nc[q];
+            }
+            int hbad = 0;
+            HIPCHK(c, hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (hbad) return mdg_fail(c, MDG_ERR_ARG, "restriction window wider than %d fine nodes", RT);
+            lv.tu = rtab{lv.rt_f.p + off[0], lv.rt_f.p + off[1], lv.rt_w.p + (size_t)off[0] * RT, lv.rt_w.p + (size_t)off[1] * RT};
+            lv.tv = rtab{lv.rt_f.p + off[2], lv.rt_f.p + off[3], lv.rt_w.p + (size_t)off[2] * RT, lv.rt_w.p + (size_t)off[3] * RT};
+        }
         g = gc;
         ncx = ncx2;
         ncz = ncz2;
@@ -1037,6 +1077,7 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
             hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.n, sc, lv.d.p, T(0), lv.x.p);
         }
         lv.lmax = 1.2 * lam;   // power iteration approaches lambda_max from below
+        lv.cheb_div = o->cheb_range > 0.0 ? o->cheb_range : 30.0;
     }
     // coarsest: dense inverse of A_g on its free unknowns, assembled by
     // applying the operator to unit vectors on the device
@@ -1290,7 +1331,7 @@ static int bfbt_rhs(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solver
 template <typename T>
 static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool zero_start, bool fused) {
     // x (+)= Chebyshev(deg) correction for A_g x = b
-    const double lmax = lv.lmax, lmin = lmax / 30.0;
+    const double lmax = lv.lmax, lmin = lmax / lv.cheb_div;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
     double rho = 1.0 / sigma;
     const int n = lv.n;
@@ -1347,7 +1388,7 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
 template <typename T>
 static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, int cycle) {
     mg_level<T> &lv = H.L[i];
-    if (lv.n <= CHEB_COARSE_N && H.deg_coarse > 0) deg = H.deg_coarse;
+    if (i > 0 && H.deg_coarse > 0) deg = H.deg_coarse;
     if (i + 1 == H.L.size()) {
         hipLaunchKernelGGL(k_fill<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.n, lv.x.p, T(0));
         hipLaunchKernelGGL(k_dense_solve<T>, dim3(1), dim3(256), 0, c->stream, lv.nfree, lv.idx.p, lv.Ainv.p, lv.b.p, lv.x.p);
@@ -1359,7 +1400,8 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, i
     for (int rep = 0; rep < reps; rep++) {
         hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p, lv.b.p,
                            lv.en.p, lv.es.p, lv.r.p);
-        hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(cl.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, lv.r.p, cl.b.p);
+        hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(cl.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, lv.tu, lv.tv,
+                           (const T *)lv.r.p, cl.b.p);
         vcycle(c, H, i + 1, gamma, deg, cycle);
         hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, cl.x.p, lv.x.p);
     }
