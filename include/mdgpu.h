@@ -261,7 +261,7 @@ typedef struct {
     int    restart;        /* GCR restart length (100)                         */
     double inner_tol;      /* relative tolerance of the velocity CG (1e-6)     */
     int    inner_max_its;  /* (80)                                             */
-    int    cheb_degree;    /* Chebyshev smoothing degree per sweep (8)         */
+    int    cheb_degree;    /* Chebyshev smoothing degree on the finest level (8) */
     int    coarse_cells;   /* coarsen until <= this many cells per side (8)    */
     int    precision;      /* MDG_PREC_FP64 or MDG_PREC_MIXED                  */
     int    precond;        /* 0 multigrid; 1 multigrid CG started from the
@@ -283,7 +283,7 @@ typedef struct {
     int    smoother;       /* 0 fused three-term Chebyshev kernels, 1 the
                               residual/update two-kernel form (0)              */
     int    cheb_degree_coarse; /* smoothing degree on every level below the
-                              finest (0: cheb_degree)                          */
+                              finest (16)                                      */
     double cheb_range;     /* the Chebyshev smoother targets eigenvalues of
                               D^-1 A in [lmax / cheb_range, lmax] (30)         */
     int    graphs;         /* 0 replay each inner CG iteration as a captured

@@ -889,7 +889,7 @@ struct mg_hier {
     dbuf<double> scal, part;                // CG scalars (rz_old, rz_new, pq, rr) and partials
     double *h_scal = nullptr;               // pinned host copy of scal
     bool fused = true;                      // three-term Chebyshev kernels
-    int deg_coarse = 0;                     // smoothing degree below the finest level (0: same)
+    int deg_coarse = 0;                     // smoothing degree below the finest level
     hipGraph_t graph = nullptr;             // one CG iteration (V-cycle included)
     hipGraphExec_t exec = nullptr;
     // pressure Poisson CG (BFBt): vectors, scalars and graph
@@ -1691,6 +1691,7 @@ static void default_opts(mdg_solver_opts *o) {
     if (o->inner_tol <= 0) o->inner_tol = 1e-6;
     if (o->inner_max_its <= 0) o->inner_max_its = 80;
     if (o->cheb_degree <= 0) o->cheb_degree = 8;
+    if (o->cheb_degree_coarse <= 0) o->cheb_degree_coarse = 16;
     if (o->coarse_cells <= 0) o->coarse_cells = 8;
     if (o->cycle <= 0) o->cycle = 1;
     if (o->check_every <= 0) o->check_every = 1;
