@@ -53,6 +53,8 @@ class PicardLoop:
         vin = dict(perc=st["perc_s"], G=st["mu_s"], T=T_s, P=P_s, exx=sr["exxd_s"],
                    ezz=sr["ezzd_s"], exz=sr["exz"], sxx0=st["sxxd0_s"], szz0=st["szzd0_s"],
                    sxz0=st["sxz0"])
+        if "pref_n" in st:   # per-node multiplier of the power-law prefactor
+            cin["pref"], vin["pref"] = st["pref_n"], st["pref_s"]
         return cin, vin
 
     def update_nonlinearity(self, st):
