@@ -1713,6 +1713,7 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
         return mdg_fail(c, MDG_ERR_STATE, "precond %d needs the network parameters (mdg_pcnet_set_params)", o->precond);
     if (o->schur < 0 || o->schur > 1) return mdg_fail(c, MDG_ERR_ARG, "schur must be 0 or 1");
     if (o->cycle != 1) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "only the V-cycle (cycle = 1) is supported");
+    if (o->smoother < 0 || o->smoother > 1) return mdg_fail(c, MDG_ERR_ARG, "smoother must be 0 or 1");
     if (o->precision == MDG_PREC_MIXED && o->schur == 1)
         return mdg_fail(c, MDG_ERR_UNSUPPORTED, "the BFBt Schur approximation runs in FP64 only (precision = MDG_PREC_FP64)");
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));

@@ -153,3 +153,16 @@ def test_operator_transpose_is_pressure_flip(oracle_mod, bc):
     lhs, rhs = y @ kx, x @ kty
     scale = np.linalg.norm(y) * np.linalg.norm(kx) + np.linalg.norm(x) * np.linalg.norm(kty)
     assert abs(lhs - rhs) <= 1e-13 * scale, (lhs, rhs)
+
+
+@pytest.mark.parametrize("opt,val,code", [("cycle", 2, -3), ("schur", 2, -1), ("precond", 4, -1),
+                                            ("smoother", 2, -1)])
+def test_solver_refuses_unsupported_options(oracle_mod, opt, val, code):
+    """Options outside the implemented set are refused before any work: the
+    W-cycle (it stalled on the ellipse workloads), unknown Schur
+    approximations, preconditioner kinds and smoother forms."""
+    g, ctx, en, es = _problem(oracle_mod, n=21, bc="pure_shear")
+    num = g.numbering()
+    with pytest.raises(mdgpu.MdgError) as e:
+        ctx.killer_solver(2 * en, 2 * en, es, np.ones(num[3]), np.zeros(num[4]), **{opt: val})
+    assert e.value.code == code
