@@ -286,6 +286,8 @@ typedef struct {
                               finest (16)                                      */
     double cheb_range;     /* the Chebyshev smoother targets eigenvalues of
                               D^-1 A in [lmax / cheb_range, lmax] (30)         */
+    int    coarsen4;       /* levels with at most this many cells per side
+                              coarsen by 4 instead of 2 (0: never) (0)         */
     int    graphs;         /* 0 replay each inner CG iteration as a captured
                               HIP graph, 1 launch its kernels one by one (for
                               profilers that do not trace graph nodes) (0)     */

@@ -451,12 +451,14 @@ __global__ void k_prolong(sgrid gf, sgrid gc, comp_maps mu, comp_maps mv, const 
 // consecutive fine indices from there (zeros included). The weights come from
 // the same pweight as the prolongation, so restriction stays its exact
 // transpose; the fine window searched is the one the bracket can reach.
-#define RT 6
+#define RT2 6     // window for coarsening by 2
+#define RT4 10    // window for coarsening by 4
 struct rtab {
     const int *fx, *fz;        // first fine index per coarse column / row
     const double *wx, *wz;     // RT weights per coarse column / row
 };
-__global__ void k_rtab(axis_map m, int nf, int nc, int *__restrict__ f0, double *__restrict__ w, int *__restrict__ bad) {
+__global__ void k_rtab(axis_map m, int nf, int nc, int RT, int *__restrict__ f0, double *__restrict__ w,
+                       int *__restrict__ bad) {
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
     if (K >= nc) return;
     const double t0 = ((K - 1 + m.off_c) * m.hc) / m.hf - m.off_f;
@@ -481,7 +483,7 @@ __global__ void k_rtab(axis_map m, int nf, int nc, int *__restrict__ f0, double 
 
 // Sum of P weights times the fine residual onto coarse node (K, L) itself
 // (no ghost folding); rows in increasing z, columns in increasing x.
-template <typename T>
+template <typename T, int RT>
 __device__ inline double restrict_node(const comp_maps &m, const rtab &t, const sgrid &gf, const T *__restrict__ rf,
                                        int K, int L) {
     const int kx = t.fx[K], lz = t.fz[L];
@@ -506,7 +508,7 @@ __device__ inline double restrict_node(const comp_maps &m, const rtab &t, const 
 }
 
 // b_c = P^T r_f, with ghost contributions folded back like P maps them.
-template <typename T>
+template <typename T, int RT>
 __global__ void k_restrict(sgrid gf, sgrid gc, comp_maps mu, comp_maps mv, rtab tu, rtab tv, const T *__restrict__ rf,
                            T *__restrict__ bc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -514,17 +516,17 @@ __global__ void k_restrict(sgrid gf, sgrid gc, comp_maps mu, comp_maps mv, rtab 
     if (i < nuc) {
         const int K = i % gc.Nx, L = i / gc.Nx;
         if (!vx_free(gc, K, L)) { bc[i] = T(0); return; }
-        double s = restrict_node<T>(mu, tu, gf, rf, K, L);
-        if (L == 1) s += (mu.ghost_lo == 1 ? 1.0 : -1.0) * restrict_node<T>(mu, tu, gf, rf, K, 0);
-        if (L == gc.Nz - 1) s += (mu.ghost_hi == 1 ? 1.0 : -1.0) * restrict_node<T>(mu, tu, gf, rf, K, gc.Nz);
+        double s = restrict_node<T, RT>(mu, tu, gf, rf, K, L);
+        if (L == 1) s += (mu.ghost_lo == 1 ? 1.0 : -1.0) * restrict_node<T, RT>(mu, tu, gf, rf, K, 0);
+        if (L == gc.Nz - 1) s += (mu.ghost_hi == 1 ? 1.0 : -1.0) * restrict_node<T, RT>(mu, tu, gf, rf, K, gc.Nz);
         bc[i] = (T)s;
     } else if (i < nuc + nvc) {
         const int j = i - nuc, K = j % (gc.Nx + 1), L = j / (gc.Nx + 1);
         if (!vz_free(gc, K, L)) { bc[i] = T(0); return; }
         const T *r = rf + nuf;
-        double s = restrict_node<T>(mv, tv, gf, r, K, L);
-        if (K == 1) s += (mv.ghost_lo == 1 ? 1.0 : -1.0) * restrict_node<T>(mv, tv, gf, r, 0, L);
-        if (K == gc.Nx - 1) s += (mv.ghost_hi == 1 ? 1.0 : -1.0) * restrict_node<T>(mv, tv, gf, r, gc.Nx, L);
+        double s = restrict_node<T, RT>(mv, tv, gf, r, K, L);
+        if (K == 1) s += (mv.ghost_lo == 1 ? 1.0 : -1.0) * restrict_node<T, RT>(mv, tv, gf, r, 0, L);
+        if (K == gc.Nx - 1) s += (mv.ghost_hi == 1 ? 1.0 : -1.0) * restrict_node<T, RT>(mv, tv, gf, r, gc.Nx, L);
         bc[i] = (T)s;
     }
 }
@@ -872,6 +874,7 @@ struct mg_level {
     dbuf<int> rt_f;        // restriction tables (Vx x, Vx z, Vz x, Vz z)
     dbuf<double> rt_w;
     rtab tu{}, tv{};
+    int rt = RT2;          // restriction window (RT2 or RT4)
     void release() {
         rt_f.release(); rt_w.release();
         en.release(); es.release(); dinv.release(); x.release(); b.release(); r.release();
@@ -1024,7 +1027,10 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
         hipLaunchKernelGGL(k_vel_diag_inv<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, g, (T)gamma, lv.en.p, lv.es.p, lv.dinv.p);
         HIPCHK(c, hipGetLastError());
         if (ncx <= cmin || ncz <= cmin) break;
-        const int ncx2 = (ncx + 1) / 2, ncz2 = (ncz + 1) / 2;
+        // coarsen by 2, or by 4 once both sides are at most coarsen4 cells
+        const bool by4 = o->coarsen4 > 0 && ncx <= o->coarsen4 && ncz <= o->coarsen4;
+        const int ncx2 = by4 ? (ncx + 3) / 4 : (ncx + 1) / 2, ncz2 = by4 ? (ncz + 3) / 4 : (ncz + 1) / 2;
+        lv.rt = by4 ? RT4 : RT2;
         sgrid gc = {ncx2 + 1, ncz2 + 1, Lx / ncx2, Lz / ncz2, g.ghostS, g.ghostN, g.ghostW, g.ghostE};
         // Vx: x at vertices (index * h), z at centres ((index - 0.5) * h)
         lv.mu = comp_maps{make_axis(g.dx, gc.dx, 0.0, gc.Nx), make_axis(g.dz, gc.dz, -0.5, gc.Nz + 1),
@@ -1039,13 +1045,14 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
             const int nc[4] = {lv.mu.nxc, lv.mu.nzc, lv.mv.nxc, lv.mv.nzc};
             const int tot = nc[0] + nc[1] + nc[2] + nc[3];
             HIPCHK(c, lv.rt_f.reserve(tot + 1));
+            const int RT = lv.rt;
             HIPCHK(c, lv.rt_w.reserve((size_t)tot * RT));
             int *bad = lv.rt_f.p + tot;
             hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, c->stream, 1, bad, 0);
             int off[4];
             for (int q = 0, o2 = 0; q < 4; q++) {
                 off[q] = o2;
-                hipLaunchKernelGGL(k_rtab, dim3(nblk(nc[q])), dim3(256), 0, c->stream, *am[q], nf[q], nc[q], lv.rt_f.p + o2,
+                hipLaunchKernelGGL(k_rtab, dim3(nblk(nc[q])), dim3(256), 0, c->stream, *am[q], nf[q], nc[q], RT, lv.rt_f.p + o2,
                                    lv.rt_w.p + (size_t)o2 * RT, bad);
                 o2 += nc[q];
             }
@@ -1400,8 +1407,12 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, i
     for (int rep = 0; rep < reps; rep++) {
         hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p, lv.b.p,
                            lv.en.p, lv.es.p, lv.r.p);
-        hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(cl.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, lv.tu, lv.tv,
-                           (const T *)lv.r.p, cl.b.p);
+        if (lv.rt == RT2)
+            hipLaunchKernelGGL((k_restrict<T, RT2>), dim3(nblk(cl.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, lv.tu,
+                               lv.tv, (const T *)lv.r.p, cl.b.p);
+        else
+            hipLaunchKernelGGL((k_restrict<T, RT4>), dim3(nblk(cl.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, lv.tu,
+                               lv.tv, (const T *)lv.r.p, cl.b.p);
         vcycle(c, H, i + 1, gamma, deg, cycle);
         hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, cl.x.p, lv.x.p);
     }
@@ -1714,6 +1725,8 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     if (o->schur < 0 || o->schur > 1) return mdg_fail(c, MDG_ERR_ARG, "schur must be 0 or 1");
     if (o->cycle != 1) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "only the V-cycle (cycle = 1) is supported");
     if (o->smoother < 0 || o->smoother > 1) return mdg_fail(c, MDG_ERR_ARG, "smoother must be 0 or 1");
+    if (o->coarsen4 > 0 && o->schur == 1)
+        return mdg_fail(c, MDG_ERR_UNSUPPORTED, "the BFBt pressure hierarchy coarsens by 2 only (coarsen4 = 0)");
     if (o->precision == MDG_PREC_MIXED && o->schur == 1)
         return mdg_fail(c, MDG_ERR_UNSUPPORTED, "the BFBt Schur approximation runs in FP64 only (precision = MDG_PREC_FP64)");
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));

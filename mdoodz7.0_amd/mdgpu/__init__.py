@@ -89,7 +89,8 @@ class SolverOpts(C.Structure):
                 ("precond", C.c_int), ("cycle", C.c_int), ("eta_avg", C.c_int),
                 ("check_every", C.c_int), ("schur", C.c_int), ("poisson_tol", C.c_double),
                 ("poisson_max_its", C.c_int), ("smoother", C.c_int),
-                ("cheb_degree_coarse", C.c_int), ("cheb_range", C.c_double), ("graphs", C.c_int),
+                ("cheb_degree_coarse", C.c_int), ("cheb_range", C.c_double), ("coarsen4", C.c_int),
+                ("graphs", C.c_int),
                 ("its", C.c_int),
                 ("inner_its", C.c_int),
                 ("poisson_its", C.c_int),
