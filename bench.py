@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--trace-solve", type=int, default=0, metavar="ITS",
                     help="profiling mode: P2G then ITS outer iterations of the solve with the "
                          "inner CG launched kernel by kernel (no HIP graphs), for kernel tracers")
+    ap.add_argument("--trace-mixed", action="store_true",
+                    help="with --trace-solve: the FP32 V-cycle (MDG_PREC_MIXED) instead of FP64")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding is "
@@ -130,7 +132,9 @@ def main():
         ctx.stokes_residual(zc, zc, zv, zc, zc, rgx, rgz, F=F)
         ctx.scale_into(en, 2.0, D11)
         try:
-            o = ctx.killer_solver_dev(D11, D11, es, F, nm, sol, max_its=a.trace_solve, graphs=1)
+            prec = mdgpu.MDG_PREC_MIXED if a.trace_mixed else mdgpu.MDG_PREC_FP64
+            o = ctx.killer_solver_dev(D11, D11, es, F, nm, sol, max_its=a.trace_solve, graphs=1,
+                                      precision=prec)
         except mdgpu.MdgError as e:
             if e.code != -6:
                 raise
