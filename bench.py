@@ -69,7 +69,7 @@ def main():
                     help="with --trace-solve: the FP32 V-cycle (MDG_PREC_MIXED) instead of FP64")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding is "
+        print("bench.py: the library runs on one GPU only (--gpus 1); multi-GPU sharding is "
               "designed, not built", file=sys.stderr)
         sys.exit(2)
     import numpy as np
@@ -291,7 +291,7 @@ def kernels_only(ctx, reps, step, f_cent, f_vert, D11, en, es, F, nm, sol, eta_s
 
 
 def cpu_baseline(wl, a, n):
-    """Oracle (CPU restatement of the reference path) on the GPU box's host:
+    """Oracle (CPU restatement of the reference path) on the host's cores:
     KillerSolver (PH + GCR(6), exact Cholesky preconditioner) on a 255^2
     sample of the same workload, extrapolated to 2047^2 with the n^3 cost of a
     nested-dissection Cholesky factor, plus the oracle P2Mastah rate."""

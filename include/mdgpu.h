@@ -37,7 +37,7 @@ typedef struct mdg_ctx mdg_ctx;
 typedef struct {
     int    Nx, Nz;                 /* vertices                               */
     double xmin, xmax, zmin, zmax;
-    int    periodic_x;             /* must be 0 in this build                */
+    int    periodic_x;             /* must be 0 (periodic x not implemented) */
 } mdg_grid_desc;
 
 /* Context lifetime; device 0 of the calling process. */

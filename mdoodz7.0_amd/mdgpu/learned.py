@@ -11,7 +11,8 @@ module only sequences the calls:
 3. train the network on those pairs (Adam, on the GPU);
 4. solve with precond = 1: every inner multigrid CG starts from the
    network's guess, and the guess is dropped whenever it does not lower the
-   residual, so the plain method remains the fallback and the judge.
+   residual, so the plain method remains the fallback and decides when the
+   solve has converged.
 
 The outer variant (collect_outer_pairs, precond = 2) captures instead the
 (residual R_k, error x* - x_k) pairs of the outer FGCR iterations and, once

@@ -4,7 +4,7 @@
  *
  * Supported subset: no free surface (tag 30), no stress BC (tag 2), no
  * periodic tags (-2/-12), no density stabilisation.  These are the
- * boundary conditions of every workload this build measures.
+ * boundary conditions of every workload the tests and bench.py use.
  */
 #include <math.h>
 #include <stdlib.h>
