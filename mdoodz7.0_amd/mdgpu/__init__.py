@@ -102,6 +102,7 @@ class SolverOpts(C.Structure):
 
 
 MDG_PREC_FP64, MDG_PREC_MIXED = 0, 1
+MDG_ERR_DIVERGED = -6
 
 
 class PhaseProp(C.Structure):
@@ -469,7 +470,7 @@ class Context:
                                                ptr(np.ascontiguousarray(x)), ptr(y)))
         return y
 
-    def newton_solver(self, Dn, Ds, D11, D33, rhs_mom, rhs_cont, **opts):
+    def newton_solver(self, Dn, Ds, D11, D33, rhs_mom, rhs_cont, accept_diverged=False, **opts):
         o = SolverOpts()
         for k, v in opts.items():
             setattr(o, k, v)
@@ -478,7 +479,7 @@ class Context:
                                        ptr(np.ascontiguousarray(Ds)), ptr(D11), ptr(D33),
                                        ptr(np.ascontiguousarray(rhs_mom)),
                                        ptr(np.ascontiguousarray(rhs_cont)), ptr(sol))
-        self._check(rc)
+        self._solved(rc, o, accept_diverged)
         return sol, o
 
     def viscosity_grid(self, phases, prm, centroid, outputs=RHEO_OUT, **arrays):
@@ -526,8 +527,18 @@ class Context:
                                                 ptr(roger_z), ptr(d), ptr(F), ptr(norms)))
         return F, norms
 
-    def killer_solver(self, D11, D22, D33, rhs_mom, rhs_cont, **opts):
-        """Replacement of KillerSolver: returns (sol, SolverOpts)."""
+    def _solved(self, rc, o, accept_diverged):
+        # MDG_ERR_DIVERGED leaves the last iterate in sol; the reference's
+        # KillerSolver carries on with it after max_its (Solvers.c:1298-1334)
+        if rc == MDG_ERR_DIVERGED and accept_diverged:
+            o.status = rc
+            return
+        self._check(rc)
+
+    def killer_solver(self, D11, D22, D33, rhs_mom, rhs_cont, accept_diverged=False, **opts):
+        """Replacement of KillerSolver: returns (sol, SolverOpts).  With
+        accept_diverged, a solve that misses its tolerance returns its last
+        iterate with o.status = MDG_ERR_DIVERGED instead of raising."""
         o = SolverOpts()
         for k, v in opts.items():
             setattr(o, k, v)
@@ -535,7 +546,7 @@ class Context:
         rc = self._L.mdg_killer_solver(self.h, C.byref(o), ptr(D11), ptr(D22), ptr(D33),
                                        ptr(np.ascontiguousarray(rhs_mom)),
                                        ptr(np.ascontiguousarray(rhs_cont)), ptr(sol))
-        self._check(rc)
+        self._solved(rc, o, accept_diverged)
         return sol, o
 
     def scale_into(self, x, a, y):

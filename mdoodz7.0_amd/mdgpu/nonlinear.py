@@ -16,9 +16,11 @@ routines:
                                           or LineSearchDecoupled's step
   ExtractSolutions2 + ApplyBC          scatter below + Context.apply_bc
 """
+import warnings
+
 import numpy as np
 
-from . import RheoParams
+from . import MDG_ERR_DIVERGED, RheoParams
 
 
 def scatter(eqn_u, eqn_v, eqn_p, dx, u, v, p, alpha=-1.0):
@@ -135,6 +137,7 @@ class PicardLoop:
         nm = c.eqn_count()[1]
         is_newton = bool(newton and not picard2newton)
         st["newton_steps"] = []
+        st["diverged_solves"] = []
         for nit in range(nit_max + 1):
             if newton and nit > 0 and not is_newton:
                 rr = hist[-1] / res0
@@ -159,10 +162,18 @@ class PicardLoop:
                 cin, vin = self.last_inputs
                 Dn = c.tangent_grid(self.phases, self.prm, 1, **cin)
                 Ds = c.tangent_grid(self.phases, self.prm, 0, **vin)
-                dx, o = c.newton_solver(Dn, Ds, D11, D33, F[:nm], F[nm:], **solver_opts)
+                dx, o = c.newton_solver(Dn, Ds, D11, D33, F[:nm], F[nm:], accept_diverged=True,
+                                        **solver_opts)
                 st["newton_steps"].append(nit)
             else:
-                dx, o = c.killer_solver(D11, D11, D33, F[:nm], F[nm:], **solver_opts)
+                dx, o = c.killer_solver(D11, D11, D33, F[:nm], F[nm:], accept_diverged=True,
+                                        **solver_opts)
+            if o.status == MDG_ERR_DIVERGED:
+                # the reference warns and carries on with the unconverged
+                # correction; the line search (or alpha) decides what it is worth
+                warnings.warn("linear solve of non-linear iteration %d stopped at relative "
+                              "residual %.3e" % (nit, o.rel_res))
+                st["diverged_solves"].append(nit)
             stats.append(o)
             alpha, stagnated = -1.0, False
             if line_search or is_newton:

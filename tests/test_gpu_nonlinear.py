@@ -151,3 +151,27 @@ def test_shear_template_newton_iteration_counts(oracle_mod, linear):
         assert nsolve == 1
     else:
         assert 1 < nsolve < 10
+
+
+def test_unconverged_newton_solves_carry_on(oracle_mod):
+    """Newton on the heavily yielding shear-band box (127^2, C = 1): the
+    linear solve on the plastic Jacobian misses its tolerance. Like the
+    reference's KillerSolver after max_its (Solvers.c:1298-1334), the loop
+    warns, records the iteration and carries on with the last iterate, the
+    line search weighing the step; the Picard solves before the switch all
+    converge."""
+    import warnings
+    from shearbox import build
+    ctx, loop, st, info = build(oracle_mod, 127, C=1.0)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        hist, nsolve, stats = loop.run(st, 8, (1e-12, 1e-8, 1e-12, 1e-8), diag_scaling=1,
+                                       line_search=True, newton=True, rel_tol=1e-8, max_its=100)
+    h = np.array(hist)
+    print("newton steps", st["newton_steps"], "unconverged", st["diverged_solves"],
+          "alphas", st["alphas"], "history", (h / h[0])[:, :3].tolist(), flush=True)
+    assert st["newton_steps"] and set(st["diverged_solves"]) <= set(st["newton_steps"])
+    assert len([x for x in w if "stopped at relative residual" in str(x.message)]) == len(st["diverged_solves"])
+    for nit, o in zip(range(nsolve), stats):
+        assert (o.status == mdgpu.MDG_ERR_DIVERGED) == (nit in st["diverged_solves"])
+    assert np.isfinite(h).all() and h[-1, 0] < h[0, 0]

@@ -25,52 +25,22 @@ import time
 import numpy as np
 import pytest
 
-import mdgpu
-from mdgpu import setups as S
-from mdgpu.nonlinear import PicardLoop
+from shearbox import build
 
 pytestmark = pytest.mark.gpu
 
 
 def test_shear_band_box_1023(oracle_mod):
-    from oracle.driver import initial_fields
-    O = oracle_mod
-    n = 1023
-    box = (0.0, 1.0, -1.0, 0.0)
     t0 = time.time()
-    g = O.Grid(n + 1, n + 1, *box)
-    g.pure_shear(1.0)
-    x, z = S.put_markers(g.xg, g.zg, g.dx, g.dz, 4, 4, seed=7)
-    ph = S.seed_phase(x, z, S.weak_seeds(9, 64, 2 * g.dx, *box)).astype(np.int32)
-    m = np.arange(x.size, dtype=np.uint64)
-    u1, u2 = S.uniform(77, 2 * m), S.uniform(77, 2 * m + np.uint64(1))
-    pref_m = np.exp(0.5 * np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2))   # Box-Muller
-    print("markers", x.size, "in weak seeds", int(ph.sum()), "set-up %.1fs" % (time.time() - t0), flush=True)
-
-    ctx = mdgpu.Context(n + 1, n + 1, *box)
-    ctx.set_bc(g.BCu_t, g.BCu_v, g.BCv_t, g.BCv_v, g.BCp_t, g.BCg_t)
-    ctx.markers_set(x, z, ph)
-    perc_n = np.array(ctx.p2mastah(np.zeros(2), prop=1, centroid=1, nb_phases=2))
-    perc_s = np.array(ctx.p2mastah(np.zeros(2), prop=1, centroid=0, nb_phases=2))
-    pref_n = ctx.p2mastah(pref_m, flag=1, avg=2, centroid=1)
-    pref_s = ctx.p2mastah(pref_m, flag=1, avg=2, centroid=0)
-    assert np.abs(perc_n.sum(axis=0) - 1.0).max() < 1e-12
-    assert np.abs(perc_s.sum(axis=0) - 1.0).max() < 1e-12
-    for f in (pref_n, pref_s):
+    ctx, loop, st, info = build(oracle_mod, 1023, C=1.5)
+    nc, pref_m = info["nc"], info["pref_m"]
+    print("markers", info["markers"], "in weak seeds", info["weak"], "set-up %.1fs" % (time.time() - t0),
+          flush=True)
+    assert np.abs(st["perc_n"].sum(axis=0) - 1.0).max() < 1e-12
+    assert np.abs(st["perc_s"].sum(axis=0) - 1.0).max() < 1e-12
+    for f in (st["pref_n"], st["pref_s"]):
         assert f.min() >= pref_m.min() and f.max() <= pref_m.max()
         assert abs(np.log(f).mean()) < 0.01
-
-    nc, nv = n * n, (n + 1) ** 2
-    u, v, p = initial_fields(g)
-    s0 = 0.2 + 0.05 * np.random.default_rng(3).standard_normal(nc)
-    st = dict(u=u, v=v, p=p, T=np.ones(nc), mu_n=np.ones(nc), mu_s=np.ones(nv), perc_n=perc_n,
-              perc_s=perc_s, sxxd0=s0, szzd0=-s0, sxz0_n=np.zeros(nc), sxxd0_s=ctx.c2v(s0),
-              szzd0_s=ctx.c2v(-s0), sxz0=np.zeros(nv), rho_n=np.ones(nc), rho_s=np.ones(nv),
-              pref_n=pref_n, pref_s=pref_s)
-    prm = mdgpu.RheoParams(average=0, step=1, elastic=1, dt=1.0, R=1.0, zeroC=0.0, min_eta=1e-4,
-                           max_eta=1e4, eta_tol=1e-13, Tiimin=1e-5, scale_E=1.0, scale_eta=1.0)
-    eu, ev, ep = g.numbering()[:3]
-    loop = PicardLoop(ctx, eu, ev, ep, [mdgpu.PhaseProp(**d) for d in S.vep_phases(C=1.5)], prm)
     t0 = time.time()
     hist, nsolve, stats = loop.run(st, 10, (1e-12, 1e-8, 1e-12, 1e-8), diag_scaling=1,
                                    line_search=True, rel_tol=1e-8)
@@ -84,6 +54,7 @@ def test_shear_band_box_1023(oracle_mod):
     assert all(o.rel_res <= 1e-8 for o in stats)
     assert rel[-1, 0] < 0.1 and rel[-1, 1] < 0.1
     assert rel[-1, 2] < 1e-8
+    assert st["diverged_solves"] == []
     _, cen, ver = loop.update_nonlinearity(st)
     npl = int((cen["eII_pl"] > 0).sum())
     print("centres yielding", npl, "of", nc, flush=True)
