@@ -65,6 +65,10 @@ def main():
     ap.add_argument("--trace-solve", type=int, default=0, metavar="ITS",
                     help="profiling mode: P2G then ITS outer iterations of the solve with the "
                          "inner CG launched kernel by kernel (no HIP graphs), for kernel tracers")
+    ap.add_argument("--rheology", type=int, default=0, metavar="REPS",
+                    help="profiling mode: only the grid rheology update (NonNewtonianViscosityGrid "
+                         "on the centres of the --cells grid, two visco-elasto-plastic phases), "
+                         "REPS times, inputs resident in HBM")
     ap.add_argument("--trace-mixed", action="store_true",
                     help="with --trace-solve: the FP32 V-cycle (MDG_PREC_MIXED) instead of FP64")
     a = ap.parse_args()
@@ -77,6 +81,9 @@ def main():
     from mdgpu import learned
     from mdgpu import workloads as W
 
+    if a.rheology:
+        rheology_only(a.cells, a.rheology)
+        return
     t0 = time.time()
     wl = W.ellipse_workload(a.cells, a.nper, a.seed, n_ell=a.ellipses)
     n = wl["x"].size
@@ -288,6 +295,50 @@ def kernels_only(ctx, reps, step, f_cent, f_vert, D11, en, es, F, nm, sol, eta_s
     out["copy_bytes"] = 16.0 * n
     out["markers"] = n
     print(json.dumps(out), flush=True)
+
+
+def rheology_only(cells, reps):
+    """The grid rheology kernel alone: ViscosityConcise per centre and phase
+    (power law n = 3 with a per-node prefactor, Maxwell elasticity,
+    Drucker-Prager yield; 10% of the centres in the weaker phase), synthetic
+    strain rates, pressures and old stresses around the yield point, every
+    array on the device. Prints the wall time per call (the kernel time
+    comes from a kernel trace of the same command)."""
+    import numpy as np
+    import mdgpu
+    from mdgpu import setups as S
+    nc = cells * cells
+    from mdgpu import learned
+    ctx = mdgpu.Context(cells + 1, cells + 1, 0.0, 1.0, -1.0, 0.0)
+    ctx.set_bc(*learned.free_slip_bc(cells + 1))
+    rng = np.random.default_rng(5)
+    weak = rng.uniform(0.0, 1.0, nc) < 0.1
+    exx = 1.0 + 0.2 * rng.standard_normal(nc)
+    s0 = 0.2 + 0.05 * rng.standard_normal(nc)
+    host = dict(perc=np.concatenate([(~weak).astype(float), weak.astype(float)]), G=np.ones(nc),
+                T=np.ones(nc), P=0.5 + 0.1 * rng.standard_normal(nc), exx=exx, ezz=-exx,
+                exz=0.1 * rng.standard_normal(nc), sxx0=s0, szz0=-s0, sxz0=np.zeros(nc),
+                pref=np.exp(0.5 * rng.standard_normal(nc)))
+    arrays = {k: ctx.device(v.size, host=v) for k, v in host.items()}
+    arrays.update({k: ctx.device(nc) for k in mdgpu.RHEO_OUT})
+    phases = [mdgpu.PhaseProp(**d) for d in S.vep_phases(C=1.5)]
+    prm = mdgpu.RheoParams(average=0, step=1, elastic=1, dt=1.0, R=1.0, zeroC=0.0, min_eta=1e-4,
+                           max_eta=1e4, eta_tol=1e-13, Tiimin=1e-5, scale_E=1.0, scale_eta=1.0)
+    ctx.viscosity_grid(phases, prm, 1, **arrays)
+    t = []
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.viscosity_grid(phases, prm, 1, **arrays)
+        t.append(ctx.last_kernel_ms())
+    wall = (time.perf_counter() - t0) / reps
+    k_ms = sorted(t)[len(t) // 2]
+    # algorithmic bytes per centre: 2 phase fractions + 10 inputs read, 7 outputs written, 1 B tag
+    nbytes = (12 + 7) * 8.0 * nc + nc
+    yielding = int((arrays["eII_pl"].numpy() > 0).sum())
+    print(json.dumps({"rheology_centres": nc, "phases": 2, "kernel_ms": k_ms,
+                      "wall_ms_per_call": 1e3 * wall, "mnodes_per_s": nc / (k_ms * 1e-3) / 1e6,
+                      "alg_bytes": nbytes, "achieved_gbs": nbytes / (k_ms * 1e-3) / 1e9,
+                      "yielding_centres": yielding}), flush=True)
 
 
 def cpu_baseline(wl, a, n):
