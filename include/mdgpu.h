@@ -305,6 +305,27 @@ int mdg_killer_solver(mdg_ctx *ctx, mdg_solver_opts *opts, const double *D11_n,
                       const double *D22_n, const double *D33_s, const double *rhs_mom,
                       const double *rhs_cont, double *sol);
 
+/* ---------------------------------------------------------------- advection
+ * V2P (ParticleRoutines.c:63-256): the grid velocity at the markers. u on
+ * the Vx nodes (Nx x (Nz+1), mesh->u_in), v on the Vz nodes ((Nx+1) x Nz,
+ * mesh->v_in); conserv_interp = model.conserv_interp (Taras' correction);
+ * inactive markers (phase -1) get 0. The reference's air-node (tag 30)
+ * substitutions are kept, though mdg_set_bc does not accept free-surface
+ * tags yet. Arrays may be host or device memory. */
+int mdg_v2p(mdg_ctx *ctx, const double *u, const double *v, int conserv_interp, int n, const double *x,
+            const double *z, const int *phase, double *vx, double *vz);
+
+/* RogerGuntherII (AdvectionRoutines.c:347-410; prototype
+ * mdoodz-private.h:593; called at Main_DOODZ.c:1304) with isoutPart
+ * (AdvectionRoutines.c:320-341, non-periodic box): fourth-order Runge-Kutta
+ * with V2P at each stage. x, z, phase are updated in place (markers leaving
+ * the box get phase -1 and keep their exit position); vx, vz (may be NULL:
+ * particles->Vx/Vz) receive the RK4 velocity of the markers still active
+ * and keep their values otherwise. The sorted copy of mdg_markers_set is
+ * not touched: call it again with the new positions before the next P2G. */
+int mdg_advect_rk4(mdg_ctx *ctx, const double *u, const double *v, double dt, int conserv_interp, int n,
+                   double *x, double *z, int *phase, double *vx, double *vz);
+
 /* ---------------------------------------------------------------- Newton
  * RheologicalOperators with Jacobian = 1 (StokesRoutines.c:250-375;
  * prototype mdoodz-private.h:607), isotropic and incompressible: the

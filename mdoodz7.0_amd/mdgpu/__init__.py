@@ -76,6 +76,8 @@ def lib():
         L.mdg_dev_free.argtypes = [vp, vp]
         L.mdg_memcpy.argtypes = [vp, vp, vp, C.c_ulonglong]
         L.mdg_dev_scale.argtypes = [vp, C.c_longlong, C.c_double, vp, vp]
+        L.mdg_v2p.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        L.mdg_advect_rk4.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         _extra_signatures(L)
         _LIB = L
     return _LIB
@@ -290,6 +292,22 @@ class Context:
         jp = np.zeros(self.n_markers, np.int32)
         self._check(self._L.mdg_marker_index(self.h, centroid, ptr(ip), ptr(jp)))
         return ip, jp
+
+    def v2p(self, u, v, x, z, phase, conserv=0, vx=None, vz=None):
+        """V2P (ParticleRoutines.c:63): (vx, vz) at the markers."""
+        n = len(x)
+        vx = np.zeros(n) if vx is None else vx
+        vz = np.zeros(n) if vz is None else vz
+        self._check(self._L.mdg_v2p(self.h, ptr(u), ptr(v), conserv, n, ptr(x), ptr(z), ptr(phase),
+                                    ptr(vx), ptr(vz)))
+        return vx, vz
+
+    def advect_rk4(self, u, v, dt, x, z, phase, conserv=0, vx=None, vz=None):
+        """RogerGuntherII (AdvectionRoutines.c:347): x, z, phase (and vx, vz
+        when given) updated in place; numpy arrays must be contiguous float64
+        / int32, or DeviceArray."""
+        self._check(self._L.mdg_advect_rk4(self.h, ptr(u), ptr(v), dt, conserv, len(x), ptr(x), ptr(z),
+                                           ptr(phase), ptr(vx), ptr(vz)))
 
     def markers_gather(self, src, dst_sorted):
         self._check(self._L.mdg_markers_gather(self.h, ptr(src), ptr(dst_sorted)))

@@ -94,6 +94,10 @@ def lib():
                                    C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, f64p,
                                    C.c_int, C.c_void_p, C.c_int, C.c_int]
         L.orc_c2v.argtypes = [gp, C.c_int, f64p, f64p]
+        L.orc_v2p.argtypes = [gp, f64p, f64p, C.c_int, C.c_int, f64p, f64p, i32p, f64p, f64p]
+        L.orc_advect_rk4.argtypes = [gp, f64p, f64p, C.c_double, C.c_int, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_int, f64p, f64p, i32p, C.c_void_p,
+                                     C.c_void_p]
         L.orc_apply_bc.argtypes = [gp, f64p, f64p]
         L.orc_exz_n.argtypes = [gp, f64p, f64p]
         L.orc_strain_rates.argtypes = [gp, C.c_int, f64p, f64p, f64p, f64p, f64p, f64p,
@@ -180,6 +184,31 @@ def marker_index(g, centroid, x, z):
     lib().orc_marker_index(g.c, centroid, len(x), np.ascontiguousarray(x),
                            np.ascontiguousarray(z), ip, jp)
     return ip, jp
+
+
+def v2p(g, u, v, x, z, phase, conserv=0):
+    """ParticleRoutines.c:63-256: (vx, vz) at the markers."""
+    n = len(x)
+    vx, vz = np.zeros(n), np.zeros(n)
+    lib().orc_v2p(g.c, np.ascontiguousarray(u, np.float64), np.ascontiguousarray(v, np.float64),
+                  conserv, n, np.ascontiguousarray(x, np.float64), np.ascontiguousarray(z, np.float64),
+                  np.ascontiguousarray(phase, np.int32), vx, vz)
+    return vx, vz
+
+
+def advect_rk4(g, u, v, dt, x, z, phase, conserv=0, vx=None, vz=None):
+    """AdvectionRoutines.c:347-410: returns new (x, z, phase, vx, vz); vx, vz
+    keep their input values (zeros if not given) for markers that end
+    inactive."""
+    n = len(x)
+    x, z = np.array(x, np.float64), np.array(z, np.float64)
+    phase = np.array(phase, np.int32)
+    vx = np.zeros(n) if vx is None else np.array(vx, np.float64)
+    vz = np.zeros(n) if vz is None else np.array(vz, np.float64)
+    lib().orc_advect_rk4(g.c, np.ascontiguousarray(u, np.float64), np.ascontiguousarray(v, np.float64),
+                         dt, conserv, g.xmin, g.xmax, g.zmin, g.zmax, n, x, z, phase,
+                         vx.ctypes.data, vz.ctypes.data)
+    return x, z, phase, vx, vz
 
 
 def p2mastah(g, x, z, phase, mat_prop, flag=0, avg=0, prop=0, centroid=1, stencil=1,

@@ -1,0 +1,158 @@
+/*
+ * Marker advection: V2P (ParticleRoutines.c:63-256) and the fourth-order
+ * Runge-Kutta step of RogerGuntherII (AdvectionRoutines.c:347-410) with
+ * isoutPart (AdvectionRoutines.c:320-341, non-periodic box).
+ * TEST INFRASTRUCTURE ONLY (see mdoracle.h).
+ */
+#include <math.h>
+
+#include "mdoracle.h"
+
+/* the cell of a staggered component: ceil(d/h) - 1 clamped to [0, n-2]
+ * (ParticleRoutines.c:81-88, 92-99) */
+static int v2p_cell(double d, double h, int n) {
+    int j = (int)ceil(d / h) - 1;
+    if (j < 0) j = 0;
+    if (j > n - 2) j = n - 2;
+    return j;
+}
+
+/* Taras' conservative correction of one interpolated row/column
+ * (ParticleRoutines.c:127-158, 216-248): 0.5 (t - 0.5)^2 times the second
+ * difference on the side of the marker. */
+static double v2p_corr(double t) { return 0.5 * pow(t - 0.5, 2.0); }
+
+/* V2P for one active marker (ParticleRoutines.c:76-252). u: Vx on
+ * Nx x (Nz+1) nodes (xg, zvx); v: Vz on (Nx+1) x Nz nodes (xvz, zg). */
+static void v2p_one(const orc_grid *g, const double *u, const double *v, int conserv, double x, double z,
+                    double *vxm, double *vzm) {
+    const double dx = g->dx, dz = g->dz;
+    const signed char *tu = g->BCu_t, *tv = g->BCv_t;
+    /* Vx */
+    {
+        const int Nx = g->Nx, Nz = g->Nz + 1;
+        const int j = v2p_cell(x - g->xg[0], dx, Nx), i = v2p_cell(z - g->zvx[0], dz, Nz);
+        const double dxm = fabs(x - g->xg[j]), dzm = fabs(z - g->zvx[i]);
+        const int iSW = j + i * Nx, iSE = iSW + 1, iNW = j + (i + 1) * Nx, iNE = iNW + 1;
+        double VNW = u[iNW], VNE = u[iNE], VSW = u[iSW], VSE = u[iSE];
+        if (tu[iSW] == 30) VSW = u[iNW];
+        if (tu[iSE] == 30) VSE = u[iNE];
+        if (tu[iNW] == 30) VNW = u[iSW];
+        if (tu[iNE] == 30) VNE = u[iSE];
+        if (tu[iNE] == 30 && tu[iSE] == 30) { VNE = VSW; VSE = VSW; }
+        if (tu[iNW] == 30 && tu[iSW] == 30) { VNW = VSE; VSW = VSE; }
+        double v13 = VSW * (1.0 - dxm / dx) + VSE * dxm / dx;
+        double v24 = VNW * (1.0 - dxm / dx) + VNE * dxm / dx;
+        if (conserv == 1) {
+            if (dxm / dx >= 0.5) {
+                if (j < Nx - 2) {
+                    const int iSEE = iSE + 1, iNEE = iNE + 1;
+                    double VSEE = u[iSEE], VNEE = u[iNEE];
+                    if (tu[iSEE] == 30) VSEE = u[iSW];
+                    if (tu[iNEE] == 30) VNEE = u[iNW];
+                    v13 = v13 + v2p_corr(dxm / dx) * (VSW - 2.0 * VSE + VSEE);
+                    v24 = v24 + v2p_corr(dxm / dx) * (VNW - 2.0 * VNE + VNEE);
+                }
+            } else if (j > 0) {
+                const int iSWW = iSW - 1, iNWW = iNW - 1;
+                double VSWW = u[iSWW], VNWW = u[iNWW];
+                if (tu[iSWW] == 30) VSWW = u[iSE];
+                if (tu[iNWW] == 30) VNWW = u[iNE];
+                v13 = v13 + v2p_corr(dxm / dx) * (VSWW - 2.0 * VSW + VSE);
+                v24 = v24 + v2p_corr(dxm / dx) * (VNWW - 2.0 * VNW + VNE);
+            }
+        }
+        *vxm = (1.0 - dzm / dz) * v13 + (dzm / dz) * v24;
+    }
+    /* Vz */
+    {
+        const int Nx = g->Nx + 1, Nz = g->Nz;
+        const int j = v2p_cell(x - g->xvz[0], dx, Nx), i = v2p_cell(z - g->zg[0], dz, Nz);
+        const double dxm = fabs(x - g->xvz[j]), dzm = fabs(z - g->zg[i]);
+        const int iSW = j + i * Nx, iSE = iSW + 1, iNW = j + (i + 1) * Nx, iNE = iNW + 1;
+        double VSE = v[iSE], VSW = v[iSW], VNE = v[iNE], VNW = v[iNW];
+        if (tv[iSW] == 30) VSW = v[iNE];
+        if (tv[iSE] == 30) VSE = v[iNE];
+        if (tv[iNW] == 30) VNW = v[iSW];
+        if (tv[iNE] == 30) VNE = v[iSE];
+        if (tv[iNE] == 30 && tv[iSE] == 30) { VNE = VSW; VSE = VSW; }
+        if (tv[iNW] == 30 && tv[iSW] == 30) { VNW = VSE; VSW = VSE; }
+        double v12 = VSW * (1.0 - dzm / dz) + VNW * dzm / dz;
+        double v34 = VSE * (1.0 - dzm / dz) + VNE * dzm / dz;
+        if (conserv == 1) {
+            if (dzm / dz >= 0.5) {
+                if (i < Nz - 2) {
+                    const int iNNW = j + (i + 2) * Nx, iNNE = iNNW + 1;
+                    double VNNW = v[iNNW], VNNE = v[iNNE];
+                    if (tv[iNNW] == 30) VNNW = v[iSW];
+                    if (tv[iNNE] == 30) VNNE = v[iSE];
+                    v12 = v12 + v2p_corr(dzm / dz) * (VSW - 2.0 * VNW + VNNW);
+                    v34 = v34 + v2p_corr(dzm / dz) * (VSE - 2.0 * VNE + VNNE);
+                }
+            } else if (i > 0) {
+                const int iSSW = j + (i - 1) * Nx, iSSE = iSSW + 1;
+                double VSSW = v[iSSW], VSSE = v[iSSE];
+                if (tv[iSSW] == 30) VSSW = v[iNW];
+                if (tv[iSSE] == 30) VSSE = v[iNE];
+                v12 = v12 + v2p_corr(dzm / dz) * (VSSW - 2.0 * VSW + VNW);
+                v34 = v34 + v2p_corr(dzm / dz) * (VSSE - 2.0 * VSE + VNE);
+            }
+        }
+        *vzm = (1.0 - dxm / dx) * v12 + (dxm / dx) * v34;
+    }
+}
+
+void orc_v2p(const orc_grid *g, const double *u, const double *v, int conserv, int n, const double *x,
+             const double *z, const int *phase, double *vx, double *vz) {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n; k++) {
+        vx[k] = 0.0;
+        vz[k] = 0.0;
+        if (phase[k] != -1) v2p_one(g, u, v, conserv, x[k], z[k], &vx[k], &vz[k]);
+    }
+}
+
+/* isoutPart, non-periodic (AdvectionRoutines.c:336-338) */
+static void isout(double x, double z, const double box[4], int *phase) {
+    if (x < box[0] || x > box[1] || z < box[2] || z > box[3]) *phase = -1;
+}
+
+/* RogerGuntherII's loop body (AdvectionRoutines.c:372-405) for every marker */
+void orc_advect_rk4(const orc_grid *g, const double *u, const double *v, double dt, int conserv,
+                    double xmin, double xmax, double zmin, double zmax, int n, double *x, double *z,
+                    int *phase, double *vx, double *vz) {
+    const double box[4] = {xmin, xmax, zmin, zmax};
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n; k++) {
+        const double xA = x[k], zA = z[k];
+        double VxA = 0.0, VzA = 0.0, VxB = 0.0, VzB = 0.0, VxC = 0.0, VzC = 0.0, VxD = 0.0, VzD = 0.0;
+        if (phase[k] != -1) v2p_one(g, u, v, conserv, x[k], z[k], &VxA, &VzA);
+        if (phase[k] != -1) {
+            x[k] = xA + 0.5 * dt * VxA;
+            z[k] = zA + 0.5 * dt * VzA;
+        }
+        isout(x[k], z[k], box, &phase[k]);
+        if (phase[k] != -1) v2p_one(g, u, v, conserv, x[k], z[k], &VxB, &VzB);
+        if (phase[k] != -1) {
+            x[k] = xA + 0.5 * dt * VxB;
+            z[k] = zA + 0.5 * dt * VzB;
+        }
+        isout(x[k], z[k], box, &phase[k]);
+        if (phase[k] != -1) v2p_one(g, u, v, conserv, x[k], z[k], &VxC, &VzC);
+        if (phase[k] != -1) {
+            x[k] = xA + 1.0 * dt * VxC;
+            z[k] = zA + 1.0 * dt * VzC;
+        }
+        isout(x[k], z[k], box, &phase[k]);
+        if (phase[k] != -1) v2p_one(g, u, v, conserv, x[k], z[k], &VxD, &VzD);
+        VxA = (1.0 / 6.0) * (VxA + 2.0 * VxB + 2.0 * VxC + VxD);
+        VzA = (1.0 / 6.0) * (VzA + 2.0 * VzB + 2.0 * VzC + VzD);
+        if (phase[k] != -1) {
+            x[k] = xA + dt * VxA;
+            z[k] = zA + dt * VzA;
+            if (vx) vx[k] = VxA;
+            if (vz) vz[k] = VzA;
+        }
+        isout(x[k], z[k], box, &phase[k]);
+    }
+}

@@ -59,6 +59,19 @@ void orc_p2mastah(const orc_grid *g, int n, const double *x, const double *z, co
 /* GridRoutines.c:108-212 */
 void orc_c2v(const orc_grid *g, int periodic_x, const double *cent, double *vert);
 
+/* ParticleRoutines.c:63-256 (V2P): grid velocity at the markers; u on the
+ * Vx nodes, v on the Vz nodes; conserv = model.conserv_interp (Taras'
+ * correction); inactive markers (phase -1) get 0. */
+void orc_v2p(const orc_grid *g, const double *u, const double *v, int conserv, int n, const double *x,
+             const double *z, const int *phase, double *vx, double *vz);
+
+/* AdvectionRoutines.c:347-410 (RogerGuntherII, RK4 with V2P at each stage)
+ * and :320-341 (isoutPart, non-periodic): x, z, phase in place; vx, vz (may
+ * be NULL) receive the RK4 velocity of the markers still active. */
+void orc_advect_rk4(const orc_grid *g, const double *u, const double *v, double dt, int conserv,
+                    double xmin, double xmax, double zmin, double zmax, int n, double *x, double *z,
+                    int *phase, double *vx, double *vz);
+
 /* StokesRoutines.c:415-~500 (ghost values from BC tags) */
 void orc_apply_bc(const orc_grid *g, double *u, double *v);
 

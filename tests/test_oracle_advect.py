@@ -1,0 +1,77 @@
+"""Oracle V2P and RK4 advection (ParticleRoutines.c:63-256,
+AdvectionRoutines.c:320-410) pinned by what the mathematics fixes: bilinear
+V2P reproduces linear velocity fields at every marker of the box (with or
+without Taras' correction, whose second differences vanish on them); the
+classic RK4 step on a linear autonomous field is the fourth-order Taylor
+polynomial of exp(dt A); markers that leave the box are flagged inactive and
+stay where they left; inactive markers do not move."""
+import numpy as np
+import pytest
+
+from mdgpu import setups as S
+
+
+def _linear(g, a):
+    """u, v on the Vx / Vz nodes of u = a0 + a1 x + a2 z, v = a3 + a4 x + a5 z."""
+    xg, zvx = np.asarray(g.xg), np.asarray(g.zvx)
+    xvz, zg = np.asarray(g.xvz), np.asarray(g.zg)
+    u = (a[0] + a[1] * xg[None, :] + a[2] * zvx[:, None]).ravel()
+    v = (a[3] + a[4] * xvz[None, :] + a[5] * zg[:, None]).ravel()
+    return u, v
+
+
+@pytest.mark.parametrize("conserv", [0, 1])
+def test_v2p_reproduces_linear_fields(oracle_mod, conserv):
+    O = oracle_mod
+    g = O.Grid(41, 33, -1.0, 1.0, 0.0, 0.8)
+    a = (0.3, 0.5, -0.2, -0.1, 0.25, -0.5)
+    u, v = _linear(g, a)
+    x, z = S.put_markers(g.xg, g.zg, g.dx, g.dz, 3, 3, seed=4)
+    ph = np.zeros(x.size, np.int32)
+    ph[::97] = -1
+    vx, vz = O.v2p(g, u, v, x, z, ph, conserv=conserv)
+    act = ph != -1
+    np.testing.assert_allclose(vx[act], (a[0] + a[1] * x + a[2] * z)[act], rtol=0, atol=2e-15)
+    np.testing.assert_allclose(vz[act], (a[3] + a[4] * x + a[5] * z)[act], rtol=0, atol=2e-15)
+    assert (vx[~act] == 0).all() and (vz[~act] == 0).all()
+
+
+def test_rk4_on_a_linear_field_is_the_taylor_polynomial(oracle_mod):
+    O = oracle_mod
+    g = O.Grid(65, 65, -1.0, 1.0, -1.0, 1.0)
+    # rotation plus pure shear about the origin: d/dt (x, z) = A (x, z)
+    A = np.array([[0.2, -1.0], [0.7, -0.2]])
+    u, v = _linear(g, (0.0, A[0, 0], A[0, 1], 0.0, A[1, 0], A[1, 1]))
+    rng = np.random.default_rng(2)
+    x, z = rng.uniform(-0.4, 0.4, 2000), rng.uniform(-0.4, 0.4, 2000)
+    dt = 0.05
+    xn, zn, ph, vx, vz = O.advect_rk4(g, u, v, dt, x, z, np.zeros(x.size, np.int32))
+    hA = dt * A
+    T = np.eye(2) + hA + hA @ hA / 2 + hA @ hA @ hA / 6 + hA @ hA @ hA @ hA / 24
+    ref = T @ np.stack([x, z])
+    assert (ph == 0).all()
+    np.testing.assert_allclose(xn, ref[0], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(zn, ref[1], rtol=0, atol=1e-15)
+    # the stored velocity is the RK4 average: (x_new - x) / dt
+    np.testing.assert_allclose(vx, (xn - x) / dt, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(vz, (zn - z) / dt, rtol=0, atol=1e-13)
+
+
+def test_markers_leaving_the_box_become_inactive(oracle_mod):
+    O = oracle_mod
+    g = O.Grid(21, 21, 0.0, 1.0, 0.0, 1.0)
+    u, v = _linear(g, (1.0, 0.0, 0.0, 0.0, 0.0, 0.0))   # uniform flow in +x
+    x = np.array([0.5, 0.97, 0.99, 0.2])
+    z = np.array([0.5, 0.5, 0.5, 0.5])
+    ph = np.array([0, 0, 0, -1], np.int32)
+    vx0 = np.full(4, 7.0)
+    xn, zn, phn, vx, vz = O.advect_rk4(g, u, v, 0.05, x, z, ph, vx=vx0, vz=vx0)
+    assert list(phn) == [0, -1, -1, -1]
+    assert xn[0] == pytest.approx(0.55, abs=1e-15)
+    # 0.99 leaves at the first half step: stays at 0.99 + 0.025
+    assert xn[2] == pytest.approx(1.015, abs=1e-15)
+    # 0.97 leaves at the final step (0.97 + 0.05 = 1.02 > 1)
+    assert xn[1] == pytest.approx(1.02, abs=1e-15)
+    assert xn[3] == 0.2 and zn[3] == 0.5
+    # velocities are written only for markers still active at the end
+    assert vx[0] == pytest.approx(1.0) and vx[1] == 7.0 and vx[2] == 7.0 and vx[3] == 7.0
