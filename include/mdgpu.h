@@ -326,6 +326,21 @@ int mdg_v2p(mdg_ctx *ctx, const double *u, const double *v, int conserv_interp, 
 int mdg_advect_rk4(mdg_ctx *ctx, const double *u, const double *v, double dt, int conserv_interp, int n,
                    double *x, double *z, int *phase, double *vx, double *vz);
 
+/* ---------------------------------------------------------------- grid -> markers
+ * Interp_Grid2P_centroids2 (ParticleRoutines.c:1332-1580; non-periodic): a
+ * centre field F ((Nx-1) x (Nz-1)) at the markers, bilinear between the
+ * centres, copied along the box edges, BCp tag 30/31 cells replaced by their
+ * vertical neighbour, normalised by the weight sum; inactive markers get 0. */
+int mdg_g2p_centroids(mdg_ctx *ctx, const double *F, int n, const double *x, const double *z,
+                      const int *phase, double *out);
+
+/* UpdateParticlePressure (RheologyParticles.c:1621-1704; called after the
+ * solve) with model.subgrid_diffusion < 2: dp = p_in - p0_n on the cells not
+ * tagged 30 (mesh->dp, returned when dp is not NULL), interpolated as above
+ * and added to the marker pressure P (particles->P) in place. */
+int mdg_update_particle_pressure(mdg_ctx *ctx, const double *p_in, const double *p0_n, double *dp, int n,
+                                 const double *x, const double *z, const int *phase, double *P);
+
 /* ---------------------------------------------------------------- Newton
  * RheologicalOperators with Jacobian = 1 (StokesRoutines.c:250-375;
  * prototype mdoodz-private.h:607), isotropic and incompressible: the

@@ -1,0 +1,181 @@
+// Grid -> marker updates: Interp_Grid2P_centroids2 (ParticleRoutines.c:
+// 1332-1580, non-periodic) and UpdateParticlePressure (RheologyParticles.c:
+// 1621-1704) without subgrid diffusion. One thread per marker gathers the
+// four centres around it (the centre field stays in L2); the marker streams
+// x, z, phase in and the field (or P read-modify-write) out. Same arithmetic
+// and order as the reference, built without contraction: bit-exact.
+#include "mdg_internal.h"
+
+namespace {
+
+struct cgrid {
+    int Nx, Nz;               // centres per row / column
+    double dx, dz;
+    const double *X, *Z;      // xvz, zvx (centre coordinates with one ghost each side)
+    const signed char *tag;   // BCp
+};
+
+__device__ double g2p_cent_one(const cgrid &g, const double *__restrict__ F, double x, double z) {
+    const int Nx = g.Nx, Nz = g.Nz;
+    const double dx = g.dx, dz = g.dz;
+    const int j = (int)ceil((x - g.X[0]) / dx) - 1;
+    const int i = (int)ceil((z - g.Z[0]) / dz) - 1;
+    int iSW = 0, iSE = 0, iNW = 0, iNE = 0;
+    if (j > 0 && j < Nx && i > 0 && i < Nz) {
+        iSW = (j - 1) + (i - 1) * Nx; iSE = iSW + 1; iNW = (j - 1) + i * Nx; iNE = iNW + 1;
+    } else if (j == 0 && i > 0 && i < Nz) {          // west edge: copy along it
+        iSW = iSE = (i - 1) * Nx; iNW = iNE = i * Nx;
+    } else if (j == Nx && i > 0 && i < Nz) {         // east edge
+        iSW = iSE = (j - 1) + (i - 1) * Nx; iNW = iNE = (j - 1) + i * Nx;
+    } else if (i == 0 && j > 0 && j < Nx) {          // south edge
+        iSW = iNW = j - 1; iSE = iNE = j;
+    } else if (i == Nz && j > 0 && j < Nx) {         // north edge
+        iSW = iNW = (j - 1) + (i - 1) * Nx; iSE = iNE = iSW + 1;
+    } else if (i == 0 && j == 0) {
+        iSW = iSE = iNW = iNE = 0;
+    } else if (i == 0 && j == Nx) {
+        iSW = iSE = iNW = iNE = j - 1;
+    } else if (i == Nz && j == 0) {
+        iSW = iSE = iNW = iNE = (i - 1) * Nx;
+    } else if (i == Nz && j == Nx) {
+        iSW = iSE = iNW = iNE = (j - 1) + (i - 1) * Nx;
+    }
+    // markers of the box always land in one of the cases above; clamp the
+    // coordinate index for any other input so every read stays in range
+    const int jj = j < 0 ? 0 : (j > Nx ? Nx : j), ii = i < 0 ? 0 : (i > Nz ? Nz : i);
+    const double dxm = x - g.X[jj], dzm = z - g.Z[ii];
+    const signed char *t = g.tag;
+    const bool aSW = t[iSW] == 30 || t[iSW] == 31, aSE = t[iSE] == 30 || t[iSE] == 31;
+    const bool aNW = t[iNW] == 30 || t[iNW] == 31, aNE = t[iNE] == 30 || t[iNE] == 31;
+    double val = 0.0, sumW = 0.0;
+    val += (1.0 - dxm / dx) * (1.0 - dzm / dz) * F[aSW ? iNW : iSW];
+    sumW += (1.0 - dxm / dx) * (1.0 - dzm / dz);
+    val += (dxm / dx) * (1.0 - dzm / dz) * F[aSE ? iNE : iSE];
+    sumW += (dxm / dx) * (1.0 - dzm / dz);
+    val += (1.0 - dxm / dx) * (dzm / dz) * F[aNW ? iSW : iNW];
+    sumW += (1.0 - dxm / dx) * (dzm / dz);
+    val += (dxm / dx) * (dzm / dz) * F[aNE ? iSE : iNE];
+    sumW += (dxm / dx) * (dzm / dz);
+    if (sumW > 1e-13) val /= sumW;
+    return val;
+}
+
+__global__ void k_g2p_centroids(cgrid g, const double *__restrict__ F, int n, const double *__restrict__ x,
+                                const double *__restrict__ z, const int *__restrict__ phase,
+                                double *__restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    out[k] = (phase[k] != -1) ? g2p_cent_one(g, F, x[k], z[k]) : 0.0;
+}
+
+__global__ void k_pressure_increment(int nc, const signed char *__restrict__ tag, const double *__restrict__ p_in,
+                                     const double *__restrict__ p0, double *__restrict__ dp) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc) dp[c] = (tag[c] != 30) ? (p_in[c] - p0[c]) : 0.0;
+}
+
+__global__ void k_add_g2p(cgrid g, const double *__restrict__ F, int n, const double *__restrict__ x,
+                          const double *__restrict__ z, const int *__restrict__ phase, double *__restrict__ P) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double inc = (phase[k] != -1) ? g2p_cent_one(g, F, x[k], z[k]) : 0.0;
+    P[k] = P[k] + inc;
+}
+
+cgrid make_cgrid(const mdg_ctx *c) {
+    return cgrid{c->Nx - 1, c->Nz - 1, c->dx, c->dz, c->xvz.p, c->zvx.p, c->BCp_t.p};
+}
+
+}  // namespace
+
+// staging buffers for host callers, one set per context (freed with it)
+struct g2p_scratch {
+    dbuf<double> F, x, z, out, p0, dp;
+    dbuf<int> ph;
+};
+
+static g2p_scratch &g2p_of(mdg_ctx *c) {
+    if (!c->g2p) c->g2p = new g2p_scratch();
+    return *c->g2p;
+}
+
+void mdg_g2p_release(mdg_ctx *c) {
+    if (!c || !c->g2p) return;
+    g2p_scratch *s = c->g2p;
+    s->F.release(); s->x.release(); s->z.release(); s->out.release(); s->p0.release(); s->dp.release();
+    s->ph.release();
+    delete s;
+    c->g2p = nullptr;
+}
+
+static int timed_end(mdg_ctx *c) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return MDG_OK;
+}
+
+extern "C" int mdg_g2p_centroids(mdg_ctx *c, const double *F, int n, const double *x, const double *z,
+                                 const int *phase, double *out) {
+    if (!c || !F || !x || !z || !phase || !out || n < 0) return mdg_fail(c, MDG_ERR_ARG, "bad argument");
+    if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
+    if (n == 0) return MDG_OK;
+    g2p_scratch &s = g2p_of(c);
+    const size_t nc = (size_t)(c->Nx - 1) * (c->Nz - 1);
+    const double *dF, *dx_, *dz_;
+    const int *dph;
+    int rc;
+    if ((rc = mdg_stage_in(c, F, nc, s.F, &dF)) || (rc = mdg_stage_in(c, x, (size_t)n, s.x, &dx_)) ||
+        (rc = mdg_stage_in(c, z, (size_t)n, s.z, &dz_)) || (rc = mdg_stage_in(c, phase, (size_t)n, s.ph, &dph)))
+        return rc;
+    double *dout = out;
+    if (!mdg_is_device_ptr(out)) {
+        HIPCHK(c, s.out.reserve((size_t)n));
+        dout = s.out.p;
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_g2p_centroids, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, make_cgrid(c), dF, n, dx_,
+                       dz_, dph, dout);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    if (dout != out) HIPCHK(c, hipMemcpyAsync(out, dout, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    return timed_end(c);
+}
+
+extern "C" int mdg_update_particle_pressure(mdg_ctx *c, const double *p_in, const double *p0_n, double *dp, int n,
+                                            const double *x, const double *z, const int *phase, double *P) {
+    if (!c || !p_in || !p0_n || !x || !z || !phase || !P || n < 0) return mdg_fail(c, MDG_ERR_ARG, "bad argument");
+    if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
+    g2p_scratch &s = g2p_of(c);
+    const int nc = (c->Nx - 1) * (c->Nz - 1);
+    const double *dpin, *dp0, *dx_, *dz_;
+    const int *dph;
+    int rc;
+    if ((rc = mdg_stage_in(c, p_in, (size_t)nc, s.F, &dpin)) || (rc = mdg_stage_in(c, p0_n, (size_t)nc, s.p0, &dp0)) ||
+        (rc = mdg_stage_in(c, x, (size_t)n, s.x, &dx_)) || (rc = mdg_stage_in(c, z, (size_t)n, s.z, &dz_)) ||
+        (rc = mdg_stage_in(c, phase, (size_t)n, s.ph, &dph)))
+        return rc;
+    double *ddp = dp;
+    if (!dp || !mdg_is_device_ptr(dp)) {
+        HIPCHK(c, s.dp.reserve((size_t)nc));
+        ddp = s.dp.p;
+    }
+    double *dP = P;
+    if (!mdg_is_device_ptr(P)) {
+        HIPCHK(c, s.out.reserve((size_t)(n > 0 ? n : 1)));
+        if (n > 0) HIPCHK(c, hipMemcpyAsync(s.out.p, P, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+        dP = s.out.p;
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_pressure_increment, dim3(mdg_grid1d(nc, 256)), dim3(256), 0, c->stream, nc,
+                       (const signed char *)c->BCp_t.p, dpin, dp0, ddp);
+    if (n > 0)
+        hipLaunchKernelGGL(k_add_g2p, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, make_cgrid(c), (const double *)ddp,
+                           n, dx_, dz_, dph, dP);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    if (dP != P && n > 0) HIPCHK(c, hipMemcpyAsync(P, dP, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (dp && ddp != dp) HIPCHK(c, hipMemcpyAsync(dp, ddp, (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
+    return timed_end(c);
+}

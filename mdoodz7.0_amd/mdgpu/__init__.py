@@ -78,6 +78,8 @@ def lib():
         L.mdg_dev_scale.argtypes = [vp, C.c_longlong, C.c_double, vp, vp]
         L.mdg_v2p.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.mdg_advect_rk4.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        L.mdg_g2p_centroids.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp]
+        L.mdg_update_particle_pressure.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
         _extra_signatures(L)
         _LIB = L
     return _LIB
@@ -308,6 +310,20 @@ class Context:
         / int32, or DeviceArray."""
         self._check(self._L.mdg_advect_rk4(self.h, ptr(u), ptr(v), dt, conserv, len(x), ptr(x), ptr(z),
                                            ptr(phase), ptr(vx), ptr(vz)))
+
+    def g2p_centroids(self, F, x, z, phase, out=None):
+        """Interp_Grid2P_centroids2 (ParticleRoutines.c:1332): a centre field
+        at the markers."""
+        out = np.zeros(len(x)) if out is None else out
+        self._check(self._L.mdg_g2p_centroids(self.h, ptr(F), len(x), ptr(x), ptr(z), ptr(phase), ptr(out)))
+        return out
+
+    def update_particle_pressure(self, p_in, p0_n, x, z, phase, P, dp=None):
+        """UpdateParticlePressure (RheologyParticles.c:1621, no subgrid
+        diffusion): P updated in place; returns dp when given."""
+        self._check(self._L.mdg_update_particle_pressure(self.h, ptr(p_in), ptr(p0_n), ptr(dp), len(x), ptr(x),
+                                                         ptr(z), ptr(phase), ptr(P)))
+        return dp
 
     def markers_gather(self, src, dst_sorted):
         self._check(self._L.mdg_markers_gather(self.h, ptr(src), ptr(dst_sorted)))

@@ -95,6 +95,9 @@ def lib():
                                    C.c_int, C.c_void_p, C.c_int, C.c_int]
         L.orc_c2v.argtypes = [gp, C.c_int, f64p, f64p]
         L.orc_v2p.argtypes = [gp, f64p, f64p, C.c_int, C.c_int, f64p, f64p, i32p, f64p, f64p]
+        L.orc_g2p_centroids.argtypes = [gp, f64p, C.c_int, f64p, f64p, i32p, f64p]
+        L.orc_update_particle_pressure.argtypes = [gp, f64p, f64p, f64p, C.c_int, f64p, f64p, i32p,
+                                                   f64p]
         L.orc_advect_rk4.argtypes = [gp, f64p, f64p, C.c_double, C.c_int, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_int, f64p, f64p, i32p, C.c_void_p,
                                      C.c_void_p]
@@ -194,6 +197,27 @@ def v2p(g, u, v, x, z, phase, conserv=0):
                   conserv, n, np.ascontiguousarray(x, np.float64), np.ascontiguousarray(z, np.float64),
                   np.ascontiguousarray(phase, np.int32), vx, vz)
     return vx, vz
+
+
+def g2p_centroids(g, F, x, z, phase):
+    """ParticleRoutines.c:1332-1580: a centre field at the markers."""
+    out = np.zeros(len(x))
+    lib().orc_g2p_centroids(g.c, np.ascontiguousarray(F, np.float64), len(x),
+                            np.ascontiguousarray(x, np.float64), np.ascontiguousarray(z, np.float64),
+                            np.ascontiguousarray(phase, np.int32), out)
+    return out
+
+
+def update_particle_pressure(g, p_in, p0_n, x, z, phase, P):
+    """RheologyParticles.c:1621-1704 (no subgrid diffusion): returns (new P, dp)."""
+    P = np.array(P, np.float64)
+    dp = np.zeros(len(p_in))
+    lib().orc_update_particle_pressure(g.c, np.ascontiguousarray(p_in, np.float64),
+                                       np.ascontiguousarray(p0_n, np.float64), dp, len(x),
+                                       np.ascontiguousarray(x, np.float64),
+                                       np.ascontiguousarray(z, np.float64),
+                                       np.ascontiguousarray(phase, np.int32), P)
+    return P, dp
 
 
 def advect_rk4(g, u, v, dt, x, z, phase, conserv=0, vx=None, vz=None):

@@ -156,3 +156,81 @@ void orc_advect_rk4(const orc_grid *g, const double *u, const double *v, double 
         isout(x[k], z[k], box, &phase[k]);
     }
 }
+
+/* Interp_Grid2P_centroids2 (ParticleRoutines.c:1332-1580), non-periodic:
+ * a centre field to the markers, bilinear between the centres around the
+ * marker (extended coordinates xvz, zvx), values copied along the box edges,
+ * air / surface cells (BCp tag 30/31) replaced by their vertical neighbour,
+ * normalised by the weight sum. Inactive markers get 0. */
+static double g2p_cent_one(const orc_grid *g, const double *F, double x, double z) {
+    const int Nx = g->Nx - 1, Nz = g->Nz - 1;
+    const double dx = g->dx, dz = g->dz;
+    const double *X = g->xvz, *Z = g->zvx;
+    const signed char *tag = g->BCp_t;
+    const int j = (int)ceil((x - X[0]) / dx) - 1;
+    const int i = (int)ceil((z - Z[0]) / dz) - 1;
+    int ic = 0, jc = 0, iSW = 0, iSE = 0, iNW = 0, iNE = 0;
+    if (j > 0 && j < Nx && i > 0 && i < Nz) {
+        ic = i - 1; jc = j - 1;
+        iSW = jc + ic * Nx; iSE = iSW + 1; iNW = jc + (ic + 1) * Nx; iNE = iNW + 1;
+    } else if (j == 0 && i > 0 && i < Nz) {
+        ic = i - 1; jc = j;
+        iSW = iSE = jc + ic * Nx; iNW = iNE = jc + (ic + 1) * Nx;
+    } else if (j == Nx && i > 0 && i < Nz) {
+        ic = i - 1; jc = j - 1;
+        iSW = iSE = jc + ic * Nx; iNW = iNE = jc + (ic + 1) * Nx;
+    } else if (i == 0 && j > 0 && j < Nx) {
+        ic = i; jc = j - 1;
+        iSW = iNW = jc + ic * Nx; iSE = iNE = jc + ic * Nx + 1;
+    } else if (i == Nz && j > 0 && j < Nx) {
+        ic = i - 1; jc = j - 1;
+        iSW = iNW = jc + ic * Nx; iSE = iNE = jc + ic * Nx + 1;
+    } else if (i == 0 && j == 0) {
+        iSW = iSE = iNW = iNE = 0;
+    } else if (i == 0 && j == Nx) {
+        iSW = iSE = iNW = iNE = j - 1;
+    } else if (i == Nz && j == 0) {
+        iSW = iSE = iNW = iNE = (i - 1) * Nx;
+    } else if (i == Nz && j == Nx) {
+        iSW = iSE = iNW = iNE = (j - 1) + (i - 1) * Nx;
+    }
+    (void)ic; (void)jc;
+    /* markers of the box always take one of the cases above; other inputs
+     * read clamped coordinates (the reference leaves them undefined) */
+    const int jj = j < 0 ? 0 : (j > Nx ? Nx : j), ii = i < 0 ? 0 : (i > Nz ? Nz : i);
+    const double dxm = x - X[jj], dzm = z - Z[ii];
+    double val = 0.0, sumW = 0.0;
+    const int air_SW = tag[iSW] == 30 || tag[iSW] == 31, air_SE = tag[iSE] == 30 || tag[iSE] == 31;
+    const int air_NW = tag[iNW] == 30 || tag[iNW] == 31, air_NE = tag[iNE] == 30 || tag[iNE] == 31;
+    val += (1.0 - dxm / dx) * (1.0 - dzm / dz) * F[air_SW ? iNW : iSW];
+    sumW += (1.0 - dxm / dx) * (1.0 - dzm / dz);
+    val += (dxm / dx) * (1.0 - dzm / dz) * F[air_SE ? iNE : iSE];
+    sumW += (dxm / dx) * (1.0 - dzm / dz);
+    val += (1.0 - dxm / dx) * (dzm / dz) * F[air_NW ? iSW : iNW];
+    sumW += (1.0 - dxm / dx) * (dzm / dz);
+    val += (dxm / dx) * (dzm / dz) * F[air_NE ? iSE : iNE];
+    sumW += (dxm / dx) * (dzm / dz);
+    if (sumW > 1e-13) val /= sumW;
+    return val;
+}
+
+void orc_g2p_centroids(const orc_grid *g, const double *F, int n, const double *x, const double *z,
+                       const int *phase, double *out) {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n; k++) out[k] = (phase[k] != -1) ? g2p_cent_one(g, F, x[k], z[k]) : 0.0;
+}
+
+/* UpdateParticlePressure (RheologyParticles.c:1621-1704) without subgrid
+ * diffusion (model.subgrid_diffusion < 2): dp = p_in - p0_n on the cells
+ * that are not air (:1631-1637), interpolated to the markers and added to
+ * their pressure (:1694-1700). dp (ncells) receives the increment. */
+void orc_update_particle_pressure(const orc_grid *g, const double *p_in, const double *p0_n, double *dp,
+                                  int n, const double *x, const double *z, const int *phase, double *P) {
+    const int nc = (g->Nx - 1) * (g->Nz - 1);
+    for (int c = 0; c < nc; c++) dp[c] = (g->BCp_t[c] != 30) ? (p_in[c] - p0_n[c]) : 0.0;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n; k++) {
+        const double inc = (phase[k] != -1) ? g2p_cent_one(g, dp, x[k], z[k]) : 0.0;
+        P[k] = P[k] + inc;
+    }
+}

@@ -72,6 +72,16 @@ void orc_advect_rk4(const orc_grid *g, const double *u, const double *v, double 
                     double xmin, double xmax, double zmin, double zmax, int n, double *x, double *z,
                     int *phase, double *vx, double *vz);
 
+/* ParticleRoutines.c:1332-1580 (Interp_Grid2P_centroids2, non-periodic):
+ * a centre field at the markers; inactive markers get 0. */
+void orc_g2p_centroids(const orc_grid *g, const double *F, int n, const double *x, const double *z,
+                       const int *phase, double *out);
+
+/* RheologyParticles.c:1621-1704 (UpdateParticlePressure, no subgrid
+ * diffusion): dp = p_in - p0_n off air cells, interpolated and added to P. */
+void orc_update_particle_pressure(const orc_grid *g, const double *p_in, const double *p0_n, double *dp,
+                                  int n, const double *x, const double *z, const int *phase, double *P);
+
 /* StokesRoutines.c:415-~500 (ghost values from BC tags) */
 void orc_apply_bc(const orc_grid *g, double *u, double *v);
 

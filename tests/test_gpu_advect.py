@@ -92,3 +92,22 @@ def test_rk4_full_size_2047(oracle_mod):
     assert np.array_equal(dx_.numpy()[sub], xr) and np.array_equal(dz_.numpy()[sub], zr)
     assert np.array_equal(dph.numpy()[sub], phr)
     assert np.array_equal(dvx.numpy()[sub], vxr) and np.array_equal(dvz.numpy()[sub], vzr)
+
+
+@pytest.mark.parametrize("n", [63, 100])
+def test_g2p_centroids_and_pressure_update_bit_exact(oracle_mod, n):
+    """Interp_Grid2P_centroids2 and UpdateParticlePressure on the GPU against
+    the oracle: random centre fields, markers up to the walls and corners,
+    inactive markers."""
+    O = oracle_mod
+    g, ctx, u, v, x, z, ph = _case(O, n, 5 + n)
+    inside = (x >= 0.0) & (x <= 1.0) & (z >= -1.0) & (z <= 0.0)
+    ph = np.where(inside, ph, -1).astype(np.int32)
+    rng = np.random.default_rng(n)
+    F = rng.standard_normal(n * n)
+    assert np.array_equal(ctx.g2p_centroids(F, x, z, ph), O.g2p_centroids(g, F, x, z, ph))
+    p0, p1, P0 = rng.standard_normal(n * n), rng.standard_normal(n * n), rng.standard_normal(x.size)
+    Pr, dpr = O.update_particle_pressure(g, p1, p0, x, z, ph, P0)
+    Pg, dpg = P0.copy(), np.zeros(n * n)
+    ctx.update_particle_pressure(p1, p0, x, z, ph, Pg, dp=dpg)
+    assert np.array_equal(dpg, dpr) and np.array_equal(Pg, Pr)

@@ -75,3 +75,51 @@ def test_markers_leaving_the_box_become_inactive(oracle_mod):
     assert xn[3] == 0.2 and zn[3] == 0.5
     # velocities are written only for markers still active at the end
     assert vx[0] == pytest.approx(1.0) and vx[1] == 7.0 and vx[2] == 7.0 and vx[3] == 7.0
+
+
+def _centre_linear(g, a):
+    xc, zc = np.asarray(g.xc), np.asarray(g.zc)
+    return (a[0] + a[1] * xc[None, :] + a[2] * zc[:, None]).ravel()
+
+
+def test_g2p_centroids_linear_constant_and_edges(oracle_mod):
+    """Interp_Grid2P_centroids2: a constant centre field is returned at every
+    marker to round-off (the weights are normalised); a linear one is reproduced at
+    the markers between the outermost centres; within half a cell of a wall
+    the value is copied along the wall (linear in the tangential coordinate
+    only); inactive markers get 0."""
+    O = oracle_mod
+    g = O.Grid(33, 25, 0.0, 2.0, -1.0, 0.5)
+    g.pure_shear(1.0)
+    x, z = S.put_markers(g.xg, g.zg, g.dx, g.dz, 3, 3, seed=8)
+    ph = np.zeros(x.size, np.int32)
+    ph[::89] = -1
+    act = ph != -1
+    out = O.g2p_centroids(g, np.full((g.Nx - 1) * (g.Nz - 1), 3.25), x, z, ph)
+    np.testing.assert_allclose(out[act], 3.25, rtol=2e-15, atol=0)
+    assert (out[~act] == 0).all()
+    a = (0.5, 1.5, -2.0)
+    out = O.g2p_centroids(g, _centre_linear(g, a), x, z, ph)
+    xc, zc = np.asarray(g.xc), np.asarray(g.zc)
+    inner = act & (x > xc[0]) & (x < xc[-1]) & (z > zc[0]) & (z < zc[-1])
+    np.testing.assert_allclose(out[inner], (a[0] + a[1] * x + a[2] * z)[inner], rtol=0, atol=5e-15)
+    west = act & (x < xc[0]) & (z > zc[0]) & (z < zc[-1])
+    np.testing.assert_allclose(out[west], (a[0] + a[1] * xc[0] + a[2] * z)[west], rtol=0, atol=5e-15)
+    corner = act & (x < xc[0]) & (z < zc[0])
+    np.testing.assert_allclose(out[corner], a[0] + a[1] * xc[0] + a[2] * zc[0], rtol=0, atol=5e-15)
+    assert inner.any() and west.any() and corner.any()
+
+
+def test_update_particle_pressure_adds_the_interpolated_increment(oracle_mod):
+    O = oracle_mod
+    g = O.Grid(21, 21, 0.0, 1.0, 0.0, 1.0)
+    g.pure_shear(1.0)
+    x, z = S.put_markers(g.xg, g.zg, g.dx, g.dz, 2, 2, seed=3)
+    ph = np.zeros(x.size, np.int32)
+    rng = np.random.default_rng(0)
+    p0 = rng.standard_normal(400)
+    p1 = p0 + _centre_linear(g, (0.1, 0.2, -0.3))
+    P0 = rng.standard_normal(x.size)
+    P, dp = O.update_particle_pressure(g, p1, p0, x, z, ph, P0)
+    np.testing.assert_allclose(dp, p1 - p0, rtol=0, atol=0)
+    np.testing.assert_allclose(P, P0 + O.g2p_centroids(g, dp, x, z, ph), rtol=0, atol=0)
