@@ -334,6 +334,13 @@ int mdg_advect_rk4(mdg_ctx *ctx, const double *u, const double *v, double dt, in
 int mdg_g2p_centroids(mdg_ctx *ctx, const double *F, int n, const double *x, const double *z,
                       const int *phase, double *out);
 
+/* Interp_Grid2P (ParticleRoutines.c:1223-1325) on the vertices (xg, zg,
+ * BCg tags): a vertex field F (Nx x Nz) at the markers, bilinear in the
+ * clamped cell, tag 30/31 vertices replaced by their vertical neighbour, not
+ * normalised (as the reference); inactive markers get 0. */
+int mdg_g2p_vertices(mdg_ctx *ctx, const double *F, int n, const double *x, const double *z,
+                     const int *phase, double *out);
+
 /* UpdateParticlePressure (RheologyParticles.c:1621-1704; called after the
  * solve) with model.subgrid_diffusion < 2: dp = p_in - p0_n on the cells not
  * tagged 30 (mesh->dp, returned when dp is not NULL), interpolated as above

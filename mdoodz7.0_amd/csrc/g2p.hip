@@ -68,6 +68,42 @@ __global__ void k_g2p_centroids(cgrid g, const double *__restrict__ F, int n, co
     out[k] = (phase[k] != -1) ? g2p_cent_one(g, F, x[k], z[k]) : 0.0;
 }
 
+// Interp_Grid2P (ParticleRoutines.c:1223-1325) on the vertices
+struct vgrid {
+    int Nx, Nz;
+    double dx, dz;
+    const double *X, *Z;      // xg, zg
+    const signed char *tag;   // BCg
+};
+__global__ void k_g2p_vertices(vgrid g, const double *__restrict__ F, int n, const double *__restrict__ x,
+                               const double *__restrict__ z, const int *__restrict__ phase,
+                               double *__restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (phase[k] == -1) {
+        out[k] = 0.0;
+        return;
+    }
+    const double xk = x[k], zk = z[k], dx = g.dx, dz = g.dz;
+    int j = (int)ceil((xk - g.X[0]) / dx) - 1;
+    if (j < 0) j = 0;
+    if (j > g.Nx - 2) j = g.Nx - 2;
+    int i = (int)ceil((zk - g.Z[0]) / dz) - 1;
+    if (i < 0) i = 0;
+    if (i > g.Nz - 2) i = g.Nz - 2;
+    const double dxm = fabs(xk - g.X[j]), dzm = fabs(zk - g.Z[i]);
+    const int iSW = j + i * g.Nx, iSE = iSW + 1, iNW = j + (i + 1) * g.Nx, iNE = iNW + 1;
+    const signed char *t = g.tag;
+    const bool aSW = t[iSW] == 30 || t[iSW] == 31, aSE = t[iSE] == 30 || t[iSE] == 31;
+    const bool aNW = t[iNW] == 30 || t[iNW] == 31, aNE = t[iNE] == 30 || t[iNE] == 31;
+    double val = 0.0;
+    val += (1.0 - dxm / dx) * (1.0 - dzm / dz) * F[aSW ? iNW : iSW];
+    val += (dxm / dx) * (1.0 - dzm / dz) * F[aSE ? iNE : iSE];
+    val += (1.0 - dxm / dx) * (dzm / dz) * F[aNW ? iSW : iNW];
+    val += (dxm / dx) * (dzm / dz) * F[aNE ? iSE : iNE];
+    out[k] = val;
+}
+
 __global__ void k_pressure_increment(int nc, const signed char *__restrict__ tag, const double *__restrict__ p_in,
                                      const double *__restrict__ p0, double *__restrict__ dp) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -137,6 +173,32 @@ extern "C" int mdg_g2p_centroids(mdg_ctx *c, const double *F, int n, const doubl
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(k_g2p_centroids, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, make_cgrid(c), dF, n, dx_,
                        dz_, dph, dout);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    if (dout != out) HIPCHK(c, hipMemcpyAsync(out, dout, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    return timed_end(c);
+}
+
+extern "C" int mdg_g2p_vertices(mdg_ctx *c, const double *F, int n, const double *x, const double *z,
+                                const int *phase, double *out) {
+    if (!c || !F || !x || !z || !phase || !out || n < 0) return mdg_fail(c, MDG_ERR_ARG, "bad argument");
+    if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
+    if (n == 0) return MDG_OK;
+    g2p_scratch &s = g2p_of(c);
+    const double *dF, *dx_, *dz_;
+    const int *dph;
+    int rc;
+    if ((rc = mdg_stage_in(c, F, (size_t)c->Nx * c->Nz, s.F, &dF)) || (rc = mdg_stage_in(c, x, (size_t)n, s.x, &dx_)) ||
+        (rc = mdg_stage_in(c, z, (size_t)n, s.z, &dz_)) || (rc = mdg_stage_in(c, phase, (size_t)n, s.ph, &dph)))
+        return rc;
+    double *dout = out;
+    if (!mdg_is_device_ptr(out)) {
+        HIPCHK(c, s.out.reserve((size_t)n));
+        dout = s.out.p;
+    }
+    const vgrid g = {c->Nx, c->Nz, c->dx, c->dz, c->xg.p, c->zg.p, c->BCg_t.p};
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_g2p_vertices, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, g, dF, n, dx_, dz_, dph, dout);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     if (dout != out) HIPCHK(c, hipMemcpyAsync(out, dout, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));

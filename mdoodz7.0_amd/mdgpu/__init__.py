@@ -79,6 +79,7 @@ def lib():
         L.mdg_v2p.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.mdg_advect_rk4.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.mdg_g2p_centroids.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp]
+        L.mdg_g2p_vertices.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp]
         L.mdg_update_particle_pressure.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
         _extra_signatures(L)
         _LIB = L
@@ -316,6 +317,12 @@ class Context:
         at the markers."""
         out = np.zeros(len(x)) if out is None else out
         self._check(self._L.mdg_g2p_centroids(self.h, ptr(F), len(x), ptr(x), ptr(z), ptr(phase), ptr(out)))
+        return out
+
+    def g2p_vertices(self, F, x, z, phase, out=None):
+        """Interp_Grid2P (ParticleRoutines.c:1223) on the vertices."""
+        out = np.zeros(len(x)) if out is None else out
+        self._check(self._L.mdg_g2p_vertices(self.h, ptr(F), len(x), ptr(x), ptr(z), ptr(phase), ptr(out)))
         return out
 
     def update_particle_pressure(self, p_in, p0_n, x, z, phase, P, dp=None):

@@ -96,6 +96,7 @@ def lib():
         L.orc_c2v.argtypes = [gp, C.c_int, f64p, f64p]
         L.orc_v2p.argtypes = [gp, f64p, f64p, C.c_int, C.c_int, f64p, f64p, i32p, f64p, f64p]
         L.orc_g2p_centroids.argtypes = [gp, f64p, C.c_int, f64p, f64p, i32p, f64p]
+        L.orc_g2p_vertices.argtypes = [gp, f64p, C.c_int, f64p, f64p, i32p, f64p]
         L.orc_update_particle_pressure.argtypes = [gp, f64p, f64p, f64p, C.c_int, f64p, f64p, i32p,
                                                    f64p]
         L.orc_advect_rk4.argtypes = [gp, f64p, f64p, C.c_double, C.c_int, C.c_double, C.c_double,
@@ -205,6 +206,15 @@ def g2p_centroids(g, F, x, z, phase):
     lib().orc_g2p_centroids(g.c, np.ascontiguousarray(F, np.float64), len(x),
                             np.ascontiguousarray(x, np.float64), np.ascontiguousarray(z, np.float64),
                             np.ascontiguousarray(phase, np.int32), out)
+    return out
+
+
+def g2p_vertices(g, F, x, z, phase):
+    """ParticleRoutines.c:1223-1325: a vertex field at the markers."""
+    out = np.zeros(len(x))
+    lib().orc_g2p_vertices(g.c, np.ascontiguousarray(F, np.float64), len(x),
+                           np.ascontiguousarray(x, np.float64), np.ascontiguousarray(z, np.float64),
+                           np.ascontiguousarray(phase, np.int32), out)
     return out
 
 

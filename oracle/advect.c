@@ -234,3 +234,35 @@ void orc_update_particle_pressure(const orc_grid *g, const double *p_in, const d
         P[k] = P[k] + inc;
     }
 }
+
+/* Interp_Grid2P (ParticleRoutines.c:1223-1325) on the vertices (xg, zg,
+ * BCg tags): bilinear in the clamped cell, tag 30/31 vertices replaced by
+ * their vertical neighbour, no normalisation (the reference sums the
+ * weights but does not divide); inactive markers get 0. */
+void orc_g2p_vertices(const orc_grid *g, const double *F, int n, const double *x, const double *z,
+                      const int *phase, double *out) {
+    const int Nx = g->Nx, Nz = g->Nz;
+    const double dx = g->dx, dz = g->dz;
+    const signed char *tag = g->BCg_t;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n; k++) {
+        out[k] = 0.0;
+        if (phase[k] == -1) continue;
+        int j = (int)ceil((x[k] - g->xg[0]) / dx) - 1;
+        if (j < 0) j = 0;
+        if (j > Nx - 2) j = Nx - 2;
+        int i = (int)ceil((z[k] - g->zg[0]) / dz) - 1;
+        if (i < 0) i = 0;
+        if (i > Nz - 2) i = Nz - 2;
+        const double dxm = fabs(x[k] - g->xg[j]), dzm = fabs(z[k] - g->zg[i]);
+        const int iSW = j + i * Nx, iSE = iSW + 1, iNW = j + (i + 1) * Nx, iNE = iNW + 1;
+        const int aSW = tag[iSW] == 30 || tag[iSW] == 31, aSE = tag[iSE] == 30 || tag[iSE] == 31;
+        const int aNW = tag[iNW] == 30 || tag[iNW] == 31, aNE = tag[iNE] == 30 || tag[iNE] == 31;
+        double val = 0.0;
+        val += (1.0 - dxm / dx) * (1.0 - dzm / dz) * F[aSW ? iNW : iSW];
+        val += (dxm / dx) * (1.0 - dzm / dz) * F[aSE ? iNE : iSE];
+        val += (1.0 - dxm / dx) * (dzm / dz) * F[aNW ? iSW : iNW];
+        val += (dxm / dx) * (dzm / dz) * F[aNE ? iSE : iNE];
+        out[k] = val;
+    }
+}

@@ -77,6 +77,11 @@ void orc_advect_rk4(const orc_grid *g, const double *u, const double *v, double 
 void orc_g2p_centroids(const orc_grid *g, const double *F, int n, const double *x, const double *z,
                        const int *phase, double *out);
 
+/* ParticleRoutines.c:1223-1325 (Interp_Grid2P) on the vertices: a vertex
+ * field at the markers, unnormalised; inactive markers get 0. */
+void orc_g2p_vertices(const orc_grid *g, const double *F, int n, const double *x, const double *z,
+                      const int *phase, double *out);
+
 /* RheologyParticles.c:1621-1704 (UpdateParticlePressure, no subgrid
  * diffusion): dp = p_in - p0_n off air cells, interpolated and added to P. */
 void orc_update_particle_pressure(const orc_grid *g, const double *p_in, const double *p0_n, double *dp,

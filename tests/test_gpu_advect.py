@@ -95,8 +95,9 @@ def test_rk4_full_size_2047(oracle_mod):
 
 
 @pytest.mark.parametrize("n", [63, 100])
-def test_g2p_centroids_and_pressure_update_bit_exact(oracle_mod, n):
-    """Interp_Grid2P_centroids2 and UpdateParticlePressure on the GPU against
+def test_g2p_and_pressure_update_bit_exact(oracle_mod, n):
+    """Interp_Grid2P_centroids2, Interp_Grid2P (vertices) and
+    UpdateParticlePressure on the GPU against
     the oracle: random centre fields, markers up to the walls and corners,
     inactive markers."""
     O = oracle_mod
@@ -106,6 +107,8 @@ def test_g2p_centroids_and_pressure_update_bit_exact(oracle_mod, n):
     rng = np.random.default_rng(n)
     F = rng.standard_normal(n * n)
     assert np.array_equal(ctx.g2p_centroids(F, x, z, ph), O.g2p_centroids(g, F, x, z, ph))
+    Fv = rng.standard_normal((n + 1) * (n + 1))
+    assert np.array_equal(ctx.g2p_vertices(Fv, x, z, ph), O.g2p_vertices(g, Fv, x, z, ph))
     p0, p1, P0 = rng.standard_normal(n * n), rng.standard_normal(n * n), rng.standard_normal(x.size)
     Pr, dpr = O.update_particle_pressure(g, p1, p0, x, z, ph, P0)
     Pg, dpg = P0.copy(), np.zeros(n * n)

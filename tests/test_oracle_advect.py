@@ -123,3 +123,22 @@ def test_update_particle_pressure_adds_the_interpolated_increment(oracle_mod):
     P, dp = O.update_particle_pressure(g, p1, p0, x, z, ph, P0)
     np.testing.assert_allclose(dp, p1 - p0, rtol=0, atol=0)
     np.testing.assert_allclose(P, P0 + O.g2p_centroids(g, dp, x, z, ph), rtol=0, atol=0)
+
+
+def test_g2p_vertices_reproduces_bilinear_fields(oracle_mod):
+    """Interp_Grid2P on the vertices: the weights of the enclosing cell sum to
+    one, so bilinear vertex fields (a + b x + c z + d x z) are reproduced at
+    every marker without the normalisation the reference leaves out."""
+    O = oracle_mod
+    g = O.Grid(29, 37, -0.5, 0.5, -1.0, 0.0)
+    g.pure_shear(1.0)
+    x, z = S.put_markers(g.xg, g.zg, g.dx, g.dz, 3, 3, seed=6)
+    ph = np.zeros(x.size, np.int32)
+    ph[::71] = -1
+    act = ph != -1
+    xg, zg = np.asarray(g.xg), np.asarray(g.zg)
+    a = (0.2, -1.0, 0.7, 1.3)
+    F = (a[0] + a[1] * xg[None, :] + a[2] * zg[:, None] + a[3] * xg[None, :] * zg[:, None]).ravel()
+    out = O.g2p_vertices(g, F, x, z, ph)
+    np.testing.assert_allclose(out[act], (a[0] + a[1] * x + a[2] * z + a[3] * x * z)[act], rtol=0, atol=5e-15)
+    assert (out[~act] == 0).all()
