@@ -341,6 +341,18 @@ int mdg_g2p_centroids(mdg_ctx *ctx, const double *F, int n, const double *x, con
 int mdg_g2p_vertices(mdg_ctx *ctx, const double *F, int n, const double *x, const double *z,
                      const int *phase, double *out);
 
+/* UpdateParticleStress (RheologyParticles.c:1725-2082) for elastic runs
+ * with model.subgrid_diffusion 0 or 1 and model.stress_rotation = 0
+ * (:1924-1960): sxxd - sxxd0, szzd - szzd0 on the cells not tagged 30/31 and
+ * sxz - sxz0 on the vertices not tagged 30, interpolated as
+ * mdg_g2p_centroids / mdg_g2p_vertices and added to the marker stresses
+ * (particles->sxxd, szzd, sxz) in place. The stress rotation step
+ * (stress_rotation 1, 2) is not part of this entry point. */
+int mdg_update_particle_stress(mdg_ctx *ctx, const double *sxxd, const double *szzd, const double *sxz,
+                               const double *sxxd0, const double *szzd0, const double *sxz0, int n,
+                               const double *x, const double *z, const int *phase, double *msxxd,
+                               double *mszzd, double *msxz);
+
 /* UpdateParticlePressure (RheologyParticles.c:1621-1704; called after the
  * solve) with model.subgrid_diffusion < 2: dp = p_in - p0_n on the cells not
  * tagged 30 (mesh->dp, returned when dp is not NULL), interpolated as above

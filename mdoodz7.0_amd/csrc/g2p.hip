@@ -75,16 +75,8 @@ struct vgrid {
     const double *X, *Z;      // xg, zg
     const signed char *tag;   // BCg
 };
-__global__ void k_g2p_vertices(vgrid g, const double *__restrict__ F, int n, const double *__restrict__ x,
-                               const double *__restrict__ z, const int *__restrict__ phase,
-                               double *__restrict__ out) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    if (phase[k] == -1) {
-        out[k] = 0.0;
-        return;
-    }
-    const double xk = x[k], zk = z[k], dx = g.dx, dz = g.dz;
+__device__ double g2p_vert_one(const vgrid &g, const double *__restrict__ F, double xk, double zk) {
+    const double dx = g.dx, dz = g.dz;
     int j = (int)ceil((xk - g.X[0]) / dx) - 1;
     if (j < 0) j = 0;
     if (j > g.Nx - 2) j = g.Nx - 2;
@@ -101,7 +93,45 @@ __global__ void k_g2p_vertices(vgrid g, const double *__restrict__ F, int n, con
     val += (dxm / dx) * (1.0 - dzm / dz) * F[aSE ? iNE : iSE];
     val += (1.0 - dxm / dx) * (dzm / dz) * F[aNW ? iSW : iNW];
     val += (dxm / dx) * (dzm / dz) * F[aNE ? iSE : iNE];
-    out[k] = val;
+    return val;
+}
+
+__global__ void k_g2p_vertices(vgrid g, const double *__restrict__ F, int n, const double *__restrict__ x,
+                               const double *__restrict__ z, const int *__restrict__ phase,
+                               double *__restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    out[k] = (phase[k] != -1) ? g2p_vert_one(g, F, x[k], z[k]) : 0.0;
+}
+
+// grid stress increments of UpdateParticleStress (RheologyParticles.c:1929-1939)
+__global__ void k_stress_increments(int nc, int nv, const signed char *__restrict__ tp,
+                                    const signed char *__restrict__ tg, const double *__restrict__ sxx,
+                                    const double *__restrict__ szz, const double *__restrict__ sxz,
+                                    const double *__restrict__ sxx0, const double *__restrict__ szz0,
+                                    const double *__restrict__ sxz0, double *__restrict__ dxx,
+                                    double *__restrict__ dzz, double *__restrict__ dxz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nc) {
+        const bool on = tp[i] != 30 && tp[i] != 31;
+        dxx[i] = on ? sxx[i] - sxx0[i] : 0.0;
+        dzz[i] = on ? szz[i] - szz0[i] : 0.0;
+    } else if (i < nc + nv) {
+        const int j = i - nc;
+        dxz[j] = (tg[j] != 30) ? sxz[j] - sxz0[j] : 0.0;
+    }
+}
+__global__ void k_add_stress(cgrid gc, vgrid gv, const double *__restrict__ dxx, const double *__restrict__ dzz,
+                             const double *__restrict__ dxz, int n, const double *__restrict__ x,
+                             const double *__restrict__ z, const int *__restrict__ phase, double *__restrict__ mxx,
+                             double *__restrict__ mzz, double *__restrict__ mxz) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const bool on = phase[k] != -1;
+    const double xk = x[k], zk = z[k];
+    mxx[k] = mxx[k] + (on ? g2p_cent_one(gc, dxx, xk, zk) : 0.0);
+    mzz[k] = mzz[k] + (on ? g2p_cent_one(gc, dzz, xk, zk) : 0.0);
+    mxz[k] = mxz[k] + (on ? g2p_vert_one(gv, dxz, xk, zk) : 0.0);
 }
 
 __global__ void k_pressure_increment(int nc, const signed char *__restrict__ tag, const double *__restrict__ p_in,
@@ -128,6 +158,7 @@ cgrid make_cgrid(const mdg_ctx *c) {
 struct g2p_scratch {
     dbuf<double> F, x, z, out, p0, dp;
     dbuf<int> ph;
+    dbuf<double> in[6], inc, m[3];   // stress update: grid inputs, increments, marker stresses
 };
 
 static g2p_scratch &g2p_of(mdg_ctx *c) {
@@ -139,7 +170,9 @@ void mdg_g2p_release(mdg_ctx *c) {
     if (!c || !c->g2p) return;
     g2p_scratch *s = c->g2p;
     s->F.release(); s->x.release(); s->z.release(); s->out.release(); s->p0.release(); s->dp.release();
-    s->ph.release();
+    s->ph.release(); s->inc.release();
+    for (auto &b : s->in) b.release();
+    for (auto &b : s->m) b.release();
     delete s;
     c->g2p = nullptr;
 }
@@ -239,5 +272,53 @@ extern "C" int mdg_update_particle_pressure(mdg_ctx *c, const double *p_in, cons
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     if (dP != P && n > 0) HIPCHK(c, hipMemcpyAsync(P, dP, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
     if (dp && ddp != dp) HIPCHK(c, hipMemcpyAsync(dp, ddp, (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
+    return timed_end(c);
+}
+
+extern "C" int mdg_update_particle_stress(mdg_ctx *c, const double *sxxd, const double *szzd, const double *sxz,
+                                          const double *sxxd0, const double *szzd0, const double *sxz0, int n,
+                                          const double *x, const double *z, const int *phase, double *msxxd,
+                                          double *mszzd, double *msxz) {
+    if (!c || !sxxd || !szzd || !sxz || !sxxd0 || !szzd0 || !sxz0 || !x || !z || !phase || !msxxd || !mszzd ||
+        !msxz || n < 0)
+        return mdg_fail(c, MDG_ERR_ARG, "bad argument");
+    if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
+    if (n == 0) return MDG_OK;
+    g2p_scratch &s = g2p_of(c);
+    const int nc = (c->Nx - 1) * (c->Nz - 1), nv = c->Nx * c->Nz;
+    const double *src[6] = {sxxd, szzd, sxz, sxxd0, szzd0, sxz0}, *d[6];
+    const size_t len[6] = {(size_t)nc, (size_t)nc, (size_t)nv, (size_t)nc, (size_t)nc, (size_t)nv};
+    int rc;
+    for (int q = 0; q < 6; q++)
+        if ((rc = mdg_stage_in(c, src[q], len[q], s.in[q], &d[q]))) return rc;
+    const double *dx_, *dz_;
+    const int *dph;
+    if ((rc = mdg_stage_in(c, x, (size_t)n, s.x, &dx_)) || (rc = mdg_stage_in(c, z, (size_t)n, s.z, &dz_)) ||
+        (rc = mdg_stage_in(c, phase, (size_t)n, s.ph, &dph)))
+        return rc;
+    HIPCHK(c, s.inc.reserve((size_t)(2 * nc + nv)));
+    double *dxx = s.inc.p, *dzz = dxx + nc, *dxz = dzz + nc;
+    double *user[3] = {msxxd, mszzd, msxz}, *dm[3];
+    for (int q = 0; q < 3; q++) {
+        if (mdg_is_device_ptr(user[q])) {
+            dm[q] = user[q];
+        } else {
+            HIPCHK(c, s.m[q].reserve((size_t)n));
+            HIPCHK(c, hipMemcpyAsync(s.m[q].p, user[q], (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+            dm[q] = s.m[q].p;
+        }
+    }
+    const vgrid gv = {c->Nx, c->Nz, c->dx, c->dz, c->xg.p, c->zg.p, c->BCg_t.p};
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_stress_increments, dim3(mdg_grid1d(nc + nv, 256)), dim3(256), 0, c->stream, nc, nv,
+                       (const signed char *)c->BCp_t.p, (const signed char *)c->BCg_t.p, d[0], d[1], d[2], d[3], d[4],
+                       d[5], dxx, dzz, dxz);
+    hipLaunchKernelGGL(k_add_stress, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, make_cgrid(c), gv,
+                       (const double *)dxx, (const double *)dzz, (const double *)dxz, n, dx_, dz_, dph, dm[0], dm[1],
+                       dm[2]);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    for (int q = 0; q < 3; q++)
+        if (dm[q] != user[q]) HIPCHK(c, hipMemcpyAsync(user[q], dm[q], (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
     return timed_end(c);
 }

@@ -80,6 +80,7 @@ def lib():
         L.mdg_advect_rk4.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.mdg_g2p_centroids.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp]
         L.mdg_g2p_vertices.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp]
+        L.mdg_update_particle_stress.argtypes = [vp] * 7 + [C.c_int] + [vp] * 6
         L.mdg_update_particle_pressure.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
         _extra_signatures(L)
         _LIB = L
@@ -324,6 +325,13 @@ class Context:
         out = np.zeros(len(x)) if out is None else out
         self._check(self._L.mdg_g2p_vertices(self.h, ptr(F), len(x), ptr(x), ptr(z), ptr(phase), ptr(out)))
         return out
+
+    def update_particle_stress(self, sxxd, szzd, sxz, sxxd0, szzd0, sxz0, x, z, phase, msxxd, mszzd, msxz):
+        """UpdateParticleStress (RheologyParticles.c:1725; no subgrid stress
+        diffusion, no rotation): marker stresses updated in place."""
+        self._check(self._L.mdg_update_particle_stress(
+            self.h, ptr(sxxd), ptr(szzd), ptr(sxz), ptr(sxxd0), ptr(szzd0), ptr(sxz0), len(x), ptr(x), ptr(z),
+            ptr(phase), ptr(msxxd), ptr(mszzd), ptr(msxz)))
 
     def update_particle_pressure(self, p_in, p0_n, x, z, phase, P, dp=None):
         """UpdateParticlePressure (RheologyParticles.c:1621, no subgrid

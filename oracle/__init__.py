@@ -97,6 +97,8 @@ def lib():
         L.orc_v2p.argtypes = [gp, f64p, f64p, C.c_int, C.c_int, f64p, f64p, i32p, f64p, f64p]
         L.orc_g2p_centroids.argtypes = [gp, f64p, C.c_int, f64p, f64p, i32p, f64p]
         L.orc_g2p_vertices.argtypes = [gp, f64p, C.c_int, f64p, f64p, i32p, f64p]
+        L.orc_update_particle_stress.argtypes = [gp] + [f64p] * 6 + [C.c_int, f64p, f64p, i32p, f64p,
+                                                                     f64p, f64p]
         L.orc_update_particle_pressure.argtypes = [gp, f64p, f64p, f64p, C.c_int, f64p, f64p, i32p,
                                                    f64p]
         L.orc_advect_rk4.argtypes = [gp, f64p, f64p, C.c_double, C.c_int, C.c_double, C.c_double,
@@ -215,6 +217,16 @@ def g2p_vertices(g, F, x, z, phase):
     lib().orc_g2p_vertices(g.c, np.ascontiguousarray(F, np.float64), len(x),
                            np.ascontiguousarray(x, np.float64), np.ascontiguousarray(z, np.float64),
                            np.ascontiguousarray(phase, np.int32), out)
+    return out
+
+
+def update_particle_stress(g, sxxd, szzd, sxz, sxxd0, szzd0, sxz0, x, z, phase, msxxd, mszzd, msxz):
+    """RheologyParticles.c:1725-2082 (no subgrid stress diffusion, no
+    rotation): returns the new marker stresses."""
+    f = lambda a: np.ascontiguousarray(a, np.float64)
+    out = [np.array(a, np.float64) for a in (msxxd, mszzd, msxz)]
+    lib().orc_update_particle_stress(g.c, f(sxxd), f(szzd), f(sxz), f(sxxd0), f(szzd0), f(sxz0), len(x),
+                                     f(x), f(z), np.ascontiguousarray(phase, np.int32), *out)
     return out
 
 

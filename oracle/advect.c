@@ -5,6 +5,7 @@
  * TEST INFRASTRUCTURE ONLY (see mdoracle.h).
  */
 #include <math.h>
+#include <stdlib.h>
 
 #include "mdoracle.h"
 
@@ -239,30 +240,60 @@ void orc_update_particle_pressure(const orc_grid *g, const double *p_in, const d
  * BCg tags): bilinear in the clamped cell, tag 30/31 vertices replaced by
  * their vertical neighbour, no normalisation (the reference sums the
  * weights but does not divide); inactive markers get 0. */
-void orc_g2p_vertices(const orc_grid *g, const double *F, int n, const double *x, const double *z,
-                      const int *phase, double *out) {
+static double g2p_vert_one(const orc_grid *g, const double *F, double x, double z) {
     const int Nx = g->Nx, Nz = g->Nz;
     const double dx = g->dx, dz = g->dz;
     const signed char *tag = g->BCg_t;
+    int j = (int)ceil((x - g->xg[0]) / dx) - 1;
+    if (j < 0) j = 0;
+    if (j > Nx - 2) j = Nx - 2;
+    int i = (int)ceil((z - g->zg[0]) / dz) - 1;
+    if (i < 0) i = 0;
+    if (i > Nz - 2) i = Nz - 2;
+    const double dxm = fabs(x - g->xg[j]), dzm = fabs(z - g->zg[i]);
+    const int iSW = j + i * Nx, iSE = iSW + 1, iNW = j + (i + 1) * Nx, iNE = iNW + 1;
+    const int aSW = tag[iSW] == 30 || tag[iSW] == 31, aSE = tag[iSE] == 30 || tag[iSE] == 31;
+    const int aNW = tag[iNW] == 30 || tag[iNW] == 31, aNE = tag[iNE] == 30 || tag[iNE] == 31;
+    double val = 0.0;
+    val += (1.0 - dxm / dx) * (1.0 - dzm / dz) * F[aSW ? iNW : iSW];
+    val += (dxm / dx) * (1.0 - dzm / dz) * F[aSE ? iNE : iSE];
+    val += (1.0 - dxm / dx) * (dzm / dz) * F[aNW ? iSW : iNW];
+    val += (dxm / dx) * (dzm / dz) * F[aNE ? iSE : iNE];
+    return val;
+}
+
+void orc_g2p_vertices(const orc_grid *g, const double *F, int n, const double *x, const double *z,
+                      const int *phase, double *out) {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n; k++) out[k] = (phase[k] != -1) ? g2p_vert_one(g, F, x[k], z[k]) : 0.0;
+}
+
+/* UpdateParticleStress (RheologyParticles.c:1725-2082) with elastic = 1,
+ * subgrid_diffusion 0 or 1 and no stress rotation (:1924-1960): the grid
+ * stress increments (sxxd - sxxd0, szzd - szzd0 on the cells not tagged
+ * 30/31, sxz - sxz0 on the vertices not tagged 30) interpolated to the
+ * markers and added to their stresses. */
+void orc_update_particle_stress(const orc_grid *g, const double *sxxd, const double *szzd, const double *sxz,
+                                const double *sxxd0, const double *szzd0, const double *sxz0, int n,
+                                const double *x, const double *z, const int *phase, double *msxxd,
+                                double *mszzd, double *msxz) {
+    const int nc = (g->Nx - 1) * (g->Nz - 1), nv = g->Nx * g->Nz;
+    double *dxx = calloc((size_t)nc, sizeof(double)), *dzz = calloc((size_t)nc, sizeof(double));
+    double *dxz = calloc((size_t)nv, sizeof(double));
+    for (int c = 0; c < nc; c++) {
+        if (g->BCp_t[c] != 30 && g->BCp_t[c] != 31) dxx[c] = sxxd[c] - sxxd0[c];
+        if (g->BCp_t[c] != 30 && g->BCp_t[c] != 31) dzz[c] = szzd[c] - szzd0[c];
+    }
+    for (int c = 0; c < nv; c++)
+        if (g->BCg_t[c] != 30) dxz[c] = sxz[c] - sxz0[c];
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < n; k++) {
-        out[k] = 0.0;
-        if (phase[k] == -1) continue;
-        int j = (int)ceil((x[k] - g->xg[0]) / dx) - 1;
-        if (j < 0) j = 0;
-        if (j > Nx - 2) j = Nx - 2;
-        int i = (int)ceil((z[k] - g->zg[0]) / dz) - 1;
-        if (i < 0) i = 0;
-        if (i > Nz - 2) i = Nz - 2;
-        const double dxm = fabs(x[k] - g->xg[j]), dzm = fabs(z[k] - g->zg[i]);
-        const int iSW = j + i * Nx, iSE = iSW + 1, iNW = j + (i + 1) * Nx, iNE = iNW + 1;
-        const int aSW = tag[iSW] == 30 || tag[iSW] == 31, aSE = tag[iSE] == 30 || tag[iSE] == 31;
-        const int aNW = tag[iNW] == 30 || tag[iNW] == 31, aNE = tag[iNE] == 30 || tag[iNE] == 31;
-        double val = 0.0;
-        val += (1.0 - dxm / dx) * (1.0 - dzm / dz) * F[aSW ? iNW : iSW];
-        val += (dxm / dx) * (1.0 - dzm / dz) * F[aSE ? iNE : iSE];
-        val += (1.0 - dxm / dx) * (dzm / dz) * F[aNW ? iSW : iNW];
-        val += (dxm / dx) * (dzm / dz) * F[aNE ? iSE : iNE];
-        out[k] = val;
+        const int on = phase[k] != -1;
+        msxxd[k] = msxxd[k] + (on ? g2p_cent_one(g, dxx, x[k], z[k]) : 0.0);
+        mszzd[k] = mszzd[k] + (on ? g2p_cent_one(g, dzz, x[k], z[k]) : 0.0);
+        msxz[k] = msxz[k] + (on ? g2p_vert_one(g, dxz, x[k], z[k]) : 0.0);
     }
+    free(dxx);
+    free(dzz);
+    free(dxz);
 }

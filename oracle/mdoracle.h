@@ -82,6 +82,14 @@ void orc_g2p_centroids(const orc_grid *g, const double *F, int n, const double *
 void orc_g2p_vertices(const orc_grid *g, const double *F, int n, const double *x, const double *z,
                       const int *phase, double *out);
 
+/* RheologyParticles.c:1725-2082 (UpdateParticleStress: elastic, no subgrid
+ * stress diffusion, no rotation): grid stress increments added to the
+ * marker stresses in place. */
+void orc_update_particle_stress(const orc_grid *g, const double *sxxd, const double *szzd, const double *sxz,
+                                const double *sxxd0, const double *szzd0, const double *sxz0, int n,
+                                const double *x, const double *z, const int *phase, double *msxxd,
+                                double *mszzd, double *msxz);
+
 /* RheologyParticles.c:1621-1704 (UpdateParticlePressure, no subgrid
  * diffusion): dp = p_in - p0_n off air cells, interpolated and added to P. */
 void orc_update_particle_pressure(const orc_grid *g, const double *p_in, const double *p0_n, double *dp,

@@ -95,9 +95,9 @@ def test_rk4_full_size_2047(oracle_mod):
 
 
 @pytest.mark.parametrize("n", [63, 100])
-def test_g2p_and_pressure_update_bit_exact(oracle_mod, n):
-    """Interp_Grid2P_centroids2, Interp_Grid2P (vertices) and
-    UpdateParticlePressure on the GPU against
+def test_g2p_and_marker_updates_bit_exact(oracle_mod, n):
+    """Interp_Grid2P_centroids2, Interp_Grid2P (vertices),
+    UpdateParticlePressure and UpdateParticleStress on the GPU against
     the oracle: random centre fields, markers up to the walls and corners,
     inactive markers."""
     O = oracle_mod
@@ -114,3 +114,10 @@ def test_g2p_and_pressure_update_bit_exact(oracle_mod, n):
     Pg, dpg = P0.copy(), np.zeros(n * n)
     ctx.update_particle_pressure(p1, p0, x, z, ph, Pg, dp=dpg)
     assert np.array_equal(dpg, dpr) and np.array_equal(Pg, Pr)
+    nc, nv = n * n, (n + 1) * (n + 1)
+    grid = [rng.standard_normal(m) for m in (nc, nc, nv, nc, nc, nv)]
+    mk = [rng.standard_normal(x.size) for _ in range(3)]
+    ref = O.update_particle_stress(g, *grid, x, z, ph, *mk)
+    got = [a.copy() for a in mk]
+    ctx.update_particle_stress(*grid, x, z, ph, *got)
+    assert all(np.array_equal(a, b) for a, b in zip(got, ref))

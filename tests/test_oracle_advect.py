@@ -142,3 +142,26 @@ def test_g2p_vertices_reproduces_bilinear_fields(oracle_mod):
     out = O.g2p_vertices(g, F, x, z, ph)
     np.testing.assert_allclose(out[act], (a[0] + a[1] * x + a[2] * z + a[3] * x * z)[act], rtol=0, atol=5e-15)
     assert (out[~act] == 0).all()
+
+
+def test_update_particle_stress_adds_the_interpolated_increments(oracle_mod):
+    """UpdateParticleStress without subgrid diffusion or rotation: each marker
+    stress grows by the interpolated grid increment (centres for the normal
+    components, vertices for the shear one); constant increments are added
+    exactly as constants."""
+    O = oracle_mod
+    g = O.Grid(17, 17, 0.0, 1.0, 0.0, 1.0)
+    g.pure_shear(1.0)
+    x, z = S.put_markers(g.xg, g.zg, g.dx, g.dz, 2, 2, seed=9)
+    ph = np.zeros(x.size, np.int32)
+    nc, nv = 16 * 16, 17 * 17
+    rng = np.random.default_rng(4)
+    s0 = [rng.standard_normal(m) for m in (nc, nc, nv)]
+    s1 = [s0[0] + 0.5, s0[1] - 0.25, s0[2] + 2.0]
+    m0 = [rng.standard_normal(x.size) for _ in range(3)]
+    mxx, mzz, mxz = O.update_particle_stress(g, *s1, *s0, x, z, ph, *m0)
+    np.testing.assert_allclose(mxx, m0[0] + 0.5, rtol=0, atol=4e-15)
+    np.testing.assert_allclose(mzz, m0[1] - 0.25, rtol=0, atol=4e-15)
+    np.testing.assert_allclose(mxz, m0[2] + 2.0, rtol=0, atol=8e-15)
+    dxx = s1[0] - s0[0]
+    np.testing.assert_allclose(mxx, m0[0] + O.g2p_centroids(g, dxx, x, z, ph), rtol=0, atol=0)
