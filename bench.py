@@ -69,6 +69,10 @@ def main():
                     help="profiling mode: only the grid rheology update (NonNewtonianViscosityGrid "
                          "on the centres of the --cells grid, two visco-elasto-plastic phases), "
                          "REPS times, inputs resident in HBM")
+    ap.add_argument("--markers", type=int, default=0, metavar="REPS",
+                    help="profiling mode: only the marker kernels on the --cells workload's markers "
+                         "(RK4 advection, centre and vertex interpolation, pressure and stress "
+                         "updates), REPS times each, arrays resident in HBM")
     ap.add_argument("--trace-mixed", action="store_true",
                     help="with --trace-solve: the FP32 V-cycle (MDG_PREC_MIXED) instead of FP64")
     a = ap.parse_args()
@@ -83,6 +87,9 @@ def main():
 
     if a.rheology:
         rheology_only(a.cells, a.rheology)
+        return
+    if a.markers:
+        markers_only(a, a.markers)
         return
     t0 = time.time()
     wl = W.ellipse_workload(a.cells, a.nper, a.seed, n_ell=a.ellipses)
@@ -339,6 +346,56 @@ def rheology_only(cells, reps):
                       "wall_ms_per_call": 1e3 * wall, "mnodes_per_s": nc / (k_ms * 1e-3) / 1e6,
                       "alg_bytes": nbytes, "achieved_gbs": nbytes / (k_ms * 1e-3) / 1e9,
                       "yielding_centres": yielding}), flush=True)
+
+
+def markers_only(a, reps):
+    """The marker kernels alone on the workload's markers (9 per cell):
+    RK4 advection in a cellular flow, centre and vertex interpolation, the
+    pressure and stress updates; every array on the device. Prints the
+    median kernel time of each (HIP events) and the marker throughput."""
+    import numpy as np
+    import mdgpu
+    from mdgpu import learned
+    from mdgpu import workloads as W
+    n = a.cells
+    wl = W.ellipse_workload(n, a.nper, a.seed, n_ell=4)
+    x, z = wl["x"], wl["z"]
+    m = x.size
+    ctx = mdgpu.Context(n + 1, n + 1, 0.0, 1.0, 0.0, 1.0)
+    ctx.set_bc(*learned.free_slip_bc(n + 1))
+    co = ctx.coords()
+    xg, zg, xvz, zvx = co["xg"], co["zg"], co["xvz"], co["zvx"]
+    u = (np.sin(np.pi * xg)[None, :] * np.pi * np.cos(np.pi * zvx)[:, None]).ravel()
+    v = (-np.pi * np.cos(np.pi * xvz)[None, :] * np.sin(np.pi * zg)[:, None]).ravel()
+    rng = np.random.default_rng(1)
+    nc, nv = n * n, (n + 1) ** 2
+    dev = lambda h: ctx.device(h.size, h.dtype, h)
+    du, dv = dev(u), dev(v)
+    dx_, dz_, dph = dev(x), dev(z), dev(np.zeros(m, np.int32))
+    dvx, dvz, dout = (dev(np.zeros(m)) for _ in range(3))
+    fc, fv = dev(rng.standard_normal(nc)), dev(rng.standard_normal(nv))
+    grid = [dev(rng.standard_normal(k)) for k in (nc, nc, nv, nc, nc, nv)]
+    mk = [dev(np.zeros(m)) for _ in range(4)]
+    dt = 0.5 * (1.0 / n) / np.pi
+    runs = {
+        "advect_rk4": lambda: ctx.advect_rk4(du, dv, dt, dx_, dz_, dph, vx=dvx, vz=dvz),
+        "g2p_centroids": lambda: ctx.g2p_centroids(fc, dx_, dz_, dph, out=dout),
+        "g2p_vertices": lambda: ctx.g2p_vertices(fv, dx_, dz_, dph, out=dout),
+        "update_particle_pressure": lambda: ctx.update_particle_pressure(grid[0], grid[3], dx_, dz_, dph, mk[0]),
+        "update_particle_stress": lambda: ctx.update_particle_stress(*grid, dx_, dz_, dph, *mk[1:]),
+    }
+    out = {"markers": m, "cells": n}
+    for name, f in runs.items():
+        t = []
+        for _ in range(reps):
+            if name == "advect_rk4":
+                dx_.upload(x); dz_.upload(z)
+            f()
+            t.append(ctx.last_kernel_ms())
+        k = sorted(t)[len(t) // 2]
+        out[name + "_ms"] = k
+        out[name + "_gmarkers_per_s"] = m / (k * 1e-3) / 1e9
+    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(wl, a, n):
