@@ -862,6 +862,8 @@ struct mg_level {
     dbuf<T> en, es, dinv, x, b, r, d;
     double lmax = 1.0;
     double cheb_div = 30.0;  // smoothed range [lmax / cheb_div, lmax]
+    const T *bsrc = nullptr;  // right-hand side read in place of b (the FP64 CG residual on level 0)
+    const T *rhs() const { return bsrc ? bsrc : b.p; }
     comp_maps mu, mv;      // transfer to the next coarser level
     // coarsest level
     int nfree = 0;
@@ -1344,7 +1346,7 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
     const int n = lv.n;
     if (fused && n <= CHEB_BLOCK_MAX) {
         hipLaunchKernelGGL(k_cheb_block<T>, dim3(1), dim3(CHEB_BLOCK), 0, c->stream, lv.g, make_stc<T>(lv.g), (T)gamma, deg,
-                           theta, delta, zero_start ? 1 : 0, lv.x.p, lv.d.p, (const T *)lv.b.p, (const T *)lv.en.p,
+                           theta, delta, zero_start ? 1 : 0, lv.x.p, lv.d.p, lv.rhs(), (const T *)lv.en.p,
                            (const T *)lv.es.p, (const T *)lv.dinv.p);
         return;
     }
@@ -1355,16 +1357,16 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
         T *cur = lv.d.p, *prev = lv.x.p;
         if (zero_start)
             hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)(1.0 / theta),
-                               (const T *)lv.dinv.p, (const T *)lv.b.p, cur, prev);
+                               (const T *)lv.dinv.p, lv.rhs(), cur, prev);
         else
             hipLaunchKernelGGL((k_cheb3<T, true>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, lv.g, cf, tg, (T)gamma, T(0),
-                               (T)(1.0 / theta), (const T *)prev, cur, (const T *)lv.b.p, (const T *)lv.en.p,
+                               (T)(1.0 / theta), (const T *)prev, cur, lv.rhs(), (const T *)lv.en.p,
                                (const T *)lv.es.p);
         for (int k = 0; k < deg - 1; k++) {
             const double rho_new = 1.0 / (2.0 * sigma - rho);
             hipLaunchKernelGGL((k_cheb3<T, false>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, lv.g, cf, tg, (T)gamma,
                                (T)(rho_new * rho), (T)(2.0 * rho_new / delta), (const T *)cur, prev,
-                               (const T *)lv.b.p, (const T *)lv.en.p, (const T *)lv.es.p);
+                               lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
             std::swap(cur, prev);
             rho = rho_new;
         }
@@ -1373,11 +1375,11 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
         return;
     }
     if (zero_start) {
-        hipLaunchKernelGGL(k_copy<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, lv.b.p, lv.r.p);
+        hipLaunchKernelGGL(k_copy<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, lv.rhs(), lv.r.p);
         hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, lv.x.p, T(0));
     } else {
         hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p,
-                           lv.b.p, lv.en.p, lv.es.p, lv.r.p);
+                           lv.rhs(), lv.en.p, lv.es.p, lv.r.p);
     }
     hipLaunchKernelGGL(k_scale_dinv<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)(1.0 / theta), lv.dinv.p, lv.r.p, lv.d.p);
     for (int k = 0; k < deg - 1; k++) {
@@ -1398,14 +1400,14 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, i
     if (i > 0 && H.deg_coarse > 0) deg = H.deg_coarse;
     if (i + 1 == H.L.size()) {
         hipLaunchKernelGGL(k_fill<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.n, lv.x.p, T(0));
-        hipLaunchKernelGGL(k_dense_solve<T>, dim3(1), dim3(256), 0, c->stream, lv.nfree, lv.idx.p, lv.Ainv.p, lv.b.p, lv.x.p);
+        hipLaunchKernelGGL(k_dense_solve<T>, dim3(1), dim3(256), 0, c->stream, lv.nfree, lv.idx.p, lv.Ainv.p, lv.rhs(), lv.x.p);
         return;
     }
     mg_level<T> &cl = H.L[i + 1];
     chebyshev(c, lv, gamma, deg, true, H.fused);
     const int reps = (i + 2 == H.L.size()) ? 1 : cycle;
     for (int rep = 0; rep < reps; rep++) {
-        hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p, lv.b.p,
+        hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p, lv.rhs(),
                            lv.en.p, lv.es.p, lv.r.p);
         if (lv.rt == RT2)
             hipLaunchKernelGGL((k_restrict<T, RT2>), dim3(nblk(cl.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, lv.tu,
@@ -1428,7 +1430,10 @@ static void pcg_iteration(mdg_ctx *c, mg_hier<T> &H, const mdg_solver_opts *o) {
     const int n = l0.n, nb = std::min(8192, mdg_grid1d(n, RB)), nd = std::min(1024, nb);
     const double gamma = o->penalty;
     double *x = H.cg_x.p, *r = H.cg_r.p, *p = H.cg_p.p, *q = H.cg_q.p;
-    hipLaunchKernelGGL((k_convert<double, T>), dim3(nblk(n)), dim3(256), 0, c->stream, n, (const double *)r, l0.b.p);
+    if constexpr (std::is_same<T, double>::value)
+        l0.bsrc = r;   // the V-cycle reads the CG residual itself
+    else
+        hipLaunchKernelGGL((k_convert<double, T>), dim3(nblk(n)), dim3(256), 0, c->stream, n, (const double *)r, l0.b.p);
     vcycle(c, H, 0, gamma, o->cheb_degree, o->cycle > 1 ? o->cycle : 1);
     const double *z;
     if constexpr (std::is_same<T, double>::value) {
