@@ -29,6 +29,8 @@ extern "C" {
 #define MDG_ERR_NOT_SPD     -5   /* non-positive diagonal (Solvers.c:1246)  */
 #define MDG_ERR_DIVERGED    -6   /* solver did not reach its tolerance      */
 #define MDG_ERR_NAN         -7   /* NaN residual (StokesRoutines.c:947-953) */
+#define MDG_ERR_INCOMPRESSIBLE -8 /* continuity residual above KillerSolver's
+                                    a-posteriori bound (Solvers.c:1333-1336)  */
 
 typedef struct mdg_ctx mdg_ctx;
 
@@ -290,7 +292,9 @@ typedef struct {
                               coarsen by 4 instead of 2 (0: never) (0)         */
     int    graphs;         /* 0 replay each inner CG iteration as a captured
                               HIP graph, 1 launch its kernels one by one (for
-                              profilers that do not trace graph nodes) (0)     */
+                              profilers that do not trace graph nodes), 2 as 1
+                              with HIP events around every fine-level
+                              smoother step (smoother_ms below) (0)           */
     /* outputs */
     int    its, inner_its, poisson_its, levels, status;
     double rel_res, time_ms;
@@ -300,6 +304,35 @@ typedef struct {
                               guesses; 2: mean ||R||^2 after/before each
                               iteration's network directions; 3: mean
                               ||R - K z0||^2/||R||^2 of the network part       */
+    /* KillerSolver's own iteration and tolerances (Solvers.c:1054-1454,
+     * defaults InputOutput.c:1174-1198).  method 0 is the solve above;
+     * method 1 runs KillerSolver's Powell-Hestenes loop with the constant
+     * penalty gamma/dV (penalty field) around GCR(6) with the reference's
+     * stopping rules, the Cholesky factor of Jts = A - B (gamma/dV) C being
+     * replaced by a multigrid-preconditioned CG solve of Jts to the relative
+     * tolerance pc_tol.                                                     */
+    int    method;         /* 0 or 1 (0)                                       */
+    int    max_its_PH;     /* method 1 (10)                                    */
+    double rel_tol_KSP;    /* method 1 (1e-4)                                  */
+    int    max_its_KSP;    /* method 1 (100)                                   */
+    double lin_abs_mom, lin_rel_mom;   /* method 1 PH stop (1e-9, 1e-5)        */
+    /* continuity tolerances: the PH stop of method 1 and, for both methods,
+     * the a-posteriori check of Solvers.c:1333: the call returns
+     * MDG_ERR_INCOMPRESSIBLE (the solution is still written) when
+     * |max_div| > 10 lin_abs_div and max_div / max_div0 > lin_rel_div.
+     * Method 0 takes both as 0 = no check unless set (method 1: 1e-9, 1e-5). */
+    double lin_abs_div, lin_rel_div;
+    double pc_tol;         /* method 1 (1e-10)                                 */
+    /* outputs: signed maxima (MinMaxArrayVal, MiscFunctions.c:342-375) of the
+     * momentum and continuity residuals of the returned solution, and of the
+     * first Powell-Hestenes iterate (method 1) or of the right-hand side
+     * (method 0, which starts from zero and has no Powell-Hestenes iterate) */
+    double max_mom, max_mom0, max_div, max_div0;
+    int    its_PH, its_KSP;/* method 1: PH iterations, GCR iterations summed
+                              as kspgcr counts them (Solvers.c:208-211)       */
+    double smoother_ms;    /* graphs = 2: mean duration of the fine-level
+                              smoother steps (k_cheb3) in this solve          */
+    int    smoother_launches;
 } mdg_solver_opts;
 int mdg_killer_solver(mdg_ctx *ctx, mdg_solver_opts *opts, const double *D11_n,
                       const double *D22_n, const double *D33_s, const double *rhs_mom,

@@ -74,16 +74,18 @@ extern "C" int mdg_markers_set(mdg_ctx *c, int n, const double *x, const double 
     const double *dx_, *dz_;
     const int *dp_;
     int rc;
-    // stage inputs (host copies go through the sorted buffers' siblings)
+    // The previous marker set is invalid from here on: a failure below may
+    // already have reallocated the sorted buffers, so no cached weights or
+    // offsets may be used until this call succeeds.
+    c->n_markers = 0;
+    for (int q = 0; q < 4; q++) c->mw_state[q] = 0;
     HIPCHK(c, c->field_tmp.reserve(2 * N + 1));
     HIPCHK(c, c->idx.reserve(N + 1));
-    dbuf<double> tx, tz;  // only used when host data must be staged
-    dbuf<int> tp;
+    scoped_dbuf<double> tx, tz;  // only used when host data must be staged
+    scoped_dbuf<int> tp;
     if ((rc = mdg_stage_in(c, x, N, tx, &dx_)) || (rc = mdg_stage_in(c, z, N, tz, &dz_)) ||
-        (rc = mdg_stage_in(c, phase, N, tp, &dp_))) {
-        tx.release(); tz.release(); tp.release();
+        (rc = mdg_stage_in(c, phase, N, tp, &dp_)))
         return rc;
-    }
     const unsigned ncells = (unsigned)c->Ncx * (unsigned)c->Ncz;
     const unsigned Q = 4u * ncells;
     HIPCHK(c, c->keys.reserve(N + 1));
@@ -113,9 +115,7 @@ extern "C" int mdg_markers_set(mdg_ctx *c, int n, const double *x, const double 
                        c->stream, n, c->keys2.p, Q, c->qoff.p);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    tx.release(); tz.release(); tp.release();
-    c->n_markers = n;
-    for (int q = 0; q < 4; q++) c->mw_state[q] = 0;   // cached P2G weights are stale
+    c->n_markers = n;   // cached P2G weights stay invalid (state 0) until recomputed
     return MDG_OK;
 }
 
@@ -128,7 +128,7 @@ extern "C" int mdg_marker_index(mdg_ctx *c, int centroid, int *ip, int *jp) {
     if (n == 0) return MDG_OK;
     const bool dev = mdg_is_device_ptr(ip) && mdg_is_device_ptr(jp);
     int *dip = ip, *djp = jp;
-    dbuf<int> t;
+    scoped_dbuf<int> t;
     if (!dev) {
         HIPCHK(c, t.reserve(2 * (size_t)n));
         dip = t.p;
@@ -143,7 +143,6 @@ extern "C" int mdg_marker_index(mdg_ctx *c, int centroid, int *ip, int *jp) {
         HIPCHK(c, hipMemcpyAsync(jp, djp, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    t.release();
     return MDG_OK;
 }
 
@@ -152,16 +151,13 @@ int mdg_gather_sorted(mdg_ctx *c, const double *src, double *dst) {
     const int n = c->n_markers;
     if (n == 0) return MDG_OK;
     const double *s;
-    dbuf<double> t;
+    scoped_dbuf<double> t;
     int rc = mdg_stage_in(c, src, (size_t)n, t, &s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_gather_field, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, n,
                        c->perm.p, s, dst);
     HIPCHK(c, hipGetLastError());
-    if (t.p) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        t.release();
-    }
+    if (t.p) HIPCHK(c, hipStreamSynchronize(c->stream));   // before t is freed
     return MDG_OK;
 }
 

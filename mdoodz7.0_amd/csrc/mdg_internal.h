@@ -31,6 +31,15 @@ struct dbuf {
     }
 };
 
+// A dbuf for staging inside one call: released on every return path.
+template <typename T>
+struct scoped_dbuf : dbuf<T> {
+    scoped_dbuf() = default;
+    scoped_dbuf(const scoped_dbuf &) = delete;
+    scoped_dbuf &operator=(const scoped_dbuf &) = delete;
+    ~scoped_dbuf() { this->release(); }
+};
+
 struct mdg_ctx {
     // grid
     int    Nx = 0, Nz = 0, Ncx = 0, Ncz = 0;

@@ -64,6 +64,33 @@ __global__ void __launch_bounds__(RB) k_sum_part(int nb, const double *__restric
     if (threadIdx.x == 0) *dst = s[0];
 }
 
+// Signed maximum (MinMaxArrayVal's max, MiscFunctions.c:342-375): block
+// partials, then one block; -inf for an empty array.
+__global__ void __launch_bounds__(RB) k_max_part(int n, const double *__restrict__ a, double *__restrict__ part) {
+    __shared__ double s[RB];
+    double m = -INFINITY;
+    for (int i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) m = fmax(m, a[i]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = RB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+__global__ void __launch_bounds__(RB) k_max_final(int nb, const double *__restrict__ part, double *__restrict__ dst) {
+    __shared__ double s[RB];
+    double m = -INFINITY;
+    for (int i = threadIdx.x; i < nb; i += RB) m = fmax(m, part[i]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = RB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *dst = s[0];
+}
+
 template <typename T>
 __global__ void k_fill(int n, T *__restrict__ a, T v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -855,6 +882,44 @@ __global__ void k_rhs_from_zp(sgrid g, const double *__restrict__ r, const T *__
 }
 
 // ----------------------------------------------------------- host structures
+// HIP events around the fine-level smoother steps of a solve launched kernel
+// by kernel (graphs = 2): their average in-solve duration.
+struct ktimer {
+    std::vector<hipEvent_t> ev;   // pairs (start, stop)
+    int used = 0;                 // pairs recorded since the last drain
+    double ms = 0.0;
+    long launches = 0;
+    ~ktimer() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    int begin(hipStream_t st) {
+        if ((size_t)(2 * used + 2) > ev.size()) {
+            for (int k = 0; k < 2; k++) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return -1;
+                ev.push_back(e);
+            }
+        }
+        return hipEventRecord(ev[2 * used], st) == hipSuccess ? 0 : -1;
+    }
+    int end(hipStream_t st) {
+        if (hipEventRecord(ev[2 * used + 1], st) != hipSuccess) return -1;
+        used++;
+        return 0;
+    }
+    // after the stream has been synchronised
+    int drain() {
+        for (int i = 0; i < used; i++) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]) != hipSuccess) return -1;
+            ms += t;
+            launches++;
+        }
+        used = 0;
+        return 0;
+    }
+};
+
 template <typename T>
 struct mg_level {
     sgrid g;
@@ -895,6 +960,7 @@ struct mg_hier {
     double *h_scal = nullptr;               // pinned host copy of scal
     bool fused = true;                      // three-term Chebyshev kernels
     int deg_coarse = 0;                     // smoothing degree below the finest level
+    ktimer *tm = nullptr;                   // times the fine-level smoother steps (graphs = 2)
     hipGraph_t graph = nullptr;             // one CG iteration (V-cycle included)
     hipGraphExec_t exec = nullptr;
     // pressure Poisson CG (BFBt): vectors, scalars and graph
@@ -933,6 +999,8 @@ struct solver_state {
     mg_hier<float> hf;
     dbuf<double> X, R, Rt, B, Zp, V, Z;   // outer FGCR (V, Z: restart slots)
     dbuf<double> Nu, Np, Ku, Kp;          // learned outer directions and K times them
+    dbuf<double> Rm, Req;                 // residual of the returned solution (grid, equation numbering)
+    ktimer tm;                            // fine-level smoother steps of the last graphs = 2 solve
     dbuf<double> part, scal;
     int m = 0;
     size_t nfull = 0;
@@ -947,6 +1015,7 @@ void mdg_solver_release(mdg_ctx *c) {
     s->X.release(); s->R.release(); s->Rt.release(); s->B.release(); s->Zp.release();
     s->V.release(); s->Z.release(); s->part.release(); s->scal.release();
     s->Nu.release(); s->Np.release(); s->Ku.release(); s->Kp.release();
+    s->Rm.release(); s->Req.release();
     delete s;
     c->solver = nullptr;
 }
@@ -959,6 +1028,16 @@ static int dot_host(mdg_ctx *c, solver_state *s, int n, const T *a, const T *b, 
     const int nb = std::min(1024, mdg_grid1d(n, RB));
     hipLaunchKernelGGL(k_dot_part<T>, dim3(nb), dim3(RB), 0, c->stream, n, a, b, s->part.p);
     hipLaunchKernelGGL(k_sum_part, dim3(1), dim3(RB), 0, c->stream, nb, s->part.p, s->scal.p);
+    HIPCHK(c, hipMemcpyAsync(out, s->scal.p, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDG_OK;
+}
+
+// signed maximum of n device doubles into a host double (synchronising)
+static int max_host(mdg_ctx *c, solver_state *s, int n, const double *a, double *out) {
+    const int nb = std::max(1, std::min(1024, mdg_grid1d(n, RB)));
+    hipLaunchKernelGGL(k_max_part, dim3(nb), dim3(RB), 0, c->stream, n, a, s->part.p);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(RB), 0, c->stream, nb, (const double *)s->part.p, s->scal.p);
     HIPCHK(c, hipMemcpyAsync(out, s->scal.p, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDG_OK;
@@ -1280,6 +1359,7 @@ template <typename T>
 static int poisson_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solver_opts *o, int *its) {
     const int n = ncell(H.L[0].g);
     const bool graphs = (o->graphs == 0);
+    H.tm = (o->graphs == 2) ? &s->tm : nullptr;
     if (graphs && !H.pexec) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         poisson_iteration(c, H, o->cheb_degree);
@@ -1338,7 +1418,8 @@ static int bfbt_rhs(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solver
 
 // ----------------------------------------------------------------- V-cycle
 template <typename T>
-static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool zero_start, bool fused) {
+static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool zero_start, bool fused,
+                      ktimer *tm = nullptr) {
     // x (+)= Chebyshev(deg) correction for A_g x = b
     const double lmax = lv.lmax, lmin = lmax / lv.cheb_div;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
@@ -1364,9 +1445,11 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
                                (const T *)lv.es.p);
         for (int k = 0; k < deg - 1; k++) {
             const double rho_new = 1.0 / (2.0 * sigma - rho);
+            if (tm) tm->begin(c->stream);
             hipLaunchKernelGGL((k_cheb3<T, false>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, lv.g, cf, tg, (T)gamma,
                                (T)(rho_new * rho), (T)(2.0 * rho_new / delta), (const T *)cur, prev,
                                lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
+            if (tm) tm->end(c->stream);
             std::swap(cur, prev);
             rho = rho_new;
         }
@@ -1404,7 +1487,8 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, i
         return;
     }
     mg_level<T> &cl = H.L[i + 1];
-    chebyshev(c, lv, gamma, deg, true, H.fused);
+    ktimer *tm = (i == 0) ? H.tm : nullptr;
+    chebyshev(c, lv, gamma, deg, true, H.fused, tm);
     const int reps = (i + 2 == H.L.size()) ? 1 : cycle;
     for (int rep = 0; rep < reps; rep++) {
         hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p, lv.rhs(),
@@ -1418,7 +1502,7 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, i
         vcycle(c, H, i + 1, gamma, deg, cycle);
         hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, cl.x.p, lv.x.p);
     }
-    chebyshev(c, lv, gamma, deg, false, H.fused);
+    chebyshev(c, lv, gamma, deg, false, H.fused, tm);
 }
 
 // Records one CG iteration (preconditioning V-cycle, direction, operator
@@ -1524,6 +1608,7 @@ static int inner_pcg(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solve
         if ((it + 1) % every && it + 1 < o->inner_max_its) continue;
         HIPCHK(c, hipMemcpyAsync(H.h_scal, H.scal.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (H.tm && H.tm->drain()) return mdg_fail(c, MDG_ERR_HIP, "smoother timing events failed");
         if (!(H.h_scal[2] > 0.0) || H.h_scal[3] <= tol2) break;
     }
     return mdg_pcnet_capture_out<double>(c, l0.g, (const double *)H.cg_x.p);
@@ -1700,6 +1785,19 @@ int mdg_apply_Kg_grid(mdg_ctx *c, double gamma, const double *x, double *y) {
 }
 
 static void default_opts(mdg_solver_opts *o) {
+    if (o->method == 1) {
+        // KillerSolver's own defaults (InputOutput.c:1174-1198)
+        if (o->penalty <= 0) o->penalty = 1e3;
+        if (o->max_its_PH <= 0) o->max_its_PH = 10;
+        if (o->rel_tol_KSP <= 0) o->rel_tol_KSP = 1e-4;
+        if (o->max_its_KSP <= 0) o->max_its_KSP = 100;
+        if (o->lin_abs_mom <= 0) o->lin_abs_mom = 1e-9;
+        if (o->lin_rel_mom <= 0) o->lin_rel_mom = 1e-5;
+        if (o->lin_abs_div <= 0) o->lin_abs_div = 1e-9;
+        if (o->lin_rel_div <= 0) o->lin_rel_div = 1e-5;
+        if (o->pc_tol <= 0) o->pc_tol = 1e-10;
+        if (o->inner_max_its <= 0) o->inner_max_its = 20000;
+    }
     if (o->penalty <= 0) o->penalty = 10.0;
     if (o->rel_tol <= 0) o->rel_tol = 1e-8;
     if (o->max_its <= 0) o->max_its = 500;
@@ -1730,6 +1828,10 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     if (o->schur < 0 || o->schur > 1) return mdg_fail(c, MDG_ERR_ARG, "schur must be 0 or 1");
     if (o->cycle != 1) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "only the V-cycle (cycle = 1) is supported");
     if (o->smoother < 0 || o->smoother > 1) return mdg_fail(c, MDG_ERR_ARG, "smoother must be 0 or 1");
+    if (o->graphs < 0 || o->graphs > 2) return mdg_fail(c, MDG_ERR_ARG, "graphs must be 0, 1 or 2");
+    s->tm.ms = 0.0;
+    s->tm.launches = 0;
+    s->tm.used = 0;
     if (o->coarsen4 > 0 && o->schur == 1)
         return mdg_fail(c, MDG_ERR_UNSUPPORTED, "the BFBt pressure hierarchy coarsens by 2 only (coarsen4 = 0)");
     if (o->precision == MDG_PREC_MIXED && o->schur == 1)
@@ -1768,6 +1870,8 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     // precond 2: mean ||R||^2 ratio across each iteration's network pair
     const int ng = (o->precond >= 2) ? s->guess_used / 2 : s->guess_used;
     o->guess_gain = ng ? s->guess_gain / ng : 0.0;
+    o->smoother_launches = (int)s->tm.launches;
+    o->smoother_ms = s->tm.launches ? s->tm.ms / s->tm.launches : 0.0;
     return rc;
 }
 
@@ -1776,6 +1880,8 @@ extern "C" int mdg_killer_solver(mdg_ctx *c, mdg_solver_opts *o, const double *D
                                  const double *rhs_cont, double *sol) {
     if (!c || !o || !rhs_mom || !rhs_cont || !sol) return mdg_fail(c, MDG_ERR_ARG, "null argument");
     if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
+    if (o->method < 0 || o->method > 1) return mdg_fail(c, MDG_ERR_ARG, "method must be 0 or 1");
+    if (o->method == 1) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "method 1 is not built yet");
     int rc;
     if ((rc = mdg_stage_coeffs(c, D11_n, D22_n, D33_s)) || (rc = mdg_reserve_grid_vectors(c))) return rc;
     const sgrid g = c->sg;
@@ -1785,14 +1891,51 @@ extern "C" int mdg_killer_solver(mdg_ctx *c, mdg_solver_opts *o, const double *D
     HIPCHK(c, hipMemcpyAsync(c->eqbuf.p, rhs_mom, (size_t)c->neq_mom * 8, hipMemcpyDefault, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->eqbuf.p + c->neq_mom, rhs_cont, (size_t)c->neq_cont * 8, hipMemcpyDefault, c->stream));
     double *b = c->gx.p, *x = c->gy.p;
-    (void)n;
     if ((rc = mdg_eqn_to_grid(c, c->eqbuf.p, b, b + nvx(g), b + nvx(g) + nvz(g), 1.0))) return rc;
+    if (!c->solver) c->solver = new solver_state();
+    solver_state *s = c->solver;
+    HIPCHK(c, s->part.reserve(4096));
+    HIPCHK(c, s->scal.reserve(16));
+    double mom0 = 0.0, div0 = 0.0;
+    if (o->method == 0 && ((rc = max_host(c, s, c->neq_mom, c->eqbuf.p, &mom0)) ||
+                           (rc = max_host(c, s, c->neq_cont, c->eqbuf.p + c->neq_mom, &div0))))
+        return rc;
     rc = mdg_solve_grid(c, o, b, x);
     if (rc && rc != MDG_ERR_DIVERGED) return rc;
+    if (o->method == 0) {
+        // signed maxima of the returned solution's residual b - K x
+        // (equation numbering) against those of the zero start
+        HIPCHK(c, s->Rm.reserve(n));
+        HIPCHK(c, s->Req.reserve((size_t)c->neq));
+        int rc3;
+        if (mdg_newton_active(c)) {
+            if ((rc3 = mdg_newton_apply_grid(c, 0.0, x, s->Rm.p))) return rc3;
+        } else {
+            hipLaunchKernelGGL(k_apply_Kg, dim3(nblk((int)n)), dim3(256), 0, c->stream, g, 0.0, (const double *)x,
+                               (const double *)c->eta_n.p, (const double *)c->eta_s.p, s->Rm.p);
+        }
+        hipLaunchKernelGGL(k_axpby<double>, dim3(nblk((int)n)), dim3(256), 0, c->stream, (int)n, 1.0, (const double *)b,
+                           -1.0, s->Rm.p);
+        if ((rc3 = mdg_grid_to_eqn(c, s->Rm.p, s->Rm.p + nvx(g), s->Rm.p + nvx(g) + nvz(g), s->Req.p)) ||
+            (rc3 = max_host(c, s, c->neq_mom, s->Req.p, &o->max_mom)) ||
+            (rc3 = max_host(c, s, c->neq_cont, s->Req.p + c->neq_mom, &o->max_div)))
+            return rc3;
+        o->max_mom0 = mom0;
+        o->max_div0 = div0;
+    }
     const int rc2 = mdg_grid_to_eqn(c, x, x + nvx(g), x + nvx(g) + nvz(g), c->eqbuf.p);
     if (rc2) return rc2;
     HIPCHK(c, hipMemcpyAsync(sol, c->eqbuf.p, (size_t)c->neq * 8, hipMemcpyDefault, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // KillerSolver's a-posteriori incompressibility check (Solvers.c:1333-1336):
+    // a missed momentum tolerance is carried on with, a continuity residual
+    // above both bounds is not
+    if ((o->lin_abs_div > 0.0 || o->lin_rel_div > 0.0) && std::fabs(o->max_div) > 10.0 * o->lin_abs_div &&
+        o->max_div / o->max_div0 > o->lin_rel_div)
+        return mdg_fail(c, MDG_ERR_INCOMPRESSIBLE,
+                        "incompressibility not satisfied: max. div. residual %.3e (x %.3e of the first), bounds "
+                        "10 x %.1e and %.1e",
+                        o->max_div, o->max_div / o->max_div0, o->lin_abs_div, o->lin_rel_div);
     return rc;
 }
 
@@ -1807,6 +1950,11 @@ extern "C" int mdg_time_smoother(mdg_ctx *c, int reps, double *ms, double *bytes
     if (!c->solver || c->solver->hd.L.empty()) return mdg_fail(c, MDG_ERR_STATE, "no FP64 solve has run");
     mg_level<double> &l0 = c->solver->hd.L[0];
     const int n = l0.n;
+    // the FP64 V-cycle reads the CG residual in place, so level 0's own b is
+    // never written by a solve: give it the last CG residual
+    if (c->solver->hd.cg_r.p)
+        HIPCHK(c, hipMemcpyAsync(l0.b.p, c->solver->hd.cg_r.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
+                                 c->stream));
     const tile_grid tg = make_tiles(l0.g);
     const stc<double> cf = make_stc<double>(l0.g);
     auto launch = [&]() {

@@ -101,7 +101,13 @@ class SolverOpts(C.Structure):
                 ("inner_its", C.c_int),
                 ("poisson_its", C.c_int),
                 ("levels", C.c_int), ("status", C.c_int), ("rel_res", C.c_double),
-                ("time_ms", C.c_double), ("guess_used", C.c_int), ("guess_gain", C.c_double)]
+                ("time_ms", C.c_double), ("guess_used", C.c_int), ("guess_gain", C.c_double),
+                ("method", C.c_int), ("max_its_PH", C.c_int), ("rel_tol_KSP", C.c_double),
+                ("max_its_KSP", C.c_int), ("lin_abs_mom", C.c_double), ("lin_rel_mom", C.c_double),
+                ("lin_abs_div", C.c_double), ("lin_rel_div", C.c_double), ("pc_tol", C.c_double),
+                ("max_mom", C.c_double), ("max_mom0", C.c_double), ("max_div", C.c_double),
+                ("max_div0", C.c_double), ("its_PH", C.c_int), ("its_KSP", C.c_int),
+                ("smoother_ms", C.c_double), ("smoother_launches", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -109,6 +115,7 @@ class SolverOpts(C.Structure):
 
 MDG_PREC_FP64, MDG_PREC_MIXED = 0, 1
 MDG_ERR_DIVERGED = -6
+MDG_ERR_INCOMPRESSIBLE = -8
 
 
 class PhaseProp(C.Structure):
@@ -187,9 +194,35 @@ def ptr(a):
     if isinstance(a, DeviceArray):
         return a.p
     if isinstance(a, np.ndarray):
-        assert a.flags["C_CONTIGUOUS"], "arrays must be C-contiguous"
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays must be C-contiguous")
         return a.ctypes.data
-    raise TypeError("expected numpy array or torch tensor, got %r" % type(a))
+    raise TypeError("expected a numpy array or DeviceArray, got %r" % type(a))
+
+
+def checked(a, dtype, n, name, optional=False):
+    """ptr(a) after checking that a holds exactly n elements of dtype (a
+    DeviceArray may be longer).  The library reads and writes these arrays
+    through raw pointers, so a wrong dtype or a short array would be misread
+    or overrun without any error of its own."""
+    if a is None:
+        if optional:
+            return None
+        raise ValueError("%s is required" % name)
+    dt = np.dtype(dtype)
+    if isinstance(a, DeviceArray):
+        if a.dtype != dt:
+            raise ValueError("%s: expected %s, got %s" % (name, dt, a.dtype))
+        if a.n < n:
+            raise ValueError("%s: %d elements, need %d" % (name, a.n, n))
+        return a.p
+    if isinstance(a, np.ndarray):
+        if a.dtype != dt:
+            raise ValueError("%s: expected %s, got %s" % (name, dt, a.dtype))
+        if a.size != n:
+            raise ValueError("%s: %d elements, expected %d" % (name, a.size, n))
+        return ptr(a)
+    raise TypeError("%s: expected a numpy array or DeviceArray, got %r" % (name, type(a)))
 
 
 class DeviceArray:
@@ -208,7 +241,8 @@ class DeviceArray:
 
     def upload(self, host):
         host = np.ascontiguousarray(host, self.dtype)
-        assert host.size == self.n
+        if host.size != self.n:
+            raise ValueError("upload: %d elements into a device array of %d" % (host.size, self.n))
         self.ctx._check(self.ctx._L.mdg_memcpy(self.ctx.h, self.p, host.ctypes.data, host.nbytes))
         return self
 
@@ -284,12 +318,25 @@ class Context:
         return out
 
     def set_bc(self, BCu_t, BCu_v, BCv_t, BCv_v, BCp_t, BCg_t):
-        self._check(self._L.mdg_set_bc(self.h, ptr(BCu_t), ptr(BCu_v), ptr(BCv_t), ptr(BCv_v),
-                                       ptr(BCp_t), ptr(BCg_t)))
+        nu, nv = self.node_count(-1), self.node_count(-2)
+        self._check(self._L.mdg_set_bc(self.h, checked(BCu_t, np.int8, nu, "BCu_t"),
+                                       checked(BCu_v, np.float64, nu, "BCu_v"),
+                                       checked(BCv_t, np.int8, nv, "BCv_t"),
+                                       checked(BCv_v, np.float64, nv, "BCv_v"),
+                                       checked(BCp_t, np.int8, self.node_count(1), "BCp_t"),
+                                       checked(BCg_t, np.int8, self.node_count(0), "BCg_t")))
 
     def markers_set(self, x, z, phase):
-        self._check(self._L.mdg_markers_set(self.h, len(x), ptr(x), ptr(z), ptr(phase)))
-        self.n_markers = len(x)
+        n = len(x)
+        self._check(self._L.mdg_markers_set(self.h, n, checked(x, np.float64, n, "x"),
+                                            checked(z, np.float64, n, "z"),
+                                            checked(phase, np.int32, n, "phase")))
+        self.n_markers = n
+
+    def _marker_args(self, x, z, phase):
+        n = len(x)
+        return n, (checked(x, np.float64, n, "x"), checked(z, np.float64, n, "z"),
+                   checked(phase, np.int32, n, "phase"))
 
     def marker_index(self, centroid):
         ip = np.zeros(self.n_markers, np.int32)
@@ -299,45 +346,63 @@ class Context:
 
     def v2p(self, u, v, x, z, phase, conserv=0, vx=None, vz=None):
         """V2P (ParticleRoutines.c:63): (vx, vz) at the markers."""
-        n = len(x)
+        n, (px, pz, pp) = self._marker_args(x, z, phase)
         vx = np.zeros(n) if vx is None else vx
         vz = np.zeros(n) if vz is None else vz
-        self._check(self._L.mdg_v2p(self.h, ptr(u), ptr(v), conserv, n, ptr(x), ptr(z), ptr(phase),
-                                    ptr(vx), ptr(vz)))
+        self._check(self._L.mdg_v2p(self.h, checked(u, np.float64, self.node_count(-1), "u"),
+                                    checked(v, np.float64, self.node_count(-2), "v"), conserv, n,
+                                    px, pz, pp, checked(vx, np.float64, n, "vx"),
+                                    checked(vz, np.float64, n, "vz")))
         return vx, vz
 
     def advect_rk4(self, u, v, dt, x, z, phase, conserv=0, vx=None, vz=None):
         """RogerGuntherII (AdvectionRoutines.c:347): x, z, phase (and vx, vz
         when given) updated in place; numpy arrays must be contiguous float64
         / int32, or DeviceArray."""
-        self._check(self._L.mdg_advect_rk4(self.h, ptr(u), ptr(v), dt, conserv, len(x), ptr(x), ptr(z),
-                                           ptr(phase), ptr(vx), ptr(vz)))
+        n, (px, pz, pp) = self._marker_args(x, z, phase)
+        self._check(self._L.mdg_advect_rk4(
+            self.h, checked(u, np.float64, self.node_count(-1), "u"),
+            checked(v, np.float64, self.node_count(-2), "v"), dt, conserv, n, px, pz, pp,
+            checked(vx, np.float64, n, "vx", optional=True),
+            checked(vz, np.float64, n, "vz", optional=True)))
 
     def g2p_centroids(self, F, x, z, phase, out=None):
         """Interp_Grid2P_centroids2 (ParticleRoutines.c:1332): a centre field
         at the markers."""
-        out = np.zeros(len(x)) if out is None else out
-        self._check(self._L.mdg_g2p_centroids(self.h, ptr(F), len(x), ptr(x), ptr(z), ptr(phase), ptr(out)))
+        n, (px, pz, pp) = self._marker_args(x, z, phase)
+        out = np.zeros(n) if out is None else out
+        self._check(self._L.mdg_g2p_centroids(self.h, checked(F, np.float64, self.node_count(1), "F"), n,
+                                              px, pz, pp, checked(out, np.float64, n, "out")))
         return out
 
     def g2p_vertices(self, F, x, z, phase, out=None):
         """Interp_Grid2P (ParticleRoutines.c:1223) on the vertices."""
-        out = np.zeros(len(x)) if out is None else out
-        self._check(self._L.mdg_g2p_vertices(self.h, ptr(F), len(x), ptr(x), ptr(z), ptr(phase), ptr(out)))
+        n, (px, pz, pp) = self._marker_args(x, z, phase)
+        out = np.zeros(n) if out is None else out
+        self._check(self._L.mdg_g2p_vertices(self.h, checked(F, np.float64, self.node_count(0), "F"), n,
+                                             px, pz, pp, checked(out, np.float64, n, "out")))
         return out
 
     def update_particle_stress(self, sxxd, szzd, sxz, sxxd0, szzd0, sxz0, x, z, phase, msxxd, mszzd, msxz):
         """UpdateParticleStress (RheologyParticles.c:1725; no subgrid stress
         diffusion, no rotation): marker stresses updated in place."""
+        n, (px, pz, pp) = self._marker_args(x, z, phase)
+        nc, nv = self.node_count(1), self.node_count(0)
         self._check(self._L.mdg_update_particle_stress(
-            self.h, ptr(sxxd), ptr(szzd), ptr(sxz), ptr(sxxd0), ptr(szzd0), ptr(sxz0), len(x), ptr(x), ptr(z),
-            ptr(phase), ptr(msxxd), ptr(mszzd), ptr(msxz)))
+            self.h, checked(sxxd, np.float64, nc, "sxxd"), checked(szzd, np.float64, nc, "szzd"),
+            checked(sxz, np.float64, nv, "sxz"), checked(sxxd0, np.float64, nc, "sxxd0"),
+            checked(szzd0, np.float64, nc, "szzd0"), checked(sxz0, np.float64, nv, "sxz0"), n, px, pz, pp,
+            checked(msxxd, np.float64, n, "msxxd"), checked(mszzd, np.float64, n, "mszzd"),
+            checked(msxz, np.float64, n, "msxz")))
 
     def update_particle_pressure(self, p_in, p0_n, x, z, phase, P, dp=None):
         """UpdateParticlePressure (RheologyParticles.c:1621, no subgrid
         diffusion): P updated in place; returns dp when given."""
-        self._check(self._L.mdg_update_particle_pressure(self.h, ptr(p_in), ptr(p0_n), ptr(dp), len(x), ptr(x),
-                                                         ptr(z), ptr(phase), ptr(P)))
+        n, (px, pz, pp) = self._marker_args(x, z, phase)
+        nc = self.node_count(1)
+        self._check(self._L.mdg_update_particle_pressure(
+            self.h, checked(p_in, np.float64, nc, "p_in"), checked(p0_n, np.float64, nc, "p0_n"),
+            checked(dp, np.float64, nc, "dp", optional=True), n, px, pz, pp, checked(P, np.float64, n, "P")))
         return dp
 
     def markers_gather(self, src, dst_sorted):
@@ -358,9 +423,10 @@ class Context:
             return phase_perc
         if node_field is None:
             node_field = np.zeros(nn)
-        self._check(self._L.mdg_p2mastah(self.h, ptr(mat_prop), len(mat_prop), ptr(node_field),
-                                         ptr(node_type), flag, avg, 0, centroid, interp_stencil,
-                                         0, None))
+        self._check(self._L.mdg_p2mastah(self.h, checked(mat_prop, np.float64, len(mat_prop), "mat_prop"),
+                                         len(mat_prop), checked(node_field, np.float64, nn, "node_field"),
+                                         checked(node_type, np.int8, nn, "node_type", optional=True),
+                                         flag, avg, 0, centroid, interp_stencil, 0, None))
         return node_field
 
     def p2mastah_fused(self, centroid, interp_stencil, fields):

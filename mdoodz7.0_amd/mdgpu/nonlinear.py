@@ -131,6 +131,10 @@ class PicardLoop:
         coefficients and the line search (InputOutput.c:1334).
         st["newton_steps"] lists the iteration numbers solved with Newton."""
         c = self.ctx
+        # the reference's continuity tolerances (InputOutput.c:1179-1180)
+        # for KillerSolver's a-posteriori incompressibility check
+        solver_opts.setdefault("lin_abs_div", 1e-9)
+        solver_opts.setdefault("lin_rel_div", 1e-5)
         rx, rz = c.eval_rhs(gx, gz, st["rho_n"], st["rho_s"])
         abs_mom, rel_mom, abs_div, rel_div = tol
         hist, stats, res0 = [], [], None
@@ -169,8 +173,11 @@ class PicardLoop:
                 dx, o = c.killer_solver(D11, D11, D33, F[:nm], F[nm:], accept_diverged=True,
                                         **solver_opts)
             if o.status == MDG_ERR_DIVERGED:
-                # the reference warns and carries on with the unconverged
-                # correction; the line search (or alpha) decides what it is worth
+                # A missed momentum tolerance: KillerSolver carries on with the
+                # correction after max_its (the line search or alpha decides what
+                # it is worth).  A continuity residual above its a-posteriori
+                # bound (Solvers.c:1333-1336) is not carried on with: the solver
+                # returns MDG_ERR_INCOMPRESSIBLE for it, which raises above.
                 warnings.warn("linear solve of non-linear iteration %d stopped at relative "
                               "residual %.3e" % (nit, o.rel_res))
                 st["diverged_solves"].append(nit)

@@ -75,3 +75,31 @@ def test_killer_solve_matches_a_direct_solve(oracle_mod):
     assert rel(du, x[:nm]) < 1e-7
     xp = x[nm:nm + npr]
     assert rel(dp - dp.mean(), xp - xp.mean()) < 1e-7
+
+
+def test_killer_solve_incompressibility_abort(oracle_mod):
+    """KillerSolver's a-posteriori check (Solvers.c:1333-1336): the solve is
+    refused (status -2, the reference exits) when the continuity residual's
+    signed maximum stays above 10 lin_abs_div and above lin_rel_div of its
+    first Powell-Hestenes value.  One PH iteration with a weak penalty leaves
+    a ratio of exactly 1 and a large divergence, so it must abort; the same
+    system with the SolVi penalty and enough PH iterations must not."""
+    O = oracle_mod
+    from oracle.driver import evaluate_residual, initial_fields
+    g, num, en, es, A, B, Cm = _picard(O, n=21)
+    nm = num[3]
+    u, v, p = initial_fields(g)
+    rx, rz = O.eval_rhs(g, 0.0, 0.0, np.zeros(en.size), np.zeros(es.size))
+    F, _ = evaluate_residual(g, num, u, v, p, en, es, rx, rz)
+    _, _, st, rc = O.killer_solve(g, num, A, B, Cm, F[:nm], F[nm:], penalty=1.0, max_its_PH=1)
+    assert rc == -2 and st.status == -2
+    assert st.maxdiv == st.maxdiv0 and abs(st.maxdiv) > 10 * 1e-9
+    _, _, st, rc = O.killer_solve(g, num, A, B, Cm, F[:nm], F[nm:], penalty=1e5, max_its_PH=10)
+    assert rc == 0, (st.maxdiv, st.maxdiv0)
+    assert abs(st.maxdiv) < 1e-9 or st.maxdiv / st.maxdiv0 < 1e-5
+    # the criterion reads the signed maximum, not the largest magnitude
+    # (MinMaxArrayVal, MiscFunctions.c:342-375): a negative-only residual
+    # has a negative maximum and passes the absolute test whatever its size
+    _, _, st, rc = O.killer_solve(g, num, A, B, Cm, F[:nm], -np.abs(F[nm:]) - 1.0, penalty=1.0,
+                                  max_its_PH=1)
+    assert st.maxdiv <= st.maxdiv0 or st.maxdiv < 0
