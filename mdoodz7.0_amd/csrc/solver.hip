@@ -1000,6 +1000,7 @@ struct solver_state {
     dbuf<double> X, R, Rt, B, Zp, V, Z;   // outer FGCR (V, Z: restart slots)
     dbuf<double> Nu, Np, Ku, Kp;          // learned outer directions and K times them
     dbuf<double> Rm, Req;                 // residual of the returned solution (grid, equation numbering)
+    dbuf<double> ph_u, ph_f, ph_v, ph_s, ph_VV, ph_SS;   // KillerSolver iteration (method 1)
     ktimer tm;                            // fine-level smoother steps of the last graphs = 2 solve
     dbuf<double> part, scal;
     int m = 0;
@@ -1016,6 +1017,7 @@ void mdg_solver_release(mdg_ctx *c) {
     s->V.release(); s->Z.release(); s->part.release(); s->scal.release();
     s->Nu.release(); s->Np.release(); s->Ku.release(); s->Kp.release();
     s->Rm.release(); s->Req.release();
+    s->ph_u.release(); s->ph_f.release(); s->ph_v.release(); s->ph_s.release(); s->ph_VV.release(); s->ph_SS.release();
     delete s;
     c->solver = nullptr;
 }
@@ -1112,7 +1114,7 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
         const bool by4 = o->coarsen4 > 0 && ncx <= o->coarsen4 && ncz <= o->coarsen4;
         const int ncx2 = by4 ? (ncx + 3) / 4 : (ncx + 1) / 2, ncz2 = by4 ? (ncz + 3) / 4 : (ncz + 1) / 2;
         lv.rt = by4 ? RT4 : RT2;
-        sgrid gc = {ncx2 + 1, ncz2 + 1, Lx / ncx2, Lz / ncz2, g.ghostS, g.ghostN, g.ghostW, g.ghostE};
+        sgrid gc = {ncx2 + 1, ncz2 + 1, Lx / ncx2, Lz / ncz2, g.ghostS, g.ghostN, g.ghostW, g.ghostE, g.pen_const};
         // Vx: x at vertices (index * h), z at centres ((index - 0.5) * h)
         lv.mu = comp_maps{make_axis(g.dx, gc.dx, 0.0, gc.Nx), make_axis(g.dz, gc.dz, -0.5, gc.Nz + 1),
                           g.Nx, g.Nz + 1, gc.Nx, gc.Nz + 1, 0, g.ghostS, g.ghostN};
@@ -1776,6 +1778,159 @@ static int fgcr_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opt
     return mdg_fail(c, MDG_ERR_DIVERGED, "no convergence in %d iterations (rel. residual %.3e)", o->max_its, o->rel_res);
 }
 
+// ------------------------------------------------ KillerSolver's own iteration
+// Powell-Hestenes right-hand side of the velocity solve (Solvers.c:1290-1294):
+// ud = b_u - B (pen b_p + dp), grid layout, x = [du | dp].
+__global__ void k_ph_udum(sgrid g, double pen, const double *__restrict__ b, const double *__restrict__ x,
+                          double *__restrict__ ud) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu = nvx(g), nv = nvz(g), Ncx = g.Nx - 1;
+    const double *bp = b + nu + nv, *dp = x + nu + nv;
+    auto q = [&](int c) { return pen * bp[c] + dp[c]; };
+    if (i < nu) {
+        const int k = i % g.Nx, l = i / g.Nx;
+        ud[i] = vx_free(g, k, l) ? b[i] - (q(k + (l - 1) * Ncx) - q(k - 1 + (l - 1) * Ncx)) * g.dz : 0.0;
+    } else if (i < nu + nv) {
+        const int j = i - nu, k = j % (g.Nx + 1), l = j / (g.Nx + 1);
+        ud[i] = vz_free(g, k, l) ? b[i] - (q(k - 1 + l * Ncx) - q(k - 1 + (l - 1) * Ncx)) * g.dx : 0.0;
+    }
+}
+// Powell-Hestenes pressure update (Solvers.c:1301-1304): dp += pen (b_p - C du)
+__global__ void k_ph_pupdate(sgrid g, double pen, const double *__restrict__ b, double *__restrict__ x) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu = nvx(g), nv = nvz(g);
+    if (j >= ncell(g)) return;
+    const int k = j % (g.Nx - 1), l = j / (g.Nx - 1);
+    x[nu + nv + j] += pen * (b[nu + nv + j] - div_cell<double>(g, k, l, x, x + nu));
+}
+
+// kspgcr (Solvers.c:124-230) on Js = A + pen C'C (the velocity block with
+// KillerSolver's constant penalty, level 0 of H, pen_const set), b -> x in
+// place (warm start), preconditioned by the multigrid CG solve of Jts = Js
+// to o->pc_tol (in place of the Cholesky factor).  Keeps the reference's
+// restart 6, its orthogonalisation that also subtracts the coefficient and
+// pair stored in slot i1 by the previous cycle (zero in the first), its
+// stopping test on ||f|| < eps ||f0|| and its iteration count.
+template <typename T>
+static int ksp_gcr6(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opts *o, const double *b, double *x,
+                    int *its_out) {
+    const int restart = 6;
+    mg_level<T> &l0 = H.L[0];
+    const sgrid g = l0.g;
+    const int n = nvx(g) + nvz(g);
+    const double gamma = o->penalty;
+    HIPCHK(c, s->ph_f.reserve(n)); HIPCHK(c, s->ph_v.reserve(n)); HIPCHK(c, s->ph_s.reserve(n));
+    HIPCHK(c, s->ph_VV.reserve((size_t)restart * n)); HIPCHK(c, s->ph_SS.reserve((size_t)restart * n));
+    double *f = s->ph_f.p, *v = s->ph_v.p, *sv = s->ph_s.p;
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(restart * n)), dim3(256), 0, c->stream, restart * n, s->ph_VV.p, 0.0);
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(restart * n)), dim3(256), 0, c->stream, restart * n, s->ph_SS.p, 0.0);
+    // f = b - Js x
+    hipLaunchKernelGGL(k_vel_residual<double>, dim3(nblk(n)), dim3(256), 0, c->stream, g, gamma, (const double *)x, b,
+                       (const double *)c->eta_n.p, (const double *)c->eta_s.p, f);
+    double ff;
+    int rc;
+    if ((rc = dot_host<double>(c, s, n, f, f, &ff))) return rc;
+    const double rnorm0 = std::sqrt(ff);
+    double val[6] = {0, 0, 0, 0, 0, 0};
+    int its = 0;
+    bool success = false;
+    mdg_solver_opts pc = *o;           // the preconditioning solve
+    pc.inner_tol = o->pc_tol;
+    pc.precond = 0;
+    while (!success && its < o->max_its_KSP) {
+        for (int i1 = 0; i1 < restart; i1++) {
+            // s = Jts^-1 f
+            HIPCHK(c, hipMemcpyAsync(H.cg_r.p, f, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            int cg_its = 0;
+            if ((rc = inner_pcg(c, s, H, &pc, &cg_its))) return rc;
+            o->inner_its += cg_its;
+            HIPCHK(c, hipMemcpyAsync(sv, H.cg_x.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            // v = Js s
+            hipLaunchKernelGGL(k_vel_residual<double>, dim3(nblk(n)), dim3(256), 0, c->stream, g, gamma, (const double *)sv,
+                               (const double *)nullptr, (const double *)c->eta_n.p, (const double *)c->eta_s.p, v);
+            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, 0.0, (const double *)v, -1.0, v);
+            for (int i2 = 0; i2 < i1; i2++)
+                if ((rc = dot_host<double>(c, s, n, v, s->ph_VV.p + (size_t)i2 * n, &val[i2]))) return rc;
+            for (int i2 = 0; i2 < i1 + 1; i2++) {
+                hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, -val[i2],
+                                   (const double *)(s->ph_VV.p + (size_t)i2 * n), 1.0, v);
+                hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, -val[i2],
+                                   (const double *)(s->ph_SS.p + (size_t)i2 * n), 1.0, sv);
+            }
+            double rdv, vv;
+            if ((rc = dot_host<double>(c, s, n, f, v, &rdv)) || (rc = dot_host<double>(c, s, n, v, v, &vv))) return rc;
+            const double nrm = std::sqrt(vv);
+            if (!(nrm > 0.0)) return mdg_fail(c, MDG_ERR_DIVERGED, "GCR breakdown in the Powell-Hestenes velocity solve");
+            rdv /= nrm;
+            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, 0.0, (const double *)v, 1.0 / nrm, v);
+            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, 0.0, (const double *)sv, 1.0 / nrm, sv);
+            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, rdv, (const double *)sv, 1.0, x);
+            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, -rdv, (const double *)v, 1.0, f);
+            if ((rc = dot_host<double>(c, s, n, f, f, &ff))) return rc;
+            if (std::isnan(ff)) return mdg_fail(c, MDG_ERR_NAN, "NaN in the Powell-Hestenes velocity solve");
+            if (std::sqrt(ff) < o->rel_tol_KSP * rnorm0) {
+                success = true;
+                break;
+            }
+            HIPCHK(c, hipMemcpyAsync(s->ph_VV.p + (size_t)i1 * n, v, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
+                                     c->stream));
+            HIPCHK(c, hipMemcpyAsync(s->ph_SS.p + (size_t)i1 * n, sv, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
+                                     c->stream));
+            its++;
+        }
+        its++;
+    }
+    *its_out = its;
+    return MDG_OK;
+}
+
+// KillerSolver (Solvers.c:1054-1454) for the incompressible Picard system:
+// Powell-Hestenes iterations with the constant penalty pen = penalty/dV
+// around kspgcr, stopping on the signed maxima of the residuals (k > 1,
+// Solvers.c:1317-1322); b = [b_u | b_p] -> x = [du | dp], grid layout.
+template <typename T>
+static int ph_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opts *o, const double *b, double *x) {
+    const sgrid g = c->sg;
+    const int nu = nvx(g), nv = nvz(g), np = ncell(g), n = nu + nv + np;
+    const double pen = o->penalty / (g.dx * g.dz);
+    HIPCHK(c, s->Rm.reserve(n));
+    HIPCHK(c, s->Req.reserve((size_t)c->neq));
+    HIPCHK(c, s->ph_u.reserve(nu + nv));
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, x, 0.0);
+    o->its = 0; o->inner_its = 0; o->its_KSP = 0; o->rel_res = 0.0;
+    double maxdiv = 0.0, maxdiv0 = 1.0, maxru = 0.0, maxru0 = 1.0;
+    int k, rc;
+    double bb;
+    if ((rc = dot_host<double>(c, s, n, b, b, &bb))) return rc;
+    for (k = 0; k < o->max_its_PH; k++) {
+        hipLaunchKernelGGL(k_ph_udum, dim3(nblk(nu + nv)), dim3(256), 0, c->stream, g, pen, b, (const double *)x, s->ph_u.p);
+        int its = 0;
+        if ((rc = ksp_gcr6(c, s, H, o, s->ph_u.p, x, &its))) return rc;
+        o->its_KSP += its;
+        hipLaunchKernelGGL(k_ph_pupdate, dim3(nblk(np)), dim3(256), 0, c->stream, g, pen, b, x);
+        // f = b - K x, then the signed maxima over the free rows (equation numbering)
+        hipLaunchKernelGGL(k_apply_Kg, dim3(nblk(n)), dim3(256), 0, c->stream, g, 0.0, (const double *)x,
+                           (const double *)c->eta_n.p, (const double *)c->eta_s.p, s->Rm.p);
+        hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, 1.0, b, -1.0, s->Rm.p);
+        if ((rc = mdg_grid_to_eqn(c, s->Rm.p, s->Rm.p + nu, s->Rm.p + nu + nv, s->Req.p)) ||
+            (rc = max_host(c, s, c->neq_cont, s->Req.p + c->neq_mom, &maxdiv)) ||
+            (rc = max_host(c, s, c->neq_mom, s->Req.p, &maxru)))
+            return rc;
+        if (k == 0) { maxdiv0 = maxdiv; maxru0 = maxru; }
+        double rr;
+        if ((rc = dot_host<double>(c, s, n, s->Rm.p, s->Rm.p, &rr))) return rc;
+        o->rel_res = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+        if (std::isnan(o->rel_res)) return mdg_fail(c, MDG_ERR_NAN, "NaN in the Stokes solve");
+        if (k > 1 && (std::fabs(maxdiv) < o->lin_abs_div || maxdiv / maxdiv0 < o->lin_rel_div) &&
+            (std::fabs(maxru) < o->lin_abs_mom || maxru / maxru0 < o->lin_rel_mom))
+            break;
+    }
+    o->its_PH = (k < o->max_its_PH) ? k + 1 : k;
+    o->its = o->its_PH;
+    o->max_div = maxdiv; o->max_div0 = maxdiv0; o->max_mom = maxru; o->max_mom0 = maxru0;
+    return MDG_OK;
+}
+
 int mdg_apply_Kg_grid(mdg_ctx *c, double gamma, const double *x, double *y) {
     const sgrid g = c->sg;
     const int n = nvx(g) + nvz(g) + ncell(g);
@@ -1838,7 +1993,25 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
         return mdg_fail(c, MDG_ERR_UNSUPPORTED, "the BFBt Schur approximation runs in FP64 only (precision = MDG_PREC_FP64)");
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int rc;
-    if (o->precision == MDG_PREC_MIXED) {
+    if (o->method == 1) {
+        if (o->precond != 0 || o->schur != 0)
+            return mdg_fail(c, MDG_ERR_UNSUPPORTED, "method 1 takes neither a learned part nor BFBt");
+        // the multigrid levels and the CG operator carry KillerSolver's
+        // constant penalty (the hierarchy keeps the flag; the context's grid
+        // returns to the viscosity-weighted form for the other entry points)
+        c->sg.pen_const = 1;
+        if (o->precision == MDG_PREC_MIXED) {
+            rc = build_hierarchy<float>(c, s, s->hf, o);
+            c->sg.pen_const = 0;
+            if (rc) return rc;
+            rc = ph_solve<float>(c, s, s->hf, o, b, x);
+        } else {
+            rc = build_hierarchy<double>(c, s, s->hd, o);
+            c->sg.pen_const = 0;
+            if (rc) return rc;
+            rc = ph_solve<double>(c, s, s->hd, o, b, x);
+        }
+    } else if (o->precision == MDG_PREC_MIXED) {
         if ((rc = build_hierarchy<float>(c, s, s->hf, o))) return rc;
         rc = fgcr_solve<float>(c, s, s->hf, o, b, x);
     } else {
@@ -1848,8 +2021,9 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     // The closed box leaves the pressure defined up to a constant.  The
     // reference's Powell-Hestenes updates p += penalty * (rhs_cont - C u)
     // only add zero-sum increments (Solvers.c:1301-1304), so its correction
-    // has zero mean; remove the mean here to select the same solution.
-    {
+    // has zero mean; remove the mean here to select the same solution
+    // (method 1 runs those updates themselves and keeps what they give).
+    if (o->method == 0) {
         const sgrid g = c->sg;
         const int np = ncell(g), nb = 256;
         double *xp = x + nvx(g) + nvz(g);
@@ -1881,7 +2055,6 @@ extern "C" int mdg_killer_solver(mdg_ctx *c, mdg_solver_opts *o, const double *D
     if (!c || !o || !rhs_mom || !rhs_cont || !sol) return mdg_fail(c, MDG_ERR_ARG, "null argument");
     if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
     if (o->method < 0 || o->method > 1) return mdg_fail(c, MDG_ERR_ARG, "method must be 0 or 1");
-    if (o->method == 1) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "method 1 is not built yet");
     int rc;
     if ((rc = mdg_stage_coeffs(c, D11_n, D22_n, D33_s)) || (rc = mdg_reserve_grid_vectors(c))) return rc;
     const sgrid g = c->sg;

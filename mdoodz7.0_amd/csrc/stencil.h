@@ -23,6 +23,9 @@ struct sgrid {
     // boundary kind of the tangential ghost rows / columns: 1 = free slip
     // (tag 13), 2 = no slip (tag 11); normal components are Dirichlet.
     int ghostS, ghostN, ghostW, ghostE;
+    // penalty weight of the grad-div term: 0 = gamma * eta_n (viscosity
+    // weighted), 1 = gamma alone (KillerSolver's constant penalty/dV)
+    int pen_const;
 };
 
 __host__ __device__ inline int nvx(const sgrid &g) { return g.Nx * (g.Nz + 1); }
@@ -76,8 +79,9 @@ __device__ inline T apply_vx_row(const sgrid &g, int k, int l, const T *__restri
         const int cW = (k - 1) + (l - 1) * Ncx, cE = k + (l - 1) * Ncx;
         const T divW = (u[c] - u[c - 1]) / dx + (v[k + l * NxV] - v[k + (l - 1) * NxV]) / dz;
         const T divE = (u[c + 1] - u[c]) / dx + (v[k + 1 + l * NxV] - v[k + 1 + (l - 1) * NxV]) / dz;
-        y += gamma * (en[cW] * divW - en[cE] * divE) / dx;
-        d += gamma * (en[cW] + en[cE]) * idx2;
+        const T wW = g.pen_const ? T(1) : en[cW], wE = g.pen_const ? T(1) : en[cE];
+        y += gamma * (wW * divW - wE * divE) / dx;
+        d += gamma * (wW + wE) * idx2;
     }
     if (diag_out) *diag_out = d * dV;
     return y * dV;
@@ -107,8 +111,9 @@ __device__ inline T apply_vz_row(const sgrid &g, int k, int l, const T *__restri
         const int cS = (k - 1) + (l - 1) * Ncx, cN = (k - 1) + l * Ncx;
         const T divS = (u[k + l * Nx] - u[k - 1 + l * Nx]) / dx + (v[c] - v[c - NxV]) / dz;
         const T divN = (u[k + (l + 1) * Nx] - u[k - 1 + (l + 1) * Nx]) / dx + (v[c + NxV] - v[c]) / dz;
-        y += gamma * (en[cS] * divS - en[cN] * divN) / dz;
-        d += gamma * (en[cS] + en[cN]) * idz2;
+        const T wS = g.pen_const ? T(1) : en[cS], wN = g.pen_const ? T(1) : en[cN];
+        y += gamma * (wS * divS - wN * divN) / dz;
+        d += gamma * (wS + wN) * idz2;
     }
     if (diag_out) *diag_out = d * dV;
     return y * dV;
@@ -148,8 +153,9 @@ __device__ inline T apply_vx_rcp(const sgrid &g, const stc<T> &cf, int k, int l,
     // gamma * (eta_W div_W - eta_E div_E) / dx
     const T divW = (uc - uw) * cf.idx + (vnw - vsw) * cf.idz;
     const T divE = (ue - uc) * cf.idx + (vne - vse) * cf.idz;
-    y += gamma * (enW * divW - enE * divE) * cf.idx;
-    *diag_out = (uC + gamma * (enW + enE) * cf.idx2) * cf.dV;
+    const T wW = g.pen_const ? T(1) : enW, wE = g.pen_const ? T(1) : enE;
+    y += gamma * (wW * divW - wE * divE) * cf.idx;
+    *diag_out = (uC + gamma * (wW + wE) * cf.idx2) * cf.dV;
     return y * cf.dV;
 }
 template <typename T>
@@ -172,8 +178,9 @@ __device__ inline T apply_vz_rcp(const sgrid &g, const stc<T> &cf, int k, int l,
     y += (D33W * (unw - usw) + D33E * (use - une)) * cf.idxz;
     const T divS = (use - usw) * cf.idx + (vc - vs) * cf.idz;
     const T divN = (une - unw) * cf.idx + (vn - vc) * cf.idz;
-    y += gamma * (enS * divS - enN * divN) * cf.idz;
-    *diag_out = (vC + gamma * (enS + enN) * cf.idz2) * cf.dV;
+    const T wS = g.pen_const ? T(1) : enS, wN = g.pen_const ? T(1) : enN;
+    y += gamma * (wS * divS - wN * divN) * cf.idz;
+    *diag_out = (vC + gamma * (wS + wN) * cf.idz2) * cf.dV;
     return y * cf.dV;
 }
 
@@ -185,8 +192,9 @@ __device__ inline T diag_vx(const sgrid &g, int k, int l, const T *__restrict__ 
     const T dx = T(g.dx), dz = T(g.dz);
     const T enW = en[(k - 1) + (l - 1) * Ncx], enE = en[k + (l - 1) * Ncx];
     const T fS = ghost_factor<T>(g.ghostS, l == 1), fN = ghost_factor<T>(g.ghostN, l == g.Nz - 1);
+    const T wW = g.pen_const ? T(1) : enW, wE = g.pen_const ? T(1) : enE;
     const T d = T(2) * (enE + enW) / (dx * dx) + (es[k + l * Nx] * fN + es[k + (l - 1) * Nx] * fS) / (dz * dz) +
-                gamma * (enW + enE) / (dx * dx);
+                gamma * (wW + wE) / (dx * dx);
     return d * T(g.dx * g.dz);
 }
 template <typename T>
@@ -196,8 +204,9 @@ __device__ inline T diag_vz(const sgrid &g, int k, int l, const T *__restrict__ 
     const T dx = T(g.dx), dz = T(g.dz);
     const T enS = en[(k - 1) + (l - 1) * Ncx], enN = en[(k - 1) + l * Ncx];
     const T fW = ghost_factor<T>(g.ghostW, k == 1), fE = ghost_factor<T>(g.ghostE, k == Nx - 1);
+    const T wS = g.pen_const ? T(1) : enS, wN = g.pen_const ? T(1) : enN;
     const T d = (es[k + l * Nx] * fE + es[(k - 1) + l * Nx] * fW) / (dx * dx) + T(2) * (enN + enS) / (dz * dz) +
-                gamma * (enS + enN) / (dz * dz);
+                gamma * (wS + wN) / (dz * dz);
     return d * T(g.dx * g.dz);
 }
 

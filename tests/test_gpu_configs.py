@@ -98,6 +98,7 @@ def test_inclusion_101_solve(oracle_mod, bc, precision):
         uu[eu >= 0] -= sol[eu[eu >= 0]]
         vv[ev >= 0] -= sol[ev[ev >= 0]]
         pp[ep >= 0] -= sol[ep[ep >= 0]]
+        O.apply_bc(g, uu, vv)          # ghost values from the new interior (ApplyBC)
         XU, ZU = np.meshgrid(g.xg, g.zvx)
         ax, _, _ = S.solvi_analytic(XU.ravel(), ZU.ravel())
         XV, ZV = np.meshgrid(g.xvz, g.zg)
@@ -140,6 +141,46 @@ def test_inclusion_101_incompressibility_check(oracle_mod):
     np.testing.assert_allclose(o.max_div, rp.max(), rtol=1e-6, atol=1e-12 * np.abs(F).max())
     assert o.max_mom0 == F[:nm].max() and o.max_div0 == F[nm:].max()
     assert abs(o.max_div) <= 10 * 1e-9 or o.max_div / o.max_div0 <= 1e-5
+
+
+@pytest.mark.parametrize("precision", [mdgpu.MDG_PREC_FP64, mdgpu.MDG_PREC_MIXED])
+def test_solcx_free_slip_and_body_force(oracle_mod, precision):
+    """SolCx (1e6 viscosity jump, density sin(pi z) cos(pi x), gz = 1) in a
+    closed free-slip box at 81 x 81: the GPU's body force, residual and
+    solve give the oracle's solution to 1e-8, and its errors against the
+    analytic free-slip solution stay inside the reference test's bounds
+    (TESTS/SolCxBenchmarkTests.cpp:216-233)."""
+    import json
+    import os
+    from oracle.driver import evaluate_residual, initial_fields, scatter_solution
+    from test_oracle_solcx import TIGHT, solcx_errors, solcx_problem
+    O = oracle_mod
+    n = 81
+    g, en, es, rho_n = solcx_problem(O, n)
+    ctx = mdgpu.Context(n, n, 0.0, 1.0, 0.0, 1.0)
+    ctx.set_bc(g.BCu_t, g.BCu_v, g.BCv_t, g.BCv_v, g.BCp_t, g.BCg_t)
+    rho_s = np.zeros(n * n)
+    rx, rz = O.eval_rhs(g, 0.0, 1.0, rho_n, rho_s)
+    grx, grz = ctx.eval_rhs(0.0, 1.0, rho_n, rho_s)
+    assert np.array_equal(grx, rx) and np.array_equal(grz, rz)
+    num = g.numbering()
+    nm = num[3]
+    u, v, p = initial_fields(g)
+    F, _ = evaluate_residual(g, num, u, v, p, en, es, rx, rz)
+    A, B, Cm = O.assemble_picard(g, num, en, es)
+    du, dp, st, rc = O.killer_solve(g, num, A, B, Cm, F[:nm], F[nm:], **TIGHT)
+    assert rc == 0
+    sol, o = ctx.killer_solver(2 * en, 2 * en, es, F[:nm], F[nm:], rel_tol=1e-12, precision=precision)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(sol[:nm], du) <= 1e-8
+    assert rel(sol[nm:] - sol[nm:].mean(), dp - dp.mean()) <= 1e-8
+    scatter_solution(num, sol, u, v, p)
+    O.apply_bc(g, u, v)
+    ex, ez, ep = solcx_errors(g, u, v, p)
+    ka = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "reference_known_answers.json")))["solcx"]
+    print("SolCx 81 on the GPU: %d outer its, L2 Vx %.3e Vz %.3e L1 P %.3e" % (o.its, ex, ez, ep))
+    assert ex < ka["L2_Vx_max"] and ez < ka["L2_Vz_max"] and ep < ka["L1_P_max"]
 
 
 # ------------------------------------------------------------ 2047^2 ellipses
@@ -209,3 +250,45 @@ def test_ellipses_2047_solves(oracle_mod, ellipses_2047):
     assert np.linalg.norm(a[:nm] - b[:nm]) <= 1e-6 * np.linalg.norm(a[:nm])
     pa, pb = a[nm:] - a[nm:].mean(), b[nm:] - b[nm:].mean()
     assert np.linalg.norm(pa - pb) <= 1e-6 * np.linalg.norm(pa)
+
+
+@pytest.mark.parametrize("n", [41, N1])
+def test_killer_iteration_counts_match_oracle(oracle_mod, n):
+    """KillerSolver's own iteration on the GPU (method 1: Powell-Hestenes
+    loop, GCR(6), signed-maximum stopping, Solvers.c:1054-1454) at the
+    reference's SolVi settings (SolViRes101.txt: penalty 1e5,
+    rel_tol_KSP 1e-4, lin_abs_div = lin_rel_div = 1e-8, defaults otherwise):
+    the same Powell-Hestenes and GCR iteration counts as the oracle, whose
+    preconditioner is an exact Cholesky factor of Jts, and the same
+    correction to 1e-8."""
+    O = oracle_mod
+    from oracle.driver import evaluate_residual, initial_fields
+    g = O.Grid(n, n, *BOX)
+    g.pure_shear(1.0)
+    x, z = S.put_markers(g.xg, g.zg, g.dx, g.dz, 4, 4, seed=101)
+    ph = ((x * x + z * z) < 0.2 ** 2).astype(np.int32)
+    ctx = mdgpu.Context(n, n, *BOX)
+    ctx.set_bc(g.BCu_t, g.BCu_v, g.BCv_t, g.BCv_v, g.BCp_t, g.BCg_t)
+    eta = np.array([1.0, 1e3])
+    en = O.p2mastah(g, x, z, ph, eta, flag=0, avg=0, centroid=1)
+    es = O.p2mastah(g, x, z, ph, eta, flag=0, avg=0, centroid=0)
+    num = g.numbering()
+    nm = num[3]
+    u, v, p = initial_fields(g)
+    rx, rz = O.eval_rhs(g, 0.0, 0.0, np.zeros(en.size), np.zeros(es.size))
+    F, _ = evaluate_residual(g, num, u, v, p, en, es, rx, rz)
+    A, B, Cm = O.assemble_picard(g, num, en, es)
+    kw = dict(penalty=1e5, max_its_PH=10, rel_tol_KSP=1e-4, max_its_KSP=100, lin_abs_div=1e-8,
+              lin_rel_div=1e-8, lin_abs_mom=1e-9, lin_rel_mom=1e-5)
+    du, dp, st, rc = O.killer_solve(g, num, A, B, Cm, F[:nm], F[nm:], **kw)
+    assert rc == 0
+    sol, o = ctx.killer_solver(2 * en, 2 * en, es, F[:nm], F[nm:], method=1, **kw)
+    print("n %d: oracle PH %d KSP %d maxdiv %.3e; gpu PH %d KSP %d maxdiv %.3e, %d CG its, %.1f ms"
+          % (n, st.its_PH, st.its_KSP_tot, st.maxdiv, o.its_PH, o.its_KSP, o.max_div, o.inner_its,
+             o.time_ms), flush=True)
+    assert (o.its_PH, o.its_KSP) == (st.its_PH, st.its_KSP_tot)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(sol[:nm], du) <= 1e-8
+    assert rel(sol[nm:], dp) <= 1e-8
+    np.testing.assert_allclose([o.max_div, o.max_div0], [st.maxdiv, st.maxdiv0], rtol=1e-6,
+                               atol=1e-12 * abs(st.maxdiv0))
