@@ -10,15 +10,9 @@
 //   D21 = 2 eta_xx Ezz, D22 = 2 eta + 2 eta_zz Ezz, D23 = 2 eta_g Ezz, D24 = 2 eta_p Ezz
 //   D31 = 2 eta_xx Exz, D32 = 2 eta_zz Exz, D33 = eta + 2 eta_g Exz, D34 = 2 eta_p Exz.
 //
-// Jacobian-vector product: J v is the linearised residual
-// (StokesAssemblyDecoupled.c:219-1085 assembles the same operator): the
-// strain rates of the correction (ghost rows and columns from the
-// homogeneous boundary relations), centre and vertex averages as in the
-// residual path (exz to the centres, exx/ezz/p to the vertices), tangent
-// stresses
-//   dsxx = D11 exx + D12 ezz + D13 gxz_c + D14 p  (centres), dszz likewise,
-//   dsxz = D31 exx_v + D32 ezz_v + D33 gxz + D34 p_v  (vertices),
-// and the momentum/continuity rows of the residual on those stresses.
+// Jacobian-vector product: the rows of the reference's assembled Newton
+// operator (BuildJacobianOperatorDecoupled, StokesAssemblyDecoupled.c:
+// 1786-2304), applied without assembling it (k_jac_rows below).
 #include "mdg_internal.h"
 #include "rheo_device.h"
 #include "stokes_internal.h"
@@ -122,107 +116,213 @@ k_tangent_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
     if (!ok) atomicOr(fail, 1);
 }
 
-// Correction velocities with ghost rows/columns from the homogeneous
-// boundary relations: free slip copies the interior value, no slip negates it;
-// Dirichlet walls stay zero.  x is the grid-layout vector [u | v | p].
-__global__ void k_hom_ghosts(sgrid g, const double *__restrict__ x, double *__restrict__ u,
-                             double *__restrict__ v) {
+// ---------------------------------------------------------------------------
+// The Jacobian rows of BuildJacobianOperatorDecoupled (Xjacobian /
+// Zjacobian_InnerNodesDecoupled3, StokesAssemblyDecoupled.c:219-1085), applied
+// matrix-free: one thread per row computes the row's coefficients from the
+// tangent D of the two centres and two vertices it touches, the node flags of
+// the tags and the vertex-averaging weights, and sums them against x.  The
+// coefficients are written in terms of the tangent stresses' responses to the
+// deviatoric normal rates exx_d = exx - co (exx + ezz)/3 (co = 1 in the
+// reference's assembly, cm = co - 3):
+//   centres  a = (cm D11 + co D12)/3, b = (co D11 + cm D12)/3 (x rows),
+//            a2 = (co D21 + cm D22)/3, b2 = (cm D21 + co D22)/3 (z rows);
+//   vertices s = (cm D31 + co D32)/3, t = (co D31 + cm D32)/3.
+// Non-free entries of x are zero, so a coefficient of a Dirichlet, ghost or
+// wall neighbour contributes nothing (AddCoeff3 sends them to the right-hand
+// side); a no-slip ghost's coefficient is first folded into its interior
+// neighbour's, as the reference does.
+struct jac_tags {
+    const signed char *u, *v, *p, *g;   // BCu, BCv, BCp, BCg types
+};
+
+__device__ inline double fl(bool c) { return c ? 1.0 : 0.0; }
+
+__device__ double jac_x_row(const sgrid &g, const jac_tags &t, const double *__restrict__ Dn,
+                            const double *__restrict__ Ds, double co, const double *__restrict__ u,
+                            const double *__restrict__ v, const double *__restrict__ p, int k, int l) {
+    const int nx = g.Nx, nz = g.Nz, nxvz = nx + 1, ncx = nx - 1, nzvx = nz + 1;
+    const size_t nc = (size_t)ncx * (nz - 1), nvt = (size_t)nx * nz;
+    const double dx = g.dx, dz = g.dz, dx2 = dx * dx, dz2 = dz * dz, dxz = dx * dz, cm = co - 3.0;
+    const int c1 = k + l * nx, c2 = (k - 1) + (l - 1) * ncx, c3 = k + l * nxvz;
+    const int S = c1 - nx, N = c1 + nx;                 // Vx south / north
+    const int vSWi = c3 - nxvz, vSEi = vSWi + 1, vNWi = c3, vNEi = c3 + 1;
+    const int pW = c2, pE = c2 + 1;
+    const int xyS = c1 - nx, xyN = c1;                  // vertices below / above
+    // flags
+    const double inW = fl(t.p[pW] == -1), inE = fl(t.p[pE] == -1);
+    const double inS = fl(t.u[S] != 30 && t.u[S] != 13), inN = fl(t.u[N] != 30 && t.u[N] != 13);
+    const double inSv = fl(t.g[xyS] != 30), inNv = fl(t.g[xyN] != 30);
+    const double cSW = (l > 1) ? fl(t.p[pW - ncx] == -1) : 0.0, cSE = (l > 1) ? fl(t.p[pE - ncx] == -1) : 0.0;
+    const double cNW = (l < nzvx - 2) ? fl(t.p[pW + ncx] == -1) : 0.0, cNE = (l < nzvx - 2) ? fl(t.p[pE + ncx] == -1) : 0.0;
+    const double vSWv = fl(t.g[xyS - 1] != 30), vNWv = fl(t.g[xyN - 1] != 30);
+    const double vSEv = fl(t.g[xyS + 1] != 30), vNEv = fl(t.g[xyN + 1] != 30);
+    const double fSWW = fl(t.v[vSWi - 1] == -1), fNWW = fl(t.v[vNWi - 1] == -1);
+    const double fSEE = fl(t.v[vSEi + 1] == -1), fNEE = fl(t.v[vNEi + 1] == -1);
+    double wS = inW + inE + cSW + cSE, wN = inW + inE + cNW + cNE;
+    if (wS > 1.0) wS = 1.0 / wS;
+    if (wN > 1.0) wN = 1.0 / wN;
+    // tangent responses
+    const double d11W = Dn[pW], d11E = Dn[pE], d12W = Dn[nc + pW], d12E = Dn[nc + pE];
+    const double d13W = Dn[2 * nc + pW], d13E = Dn[2 * nc + pE], d14W = Dn[3 * nc + pW], d14E = Dn[3 * nc + pE];
+    const double d31S = Ds[xyS], d31N = Ds[xyN], d32S = Ds[nvt + xyS], d32N = Ds[nvt + xyN];
+    const double d33S = Ds[2 * nvt + xyS], d33N = Ds[2 * nvt + xyN], d34S = Ds[3 * nvt + xyS], d34N = Ds[3 * nvt + xyN];
+    const double aW = (cm * d11W + co * d12W) / 3.0, aE = (cm * d11E + co * d12E) / 3.0;
+    const double bW = (co * d11W + cm * d12W) / 3.0, bE = (co * d11E + cm * d12E) / 3.0;
+    const double sS = (cm * d31S + co * d32S) / 3.0, sN = (cm * d31N + co * d32N) / 3.0;
+    const double tS = (co * d31S + cm * d32S) / 3.0, tN = (co * d31N + cm * d32N) / 3.0;
+    const double qS = inS * inS * inSv, qN = inN * inN * inNv;
+    const double sh = wS * sS - wN * sN;
+    // velocity coefficients
+    double cW = inW * aW / dx2 + inW * sh / dxz - 0.25 * d13W * (inN * vNWv - inS * vSWv) / dxz;
+    double cC = (d33N * qN + d33S * qS) / dz2 + (inE - inW) * sh / dxz - 0.25 * (d13W - d13E) * (qN - qS) / dxz -
+                (inE * aE + inW * aW) / dx2;
+    double cE = inE * aE / dx2 - inE * sh / dxz + 0.25 * d13E * (inN * vNEv - inS * vSEv) / dxz;
+    const double cS = -d33S * qS / dz2 + 0.25 * qS * (d13E - d13W) / dxz + wS * (cSE - cSW) * sS / dxz;
+    const double cN = -d33N * qN / dz2 + 0.25 * qN * (d13W - d13E) / dxz - wN * (cNE - cNW) * sN / dxz;
+    const double cSWu = (-0.25 * d13W * inS * vSWv + cSW * wS * sS) / dxz;
+    const double cSEu = (0.25 * d13E * inS * vSEv - cSE * wS * sS) / dxz;
+    const double cNWu = (0.25 * d13W * inN * vNWv - cNW * wN * sN) / dxz;
+    const double cNEu = (-0.25 * d13E * inN * vNEv + cNE * wN * sN) / dxz;
+    double gSW = -d33S * inS * inSv / dxz - (inW * wN * tN + (cSW - inW) * wS * tS) / dz2 + inW * bW / dxz +
+                 0.25 * (d13E * inS * inSv - d13W * (inS * inSv - fSWW * vSWv)) / dx2;
+    double gSE = d33S * inS * inSv / dxz + (-inE * wN * tN + (inE - cSE) * wS * tS) / dz2 - inE * bE / dxz -
+                 0.25 * (d13E * (inS * inSv - fSEE * vSEv) - d13W * inS * inSv) / dx2;
+    double gNW = d33N * inN * inNv / dxz - ((cNW - inW) * wN * tN + inW * wS * tS) / dz2 - inW * bW / dxz -
+                 0.25 * (-d13E * inN * inNv + d13W * (inN * inNv - fNWW * vNWv)) / dx2;
+    double gNE = -d33N * inN * inNv / dxz + ((inE - cNE) * wN * tN - inE * wS * tS) / dz2 + inE * bE / dxz +
+                 0.25 * (-d13E * (inN * inNv - fNEE * vNEv) + d13W * inN * inNv) / dx2;
+    const double gSWW = -0.25 * d13W * fSWW * vSWv / dx2, gSEE = -0.25 * d13E * fSEE * vSEv / dx2;
+    const double gNWW = -0.25 * d13W * fNWW * vNWv / dx2, gNEE = -0.25 * d13E * fNEE * vNEv / dx2;
+    const double gSSW = cSW * wS * tS / dz2, gSSE = cSE * wS * tS / dz2;
+    const double gNNW = cNW * wN * tN / dz2, gNNE = cNE * wN * tN / dz2;
+    const double psh = (d34S * inS * wS - d34N * inN * wN) / dz;
+    const double qW = -inW / dx + inW * (d14W / dx + psh), qE = inE / dx + inE * (-d14E / dx + psh);
+    // no-slip ghosts fold into the interior neighbour
+    if (t.u[S] == 11) cC -= cS;
+    if (t.u[N] == 11) cC -= cN;
+    if (t.u[S - 1] == 11) cW -= cSWu;
+    if (t.u[S + 1] == 11) cE -= cSEu;
+    if (t.u[N - 1] == 11) cW -= cNWu;
+    if (t.u[N + 1] == 11) cE -= cNEu;
+    if (t.v[vNWi - 1] == 11) gNW -= gNWW;
+    if (t.v[vNEi + 1] == 11) gNE -= gNEE;
+    if (t.v[vSWi - 1] == 11) gSW -= gSWW;
+    if (t.v[vSEi + 1] == 11) gSE -= gSEE;
+    double y = cS * u[S] + cW * u[c1 - 1] + cC * u[c1] + cE * u[c1 + 1] + cN * u[N];
+    y += cSWu * u[S - 1] + cSEu * u[S + 1] + cNWu * u[N - 1] + cNEu * u[N + 1];
+    y += gSW * v[vSWi] + gSE * v[vSEi] + gNW * v[vNWi] + gNE * v[vNEi];
+    if (k > 1 || t.v[vSWi - 1] == -1) y += gSWW * v[vSWi - 1];
+    if (k < nx - 2 || t.v[vSEi + 1] == -1) y += gSEE * v[vSEi + 1];
+    if (k > 1 || t.v[vNWi - 1] == -1) y += gNWW * v[vNWi - 1];
+    if (k < nx - 2 || t.v[vNEi + 1] == -1) y += gNEE * v[vNEi + 1];
+    if (l > 1) y += gSSW * v[vSWi - nxvz] + gSSE * v[vSEi - nxvz];
+    if (l < nz - 1) y += gNNW * v[vNWi + nxvz] + gNNE * v[vNEi + nxvz];
+    y += qW * p[pW] + qE * p[pE];
+    if (l > 1) y += d34S * inS * wS / dz * (cSW * p[pW - ncx] + cSE * p[pE - ncx]);
+    if (l < nz - 1) y -= d34N * inN * wN / dz * (cNW * p[pW + ncx] + cNE * p[pE + ncx]);
+    return y * dxz;
+}
+
+__device__ double jac_z_row(const sgrid &g, const jac_tags &t, const double *__restrict__ Dn,
+                            const double *__restrict__ Ds, double co, const double *__restrict__ u,
+                            const double *__restrict__ v, const double *__restrict__ p, int k, int l) {
+    const int nx = g.Nx, nz = g.Nz, nxvz = nx + 1, ncx = nx - 1;
+    const size_t nc = (size_t)ncx * (nz - 1), nvt = (size_t)nx * nz;
+    const double dx = g.dx, dz = g.dz, dx2 = dx * dx, dz2 = dz * dz, dxz = dx * dz, cm = co - 3.0;
+    const int c1 = k + l * nx, c2 = (k - 1) + (l - 1) * ncx, c3 = k + l * nxvz;
+    const int W = c3 - 1, E = c3 + 1, S = c3 - nxvz, N = c3 + nxvz;     // Vz neighbours
+    const int uSWi = c1 - 1, uSEi = c1, uNWi = c1 + nx - 1, uNEi = c1 + nx;
+    const int pS = c2, pN = c2 + ncx;
+    const int xyW = c1 - 1, xyE = c1;                                      // vertices left / right
+    const double inS = fl(t.p[pS] == -1), inN = fl(t.p[pN] == -1);
+    const double inW = fl(t.v[W] != 30 && t.v[W] != 13), inE = fl(t.v[E] != 30 && t.v[E] != 13);
+    const double inWv = fl(t.g[xyW] != 30), inEv = fl(t.g[xyE] != 30);
+    double cSW = 0, cNW = 0, cSE = 0, cNE = 0;
+    if (k > 1 || (k == 1 && t.v[W] == -1)) { cSW = fl(t.p[pS - 1] == -1); cNW = fl(t.p[pN - 1] == -1); }
+    if (k < nxvz - 2 || (k == nxvz - 2 && t.v[E] == -1)) { cSE = fl(t.p[pS + 1] == -1); cNE = fl(t.p[pN + 1] == -1); }
+    const double vSWv = (l > 0) ? fl(t.g[xyW - nx] != 30) : 0.0, vSEv = (l > 0) ? fl(t.g[xyE - nx] != 30) : 0.0;
+    const double vNWv = (l < nz - 1) ? fl(t.g[xyW + nx] != 30) : 0.0, vNEv = (l < nz - 1) ? fl(t.g[xyE + nx] != 30) : 0.0;
+    const double fSSW = fl(t.u[uSWi - nx] == -1), fSSE = fl(t.u[uSEi - nx] == -1);
+    const double fNNW = fl(t.u[uNWi + nx] == -1), fNNE = fl(t.u[uNEi + nx] == -1);
+    double wE = inN + inS + cNE + cSE, wW = inN + inS + cNW + cSW;
+    if (wW > 1.0) wW = 1.0 / wW;
+    if (wE > 1.0) wE = 1.0 / wE;
+    const double d21S = Dn[4 * nc + pS], d21N = Dn[4 * nc + pN], d22S = Dn[5 * nc + pS], d22N = Dn[5 * nc + pN];
+    const double d23S = Dn[6 * nc + pS], d23N = Dn[6 * nc + pN], d24S = Dn[7 * nc + pS], d24N = Dn[7 * nc + pN];
+    const double d31W = Ds[xyW], d31E = Ds[xyE], d32W = Ds[nvt + xyW], d32E = Ds[nvt + xyE];
+    const double d33W = Ds[2 * nvt + xyW], d33E = Ds[2 * nvt + xyE], d34W = Ds[3 * nvt + xyW], d34E = Ds[3 * nvt + xyE];
+    const double a2S = (co * d21S + cm * d22S) / 3.0, a2N = (co * d21N + cm * d22N) / 3.0;
+    const double b2S = (cm * d21S + co * d22S) / 3.0, b2N = (cm * d21N + co * d22N) / 3.0;
+    const double sW = (cm * d31W + co * d32W) / 3.0, sE = (cm * d31E + co * d32E) / 3.0;
+    const double tW = (co * d31W + cm * d32W) / 3.0, tE = (co * d31E + cm * d32E) / 3.0;
+    const double qW = inW * inW * inWv, qE = inE * inE * inEv;
+    const double sh = wW * tW - wE * tE;
+    double gW = -d33W * qW / dx2 + 0.25 * qW * (d23N * inN - d23S * inS) / dxz + wW * (cNW - cSW) * tW / dxz;
+    double gC = -(inN * a2N + inS * a2S) / dz2 - 0.25 * (d23S * inS - d23N * inN) * (qE - qW) / dxz +
+                (inN - inS) * sh / dxz + (d33E * qE + d33W * qW) / dx2;
+    double gE = -d33E * qE / dx2 + 0.25 * qE * (d23S * inS - d23N * inN) / dxz - wE * (cNE - cSE) * tE / dxz;
+    const double gS = inS * (-0.25 * d23S * (inE * vSEv - vSWv * inW) / dxz + a2S / dz2 + sh / dxz);
+    const double gN = inN * (0.25 * d23N * (inE * vNEv - vNWv * inW) / dxz + a2N / dz2 - sh / dxz);
+    double hSW = 0.25 * (d23N * inN * inW * inWv + d23S * inS * (fSSW * vSWv - inW * inWv)) / dz2 + inS * b2S / dxz -
+                 d33W * inW * inWv / dxz - (inS * wE * sE - (inS - cSW) * wW * sW) / dx2;
+    double hSE = -0.25 * (-d23N * inE * inEv * inN + d23S * inS * (inE * inEv - vSEv * fSSE)) / dz2 - inS * b2S / dxz +
+                 d33E * inE * inEv / dxz + ((inS - cSE) * wE * sE - inS * wW * sW) / dx2;
+    double hNW = 0.25 * (d23N * inN * (fNNW * vNWv - inW * inWv) + d23S * inS * inW * inWv) / dz2 - inN * b2N / dxz +
+                 d33W * inW * inWv / dxz + (-inN * wE * sE + (inN - cNW) * wW * sW) / dx2;
+    double hNE = 0.25 * (-d23N * inN * (inE * inEv - vNEv * fNNE) + d23S * inE * inEv * inS) / dz2 + inN * b2N / dxz -
+                 d33E * inE * inEv / dxz + ((inN - cNE) * wE * sE - inN * wW * sW) / dx2;
+    const double gSWv = (-0.25 * d23S * inS * vSWv * inW + cSW * wW * tW) / dxz;
+    const double gSEv = (0.25 * d23S * inE * inS * vSEv - cSE * wE * tE) / dxz;
+    const double gNWv = (0.25 * d23N * inN * vNWv * inW - cNW * wW * tW) / dxz;
+    const double gNEv = (-0.25 * d23N * inE * inN * vNEv + cNE * wE * tE) / dxz;
+    const double hSWW = cSW * wW * sW / dx2, hSEE = cSE * wE * sE / dx2;
+    const double hNWW = cNW * wW * sW / dx2, hNEE = cNE * wE * sE / dx2;
+    const double hSSW = -0.25 * d23S * inS * fSSW * vSWv / dz2, hSSE = -0.25 * d23S * inS * vSEv * fSSE / dz2;
+    const double hNNW = -0.25 * d23N * inN * fNNW * vNWv / dz2, hNNE = -0.25 * d23N * inN * vNEv * fNNE / dz2;
+    const double psh = (d34W * inW * wW - d34E * inE * wE) / dx;
+    const double qS = -inS / dz + inS * (d24S / dz + psh), qN = inN / dz + inN * (-d24N / dz + psh);
+    if (t.v[W] == 11) gC -= gW;
+    if (t.v[E] == 11) gC -= gE;
+    if (t.v[S - 1] == 11) gW -= gSWv;
+    if (t.v[S + 1] == 11) gE -= gSEv;
+    if (t.v[N - 1] == 11) gW -= gNWv;
+    if (t.v[N + 1] == 11) gE -= gNEv;
+    if (t.u[uNWi + nx] == 11) hNW -= hNNW;
+    if (t.u[uNEi + nx] == 11) hNE -= hNNE;
+    if (t.u[uSWi - nx] == 11) hSW -= hSSW;
+    if (t.u[uSEi - nx] == 11) hSE -= hSSE;
+    double y = gS * v[S] + gW * v[W] + gC * v[c3] + gE * v[E] + gN * v[N];
+    y += gSWv * v[S - 1] + gSEv * v[S + 1] + gNWv * v[N - 1] + gNEv * v[N + 1];
+    y += hSW * u[uSWi] + hSE * u[uSEi] + hNW * u[uNWi] + hNE * u[uNEi];
+    y += hSSW * u[uSWi - nx] + hSSE * u[uSEi - nx] + hNNW * u[uNWi + nx] + hNNE * u[uNEi + nx];
+    if (k > 1 || t.u[uSWi - 1] == -1) y += hSWW * u[uSWi - 1];
+    if (k < nx - 1 || t.u[uSEi + 1] == -1) y += hSEE * u[uSEi + 1];
+    if (k > 1 || t.u[uNWi - 1] == -1) y += hNWW * u[uNWi - 1];
+    if (k < nx - 1 || t.u[uNEi + 1] == -1) y += hNEE * u[uNEi + 1];
+    y += qS * p[pS] + qN * p[pN];
+    if (k > 1) y += d34W * inW * wW / dx * (cSW * p[pS - 1] + cNW * p[pN - 1]);
+    if (k < nx - 1) y -= d34E * inE * wE / dx * (cSE * p[pS + 1] + cNE * p[pN + 1]);
+    return y * dxz;
+}
+
+// y = J_gamma x (grid layout): the Jacobian rows above on free velocity
+// nodes (plus gamma * C'diag(eta_n/dV)C when gamma != 0), C x on the cells.
+__global__ void __launch_bounds__(256)
+k_jac_rows(sgrid g, jac_tags t, const double *__restrict__ Dn, const double *__restrict__ Ds, double co, double gamma,
+           const double *__restrict__ x, const double *__restrict__ en, double *__restrict__ y) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int Nx = g.Nx, Nz = g.Nz, nu = nvx(g), nv = nvz(g);
-    if (i < nu) {
-        const int k = i % Nx, l = i / Nx;
-        double val = vx_free(g, k, l) ? x[i] : 0.0;
-        if ((l == 0 || l == Nz) && k > 0 && k < Nx - 1) {
-            const int ln = (l == 0) ? 1 : Nz - 1;
-            const double in = x[k + ln * Nx];
-            const int kind = (l == 0) ? g.ghostS : g.ghostN;
-            val = (kind == 2) ? -in : in;
-        }
-        u[i] = val;
-    } else if (i < nu + nv) {
-        const int j = i - nu, k = j % (Nx + 1), l = j / (Nx + 1);
-        double val = vz_free(g, k, l) ? x[nu + j] : 0.0;
-        if ((k == 0 || k == Nx) && l > 0 && l < Nz - 1) {
-            const int kn = (k == 0) ? 1 : Nx - 1;
-            const double in = x[nu + kn + l * (Nx + 1)];
-            const int kind = (k == 0) ? g.ghostW : g.ghostE;
-            val = (kind == 2) ? -in : in;
-        }
-        v[j] = val;
-    }
-}
-
-// Strain rates of the correction: div, exx on centres, gxz = 2 exz on vertices.
-__global__ void k_jv_strain(sgrid g, const double *__restrict__ u, const double *__restrict__ v,
-                            double *__restrict__ exx, double *__restrict__ ezz, double *__restrict__ gxz) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int Nx = g.Nx, Ncx = Nx - 1, nc = ncell(g), nvt = g.Nx * g.Nz;
-    if (i < nc) {
-        const int k = i % Ncx, l = i / Ncx;
-        const int c1 = k + l * Nx, c2 = k + l * (Nx + 1);
-        exx[i] = (u[c1 + 1 + Nx] - u[c1 + Nx]) / g.dx;
-        ezz[i] = (v[c2 + Nx + 1 + 1] - v[c2 + 1]) / g.dz;
-        return;
-    }
-    i -= nc;
-    if (i < nvt) {
-        const int k = i % Nx, l = i / Nx;
-        const int c1 = k + l * Nx, c2 = k + l * (Nx + 1);
-        gxz[i] = 2.0 * (0.5 * ((u[c1 + Nx] - u[c1]) / g.dz + (v[c2 + 1] - v[c2]) / g.dx));
-    }
-}
-
-// Tangent stresses: centres from the centre rates and the 4-vertex mean of
-// gxz; vertices from the mean of the surrounding centres (edge copy as in
-// InterpCentroidsToVerticesDouble for a box without air) and gxz.
-__global__ void k_jv_stress(sgrid g, const double *__restrict__ exx, const double *__restrict__ ezz,
-                            const double *__restrict__ gxz, const double *__restrict__ p,
-                            const double *__restrict__ Dn, const double *__restrict__ Ds,
-                            double *__restrict__ sxx, double *__restrict__ szz, double *__restrict__ sxz) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int Nx = g.Nx, Nz = g.Nz, Ncx = Nx - 1, Ncz = Nz - 1, nc = ncell(g), nvt = Nx * Nz;
-    if (i < nc) {
-        const int k = i % Ncx, l = i / Ncx, v0 = k + l * Nx;
-        const double gc = 0.25 * (gxz[v0] + gxz[v0 + 1] + gxz[v0 + Nx] + gxz[v0 + Nx + 1]);
-        const double a = exx[i], b = ezz[i], pp = p[i];
-        sxx[i] = Dn[0 * (size_t)nc + i] * a + Dn[1 * (size_t)nc + i] * b + Dn[2 * (size_t)nc + i] * gc + Dn[3 * (size_t)nc + i] * pp;
-        szz[i] = Dn[4 * (size_t)nc + i] * a + Dn[5 * (size_t)nc + i] * b + Dn[6 * (size_t)nc + i] * gc + Dn[7 * (size_t)nc + i] * pp;
-        return;
-    }
-    i -= nc;
-    if (i < nvt) {
-        const int k = i % Nx, l = i / Nx;
-        double ax = 0.0, az = 0.0, ap = 0.0;
-        for (int dl = -1; dl <= 0; dl++) {
-            const int lc = min(max(l + dl, 0), Ncz - 1);
-            for (int dk = -1; dk <= 0; dk++) {
-                const int kc = min(max(k + dk, 0), Ncx - 1), c = kc + lc * Ncx;
-                ax += exx[c]; az += ezz[c]; ap += p[c];
-            }
-        }
-        ax *= 0.25; az *= 0.25; ap *= 0.25;
-        sxz[i] = Ds[0 * (size_t)nvt + i] * ax + Ds[1 * (size_t)nvt + i] * az + Ds[2 * (size_t)nvt + i] * gxz[i] +
-                 Ds[3 * (size_t)nvt + i] * ap;
-    }
-}
-
-// y = J_gamma x (grid layout): momentum rows -dV (d sxx/dx - dp/dx + d sxz/dz)
-// plus the viscosity-weighted penalty term, continuity rows dV div.
-__global__ void k_jv_rows(sgrid g, double gamma, const double *__restrict__ u, const double *__restrict__ v,
-                          const double *__restrict__ p, const double *__restrict__ sxx,
-                          const double *__restrict__ szz, const double *__restrict__ sxz,
-                          const double *__restrict__ en, double *__restrict__ y) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int Nx = g.Nx, Ncx = Nx - 1, nu = nvx(g), nv = nvz(g), nc = ncell(g), NxV = Nx + 1;
+    const int Nx = g.Nx, Nz = g.Nz, Ncx = Nx - 1, nu = nvx(g), nv = nvz(g), nc = ncell(g), NxV = Nx + 1;
+    const double *u = x, *v = x + nu, *p = x + nu + nv;
     const double dV = g.dx * g.dz;
     if (i < nu) {
         const int k = i % Nx, l = i / Nx;
+        const signed char tt = t.u[i];
         double f = 0.0;
-        if (vx_free(g, k, l)) {
-            const int cW = k - 1 + (l - 1) * Ncx, cE = cW + 1;
-            f = (sxx[cE] - sxx[cW]) / g.dx - (p[cE] - p[cW]) / g.dx + (sxz[i] - sxz[i - Nx]) / g.dz;
-            f *= -dV;
+        if ((tt == -1 || tt == 2) && l > 0 && l < Nz && k > 0 && k < Nx - 1) {
+            f = jac_x_row(g, t, Dn, Ds, co, u, v, p, k, l);
             if (gamma != 0.0) {
+                const int cW = k - 1 + (l - 1) * Ncx, cE = cW + 1;
                 const double divW = (u[i] - u[i - 1]) / g.dx + (v[k + l * NxV] - v[k + (l - 1) * NxV]) / g.dz;
                 const double divE = (u[i + 1] - u[i]) / g.dx + (v[k + 1 + l * NxV] - v[k + 1 + (l - 1) * NxV]) / g.dz;
                 f += gamma * (en[cW] * divW - en[cE] * divE) / g.dx * dV;
@@ -231,12 +331,12 @@ __global__ void k_jv_rows(sgrid g, double gamma, const double *__restrict__ u, c
         y[i] = f;
     } else if (i < nu + nv) {
         const int j = i - nu, k = j % NxV, l = j / NxV;
+        const signed char tt = t.v[j];
         double f = 0.0;
-        if (vz_free(g, k, l)) {
-            const int cS = k - 1 + (l - 1) * Ncx, cN = cS + Ncx, c1 = k + l * Nx;
-            f = (szz[cN] - szz[cS]) / g.dz - (p[cN] - p[cS]) / g.dz + (sxz[c1] - sxz[c1 - 1]) / g.dx;
-            f *= -dV;
+        if ((tt == -1 || tt == 2) && k > 0 && k < NxV - 1 && l > 0 && l < Nz - 1) {
+            f = jac_z_row(g, t, Dn, Ds, co, u, v, p, k, l);
             if (gamma != 0.0) {
+                const int cS = k - 1 + (l - 1) * Ncx, cN = cS + Ncx;
                 const double divS = (u[k + l * Nx] - u[k - 1 + l * Nx]) / g.dx + (v[j] - v[j - NxV]) / g.dz;
                 const double divN = (u[k + (l + 1) * Nx] - u[k - 1 + (l + 1) * Nx]) / g.dx + (v[j + NxV] - v[j]) / g.dz;
                 f += gamma * (en[cS] * divS - en[cN] * divN) / g.dz * dV;
@@ -255,7 +355,6 @@ __global__ void k_jv_rows(sgrid g, double gamma, const double *__restrict__ u, c
 // Device state of the Newton operator (coefficients and work arrays).
 struct newton_state {
     dbuf<double> Dn, Ds;                 // 8 x cells, 4 x vertices
-    dbuf<double> u, v, exx, ezz, gxz, sxx, szz, sxz;
     dbuf<mdg_phase_prop> ph;
     bool active = false;
 };
@@ -268,9 +367,7 @@ newton_state *newton_of(mdg_ctx *c) {
 void mdg_newton_release(mdg_ctx *c) {
     newton_state *s = c->newton;
     if (!s) return;
-    s->Dn.release(); s->Ds.release(); s->u.release(); s->v.release(); s->exx.release();
-    s->ezz.release(); s->gxz.release(); s->sxx.release(); s->szz.release(); s->sxz.release();
-    s->ph.release();
+    s->Dn.release(); s->Ds.release(); s->ph.release();
     delete s;
     c->newton = nullptr;
 }
@@ -280,16 +377,12 @@ void mdg_newton_release(mdg_ctx *c) {
 int mdg_newton_apply_grid(mdg_ctx *c, double gamma, const double *x, double *y) {
     newton_state *s = c->newton;
     const sgrid g = c->sg;
-    const int nu = nvx(g), nv = nvz(g), nc = ncell(g), nvt = g.Nx * g.Nz;
-    hipLaunchKernelGGL(k_hom_ghosts, dim3(mdg_grid1d(nu + nv, 256)), dim3(256), 0, c->stream, g, x, s->u.p, s->v.p);
-    hipLaunchKernelGGL(k_jv_strain, dim3(mdg_grid1d(nc + nvt, 256)), dim3(256), 0, c->stream, g, (const double *)s->u.p,
-                       (const double *)s->v.p, s->exx.p, s->ezz.p, s->gxz.p);
-    hipLaunchKernelGGL(k_jv_stress, dim3(mdg_grid1d(nc + nvt, 256)), dim3(256), 0, c->stream, g, (const double *)s->exx.p,
-                       (const double *)s->ezz.p, (const double *)s->gxz.p, x + nu + nv, (const double *)s->Dn.p,
-                       (const double *)s->Ds.p, s->sxx.p, s->szz.p, s->sxz.p);
-    hipLaunchKernelGGL(k_jv_rows, dim3(mdg_grid1d(nu + nv + nc, 256)), dim3(256), 0, c->stream, g, gamma,
-                       (const double *)s->u.p, (const double *)s->v.p, x + nu + nv, (const double *)s->sxx.p,
-                       (const double *)s->szz.p, (const double *)s->sxz.p, (const double *)c->eta_n.p, y);
+    const int n = nvx(g) + nvz(g) + ncell(g);
+    const jac_tags t{c->BCu_t.p, c->BCv_t.p, c->BCp_t.p, c->BCg_t.p};
+    // comp = 1: BuildJacobianOperatorDecoupled assembles with the deviatoric
+    // normal rates whatever model.compressible is (StokesAssemblyDecoupled.c:1803)
+    hipLaunchKernelGGL(k_jac_rows, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, g, t, (const double *)s->Dn.p,
+                       (const double *)s->Ds.p, 1.0, gamma, x, (const double *)c->eta_n.p, y);
     HIPCHK(c, hipGetLastError());
     return MDG_OK;
 }
@@ -360,12 +453,9 @@ extern "C" int mdg_tangent_grid(mdg_ctx *c, const mdg_phase_prop *phases, int np
 static int newton_set(mdg_ctx *c, const double *Dn, const double *Ds) {
     newton_state *s = newton_of(c);
     const sgrid g = c->sg;
-    const size_t nc = ncell(g), nvt = (size_t)g.Nx * g.Nz, nu = nvx(g), nv = nvz(g);
+    const size_t nc = ncell(g), nvt = (size_t)g.Nx * g.Nz;
     HIPCHK(c, s->Dn.reserve(8 * nc));
     HIPCHK(c, s->Ds.reserve(4 * nvt));
-    HIPCHK(c, s->u.reserve(nu)); HIPCHK(c, s->v.reserve(nv));
-    HIPCHK(c, s->exx.reserve(nc)); HIPCHK(c, s->ezz.reserve(nc)); HIPCHK(c, s->gxz.reserve(nvt));
-    HIPCHK(c, s->sxx.reserve(nc)); HIPCHK(c, s->szz.reserve(nc)); HIPCHK(c, s->sxz.reserve(nvt));
     HIPCHK(c, hipMemcpyAsync(s->Dn.p, Dn, 8 * nc * 8, hipMemcpyDefault, c->stream));
     HIPCHK(c, hipMemcpyAsync(s->Ds.p, Ds, 4 * nvt * 8, hipMemcpyDefault, c->stream));
     return MDG_OK;

@@ -114,6 +114,9 @@ def lib():
         L.orc_assemble_picard.argtypes = [gp, i32p, i32p, i32p, C.c_int, C.c_int, f64p, f64p,
                                           f64p, i32p, i32p, f64p, i32p, i32p, f64p, i32p,
                                           i32p, f64p]
+        L.orc_assemble_jacobian.argtypes = [gp, i32p, i32p, i32p, C.c_int, C.c_int, f64p, f64p,
+                                            C.c_double, i32p, i32p, f64p, i32p, i32p, f64p]
+        L.orc_assemble_jacobian.restype = None
         L.orc_killer_solve.argtypes = [C.c_int, C.c_int, i32p, i32p, f64p, i32p, i32p, f64p,
                                        i32p, i32p, f64p, f64p, f64p, f64p, f64p,
                                        C.POINTER(KillerOpts)]
@@ -329,6 +332,24 @@ def assemble_picard(g, num, eta_n, eta_s):
     B = sp.csr_matrix((Bx[:Bp[-1]], Bj[:Bp[-1]], Bp), shape=(nm, nc))
     Cm = sp.csr_matrix((Cx[:Cp[-1]], Cj[:Cp[-1]], Cp), shape=(nc, nm))
     return A, B, Cm
+
+
+def assemble_jacobian(g, num, Dn, Ds, co=1.0):
+    """StokesAssemblyDecoupled.c:219-1085,1786-2304: the Newton momentum
+    blocks (AJ, BJ) as scipy CSR; Dn (8, cells), Ds (4, vertices); co =
+    comp*out_of_plane (BuildJacobianOperatorDecoupled uses comp = 1)."""
+    eu, ev, ep, nm, nc, neq = num
+    Dn = np.ascontiguousarray(Dn, np.float64).reshape(-1)
+    Ds = np.ascontiguousarray(Ds, np.float64).reshape(-1)
+    Ap = np.zeros(nm + 1, np.int32); Aj = np.zeros(21 * nm + 1, np.int32); Ax = np.zeros(21 * nm + 1)
+    Bp = np.zeros(nm + 1, np.int32); Bj = np.zeros(6 * nm + 1, np.int32); Bx = np.zeros(6 * nm + 1)
+    lib().orc_assemble_jacobian(g.c, eu, ev, ep, nm, nc, Dn, Ds, float(co), Ap, Aj, Ax, Bp, Bj, Bx)
+    import scipy.sparse as sp
+    AJ = sp.csr_matrix((Ax[:Ap[-1]], Aj[:Ap[-1]], Ap), shape=(nm, nm))
+    BJ = sp.csr_matrix((Bx[:Bp[-1]], Bj[:Bp[-1]], Bp), shape=(nm, nc))
+    AJ.sum_duplicates()
+    BJ.sum_duplicates()
+    return AJ, BJ
 
 
 def killer_solve(g, num, A, B, Cm, bu, bp, penalty=1e5, max_its_PH=10, rel_tol_KSP=1e-4,

@@ -56,6 +56,18 @@ void orc_p2mastah(const orc_grid *g, int n, const double *x, const double *z, co
                   double *node_field, int nb_phases, double *phase_perc, int mode,
                   int nthreads);
 
+/* StokesAssemblyDecoupled.c:219-556 (Xjacobian_InnerNodesDecoupled3),
+ * :732-1085 (Zjacobian_InnerNodesDecoupled3), :1786-2304 (row loops): the
+ * Newton momentum blocks AJ (neq_mom x neq_mom) and BJ (neq_mom x neq_cont)
+ * as CSR, for inner nodes of a box without periodicity or stress walls.
+ * Dn = D11,D12,D13,D14,D21,D22,D23,D24 on the centres (8 x cells), Ds =
+ * D31..D34 on the vertices (4 x vertices); co = comp*out_of_plane (the
+ * reference fixes comp = 1 in this assembly).  Capacity: 21 entries per
+ * row in AJ, 6 in BJ. */
+void orc_assemble_jacobian(const orc_grid *g, const int *eqn_u, const int *eqn_v, const int *eqn_p,
+                           int neq_mom, int neq_cont, const double *Dn, const double *Ds, double co,
+                           int *Ap, int *Aj, double *Ax, int *Bp, int *Bj, double *Bx);
+
 /* GridRoutines.c:108-212 */
 void orc_c2v(const orc_grid *g, int periodic_x, const double *cent, double *vert);
 

@@ -150,23 +150,3 @@ def picard_loop(g, st, md, phases, nit_max, tol, average=0, diag_scaling=1, gx=0
             break
     st["eta_n"], st["eta_s"] = cen["eta"], ver["eta"]
     return hist, nsolve
-
-
-def residual_vector(g, num, st, md, phases, average, rx, rz):
-    """Unscaled residual F (equation numbering) of the state st."""
-    sr, cen, ver = update_nonlinearity(g, st, md, phases, average)
-    F, _ = residual(g, num, cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"], sr["div"], rx, rz)
-    return F
-
-
-def jacobian_fd_apply(g, num, st, md, phases, average, rx, rz, x, eps):
-    """(F(s + eps x) - F(s - eps x)) / (2 eps): the directional derivative of
-    the discrete residual, the operator the Newton Jacobian linearises."""
-    out = []
-    for sgn in (1.0, -1.0):
-        s2 = dict(st)
-        s2["u"], s2["v"], s2["p"] = st["u"].copy(), st["v"].copy(), st["p"].copy()
-        scatter_solution(num, x, s2["u"], s2["v"], s2["p"], alpha=sgn * eps)
-        apply_bc(g, s2["u"], s2["v"])
-        out.append(residual_vector(g, num, s2, md, phases, average, rx, rz))
-    return (out[0] - out[1]) / (2.0 * eps)
