@@ -278,8 +278,10 @@ typedef struct {
                               2 harmonic, 3 maximum (0)                        */
     int    check_every;    /* inner CG convergence test period (1)             */
     int    schur;          /* Schur complement approximation: 0 viscosity-
-                              weighted mass, 1 scaled BFBt (0; BFBt needs
-                              precision = MDG_PREC_FP64)                       */
+                              weighted mass, 1 scaled BFBt (needs precision =
+                              MDG_PREC_FP64), 2 and 3 the mass weighted by
+                              the largest viscosity over the 3x3 and 5x5 cells
+                              around each cell (0)                              */
     double poisson_tol;    /* BFBt pressure-Poisson CG tolerance (1e-3)        */
     int    poisson_max_its;/* (30)                                             */
     int    smoother;       /* 0 fused three-term Chebyshev kernels, 1 the

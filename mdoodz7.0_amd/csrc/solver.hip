@@ -674,6 +674,54 @@ __global__ void k_schur_rhs(sgrid g, double gamma, const double *__restrict__ r,
     }
 }
 
+// Viscosity seen by the pressure of a cell for the Schur approximation: the
+// largest eta_n over the (2 r + 1)^2 cells around it (clamped at the walls).
+// Where a weak cell sits in strong material, its pressure responds like the
+// surrounding strong material, which the cell's own eta misses by the contrast.
+__global__ void k_eta_maxfilter(sgrid g, int r, const double *__restrict__ en, double *__restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncx = g.Nx - 1, ncz = g.Nz - 1;
+    if (j >= ncx * ncz) return;
+    const int k = j % ncx, l = j / ncx;
+    double m = en[j];
+    for (int dl = -r; dl <= r; dl++) {
+        const int ll = l + dl;
+        if (ll < 0 || ll >= ncz) continue;
+        for (int dk = -r; dk <= r; dk++) {
+            const int kk = k + dk;
+            if (kk < 0 || kk >= ncx) continue;
+            m = fmax(m, en[kk + ll * ncx]);
+        }
+    }
+    out[j] = m;
+}
+
+// As k_schur_rhs with S_g^-1 ~ -(eta_f + gamma eta_n)/dV: the exact inverse
+// Schur complement of the augmented system is S_0^-1 + gamma eta_n/dV, and
+// S_0^-1 is approximated by eta_f/dV with eta_f the filtered viscosity.
+template <typename T>
+__global__ void k_schur_rhs_f(sgrid g, double gamma, const double *__restrict__ r, const double *__restrict__ en,
+                              const double *__restrict__ ef, double *__restrict__ zout, T *__restrict__ ru) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu = nvx(g), nv = nvz(g), np = ncell(g), Ncx = g.Nx - 1;
+    const double idV = 1.0 / (g.dx * g.dz);
+    const double *rp = r + nu + nv;
+    auto zp = [&](int c) { return -(ef[c] + gamma * en[c]) * idV * rp[c]; };
+    if (i < nu) {
+        const int k = i % g.Nx, l = i / g.Nx;
+        double v = 0.0;
+        if (vx_free(g, k, l)) v = r[i] - (zp(k + (l - 1) * Ncx) - zp(k - 1 + (l - 1) * Ncx)) * g.dz;
+        ru[i] = (T)v;
+    } else if (i < nu + nv) {
+        const int j = i - nu, k = j % (g.Nx + 1), l = j / (g.Nx + 1);
+        double v = 0.0;
+        if (vz_free(g, k, l)) v = r[i] - (zp(k - 1 + l * Ncx) - zp(k - 1 + (l - 1) * Ncx)) * g.dx;
+        ru[i] = (T)v;
+    } else if (i < nu + nv + np) {
+        zout[i] = zp(i - nu - nv);
+    }
+}
+
 // true residual of the original system from the augmented one:
 // r_u = r_g,u + B W r_p with W = gamma*eta_n/dV (grad of W r_p), r_p unchanged.
 __global__ void k_true_residual(sgrid g, double gamma, const double *__restrict__ rg, const double *__restrict__ en,
@@ -1000,7 +1048,8 @@ struct solver_state {
     dbuf<double> X, R, Rt, B, Zp, V, Z;   // outer FGCR (V, Z: restart slots)
     dbuf<double> Nu, Np, Ku, Kp;          // learned outer directions and K times them
     dbuf<double> Rm, Req;                 // residual of the returned solution (grid, equation numbering)
-    dbuf<double> ph_u, ph_f, ph_v, ph_s, ph_VV, ph_SS;   // KillerSolver iteration (method 1)
+    dbuf<double> etaf;                    // filtered viscosity of the Schur approximation (schur 2, 3)
+    dbuf<double> ph_u, ph_f, ph_v, ph_s, ph_VV, ph_SS, ph_w, ph_y;   // KillerSolver iteration (method 1)
     ktimer tm;                            // fine-level smoother steps of the last graphs = 2 solve
     dbuf<double> part, scal;
     int m = 0;
@@ -1016,8 +1065,9 @@ void mdg_solver_release(mdg_ctx *c) {
     s->X.release(); s->R.release(); s->Rt.release(); s->B.release(); s->Zp.release();
     s->V.release(); s->Z.release(); s->part.release(); s->scal.release();
     s->Nu.release(); s->Np.release(); s->Ku.release(); s->Kp.release();
-    s->Rm.release(); s->Req.release();
+    s->Rm.release(); s->Req.release(); s->etaf.release();
     s->ph_u.release(); s->ph_f.release(); s->ph_v.release(); s->ph_s.release(); s->ph_VV.release(); s->ph_SS.release();
+    s->ph_w.release(); s->ph_y.release();
     delete s;
     c->solver = nullptr;
 }
@@ -1627,6 +1677,9 @@ static int precondition(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_o
         int pits = 0;
         if ((rc = bfbt_rhs(c, s, H, o, r, z, &pits))) return rc;
         o->poisson_its += pits;
+    } else if (o->schur >= 2) {
+        hipLaunchKernelGGL(k_schur_rhs_f<double>, dim3(nblk(nfull)), dim3(256), 0, c->stream, g, o->penalty, r, c->eta_n.p,
+                           (const double *)s->etaf.p, z, H.cg_r.p);
     } else {
         hipLaunchKernelGGL(k_schur_rhs<double>, dim3(nblk(nfull)), dim3(256), 0, c->stream, g, o->penalty, r, c->eta_n.p, z,
                            H.cg_r.p);
@@ -1647,6 +1700,11 @@ static int fgcr_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opt
     HIPCHK(c, s->R.reserve(n)); HIPCHK(c, s->Rt.reserve(n)); HIPCHK(c, s->B.reserve(n));
     HIPCHK(c, s->V.reserve((size_t)m * n)); HIPCHK(c, s->Z.reserve((size_t)m * n));
     double *R = s->R.p, *Rt = s->Rt.p;
+    if (o->schur >= 2) {
+        HIPCHK(c, s->etaf.reserve(ncell(g)));
+        hipLaunchKernelGGL(k_eta_maxfilter, dim3(nblk(ncell(g))), dim3(256), 0, c->stream, g, o->schur - 1,
+                           (const double *)c->eta_n.p, s->etaf.p);
+    }
     hipLaunchKernelGGL(k_aug_rhs, dim3(nblk(n)), dim3(256), 0, c->stream, g, o->penalty, b, c->eta_n.p, R);
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, x, 0.0);
     double bb;
@@ -1779,21 +1837,32 @@ static int fgcr_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opt
 }
 
 // ------------------------------------------------ KillerSolver's own iteration
-// Powell-Hestenes right-hand side of the velocity solve (Solvers.c:1290-1294):
-// ud = b_u - B (pen b_p + dp), grid layout, x = [du | dp].
-__global__ void k_ph_udum(sgrid g, double pen, const double *__restrict__ b, const double *__restrict__ x,
-                          double *__restrict__ ud) {
+// Inputs of the operator applications of KillerSolver's iteration, built
+// on full grid-layout vectors [velocity | pressure] so that one operator
+// (Picard K or the Newton J) gives every product:
+//   Js s = (K [s | -pen C s])_u,            (A - B pen C) s on the velocity rows;
+//   B q  = (K [0 | q])_u with q = pen b_p + dp   (Solvers.c:1290-1294).
+__global__ void k_ph_js_in(sgrid g, double pen, const double *__restrict__ su, double *__restrict__ w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nu = nvx(g), nv = nvz(g), Ncx = g.Nx - 1;
-    const double *bp = b + nu + nv, *dp = x + nu + nv;
-    auto q = [&](int c) { return pen * bp[c] + dp[c]; };
-    if (i < nu) {
-        const int k = i % g.Nx, l = i / g.Nx;
-        ud[i] = vx_free(g, k, l) ? b[i] - (q(k + (l - 1) * Ncx) - q(k - 1 + (l - 1) * Ncx)) * g.dz : 0.0;
-    } else if (i < nu + nv) {
-        const int j = i - nu, k = j % (g.Nx + 1), l = j / (g.Nx + 1);
-        ud[i] = vz_free(g, k, l) ? b[i] - (q(k - 1 + l * Ncx) - q(k - 1 + (l - 1) * Ncx)) * g.dx : 0.0;
+    const int nu = nvx(g), nv = nvz(g);
+    if (i < nu + nv) {
+        w[i] = su[i];
+    } else if (i < nu + nv + ncell(g)) {
+        const int j = i - nu - nv, k = j % (g.Nx - 1), l = j / (g.Nx - 1);
+        w[i] = -pen * div_cell<double>(g, k, l, su, su + nu);
     }
+}
+__global__ void k_ph_q_in(sgrid g, double pen, const double *__restrict__ b, const double *__restrict__ x,
+                          double *__restrict__ w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nvel = nvx(g) + nvz(g);
+    if (i < nvel) w[i] = 0.0;
+    else if (i < nvel + ncell(g)) w[i] = pen * b[i] + x[i];
+}
+// out = a - b
+__global__ void k_diff(int n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] - b[i];
 }
 // Powell-Hestenes pressure update (Solvers.c:1301-1304): dp += pen (b_p - C du)
 __global__ void k_ph_pupdate(sgrid g, double pen, const double *__restrict__ b, double *__restrict__ x) {
@@ -1804,31 +1873,53 @@ __global__ void k_ph_pupdate(sgrid g, double pen, const double *__restrict__ b, 
     x[nu + nv + j] += pen * (b[nu + nv + j] - div_cell<double>(g, k, l, x, x + nu));
 }
 
-// kspgcr (Solvers.c:124-230) on Js = A + pen C'C (the velocity block with
-// KillerSolver's constant penalty, level 0 of H, pen_const set), b -> x in
-// place (warm start), preconditioned by the multigrid CG solve of Jts = Js
-// to o->pc_tol (in place of the Cholesky factor).  Keeps the reference's
+// kspgcr (Solvers.c:124-230) on Js = J_A - J_B pen C (the Picard blocks, or
+// the Newton blocks when staged), b -> x in place (warm start),
+// preconditioned by the multigrid CG solve of the Picard Jts = A - B pen C
+// (hierarchy H, built with the constant penalty) to o->pc_tol, in place of
+// the Cholesky factor of the same matrix.  Keeps the reference's
 // restart 6, its orthogonalisation that also subtracts the coefficient and
 // pair stored in slot i1 by the previous cycle (zero in the first), its
 // stopping test on ||f|| < eps ||f0|| and its iteration count.
+// y = K x (full grid vectors): the Picard operator, or the Newton operator
+// when mdg_newton_solver has staged one
+static int ph_apply_K(mdg_ctx *c, const double *x, double *y) {
+    if (mdg_newton_active(c)) return mdg_newton_apply_grid(c, 0.0, x, y);
+    const sgrid g = c->sg;
+    const int n = nvx(g) + nvz(g) + ncell(g);
+    hipLaunchKernelGGL(k_apply_Kg, dim3(nblk(n)), dim3(256), 0, c->stream, g, 0.0, x, (const double *)c->eta_n.p,
+                       (const double *)c->eta_s.p, y);
+    return MDG_OK;
+}
+
+// y_u = Js s_u on the velocity part of grid vectors (w, y: full-size work)
+static int ph_js_apply(mdg_ctx *c, solver_state *s, double pen, const double *su, double *yu) {
+    const sgrid g = c->sg;
+    const int nvel = nvx(g) + nvz(g), n = nvel + ncell(g);
+    hipLaunchKernelGGL(k_ph_js_in, dim3(nblk(n)), dim3(256), 0, c->stream, g, pen, su, s->ph_w.p);
+    int rc;
+    if ((rc = ph_apply_K(c, s->ph_w.p, s->ph_y.p))) return rc;
+    HIPCHK(c, hipMemcpyAsync(yu, s->ph_y.p, (size_t)nvel * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return MDG_OK;
+}
+
 template <typename T>
 static int ksp_gcr6(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opts *o, const double *b, double *x,
                     int *its_out) {
     const int restart = 6;
-    mg_level<T> &l0 = H.L[0];
-    const sgrid g = l0.g;
+    const sgrid g = c->sg;
     const int n = nvx(g) + nvz(g);
-    const double gamma = o->penalty;
+    const double pen = o->penalty / (g.dx * g.dz);
     HIPCHK(c, s->ph_f.reserve(n)); HIPCHK(c, s->ph_v.reserve(n)); HIPCHK(c, s->ph_s.reserve(n));
     HIPCHK(c, s->ph_VV.reserve((size_t)restart * n)); HIPCHK(c, s->ph_SS.reserve((size_t)restart * n));
     double *f = s->ph_f.p, *v = s->ph_v.p, *sv = s->ph_s.p;
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk(restart * n)), dim3(256), 0, c->stream, restart * n, s->ph_VV.p, 0.0);
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk(restart * n)), dim3(256), 0, c->stream, restart * n, s->ph_SS.p, 0.0);
-    // f = b - Js x
-    hipLaunchKernelGGL(k_vel_residual<double>, dim3(nblk(n)), dim3(256), 0, c->stream, g, gamma, (const double *)x, b,
-                       (const double *)c->eta_n.p, (const double *)c->eta_s.p, f);
-    double ff;
     int rc;
+    // f = b - Js x
+    if ((rc = ph_js_apply(c, s, pen, x, f))) return rc;
+    hipLaunchKernelGGL(k_diff, dim3(nblk(n)), dim3(256), 0, c->stream, n, b, (const double *)f, f);
+    double ff;
     if ((rc = dot_host<double>(c, s, n, f, f, &ff))) return rc;
     const double rnorm0 = std::sqrt(ff);
     double val[6] = {0, 0, 0, 0, 0, 0};
@@ -1846,9 +1937,7 @@ static int ksp_gcr6(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opts 
             o->inner_its += cg_its;
             HIPCHK(c, hipMemcpyAsync(sv, H.cg_x.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
             // v = Js s
-            hipLaunchKernelGGL(k_vel_residual<double>, dim3(nblk(n)), dim3(256), 0, c->stream, g, gamma, (const double *)sv,
-                               (const double *)nullptr, (const double *)c->eta_n.p, (const double *)c->eta_s.p, v);
-            hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, 0.0, (const double *)v, -1.0, v);
+            if ((rc = ph_js_apply(c, s, pen, sv, v))) return rc;
             for (int i2 = 0; i2 < i1; i2++)
                 if ((rc = dot_host<double>(c, s, n, v, s->ph_VV.p + (size_t)i2 * n, &val[i2]))) return rc;
             for (int i2 = 0; i2 < i1 + 1; i2++) {
@@ -1884,7 +1973,8 @@ static int ksp_gcr6(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opts 
     return MDG_OK;
 }
 
-// KillerSolver (Solvers.c:1054-1454) for the incompressible Picard system:
+// KillerSolver (Solvers.c:1054-1454) for the incompressible system (the
+// Picard operator, or the Newton operator staged by mdg_newton_solver):
 // Powell-Hestenes iterations with the constant penalty pen = penalty/dV
 // around kspgcr, stopping on the signed maxima of the residuals (k > 1,
 // Solvers.c:1317-1322); b = [b_u | b_p] -> x = [du | dp], grid layout.
@@ -1896,6 +1986,8 @@ static int ph_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opts 
     HIPCHK(c, s->Rm.reserve(n));
     HIPCHK(c, s->Req.reserve((size_t)c->neq));
     HIPCHK(c, s->ph_u.reserve(nu + nv));
+    HIPCHK(c, s->ph_w.reserve(n));
+    HIPCHK(c, s->ph_y.reserve(n));
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, x, 0.0);
     o->its = 0; o->inner_its = 0; o->its_KSP = 0; o->rel_res = 0.0;
     double maxdiv = 0.0, maxdiv0 = 1.0, maxru = 0.0, maxru0 = 1.0;
@@ -1903,14 +1995,17 @@ static int ph_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, mdg_solver_opts 
     double bb;
     if ((rc = dot_host<double>(c, s, n, b, b, &bb))) return rc;
     for (k = 0; k < o->max_its_PH; k++) {
-        hipLaunchKernelGGL(k_ph_udum, dim3(nblk(nu + nv)), dim3(256), 0, c->stream, g, pen, b, (const double *)x, s->ph_u.p);
+        // ud = b_u - B (pen b_p + dp)
+        hipLaunchKernelGGL(k_ph_q_in, dim3(nblk(n)), dim3(256), 0, c->stream, g, pen, b, (const double *)x, s->ph_w.p);
+        if ((rc = ph_apply_K(c, s->ph_w.p, s->ph_y.p))) return rc;
+        hipLaunchKernelGGL(k_diff, dim3(nblk(nu + nv)), dim3(256), 0, c->stream, nu + nv, b, (const double *)s->ph_y.p,
+                           s->ph_u.p);
         int its = 0;
         if ((rc = ksp_gcr6(c, s, H, o, s->ph_u.p, x, &its))) return rc;
         o->its_KSP += its;
         hipLaunchKernelGGL(k_ph_pupdate, dim3(nblk(np)), dim3(256), 0, c->stream, g, pen, b, x);
         // f = b - K x, then the signed maxima over the free rows (equation numbering)
-        hipLaunchKernelGGL(k_apply_Kg, dim3(nblk(n)), dim3(256), 0, c->stream, g, 0.0, (const double *)x,
-                           (const double *)c->eta_n.p, (const double *)c->eta_s.p, s->Rm.p);
+        if ((rc = ph_apply_K(c, x, s->Rm.p))) return rc;
         hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, c->stream, n, 1.0, b, -1.0, s->Rm.p);
         if ((rc = mdg_grid_to_eqn(c, s->Rm.p, s->Rm.p + nu, s->Rm.p + nu + nv, s->Req.p)) ||
             (rc = max_host(c, s, c->neq_cont, s->Req.p + c->neq_mom, &maxdiv)) ||
@@ -1980,7 +2075,7 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
     if (o->precond < 0 || o->precond > 3) return mdg_fail(c, MDG_ERR_ARG, "precond must be 0, 1, 2 or 3");
     if (o->precond != 0 && !mdg_pcnet_ready(c))
         return mdg_fail(c, MDG_ERR_STATE, "precond %d needs the network parameters (mdg_pcnet_set_params)", o->precond);
-    if (o->schur < 0 || o->schur > 1) return mdg_fail(c, MDG_ERR_ARG, "schur must be 0 or 1");
+    if (o->schur < 0 || o->schur > 3) return mdg_fail(c, MDG_ERR_ARG, "schur must be 0, 1, 2 or 3");
     if (o->cycle != 1) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "only the V-cycle (cycle = 1) is supported");
     if (o->smoother < 0 || o->smoother > 1) return mdg_fail(c, MDG_ERR_ARG, "smoother must be 0 or 1");
     if (o->graphs < 0 || o->graphs > 2) return mdg_fail(c, MDG_ERR_ARG, "graphs must be 0, 1 or 2");

@@ -121,6 +121,9 @@ def lib():
                                        i32p, i32p, f64p, f64p, f64p, f64p, f64p,
                                        C.POINTER(KillerOpts)]
         L.orc_killer_solve.restype = C.c_int
+        L.orc_killer_solve_newton.argtypes = [C.c_int, C.c_int] + [i32p, i32p, f64p] * 5 + \
+            [f64p, f64p, f64p, f64p, C.POINTER(KillerOpts)]
+        L.orc_killer_solve_newton.restype = C.c_int
         L.orc_set_band_order.argtypes = [gp, i32p, i32p, C.c_int]
         L.orc_viscosity_concise.argtypes = [C.POINTER(OrcPhase), C.POINTER(OrcRheoModel)] + \
             [C.c_double] * 10 + [_dp, _dp, _dp, C.POINTER(C.c_int)]
@@ -354,8 +357,10 @@ def assemble_jacobian(g, num, Dn, Ds, co=1.0):
 
 def killer_solve(g, num, A, B, Cm, bu, bp, penalty=1e5, max_its_PH=10, rel_tol_KSP=1e-4,
                  max_its_KSP=100, lin_abs_div=1e-9, lin_rel_div=1e-5, lin_abs_mom=1e-9,
-                 lin_rel_mom=1e-5):
-    """Solvers.c:1054-1454 (PH + GCR(6), exact band-Cholesky preconditioner)."""
+                 lin_rel_mom=1e-5, AJ=None, BJ=None):
+    """Solvers.c:1054-1454 (PH + GCR(6), exact band-Cholesky preconditioner
+    of the Picard Jts).  With the Newton blocks AJ, BJ (assemble_jacobian),
+    GCR iterates on the Jacobian system as the reference does."""
     eu, ev, ep, nm, nc, neq = num
     lib().orc_set_band_order(g.c, eu, ev, nm)
     o = KillerOpts(penalty, g.dx * g.dz, max_its_PH, rel_tol_KSP, max_its_KSP,
@@ -366,8 +371,13 @@ def killer_solve(g, num, A, B, Cm, bu, bp, penalty=1e5, max_its_PH=10, rel_tol_K
         return (np.ascontiguousarray(M.indptr, np.int32), np.ascontiguousarray(M.indices, np.int32),
                 np.ascontiguousarray(M.data, np.float64))
     a, b, c = csr(A), csr(B), csr(Cm)
-    st = lib().orc_killer_solve(nm, nc, *a, *b, *c, np.ascontiguousarray(bu),
-                                np.ascontiguousarray(bp), du, dp, C.byref(o))
+    if AJ is None:
+        st = lib().orc_killer_solve(nm, nc, *a, *b, *c, np.ascontiguousarray(bu),
+                                    np.ascontiguousarray(bp), du, dp, C.byref(o))
+    else:
+        ja, jb = csr(AJ.tocsr()), csr(BJ.tocsr())
+        st = lib().orc_killer_solve_newton(nm, nc, *a, *b, *ja, *jb, *c, np.ascontiguousarray(bu),
+                                           np.ascontiguousarray(bp), du, dp, C.byref(o))
     return du, dp, o, st
 
 

@@ -189,10 +189,16 @@ static double signed_max(const double *a, int n) {
     return m;
 }
 
-int orc_killer_solve(int nu, int np, const int *Ap, const int *Aj, const double *Ax,
-                     const int *Bp, const int *Bj, const double *Bx, const int *Cp,
-                     const int *Cj, const double *Cx, const double *bu, const double *bp,
-                     double *du, double *dp, orc_killer_opts *o) {
+/* KillerSolver with the Picard blocks A, B (preconditioner Jts = A - B pen C,
+ * Solvers.c:1150-1243 with preconditioner = 0) and the blocks JA, JB of the
+ * operator GCR iterates on and the residuals are taken with (the Jacobian
+ * blocks AcmJ, BcmJ, Solvers.c:1176-1193,1276-1314; A, B themselves for a
+ * Picard solve). */
+static int killer_core(int nu, int np, const int *Ap, const int *Aj, const double *Ax,
+                       const int *Bp, const int *Bj, const double *Bx, const int *JAp, const int *JAj,
+                       const double *JAx, const int *JBp, const int *JBj, const double *JBx,
+                       const int *Cp, const int *Cj, const double *Cx, const double *bu, const double *bp,
+                       double *du, double *dp, orc_killer_opts *o) {
     /* The band order interleaves rows: callers pass DOFs numbered Vx rows
      * then Vz rows; the row-interleaved order is supplied by
      * orc_band_order() through opts-independent numbering below. */
@@ -216,7 +222,7 @@ int orc_killer_solve(int nu, int np, const int *Ap, const int *Aj, const double 
     double *tp = (double *)calloc(np, sizeof(double));
     double *ud = (double *)calloc(nu, sizeof(double)), *fu = (double *)calloc(nu, sizeof(double));
     double *pd = (double *)calloc(np, sizeof(double)), *fpv = (double *)calloc(np, sizeof(double));
-    js_op M = {nu, np, Ap, Aj, Bp, Bj, Cp, Cj, Ax, Bx, Cx, pen, tp};
+    js_op M = {nu, np, JAp, JAj, JBp, JBj, Cp, Cj, JAx, JBx, Cx, pen, tp};
     memset(du, 0, sizeof(double) * nu);
     memset(dp, 0, sizeof(double) * np);
     double maxdiv = 0, maxdiv0 = 1, maxru = 0, maxru0 = 1;
@@ -225,15 +231,15 @@ int orc_killer_solve(int nu, int np, const int *Ap, const int *Aj, const double 
         /* udum = bu - B*(pen*bp) - B*dp  (Solvers.c:1290-1294) */
         for (int i = 0; i < np; i++) pd[i] = pen * bp[i] + dp[i];
         memcpy(ud, bu, sizeof(double) * nu);
-        csr_mv(nu, Bp, Bj, Bx, pd, ud, -1.0, 1.0);
+        csr_mv(nu, JBp, JBj, JBx, pd, ud, -1.0, 1.0);
         its_tot += kspgcr(&M, ud, du, &L, nu, o->rel_tol_KSP, o->max_its_KSP, work);
         /* dp += pen*(bp - C*du)  (Solvers.c:1302-1305) */
         csr_mv(np, Cp, Cj, Cx, du, pd, 1.0, 0.0);
         for (int i = 0; i < np; i++) dp[i] += pen * (bp[i] - pd[i]);
         /* fu = bu - A du - B dp ; fp = bp - C du */
         memcpy(fu, bu, sizeof(double) * nu);
-        csr_mv(nu, Ap, Aj, Ax, du, fu, -1.0, 1.0);
-        csr_mv(nu, Bp, Bj, Bx, dp, fu, -1.0, 1.0);
+        csr_mv(nu, JAp, JAj, JAx, du, fu, -1.0, 1.0);
+        csr_mv(nu, JBp, JBj, JBx, dp, fu, -1.0, 1.0);
         csr_mv(np, Cp, Cj, Cx, du, fpv, 1.0, 0.0);
         for (int i = 0; i < np; i++) fpv[i] = bp[i] - fpv[i];
         maxdiv = signed_max(fpv, np);
@@ -253,6 +259,23 @@ int orc_killer_solve(int nu, int np, const int *Ap, const int *Aj, const double 
     free(work); free(tp); free(ud); free(fu); free(pd); free(fpv);
     free(L.perm); free(L.L); free(ident);
     return o->status;
+}
+
+int orc_killer_solve(int nu, int np, const int *Ap, const int *Aj, const double *Ax,
+                     const int *Bp, const int *Bj, const double *Bx, const int *Cp,
+                     const int *Cj, const double *Cx, const double *bu, const double *bp,
+                     double *du, double *dp, orc_killer_opts *o) {
+    return killer_core(nu, np, Ap, Aj, Ax, Bp, Bj, Bx, Ap, Aj, Ax, Bp, Bj, Bx, Cp, Cj, Cx, bu, bp, du, dp, o);
+}
+
+int orc_killer_solve_newton(int nu, int np, const int *Ap, const int *Aj, const double *Ax,
+                            const int *Bp, const int *Bj, const double *Bx, const int *JAp,
+                            const int *JAj, const double *JAx, const int *JBp, const int *JBj,
+                            const double *JBx, const int *Cp, const int *Cj, const double *Cx,
+                            const double *bu, const double *bp, double *du, double *dp,
+                            orc_killer_opts *o) {
+    return killer_core(nu, np, Ap, Aj, Ax, Bp, Bj, Bx, JAp, JAj, JAx, JBp, JBj, JBx, Cp, Cj, Cx, bu, bp, du, dp,
+                       o);
 }
 
 /* Row-interleaved DOF order used for the band factor: for the momentum

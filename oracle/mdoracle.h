@@ -163,6 +163,19 @@ int orc_killer_solve(int neq_mom, int neq_cont,
                      const double *rhs_mom, const double *rhs_cont, double *du, double *dp,
                      orc_killer_opts *opts);
 
+/* The same on the Newton system (Solvers.c:1176-1193,1276-1314): GCR iterates
+ * on Js = JA - JB (penalty/dV) C and the residuals use JA, JB, while the
+ * factor stays that of the Picard Jts = A - B (penalty/dV) C
+ * (preconditioner = 0, the default, InputOutput.c:1177). */
+int orc_killer_solve_newton(int neq_mom, int neq_cont,
+                            const int *Ap, const int *Aj, const double *Ax,
+                            const int *Bp, const int *Bj, const double *Bx,
+                            const int *JAp, const int *JAj, const double *JAx,
+                            const int *JBp, const int *JBj, const double *JBx,
+                            const int *Cp, const int *Cj, const double *Cx,
+                            const double *rhs_mom, const double *rhs_cont, double *du, double *dp,
+                            orc_killer_opts *opts);
+
 /* Material parameters of one phase (mat_prop of mdoodz.h, the subset used
  * by ViscosityConcise below; units are the model's scaled units). */
 typedef struct {

@@ -149,11 +149,12 @@ def test_solcx_free_slip_and_body_force(oracle_mod, precision):
     closed free-slip box at 81 x 81: the GPU's body force, residual and
     solve give the oracle's solution to 1e-8, and its errors against the
     analytic free-slip solution stay inside the reference test's bounds
-    (TESTS/SolCxBenchmarkTests.cpp:216-233)."""
+    (TESTS/SolCxBenchmarkTests.cpp:216-233).  The oracle's system is solved
+    directly (LU) for this comparison."""
     import json
     import os
     from oracle.driver import evaluate_residual, initial_fields, scatter_solution
-    from test_oracle_solcx import TIGHT, solcx_errors, solcx_problem
+    from test_oracle_solcx import solcx_errors, solcx_problem
     O = oracle_mod
     n = 81
     g, en, es, rho_n = solcx_problem(O, n)
@@ -168,8 +169,16 @@ def test_solcx_free_slip_and_body_force(oracle_mod, precision):
     u, v, p = initial_fields(g)
     F, _ = evaluate_residual(g, num, u, v, p, en, es, rx, rz)
     A, B, Cm = O.assemble_picard(g, num, en, es)
-    du, dp, st, rc = O.killer_solve(g, num, A, B, Cm, F[:nm], F[nm:], **TIGHT)
-    assert rc == 0
+    # the oracle's system solved directly (LU with a pressure-gauge row): with
+    # a 1e6 jump the Powell-Hestenes iteration's own accuracy (its penalised
+    # factor) is ~1e-6 on the momentum rows, below what this comparison needs
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    npr = Cm.shape[0]
+    one = sp.csr_matrix(np.ones((npr, 1)))
+    KK = sp.bmat([[A, B, None], [Cm, None, one], [None, one.T, None]]).tocsc()
+    xd = spla.spsolve(KK, np.concatenate([F, [0.0]]))
+    du, dp = xd[:nm], xd[nm:nm + npr]
     sol, o = ctx.killer_solver(2 * en, 2 * en, es, F[:nm], F[nm:], rel_tol=1e-12, precision=precision)
     rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
     assert rel(sol[:nm], du) <= 1e-8

@@ -161,3 +161,39 @@ def test_newton_loop_converges_to_picard_solution(oracle_mod):
     assert rel(st_gpu["u"], st["u"]) < 1e-8
     assert rel(st_gpu["v"], st["v"]) < 1e-8
     assert rel(st_gpu["p"], st["p"]) < 1e-8
+
+
+@pytest.mark.parametrize("C", [50.0, 1.0])
+def test_killer_iteration_counts_on_newton_system(oracle_mod, C):
+    """KillerSolver's iteration on a Newton system (GCR on the Jacobian,
+    preconditioned by the Picard factor, Solvers.c:1176-1193): the GPU's
+    method 1 (Picard multigrid CG in place of the factor) takes the same
+    Powell-Hestenes and GCR iteration counts as the oracle (exact factor)
+    and returns the same correction, at a power-law state (C = 50) and at a
+    yielding one (C = 1)."""
+    O = oracle_mod
+    from mdgpu.nonlinear import PicardLoop
+    g, st, ctx, o_ph, g_ph, common = _setup(O, 23, C=C)
+    rng = np.random.default_rng(9)
+    _perturbed_state(O, g, st, rng)
+    num = g.numbering()
+    eu, ev, ep, nm, nc, neq = num
+    loop = PicardLoop(ctx, eu, ev, ep, g_ph, mdgpu.RheoParams(average=0, **common))
+    sr, cen, ver = loop.update_nonlinearity(st)
+    cin, vin = loop.last_inputs
+    Dn = ctx.tangent_grid(g_ph, loop.prm, 1, **cin)
+    Ds = ctx.tangent_grid(g_ph, loop.prm, 0, **vin)
+    en, es = cen["eta"], ver["eta"]
+    rx, rz = ctx.eval_rhs(0.0, -1.0, st["rho_n"], st["rho_s"])
+    F, _ = ctx.stokes_residual(cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"], sr["div"], rx, rz)
+    A, B, Cm = O.assemble_picard(g, num, en, es)
+    AJ, BJ = O.assemble_jacobian(g, num, Dn, Ds, co=1.0)
+    kw = dict(penalty=1e3, max_its_PH=10, rel_tol_KSP=1e-4, max_its_KSP=100, lin_abs_div=1e-9,
+              lin_rel_div=1e-5, lin_abs_mom=1e-9, lin_rel_mom=1e-5)
+    du, dp, so, rc = O.killer_solve(g, num, A, B, Cm, F[:nm], F[nm:], AJ=AJ, BJ=BJ, **kw)
+    sol, o = ctx.newton_solver(Dn, Ds, 2 * en, es, F[:nm], F[nm:], method=1, accept_diverged=True, **kw)
+    print("C %g: oracle PH %d KSP %d (status %d); gpu PH %d KSP %d, %d CG its" % (
+        C, so.its_PH, so.its_KSP_tot, rc, o.its_PH, o.its_KSP, o.inner_its), flush=True)
+    assert (o.its_PH, o.its_KSP) == (so.its_PH, so.its_KSP_tot)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(sol[:nm], du) <= 1e-8 and rel(sol[nm:], dp) <= 1e-8

@@ -155,7 +155,7 @@ def test_operator_transpose_is_pressure_flip(oracle_mod, bc):
     assert abs(lhs - rhs) <= 1e-13 * scale, (lhs, rhs)
 
 
-@pytest.mark.parametrize("opt,val,code", [("cycle", 2, -3), ("schur", 2, -1), ("precond", 4, -1),
+@pytest.mark.parametrize("opt,val,code", [("cycle", 2, -3), ("schur", 4, -1), ("precond", 4, -1),
                                             ("smoother", 2, -1)])
 def test_solver_refuses_unsupported_options(oracle_mod, opt, val, code):
     """Options outside the implemented set are refused before any work: the
