@@ -395,6 +395,54 @@ int mdg_update_particle_stress(mdg_ctx *ctx, const double *sxxd, const double *s
 int mdg_update_particle_pressure(mdg_ctx *ctx, const double *p_in, const double *p0_n, double *dp, int n,
                                  const double *x, const double *z, const int *phase, double *P);
 
+/* AccumulatedStrainII (RheologyParticles.c:346-649; called at
+ * Main_DOODZ.c:1251): increments dt * eII on the centres (the total being
+ * the sum of the seven mechanisms), interpolated to each active marker from
+ * the four centres around it (BCp 30/31 centres left out, weights
+ * renormalised) and added to its accumulated strains.  eII holds 7 centre
+ * arrays: eII_pl, eII_pwl, eII_exp, eII_lin, eII_gbs, eII_el, eII_cst (an
+ * entry may be NULL = 0); div_u_pl may be NULL; strain holds 8 marker arrays
+ * updated in place: strain, strain_el, strain_pl, strain_pl_vol,
+ * strain_pwl, strain_exp, strain_lin, strain_gbs (an entry may be NULL =
+ * skipped).  A negative total increment returns MDG_ERR_ARG (the reference
+ * exits). */
+int mdg_accumulated_strain(mdg_ctx *ctx, double dt, const double *const *eII, const double *div_u_pl, int n,
+                           const double *x, const double *z, const int *phase, double *const *strain);
+
+/* The stress rotation of UpdateParticleStress for elastic runs
+ * (RheologyParticles.c:1965-1988): with the markers' spin wxzm, mode 1
+ * (stress_rotation = 1) the Jaumann rate, mode 2 the rotation by the angle
+ * dt * wxzm, mode 0 nothing; active markers only, in place.  The reference
+ * fills wxzm with P2Mastah(wxzm, mesh->wxz, flag 1) at :1971, which
+ * interpolates the zero-initialised marker array onto the vertex spin
+ * instead of the spin onto the markers: there wxzm stays 0, the stresses
+ * are unchanged and mesh->wxz is overwritten with zeros on every vertex
+ * with markers.  The same sequence here is mdg_p2mastah of a zero marker
+ * array onto the vertices (centroid 0, flag 1) followed by this call with
+ * wxzm = 0. */
+int mdg_rotate_particle_stress(mdg_ctx *ctx, int mode, double dt, int n, const int *phase, const double *wxzm,
+                               double *sxxd, double *szzd, double *sxz);
+
+/* ---------------------------------------------------------------- energy
+ * One implicit energy step: the matrix of EnergyDirectSolve
+ * (ThermalRoutines.c:74-470; prototype mdoodz-private.h, called from
+ * Main_DOODZ.c's time loop) for a box without air cells (theta = 1,
+ * transient), solved with SolveThermalPCG (ThermalSolver.c:338-480,
+ * model.thermal_solver = 1, prototype mdoodz-private.h:471): Jacobi PCG from
+ * x0 = T, stopping when ||r|| / ||r0|| < rel_tol.
+ * Cells (Ncx*Ncz, x fastest): rho_n, Cp, T (in/out) and the optional source
+ * terms Qr, Wdiss, Ha (NULL: absent); kx on the Vx nodes (Nx*(Nz+1)), kz on
+ * the Vz nodes ((Nx+1)*Nz); BCT_t/BCT_v on the cells with a ghost ring
+ * ((Ncx+2)*(Ncz+2)): type 1 Dirichlet at the wall face, 0 insulated (the
+ * south ghost adds the reference's -val/dx flux term).  *its receives the
+ * reference's return value: iterations (> 0), 0 (zero initial residual),
+ * -1 (max_its reached, T holds the last iterate), -2 / -3 (breakdown); the
+ * reference then falls back to its direct solver, which is the caller's
+ * choice here. */
+int mdg_energy_step(mdg_ctx *ctx, double dt, const double *rho_n, const double *Cp, const double *Qr,
+                    const double *Wdiss, const double *Ha, const double *kx, const double *kz,
+                    const signed char *BCT_t, const double *BCT_v, int max_its, double rel_tol, double *T, int *its);
+
 /* ---------------------------------------------------------------- Newton
  * RheologicalOperators with Jacobian = 1 (StokesRoutines.c:250-375;
  * prototype mdoodz-private.h:607), isotropic and incompressible: the
@@ -473,7 +521,8 @@ int mdg_pcnet_train_residual(mdg_ctx *ctx, int steps, int batch, float lr, unsig
  * reference; it feeds the roofline figure of the benchmark. */
 int mdg_time_smoother(mdg_ctx *ctx, int reps, double *ms_per_launch, double *bytes_per_launch);
 
-/* Time of the last P2G kernel (ms, HIP events on the context stream). */
+/* Time of the last timed call (P2G kernel, G2P and advection kernels,
+ * energy step: ms, HIP events on the context stream). */
 int mdg_last_kernel_ms(const mdg_ctx *ctx, double *ms);
 /* Time of the last computation of the cached per-marker P2G weights (ms):
  * the first 1-cell-stencil P2G call on a node set after mdg_markers_set

@@ -148,6 +148,89 @@ __global__ void k_add_g2p(cgrid g, const double *__restrict__ F, int n, const do
     P[k] = P[k] + inc;
 }
 
+// AccumulatedStrainII (RheologyParticles.c:346-649): per-centre increments
+// dt * eII in the reference's order (the total as the sum of the seven
+// mechanisms pl, pwl, exp, lin, gbs, el, cst), into 8 arrays of nc; a
+// negative total sets *bad to the smallest such cell + 1 (the reference
+// exits there).
+struct strain_rates7 {
+    const double *e[7];      // eII_pl, eII_pwl, eII_exp, eII_lin, eII_gbs, eII_el, eII_cst (NULL = 0)
+    const double *div_pl;    // div_u_pl (NULL = 0)
+};
+__global__ void k_strain_increments(int nc, double dt, strain_rates7 r, double *__restrict__ inc, int *__restrict__ bad) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double e[7];
+    for (int q = 0; q < 7; q++) e[q] = r.e[q] ? r.e[q][c] : 0.0;
+    const double tot = dt * (e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6]);
+    if (tot < 0) atomicMin(bad, c + 1);
+    inc[c] = tot;
+    inc[1 * (size_t)nc + c] = dt * e[5];
+    inc[2 * (size_t)nc + c] = dt * e[0];
+    inc[3 * (size_t)nc + c] = dt * (r.div_pl ? r.div_pl[c] : 0.0);
+    inc[4 * (size_t)nc + c] = dt * e[1];
+    inc[5 * (size_t)nc + c] = dt * e[2];
+    inc[6 * (size_t)nc + c] = dt * e[3];
+    inc[7 * (size_t)nc + c] = dt * e[4];
+}
+
+// Marker part: the four centres around the marker (the cell index clamped to
+// the centre grid, BCp 30/31 centres left out, weights renormalised), each
+// increment interpolated and added to the marker's strain (NULL arrays are
+// skipped; inactive markers add zero, as the reference's loop does).
+struct strain_out8 {
+    double *s[8];            // strain, el, pl, pl_vol, pwl, exp, lin, gbs
+};
+__global__ void k_accumulate_strain(int Ncx, int Ncz, double dx, double dz, const double *__restrict__ xc,
+                                    const double *__restrict__ zc, const signed char *__restrict__ tag,
+                                    const double *__restrict__ inc, int n, const double *__restrict__ x,
+                                    const double *__restrict__ z, const int *__restrict__ phase, strain_out8 out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const size_t nc = (size_t)Ncx * Ncz;
+    double dE[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (phase[k] != -1) {
+        const double xk = x[k], zk = z[k];
+        int j = (int)ceil((xk - xc[0]) / dx) - 1;
+        j = j < 0 ? 0 : (j > Ncx - 2 ? Ncx - 2 : j);
+        int i = (int)ceil((zk - zc[0]) / dz) - 1;
+        i = i < 0 ? 0 : (i > Ncz - 2 ? Ncz - 2 : i);
+        const double dxm = xk - xc[j], dzm = zk - zc[i];
+        const int nd[4] = {j + i * Ncx, j + i * Ncx + 1, j + (i + 1) * Ncx, j + (i + 1) * Ncx + 1};
+        const double w[4] = {(1.0 - dxm / dx) * (1.0 - dzm / dz), (dxm / dx) * (1.0 - dzm / dz),
+                             (1.0 - dxm / dx) * (dzm / dz), (dxm / dx) * (dzm / dz)};
+        double sumW = 0.0;
+        for (int q = 0; q < 4; q++) {
+            const signed char t = tag[nd[q]];
+            if (t == 30 || t == 31) continue;
+            for (int f = 0; f < 8; f++) dE[f] += w[q] * inc[f * nc + nd[q]];
+            sumW += w[q];
+        }
+        if (sumW > 1e-13)
+            for (int f = 0; f < 8; f++) dE[f] /= sumW;
+    }
+    for (int f = 0; f < 8; f++)
+        if (out.s[f]) out.s[f][k] = out.s[f][k] + dE[f];
+}
+
+// Stress rotation of UpdateParticleStress (RheologyParticles.c:1965-1988)
+__global__ void k_rotate_stress(int mode, double dt, int n, const int *__restrict__ phase, const double *__restrict__ w,
+                                double *__restrict__ sxx, double *__restrict__ szz, double *__restrict__ sxz) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || phase[k] == -1) return;
+    const double txx = sxx[k], tzz = szz[k], txz = sxz[k], wk = w[k];
+    if (mode == 1) {   // Jaumann rate
+        sxx[k] = txx - dt * 2.0 * txz * wk;
+        szz[k] = tzz - -dt * 2.0 * txz * wk;
+        sxz[k] = txz - dt * (tzz - txx) * wk;
+    } else {           // rotation by dt * w
+        const double a = dt * wk, c = cos(a), s = sin(a);
+        sxx[k] = (txx * c + txz * s) * c + (txz * c + tzz * s) * s;
+        szz[k] = -(-txx * s + txz * c) * s + (-txz * s + tzz * c) * c;
+        sxz[k] = -(txx * c + txz * s) * s + (txz * c + tzz * s) * c;
+    }
+}
+
 cgrid make_cgrid(const mdg_ctx *c) {
     return cgrid{c->Nx - 1, c->Nz - 1, c->dx, c->dz, c->xvz.p, c->zvx.p, c->BCp_t.p};
 }
@@ -159,6 +242,7 @@ struct g2p_scratch {
     dbuf<double> F, x, z, out, p0, dp;
     dbuf<int> ph;
     dbuf<double> in[6], inc, m[3];   // stress update: grid inputs, increments, marker stresses
+    dbuf<double> e[8], ms[8];        // accumulated strain: grid rates, marker strains (host callers)
 };
 
 static g2p_scratch &g2p_of(mdg_ctx *c) {
@@ -173,6 +257,8 @@ void mdg_g2p_release(mdg_ctx *c) {
     s->ph.release(); s->inc.release();
     for (auto &b : s->in) b.release();
     for (auto &b : s->m) b.release();
+    for (auto &b : s->e) b.release();
+    for (auto &b : s->ms) b.release();
     delete s;
     c->g2p = nullptr;
 }
@@ -316,6 +402,84 @@ extern "C" int mdg_update_particle_stress(mdg_ctx *c, const double *sxxd, const 
     hipLaunchKernelGGL(k_add_stress, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, make_cgrid(c), gv,
                        (const double *)dxx, (const double *)dzz, (const double *)dxz, n, dx_, dz_, dph, dm[0], dm[1],
                        dm[2]);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    for (int q = 0; q < 3; q++)
+        if (dm[q] != user[q]) HIPCHK(c, hipMemcpyAsync(user[q], dm[q], (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    return timed_end(c);
+}
+
+extern "C" int mdg_accumulated_strain(mdg_ctx *c, double dt, const double *const *eII, const double *div_u_pl, int n,
+                                      const double *x, const double *z, const int *phase, double *const *strain) {
+    if (!c || !eII || !strain || !x || !z || !phase || n < 0) return mdg_fail(c, MDG_ERR_ARG, "bad argument");
+    if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
+    g2p_scratch &s = g2p_of(c);
+    const int Ncx = c->Nx - 1, Ncz = c->Nz - 1, nc = Ncx * Ncz;
+    strain_rates7 r{};
+    int rc;
+    for (int q = 0; q < 7; q++)
+        if ((rc = mdg_stage_in(c, eII[q], (size_t)nc, s.e[q], &r.e[q]))) return rc;
+    if ((rc = mdg_stage_in(c, div_u_pl, (size_t)nc, s.e[7], &r.div_pl))) return rc;
+    const double *dx_, *dz_;
+    const int *dph;
+    if ((rc = mdg_stage_in(c, x, (size_t)n, s.x, &dx_)) || (rc = mdg_stage_in(c, z, (size_t)n, s.z, &dz_)) ||
+        (rc = mdg_stage_in(c, phase, (size_t)n, s.ph, &dph)))
+        return rc;
+    strain_out8 out{};
+    for (int f = 0; f < 8; f++) {
+        if (!strain[f] || mdg_is_device_ptr(strain[f])) {
+            out.s[f] = strain[f];
+        } else {
+            HIPCHK(c, s.ms[f].reserve((size_t)(n > 0 ? n : 1)));
+            if (n > 0) HIPCHK(c, hipMemcpyAsync(s.ms[f].p, strain[f], (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+            out.s[f] = s.ms[f].p;
+        }
+    }
+    HIPCHK(c, s.inc.reserve(8 * (size_t)nc));
+    int *bad = c->flag_dev.p, h_bad = 0;
+    HIPCHK(c, hipMemsetAsync(bad, 0x7f, sizeof(int), c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_strain_increments, dim3(mdg_grid1d(nc, 256)), dim3(256), 0, c->stream, nc, dt, r, s.inc.p, bad);
+    HIPCHK(c, hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (h_bad != 0x7f7f7f7f)
+        return mdg_fail(c, MDG_ERR_ARG, "negative strain increment at cell %d (AccumulatedStrainII exits there)", h_bad - 1);
+    if (n > 0)
+        hipLaunchKernelGGL(k_accumulate_strain, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, Ncx, Ncz, c->dx, c->dz,
+                           (const double *)c->xc.p, (const double *)c->zc.p, (const signed char *)c->BCp_t.p,
+                           (const double *)s.inc.p, n, dx_, dz_, dph, out);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    for (int f = 0; f < 8; f++)
+        if (strain[f] && out.s[f] != strain[f] && n > 0)
+            HIPCHK(c, hipMemcpyAsync(strain[f], out.s[f], (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    return timed_end(c);
+}
+
+extern "C" int mdg_rotate_particle_stress(mdg_ctx *c, int mode, double dt, int n, const int *phase, const double *wxzm,
+                                          double *sxxd, double *szzd, double *sxz) {
+    if (!c || !phase || !wxzm || !sxxd || !szzd || !sxz || n < 0) return mdg_fail(c, MDG_ERR_ARG, "bad argument");
+    if (mode < 0 || mode > 2) return mdg_fail(c, MDG_ERR_ARG, "stress_rotation must be 0, 1 or 2");
+    if (mode == 0 || n == 0) return MDG_OK;
+    g2p_scratch &s = g2p_of(c);
+    const int *dph;
+    const double *dw;
+    int rc;
+    if ((rc = mdg_stage_in(c, phase, (size_t)n, s.ph, &dph)) || (rc = mdg_stage_in(c, wxzm, (size_t)n, s.x, &dw)))
+        return rc;
+    double *user[3] = {sxxd, szzd, sxz}, *dm[3];
+    for (int q = 0; q < 3; q++) {
+        if (mdg_is_device_ptr(user[q])) {
+            dm[q] = user[q];
+        } else {
+            HIPCHK(c, s.m[q].reserve((size_t)n));
+            HIPCHK(c, hipMemcpyAsync(s.m[q].p, user[q], (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+            dm[q] = s.m[q].p;
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_rotate_stress, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, mode, dt, n, dph, dw, dm[0],
+                       dm[1], dm[2]);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     for (int q = 0; q < 3; q++)

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <string>
@@ -86,6 +87,7 @@ struct mdg_ctx {
     struct newton_state *newton = nullptr;
     struct advect_scratch *advect = nullptr;
     struct g2p_scratch *g2p = nullptr;
+    struct thermal_scratch *thermal = nullptr;
     // runtime
     hipStream_t  stream = nullptr;
     hipEvent_t   ev0 = nullptr, ev1 = nullptr;
@@ -133,6 +135,7 @@ void mdg_pcnet_release(mdg_ctx *c);
 void mdg_newton_release(mdg_ctx *c);
 void mdg_advect_release(mdg_ctx *c);
 void mdg_g2p_release(mdg_ctx *c);
+void mdg_thermal_release(mdg_ctx *c);
 // Newton operator staged by mdg_newton_solver: y = J_gamma x (grid layout).
 bool mdg_newton_active(const mdg_ctx *c);
 int mdg_newton_apply_grid(mdg_ctx *c, double gamma, const double *x, double *y);

@@ -82,6 +82,10 @@ def lib():
         L.mdg_g2p_vertices.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp]
         L.mdg_update_particle_stress.argtypes = [vp] * 7 + [C.c_int] + [vp] * 6
         L.mdg_update_particle_pressure.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
+        L.mdg_accumulated_strain.argtypes = [vp, C.c_double, vp, vp, C.c_int, vp, vp, vp, vp]
+        L.mdg_rotate_particle_stress.argtypes = [vp, C.c_int, C.c_double, C.c_int, vp, vp, vp, vp, vp]
+        L.mdg_energy_step.argtypes = [vp, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_double, vp,
+                                      C.POINTER(C.c_int)]
         _extra_signatures(L)
         _LIB = L
     return _LIB
@@ -404,6 +408,47 @@ class Context:
             self.h, checked(p_in, np.float64, nc, "p_in"), checked(p0_n, np.float64, nc, "p0_n"),
             checked(dp, np.float64, nc, "dp", optional=True), n, px, pz, pp, checked(P, np.float64, n, "P")))
         return dp
+
+    STRAIN_RATES = ("eII_pl", "eII_pwl", "eII_exp", "eII_lin", "eII_gbs", "eII_el", "eII_cst")
+    STRAINS = ("strain", "strain_el", "strain_pl", "strain_pl_vol", "strain_pwl", "strain_exp", "strain_lin",
+               "strain_gbs")
+
+    def accumulated_strain(self, dt, x, z, phase, rates, strains, div_u_pl=None):
+        """AccumulatedStrainII (RheologyParticles.c:346): rates maps names of
+        STRAIN_RATES to centre arrays (missing = 0), strains maps names of
+        STRAINS to marker arrays updated in place (missing = skipped)."""
+        n, (px, pz, pp) = self._marker_args(x, z, phase)
+        nc = self.node_count(1)
+        e = (C.c_void_p * 7)(*[checked(rates.get(k), np.float64, nc, k, optional=True) for k in self.STRAIN_RATES])
+        st = (C.c_void_p * 8)(*[checked(strains.get(k), np.float64, n, k, optional=True) for k in self.STRAINS])
+        self._check(self._L.mdg_accumulated_strain(self.h, dt, C.cast(e, C.c_void_p),
+                                                   checked(div_u_pl, np.float64, nc, "div_u_pl", optional=True), n,
+                                                   px, pz, pp, C.cast(st, C.c_void_p)))
+
+    def rotate_particle_stress(self, mode, dt, phase, wxzm, sxxd, szzd, sxz):
+        """Stress rotation of UpdateParticleStress (RheologyParticles.c:1965):
+        marker stresses updated in place."""
+        n = len(phase)
+        self._check(self._L.mdg_rotate_particle_stress(
+            self.h, int(mode), dt, n, checked(phase, np.int32, n, "phase"), checked(wxzm, np.float64, n, "wxzm"),
+            checked(sxxd, np.float64, n, "sxxd"), checked(szzd, np.float64, n, "szzd"),
+            checked(sxz, np.float64, n, "sxz")))
+
+    def energy_step(self, dt, rho_n, Cp, T, kx, kz, BCT_t, BCT_v, Qr=None, Wdiss=None, Ha=None, max_its=1000,
+                    rel_tol=1e-8):
+        """EnergyDirectSolve with SolveThermalPCG (ThermalRoutines.c:74,
+        ThermalSolver.c:338): T (cells) updated in place; returns the PCG's
+        return value (iterations, or 0 / -1 / -2 / -3)."""
+        nc = (self.Nx - 1) * (self.Nz - 1)
+        nbc = (self.Nx + 1) * (self.Nz + 1)
+        its = C.c_int(0)
+        f = lambda a, n, name: checked(a, np.float64, n, name)
+        o = lambda a, name: checked(a, np.float64, nc, name, optional=True)
+        self._check(self._L.mdg_energy_step(
+            self.h, dt, f(rho_n, nc, "rho_n"), f(Cp, nc, "Cp"), o(Qr, "Qr"), o(Wdiss, "Wdiss"), o(Ha, "Ha"), f(kx, self.Nx * (self.Nz + 1), "kx"), f(kz, (self.Nx + 1) * self.Nz, "kz"),
+            checked(BCT_t, np.int8, nbc, "BCT_t"), f(BCT_v, nbc, "BCT_v"), int(max_its), rel_tol,
+            checked(T, np.float64, nc, "T"), C.byref(its)))
+        return its.value
 
     def markers_gather(self, src, dst_sorted):
         self._check(self._L.mdg_markers_gather(self.h, ptr(src), ptr(dst_sorted)))

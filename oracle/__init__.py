@@ -14,6 +14,7 @@ _LIB = None
 
 i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
 i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_bp_nd = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _dp = C.POINTER(C.c_double)
 _bp = C.POINTER(C.c_int8)
@@ -124,6 +125,17 @@ def lib():
         L.orc_killer_solve_newton.argtypes = [C.c_int, C.c_int] + [i32p, i32p, f64p] * 5 + \
             [f64p, f64p, f64p, f64p, C.POINTER(KillerOpts)]
         L.orc_killer_solve_newton.restype = C.c_int
+        L.orc_energy_matrix.argtypes = [gp, C.c_double, f64p, f64p, f64p, f64p, _bp_nd, f64p, i32p, i32p, f64p,
+                                        f64p]
+        L.orc_energy_matrix.restype = None
+        L.orc_energy_step.argtypes = [gp, C.c_double, f64p, f64p, C.c_void_p, C.c_void_p, C.c_void_p, f64p, f64p,
+                                      _bp_nd, f64p, C.c_int, C.c_double, f64p]
+        L.orc_energy_step.restype = C.c_int
+        L.orc_accumulated_strain.argtypes = [gp, C.c_double, C.c_void_p, C.c_void_p, C.c_int, f64p, f64p, i32p,
+                                             C.c_void_p]
+        L.orc_accumulated_strain.restype = C.c_int
+        L.orc_rotate_stress.argtypes = [C.c_int, C.c_double, C.c_int, i32p, f64p, f64p, f64p, f64p]
+        L.orc_rotate_stress.restype = None
         L.orc_set_band_order.argtypes = [gp, i32p, i32p, C.c_int]
         L.orc_viscosity_concise.argtypes = [C.POINTER(OrcPhase), C.POINTER(OrcRheoModel)] + \
             [C.c_double] * 10 + [_dp, _dp, _dp, C.POINTER(C.c_int)]
@@ -246,6 +258,58 @@ def update_particle_pressure(g, p_in, p0_n, x, z, phase, P):
                                        np.ascontiguousarray(z, np.float64),
                                        np.ascontiguousarray(phase, np.int32), P)
     return P, dp
+
+
+STRAIN_RATES = ("eII_pl", "eII_pwl", "eII_exp", "eII_lin", "eII_gbs", "eII_el", "eII_cst")
+STRAINS = ("strain", "strain_el", "strain_pl", "strain_pl_vol", "strain_pwl", "strain_exp", "strain_lin",
+           "strain_gbs")
+
+
+def accumulated_strain(g, dt, x, z, phase, rates, strains, div_u_pl=None):
+    """RheologyParticles.c:346-649: returns updated copies of the marker
+    strains (dict), or raises on a negative increment."""
+    f = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
+    keep = [f(rates.get(k)) for k in STRAIN_RATES]
+    out = {k: np.array(v, np.float64) for k, v in strains.items()}
+    e = (C.c_void_p * 7)(*[None if a is None else a.ctypes.data for a in keep])
+    st = (C.c_void_p * 8)(*[out[k].ctypes.data if k in out else None for k in STRAINS])
+    dp = f(div_u_pl)
+    bad = lib().orc_accumulated_strain(g.c, dt, C.cast(e, C.c_void_p), None if dp is None else dp.ctypes.data, len(x),
+                                       f(x), f(z), np.ascontiguousarray(phase, np.int32), C.cast(st, C.c_void_p))
+    if bad >= 0:
+        raise ValueError("negative strain increment at cell %d" % bad)
+    return out
+
+
+def energy_matrix(g, dt, rho_n, Cp, kx, kz, BCT_t, BCT_v):
+    """ThermalRoutines.c:185-390: (scipy CSR matrix, boundary rhs)."""
+    n = (g.Nx - 1) * (g.Nz - 1)
+    Ap = np.zeros(n + 1, np.int32); Aj = np.zeros(5 * n, np.int32); Ax = np.zeros(5 * n); bbc = np.zeros(n)
+    f = lambda a: np.ascontiguousarray(a, np.float64)
+    lib().orc_energy_matrix(g.c, dt, f(rho_n), f(Cp), f(kx), f(kz), np.ascontiguousarray(BCT_t, np.int8), f(BCT_v),
+                            Ap, Aj, Ax, bbc)
+    import scipy.sparse as sp
+    return sp.csr_matrix((Ax[:Ap[-1]], Aj[:Ap[-1]], Ap), shape=(n, n)), bbc
+
+
+def energy_step(g, dt, rho_n, Cp, T, kx, kz, BCT_t, BCT_v, Qr=None, Wdiss=None, Ha=None, max_its=1000,
+                rel_tol=1e-8):
+    """EnergyDirectSolve + SolveThermalPCG: (new T, iterations)."""
+    f = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
+    Tn = np.array(T, np.float64)
+    extra = [f(a) for a in (Qr, Wdiss, Ha)]
+    its = lib().orc_energy_step(g.c, dt, f(rho_n), f(Cp), *[None if a is None else a.ctypes.data for a in extra],
+                                f(kx), f(kz), np.ascontiguousarray(BCT_t, np.int8), f(BCT_v), int(max_its),
+                                rel_tol, Tn)
+    return Tn, its
+
+
+def rotate_stress(mode, dt, phase, wxzm, sxxd, szzd, sxz):
+    """RheologyParticles.c:1965-1988: rotated copies of the marker stresses."""
+    out = [np.array(a, np.float64) for a in (sxxd, szzd, sxz)]
+    lib().orc_rotate_stress(int(mode), dt, len(phase), np.ascontiguousarray(phase, np.int32),
+                            np.ascontiguousarray(wxzm, np.float64), *out)
+    return out
 
 
 def advect_rk4(g, u, v, dt, x, z, phase, conserv=0, vx=None, vz=None):

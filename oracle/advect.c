@@ -297,3 +297,84 @@ void orc_update_particle_stress(const orc_grid *g, const double *sxxd, const dou
     free(dzz);
     free(dxz);
 }
+
+/* AccumulatedStrainII (RheologyParticles.c:346-649): per-centre increments
+ * dt * eII (the total being the sum of the seven mechanisms in the
+ * reference's order), then per active marker the bilinear interpolation of
+ * each increment from the four centres around it (centre coordinates xc,
+ * zc; the cell index clamped to the centre grid; BCp tag 30/31 centres left
+ * out and the weights renormalised when their sum exceeds 1e-13), added to
+ * the marker's accumulated strains.  eII[0..6] = pl, pwl, exp, lin, gbs, el,
+ * cst (NULL = 0); strain[0..7] = total, el, pl, pl_vol, pwl, exp, lin, gbs
+ * (NULL = skipped).  Returns the first cell with a negative total increment
+ * (the reference exits there), or -1. */
+int orc_accumulated_strain(const orc_grid *g, double dt, const double *const *eII, const double *div_u_pl, int n,
+                           const double *x, const double *z, const int *phase, double *const *strain) {
+    const int Nx = g->Nx - 1, Nz = g->Nz - 1, nc = Nx * Nz;   /* centre grid */
+    const double dx = g->dx, dz = g->dz;
+    double *inc = (double *)malloc(sizeof(double) * 8 * (size_t)nc);
+    int bad = -1;
+    for (int c = 0; c < nc; c++) {
+        double e[7];
+        for (int q = 0; q < 7; q++) e[q] = eII[q] ? eII[q][c] : 0.0;
+        inc[c] = dt * (e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6]);
+        if (inc[c] < 0 && bad < 0) bad = c;
+        inc[1 * nc + c] = dt * e[5];                                   /* elastic    */
+        inc[2 * nc + c] = dt * e[0];                                   /* plastic    */
+        inc[3 * nc + c] = dt * (div_u_pl ? div_u_pl[c] : 0.0);         /* pl. volume */
+        inc[4 * nc + c] = dt * e[1];                                   /* power law  */
+        inc[5 * nc + c] = dt * e[2];                                   /* exponential*/
+        inc[6 * nc + c] = dt * e[3];                                   /* linear     */
+        inc[7 * nc + c] = dt * e[4];                                   /* GBS        */
+    }
+    if (bad >= 0) { free(inc); return bad; }
+    for (int k = 0; k < n; k++) {
+        double dE[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (phase[k] != -1) {
+            int j = (int)ceil((x[k] - g->xc[0]) / dx) - 1;
+            if (j < 0) j = 0;
+            if (j > Nx - 2) j = Nx - 2;
+            int i = (int)ceil((z[k] - g->zc[0]) / dz) - 1;
+            if (i < 0) i = 0;
+            if (i > Nz - 2) i = Nz - 2;
+            const double dxm = x[k] - g->xc[j], dzm = z[k] - g->zc[i];
+            const int nd[4] = {j + i * Nx, j + i * Nx + 1, j + (i + 1) * Nx, j + (i + 1) * Nx + 1};
+            const double w[4] = {(1.0 - dxm / dx) * (1.0 - dzm / dz), (dxm / dx) * (1.0 - dzm / dz),
+                                 (1.0 - dxm / dx) * (dzm / dz), (dxm / dx) * (dzm / dz)};
+            double sumW = 0.0;
+            for (int q = 0; q < 4; q++) {
+                const signed char t = g->BCp_t[nd[q]];
+                if (t == 30 || t == 31) continue;
+                for (int f = 0; f < 8; f++) dE[f] += w[q] * inc[(size_t)f * nc + nd[q]];
+                sumW += w[q];
+            }
+            if (sumW > 1e-13)
+                for (int f = 0; f < 8; f++) dE[f] /= sumW;
+        }
+        for (int f = 0; f < 8; f++)
+            if (strain[f]) strain[f][k] += dE[f];
+    }
+    free(inc);
+    return -1;
+}
+
+/* Stress rotation of UpdateParticleStress (RheologyParticles.c:1965-1988) on
+ * the active markers with their spin wxzm: mode 1 the Jaumann rate, mode 2
+ * the rotation by the angle dt*wxzm. */
+void orc_rotate_stress(int mode, double dt, int n, const int *phase, const double *wxzm, double *sxxd, double *szzd,
+                       double *sxz) {
+    for (int k = 0; k < n; k++) {
+        if (phase[k] == -1) continue;
+        const double txx = sxxd[k], tzz = szzd[k], txz = sxz[k];
+        if (mode == 1) {
+            sxxd[k] -= dt * 2.0 * txz * wxzm[k];
+            szzd[k] -= -dt * 2.0 * txz * wxzm[k];
+            sxz[k] -= dt * (tzz - txx) * wxzm[k];
+        } else if (mode == 2) {
+            const double a = dt * wxzm[k], c = cos(a), s = sin(a);
+            sxxd[k] = (txx * c + txz * s) * c + (txz * c + tzz * s) * s;
+            szzd[k] = -(-txx * s + txz * c) * s + (-txz * s + tzz * c) * c;
+            sxz[k] = -(txx * c + txz * s) * s + (txz * c + tzz * s) * c;
+        }
+    }
+}

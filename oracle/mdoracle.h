@@ -176,6 +176,31 @@ int orc_killer_solve_newton(int neq_mom, int neq_cont,
                             const double *rhs_mom, const double *rhs_cont, double *du, double *dp,
                             orc_killer_opts *opts);
 
+/* AccumulatedStrainII (RheologyParticles.c:346-649): eII[0..6] = eII_pl,
+ * eII_pwl, eII_exp, eII_lin, eII_gbs, eII_el, eII_cst on the centres (NULL =
+ * 0), strain[0..7] = strain, strain_el, strain_pl, strain_pl_vol,
+ * strain_pwl, strain_exp, strain_lin, strain_gbs of the markers (NULL =
+ * skipped).  Returns -1, or the first cell with a negative increment. */
+int orc_accumulated_strain(const orc_grid *g, double dt, const double *const *eII, const double *div_u_pl, int n,
+                           const double *x, const double *z, const int *phase, double *const *strain);
+
+/* Stress rotation of UpdateParticleStress (RheologyParticles.c:1965-1988):
+ * mode 1 Jaumann rate, mode 2 analytical rotation, with the markers' spin. */
+void orc_rotate_stress(int mode, double dt, int n, const int *phase, const double *wxzm, double *sxxd, double *szzd,
+                       double *sxz);
+
+/* EnergyDirectSolve (ThermalRoutines.c:74-470) matrix for a box without
+ * air cells: CSR rows in cell order (<= 5 entries), boundary rhs bbc. */
+void orc_energy_matrix(const orc_grid *g, double dt, const double *rho_n, const double *Cp, const double *kx,
+                       const double *kz, const signed char *BCT_t, const double *BCT_v, int *Ap, int *Aj, double *Ax,
+                       double *bbc);
+/* One implicit energy step with SolveThermalPCG (ThermalSolver.c:338-480):
+ * T updated in place; returns the PCG iterations (or the reference's
+ * negative failure codes). */
+int orc_energy_step(const orc_grid *g, double dt, const double *rho_n, const double *Cp, const double *Qr,
+                    const double *Wdiss, const double *Ha, const double *kx, const double *kz, const signed char *BCT_t,
+                    const double *BCT_v, int max_its, double rel_tol, double *T);
+
 /* Material parameters of one phase (mat_prop of mdoodz.h, the subset used
  * by ViscosityConcise below; units are the model's scaled units). */
 typedef struct {

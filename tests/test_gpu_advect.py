@@ -121,3 +121,54 @@ def test_g2p_and_marker_updates_bit_exact(oracle_mod, n):
     got = [a.copy() for a in mk]
     ctx.update_particle_stress(*grid, x, z, ph, *got)
     assert all(np.array_equal(a, b) for a, b in zip(got, ref))
+
+
+def test_accumulated_strain_and_rotation_match_oracle(oracle_mod):
+    """AccumulatedStrainII and the stress rotation on the GPU against the
+    oracle: bit-exact for the strain accumulation and the Jaumann rate (same
+    arithmetic, no contraction), 1e-15 relative for the analytical rotation
+    (device cos/sin against glibc's); the reference's own sequence (the spin
+    P2Mastah of a zero marker array, then rotation with wxzm = 0) leaves the
+    stresses unchanged and zeroes the vertex spin."""
+    O = oracle_mod
+    n = 41
+    g = O.Grid(n, n + 6, 0.0, 1.0, -1.3, 0.0)
+    g.pure_shear(1.0)
+    ctx = mdgpu.Context(n, n + 6, 0.0, 1.0, -1.3, 0.0)
+    ctx.set_bc(g.BCu_t, g.BCu_v, g.BCv_t, g.BCv_v, g.BCp_t, g.BCg_t)
+    x, z = S.put_markers(g.xg, g.zg, g.dx, g.dz, 4, 4, seed=7)
+    ph = np.zeros(x.size, np.int32)
+    ph[::13] = -1
+    rng = np.random.default_rng(3)
+    nc = (n - 1) * (n + 5)
+    rates = {k: np.abs(rng.standard_normal(nc)) for k in O.STRAIN_RATES[:6]}
+    strains = {k: rng.standard_normal(x.size) for k in O.STRAINS}
+    divpl = rng.standard_normal(nc)
+    ref = O.accumulated_strain(g, 0.7, x, z, ph, rates, strains, div_u_pl=divpl)
+    got = {k: v.copy() for k, v in strains.items()}
+    ctx.accumulated_strain(0.7, x, z, ph, rates, got, div_u_pl=divpl)
+    for k in O.STRAINS:
+        assert np.array_equal(got[k], ref[k]), k
+    with pytest.raises(mdgpu.MdgError):
+        ctx.accumulated_strain(0.7, x, z, ph, {"eII_pl": -np.ones(nc)}, {"strain": np.zeros(x.size)})
+    txx, tzz, txz, w = rng.standard_normal((4, x.size))
+    for mode in (1, 2):
+        r = O.rotate_stress(mode, 0.2, ph, w, txx, tzz, txz)
+        s = [a.copy() for a in (txx, tzz, txz)]
+        ctx.rotate_particle_stress(mode, 0.2, ph, w, *s)
+        for a, b in zip(s, r):
+            if mode == 1:
+                assert np.array_equal(a, b)
+            else:
+                np.testing.assert_allclose(a, b, rtol=1e-15, atol=1e-15 * np.abs(b).max())
+    # the reference's call sequence: P2Mastah of the zero marker spin onto
+    # the vertices (flag 1), then the rotation with that zero spin
+    wxz = rng.standard_normal(n * (n + 6))
+    wxzm = np.zeros(x.size)
+    ctx.markers_set(x, z, ph)
+    ctx.p2mastah(wxzm, node_field=wxz, node_type=g.BCg_t, flag=1, avg=0, centroid=0)
+    assert np.all(wxz == 0.0)
+    s = [a.copy() for a in (txx, tzz, txz)]
+    ctx.rotate_particle_stress(2, 0.2, ph, wxzm, *s)
+    for a, b in zip(s, (txx, tzz, txz)):
+        assert np.array_equal(a, b)

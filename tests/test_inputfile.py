@@ -1,0 +1,111 @@
+"""Model-file reader (ReadInputFile and its ReadDou2 / ReadInt2 / ReadChar /
+ReadMatProps lookups, InputOutput.c:1077-1560, 1873-2350) on three of the
+reference's own test inputs, kept as fixtures under tests/golden/inputs
+(TESTS/Thermal/GaussianDiffusionL2.txt, TESTS/RheologyCreep/PowerLawCreep.txt,
+TESTS/SolViBenchmark/SolViRes101.txt), and on small texts for the lookup
+rules.  The olivine law of PowerLawCreep is checked end to end: reader,
+flow-law database and the oracle's ViscosityConcise give, back in Pa s, the
+viscosity of the published law written in SI units."""
+import os
+
+import pytest
+
+from mdgpu import flowlaws as FL
+from mdgpu import inputfile as IF
+
+HERE = os.path.join(os.path.dirname(__file__), "golden", "inputs")
+
+
+def _model(name):
+    return IF.Model.read(os.path.join(HERE, name))
+
+
+def test_gaussian_diffusion_file():
+    m = _model("GaussianDiffusionL2.txt")
+    s = m.scaling
+    assert (s.eta, s.L, s.V, s.T) == (1e22, 1e4, 1e-10, 500.0)
+    assert m["Nx"] == 51 and m["Nz"] == 51 and m["Nt"] == 5
+    assert m["xmin"] == pytest.approx(-5e4 / 1e4) and m["zmax"] == pytest.approx(5.0)
+    assert m["dt"] == pytest.approx(1e12 / s.t)
+    assert m["thermal"] == 1 and m["advection"] == 0
+    # Newton = 1 forces the line search and KillerSolver (InputOutput.c:1334-1341)
+    assert m["Newton"] == 1 and m["line_search"] == 1 and m["lin_solver"] == 2
+    assert m["penalty"] == 1e5 and m["lin_rel_div"] == 1e-9 and m["rel_tol_KSP"] == 1e-4
+    assert m["user0"] == 500.0 and m["user3"] == 10000.0
+    assert m.writer_subfolder == "GaussianDiffusionL2"
+    assert len(m.phases) == 1
+    # unset keys keep the reference defaults
+    assert m["max_its_PH"] == 10 and m["thermal_solver"] == 0 and m["rel_tol_thermal"] == 1e-8
+    assert m["max_its_thermal"] == 1000
+
+
+def test_solvi_file():
+    m = _model("SolViRes101.txt")
+    assert m["Nx"] == 101 and m["penalty"] == 1e5 and m["lin_abs_div"] == 1e-8
+    assert m["user1"] == 0.2
+    props = m.phase_props()
+    assert [p.cstv for p in props] == [1, 1]
+    assert props[0].eta0 * m.scaling.eta == pytest.approx(1.0)
+    assert props[1].eta0 * m.scaling.eta == pytest.approx(1000.0)
+    opts = m.solver_opts()
+    assert opts.method == 1 and opts.penalty == 1e5 and opts.lin_rel_div == 1e-8
+
+
+def test_power_law_creep_olivine(oracle_mod):
+    O = oracle_mod
+    m = _model("PowerLawCreep.txt")
+    s = m.scaling
+    ph = m.phases[0]
+    assert ph.pwlv == 40 and ph.pwl["n"] == 3.5
+    assert ph.pwl["V"] == pytest.approx(11.0e-6 / s.L ** 3)
+    pp = m.phase_props()[0]
+    assert pp.pwlv == 1 and pp.cstv == 0 and pp.F_pwl == pytest.approx(FL.correction_factor(1, 3.5))
+    md = m.rheo_params()
+    orc_md = O.rheo_model(step=md.step, elastic=md.elastic, dt=md.dt, R=md.R, zeroC=md.zeroC,
+                          min_eta=md.min_eta, max_eta=md.max_eta, eta_tol=md.eta_tol, Tiimin=md.Tiimin,
+                          scale_E=md.scale_E, scale_eta=md.scale_eta)
+    orc_ph = O.phase(**{k: getattr(pp, k) for k, _ in pp._fields_ if k != "beta"}, beta_for_K=pp.beta)
+    T_K, E = 600.0 + 273.15, 1e-14
+    args = (ph.G, T_K / s.T, 0.0, E / s.E, -E / s.E, 0.0)
+    # the file's cut-offs (min_eta = 1e-6, max_eta = 1e6) are divided by the
+    # viscosity scale like any other, i.e. read as Pa s: the olivine matrix
+    # is clipped to 1e6 Pa s, in the reference run as here
+    eta, *_ = O.viscosity_concise(orc_ph, orc_md, *args)
+    assert eta * s.eta == pytest.approx(1e6, rel=1e-14)
+    orc_md.max_eta = 1e50
+    eta, *_ = O.viscosity_concise(orc_ph, orc_md, *args)
+    eta_si = FL.viscosity_si(FL.POWER_LAWS[40][1:], E, T_K, 0.0, R=IF.RG)
+    assert eta * s.eta == pytest.approx(eta_si, rel=1e-10)
+    assert 1e20 < eta_si < 1e24
+
+
+def test_lookup_rules():
+    text = "\n".join([
+        "/**** top ****/",
+        "Nx=99",                       # no space: not a key line
+        "  Nz = 7",                    # leading space: not a key line
+        "dt  = 2.5e3  / a comment",
+        "penalty = 1e3xyz",
+        "Nb_phases = 2",
+        "ID = 0",
+        "rho = 3000",
+        "eta0 = 1e21",
+        "ID = 1",
+        "rho = 2500",
+        "Nz = 42",                     # first match anywhere in the file
+        "writer_subfolder = out_dir extra",
+    ]) + "\n"
+    f = IF.InputFile(text)
+    assert f.int("Nx", 10) == 10
+    assert f.int("Nz", 10) == 42
+    assert f.dou("dt", 0.0) == 2.5e3
+    assert f.dou("penalty", 0.0) == 1e3
+    assert f.mat("rho", 0, 1.0) == 3000.0 and f.mat("rho", 1, 1.0) == 2500.0
+    assert f.mat("eta0", 1, -1.0) == -1.0          # stops at the next ID / end of file
+    assert f.mat("eta0", 0, -1.0) == 1e21
+    assert f.char("writer_subfolder", "./") == "out_dir"
+    assert f.char("missing", "./") == "./"
+    with pytest.raises(ValueError):
+        f.mat("rho", 5, 1.0)
+    assert IF.c_atof("  -3.5e-2abc") == -3.5e-2 and IF.c_atof("abc") == 0.0
+    assert IF.c_atoi(" 000005") == 5 and IF.c_atoi("1e3") == 1
