@@ -244,6 +244,21 @@ int mdg_stokes_residual(mdg_ctx *ctx, const double *sxxd, const double *szzd, co
                         const double *p_corr, const double *div, const double *roger_x,
                         const double *roger_z, const double *d, double *F, double norms[3]);
 
+/* mdg_stokes_residual without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops): no counterpart in the reference.  Same arguments and results as
+ * mdg_stokes_residual, but every array must be device memory (norms is a
+ * host array of 3) and the reduction runs in a static device scratch of
+ * MDG_STATIC_SCRATCH_DOUBLES doubles that each GPU holds until the process
+ * ends.  All calls on one GPU share it: calls that overlap on one GPU race.
+ * Grids needing more than the scratch (3 partial sums per 256 unknowns,
+ * about 3800 x 3800 cells) are refused with MDG_ERR_ARG. */
+#define MDG_STATIC_SCRATCH_DOUBLES (1 << 19)
+int mdg_stokes_residual_static(mdg_ctx *ctx, const double *sxxd, const double *szzd,
+                               const double *sxz, const double *p_corr, const double *div,
+                               const double *roger_x, const double *roger_z, const double *d,
+                               double *F, double norms[3]);
+
 /* --------------------------------------------------------------- Stokes solve
  * Replaces KillerSolver (Solvers.c:1054-1454; prototype mdoodz-private.h:494;
  * called from SolveStokesDefectDecoupled, StokesRoutines.c:845-848) for the Picard

@@ -64,7 +64,8 @@ extern "C" int mdg_create(const mdg_grid_desc *d, mdg_ctx **out) {
     if ((rc = upload(c, c->xg, c->h_xg)) || (rc = upload(c, c->zg, c->h_zg)) ||
         (rc = upload(c, c->xc, c->h_xc)) || (rc = upload(c, c->zc, c->h_zc)) ||
         (rc = upload(c, c->xvz, c->h_xvz)) || (rc = upload(c, c->zvx, c->h_zvx)) ||
-        (rc = (c->flag_dev.reserve(16) == hipSuccess) ? MDG_OK : MDG_ERR_HIP)) {
+        (rc = (c->flag_dev.reserve(16) == hipSuccess) ? MDG_OK : MDG_ERR_HIP) ||
+        (rc = mdg_static_scratch_init(c))) {
         mdg_destroy(c);
         return rc;
     }

@@ -88,6 +88,7 @@ struct mdg_ctx {
     struct advect_scratch *advect = nullptr;
     struct g2p_scratch *g2p = nullptr;
     struct thermal_scratch *thermal = nullptr;
+    double      *static_scratch = nullptr;   // this device's static scratch (mdg_stokes_residual_static)
     // runtime
     hipStream_t  stream = nullptr;
     hipEvent_t   ev0 = nullptr, ev1 = nullptr;
@@ -95,6 +96,7 @@ struct mdg_ctx {
     std::string  err;
 };
 
+#define MDG_MAX_DEVICES 64
 int mdg_fail(mdg_ctx *c, int code, const char *fmt, ...);
 #define HIPCHK(ctx, expr)                                                                   \
     do {                                                                                    \

@@ -1,5 +1,7 @@
 // Stokes operator, residual and strain-rate kernels (FP64) and the DOF
 // numbering of the reference (StokesRoutines.c:1162-1220).
+#include <mutex>
+
 #include "stokes_internal.h"
 
 // ------------------------------------------------------------------ numbering
@@ -455,6 +457,76 @@ extern "C" int mdg_stokes_residual(mdg_ctx *c, const double *sxxd, const double 
     double s[3] = {0, 0, 0};
     for (int b = 0; b < nb; b++)
         for (int q = 0; q < 3; q++) s[q] += part[3 * b + q];
+    norms[0] = sqrt(s[0] / c->ndof_norm[0]);
+    norms[1] = sqrt(s[1] / c->ndof_norm[1]);
+    norms[2] = sqrt(s[2] / (double)c->neq_cont);
+    if (std::isnan(norms[0]) || std::isnan(norms[1]) || std::isnan(norms[2]))
+        return mdg_fail(c, MDG_ERR_NAN, "NaN residual");
+    return MDG_OK;
+}
+
+// ------------------------------------------------ allocation-free residual
+// Device scratch of mdg_stokes_residual_static: the residual norms' partial
+// sums (3 per block) and their totals.  One copy per GPU, held until the
+// process ends; every call on that GPU shares it, so calls that overlap on
+// one GPU race.  Host code never uses the array's name as an address: the
+// address comes from hipGetSymbolAddress, once per device, at context set-up.
+static __device__ __align__(256) double scratch[MDG_STATIC_SCRATCH_DOUBLES];
+static double *g_scratch_dev[MDG_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+int mdg_static_scratch_init(mdg_ctx *c) {
+    int dev = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (dev < 0 || dev >= MDG_MAX_DEVICES) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        HIPCHK(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return mdg_fail(c, MDG_ERR_HIP, "no device address for the static scratch");
+        g_scratch_dev[dev] = static_cast<double *>(p);
+    }
+    c->static_scratch = g_scratch_dev[dev];
+    return MDG_OK;
+}
+
+// totals of the 3 x nb partial sums, each summed in block order (the order
+// of mdg_stokes_residual's host loop, so both give the same bits)
+__global__ void k_sum3_ordered(int nb, const double *__restrict__ part, double *__restrict__ tot) {
+    const int q = threadIdx.x;
+    if (q >= 3) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += part[3 * b + q];
+    tot[q] = s;
+}
+
+extern "C" int mdg_stokes_residual_static(mdg_ctx *c, const double *sxxd, const double *szzd,
+                                          const double *sxz, const double *p_corr, const double *div,
+                                          const double *roger_x, const double *roger_z,
+                                          const double *d, double *F, double norms[3]) {
+    if (!c || !sxxd || !szzd || !sxz || !p_corr || !div || !roger_x || !roger_z || !F || !norms)
+        return mdg_fail(c, MDG_ERR_ARG, "null array");
+    if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
+    if (!c->static_scratch) return mdg_fail(c, MDG_ERR_STATE, "no static scratch on this device");
+    const double *in[9] = {sxxd, szzd, sxz, p_corr, div, roger_x, roger_z, d, F};
+    for (int q = 0; q < 9; q++)
+        if (in[q] && !mdg_is_device_ptr(in[q]))
+            return mdg_fail(c, MDG_ERR_ARG, "mdg_stokes_residual_static takes device arrays only (argument %d)", q + 2);
+    const sgrid g = c->sg;
+    const int n = nvx(g) + nvz(g) + ncell(g);
+    const int nb = mdg_grid1d(n, 256);
+    const long need = 3L * nb + 3;
+    if (need > MDG_STATIC_SCRATCH_DOUBLES)
+        return mdg_fail(c, MDG_ERR_ARG, "grid too large for the static scratch: %ld partial sums, at most %d", need,
+                        MDG_STATIC_SCRATCH_DOUBLES);
+    double *part = c->static_scratch, *tot = part + 3L * nb;
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, c->stream, g, sxxd, szzd, sxz, p_corr, div, roger_x,
+                       roger_z, d, c->eqn_u.p, c->eqn_v.p, c->eqn_p.p, F, part);
+    hipLaunchKernelGGL(k_sum3_ordered, dim3(1), dim3(64), 0, c->stream, nb, (const double *)part, tot);
+    HIPCHK(c, hipGetLastError());
+    double s[3];
+    HIPCHK(c, hipMemcpyAsync(s, tot, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     norms[0] = sqrt(s[0] / c->ndof_norm[0]);
     norms[1] = sqrt(s[1] / c->ndof_norm[1]);
     norms[2] = sqrt(s[2] / (double)c->neq_cont);

@@ -12,3 +12,6 @@ int mdg_apply_K(mdg_ctx *c, const double *u, const double *v, const double *p, c
                 const double *es, double *yu, double *yv, double *yp);
 int mdg_stage_coeffs(mdg_ctx *c, const double *D11, const double *D22, const double *D33);
 int mdg_reserve_grid_vectors(mdg_ctx *c);
+// Looks up this device's copy of the static scratch of the allocation-free
+// entry points (hipGetSymbolAddress, once per device) for the context.
+int mdg_static_scratch_init(mdg_ctx *c);

@@ -165,6 +165,7 @@ def _extra_signatures(L):
     L.mdg_strain_rates.argtypes = [vp] * 7
     L.mdg_eval_rhs.argtypes = [vp, C.c_double, C.c_double, vp, vp, vp, vp]
     L.mdg_stokes_residual.argtypes = [vp] * 11
+    L.mdg_stokes_residual_static.argtypes = [vp] * 11
     L.mdg_killer_solver.argtypes = [vp, C.POINTER(SolverOpts), vp, vp, vp, vp, vp, vp]
     L.mdg_time_smoother.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mdg_c2v.argtypes = [vp, C.c_int, vp, vp]
@@ -686,6 +687,18 @@ class Context:
                                                 ptr(p_corr), ptr(div), ptr(roger_x),
                                                 ptr(roger_z), ptr(d), ptr(F), ptr(norms)))
         return F, norms
+
+    def stokes_residual_static(self, sxxd, szzd, sxz, p_corr, div, roger_x, roger_z, F, d=None):
+        """mdg_stokes_residual_static: the residual without any allocation;
+        every array a DeviceArray (F receives the residual), returns norms."""
+        arrs = (sxxd, szzd, sxz, p_corr, div, roger_x, roger_z, F) + ((d,) if d is not None else ())
+        if not all(isinstance(a, DeviceArray) for a in arrs):
+            raise TypeError("stokes_residual_static takes DeviceArray arguments only")
+        norms = np.zeros(3)
+        self._check(self._L.mdg_stokes_residual_static(self.h, ptr(sxxd), ptr(szzd), ptr(sxz), ptr(p_corr),
+                                                        ptr(div), ptr(roger_x), ptr(roger_z), ptr(d), ptr(F),
+                                                        ptr(norms)))
+        return norms
 
     def _solved(self, rc, o, accept_diverged):
         # MDG_ERR_DIVERGED leaves the last iterate in sol; the reference's
