@@ -31,6 +31,8 @@ extern "C" {
 #define MDG_ERR_NAN         -7   /* NaN residual (StokesRoutines.c:947-953) */
 #define MDG_ERR_INCOMPRESSIBLE -8 /* continuity residual above KillerSolver's
                                     a-posteriori bound (Solvers.c:1333-1336)  */
+#define MDG_ERR_NO_MARKERS  -9   /* re-seeding found no marker to copy (the
+                                    reference exits, ParticleRoutines.c:3812) */
 
 typedef struct mdg_ctx mdg_ctx;
 
@@ -300,7 +302,14 @@ typedef struct {
     double poisson_tol;    /* BFBt pressure-Poisson CG tolerance (1e-3)        */
     int    poisson_max_its;/* (30)                                             */
     int    smoother;       /* 0 fused three-term Chebyshev kernels, 1 the
-                              residual/update two-kernel form (0)              */
+                              residual/update two-kernel form, 2 the two-kernel
+                              form preconditioned by vertex-patch additive
+                              Schwarz (exact solves on the four faces around
+                              each vertex, which hold the divergence-free
+                              modes the penalty term leaves to the smoother)
+                              instead of the diagonal, 3 the same patch
+                              preconditioning fused into one kernel per step
+                              with the patch systems solved in the kernel (0) */
     int    cheb_degree_coarse; /* smoothing degree on every level below the
                               finest (16)                                      */
     double cheb_range;     /* the Chebyshev smoother targets eigenvalues of
@@ -403,12 +412,41 @@ int mdg_update_particle_stress(mdg_ctx *ctx, const double *sxxd, const double *s
                                const double *x, const double *z, const int *phase, double *msxxd,
                                double *mszzd, double *msxz);
 
+/* UpdateParticleStress with model.subgrid_diffusion = 2 (RheologyParticles.c:
+ * 1811-1895): the old grid stresses (sxxd0, szzd0 on the cells, sxz0 on the
+ * vertices) and the vertex viscosity eta_phys_s at the markers; on active
+ * markers the subgrid relaxation dtau_s = -(tau_m - tau0_m)
+ * (1 - exp(-dt / (eta_m / G[phase]))); dtau_s onto the grid (P2Mastah,
+ * flag 1, arithmetic, interp_stencil, BCp / BCg node tags); the remaining
+ * grid increments (tau - tau0) - dtau_s on non-air nodes back at the
+ * markers; tau_m += dtau_s + dtau_r in place.  G_phase holds mat_prop G of
+ * the nph phases.  The markers must be the ones set by mdg_markers_set. */
+int mdg_update_particle_stress_subgrid(mdg_ctx *ctx, double dt, const double *G_phase, int nph, int interp_stencil,
+                                       const double *sxxd, const double *szzd, const double *sxz,
+                                       const double *sxxd0, const double *szzd0, const double *sxz0,
+                                       const double *eta_phys_s, int n, const double *x, const double *z,
+                                       const int *phase, double *msxxd, double *mszzd, double *msxz);
+
 /* UpdateParticlePressure (RheologyParticles.c:1621-1704; called after the
  * solve) with model.subgrid_diffusion < 2: dp = p_in - p0_n on the cells not
  * tagged 30 (mesh->dp, returned when dp is not NULL), interpolated as above
  * and added to the marker pressure P (particles->P) in place. */
 int mdg_update_particle_pressure(mdg_ctx *ctx, const double *p_in, const double *p0_n, double *dp, int n,
                                  const double *x, const double *z, const int *phase, double *P);
+
+/* UpdateParticlePressure with model.subgrid_diffusion >= 2
+ * (RheologyParticles.c:1621-1695): dp = p_in - p0_n on non-air cells (dp may
+ * be NULL); eta_phys_n and p0_n at the markers; dP_s = -(P_m - P0_m)
+ * (1 - exp(-dt / (eta_m / beta[phase]))) on active markers (the reference's
+ * relaxation time, kept as written); dP_s onto the centres (P2Mastah, flag
+ * 1, 1-cell stencil as the reference hard-codes), the remaining dp - dP_s
+ * back at the markers, P += dP_s + dP_r in place.  beta_phase holds mat_prop
+ * bet of the nph phases.  The reference leaves BCt-30 cells out of the old
+ * pressure; the BCp tags used here mark the same air cells. */
+int mdg_update_particle_pressure_subgrid(mdg_ctx *ctx, double dt, const double *beta_phase, int nph,
+                                         const double *p_in, const double *p0_n, const double *eta_phys_n,
+                                         double *dp, int n, const double *x, const double *z, const int *phase,
+                                         double *P);
 
 /* AccumulatedStrainII (RheologyParticles.c:346-649; called at
  * Main_DOODZ.c:1251): increments dt * eII on the centres (the total being
@@ -437,6 +475,28 @@ int mdg_accumulated_strain(mdg_ctx *ctx, double dt, const double *const *eII, co
  * wxzm = 0. */
 int mdg_rotate_particle_stress(mdg_ctx *ctx, int mode, double dt, int n, const int *phase, const double *wxzm,
                                double *sxxd, double *szzd, double *sxz);
+
+/* The re-seeding of CountPartCell_OLD (ParticleRoutines.c:3224-4050,
+ * reseed_mode = 0, reseed_markers = 1; prototype mdoodz-private.h; called at
+ * Main_DOODZ.c:1401) for a box without free surface or periodic x: fine
+ * cells of dx/2 x dz/2 (the box plus one coarse cell of margin) count the
+ * active markers nearest to their centres; every interior fine cell of a
+ * non-air coarse cell (cell_tag, the reference's BCt.type on the cells:
+ * 30 = air; NULL = no air) holding fewer than floor(min_part_cell / 4) markers
+ * gets one new marker at its centre that copies the phase of the closest
+ * marker of the 5 x 5 fine cells around it (FindClosestPhase2, ties to the
+ * reference's list order).  New markers, in x-major fine-cell order, take
+ * the inactive slots (phase -1) in increasing order, then slots n, n+1, ...
+ * x, z, phase hold capacity entries (n used on entry); new_slot and new_src
+ * (at least 4 Ncx Ncz entries) receive each new marker's slot and source
+ * marker, so the caller copies its other carried fields (AssignMarker-
+ * Properties, ParticleRoutines.c:695-766) and interpolates the grid fields to
+ * the new slots (mdg_g2p_centroids / mdg_g2p_vertices).  *n_out is the new
+ * marker count.  The phase fractions the reference computes first are
+ * mdg_p2mastah with prop = 1.  Errors: MDG_ERR_NO_MARKERS (empty
+ * neighbourhood), MDG_ERR_ARG (capacity exceeded); nothing is written then. */
+int mdg_count_part_cell(mdg_ctx *ctx, int min_part_cell, const signed char *cell_tag, int n, int capacity, double *x,
+                        double *z, int *phase, int *new_slot, int *new_src, int *n_new, int *n_out);
 
 /* ---------------------------------------------------------------- energy
  * One implicit energy step: the matrix of EnergyDirectSolve

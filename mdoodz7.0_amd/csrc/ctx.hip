@@ -95,6 +95,7 @@ extern "C" void mdg_destroy(mdg_ctx *c) {
     mdg_advect_release(c);
     mdg_g2p_release(c);
     mdg_thermal_release(c);
+    mdg_reseed_release(c);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -235,6 +235,56 @@ cgrid make_cgrid(const mdg_ctx *c) {
     return cgrid{c->Nx - 1, c->Nz - 1, c->dx, c->dz, c->xvz.p, c->zvx.p, c->BCp_t.p};
 }
 
+// Subgrid relaxation of UpdateParticleStress / UpdateParticlePressure
+// (RheologyParticles.c:1845-1850, :1660-1666): on active markers
+//   ms = -(m - m0) (1 - exp(-d dt / (eta_m / coef[phase]))),  d = 1.
+struct relax3 {
+    const double *m[3], *m0[3];
+    double *ms[3];
+};
+__global__ void k_subgrid_relax(int n, int nf, double dt, const double *__restrict__ coef, const int *__restrict__ phase,
+                                const double *__restrict__ etam, relax3 a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int p = phase[k];
+    if (p == -1) {
+        for (int q = 0; q < nf; q++) a.ms[q][k] = 0.0;
+        return;
+    }
+    const double d = 1.0, trel = etam[k] / coef[p];
+    for (int q = 0; q < nf; q++) a.ms[q][k] = -(a.m[q][k] - a.m0[q][k]) * (1.0 - exp(-d * dt / trel));
+}
+// remaining grid increments (tau - tau0) - dtau_s on non-air nodes
+__global__ void k_stress_remaining(int nc, int nv, const signed char *__restrict__ tp, const signed char *__restrict__ tg,
+                                   const double *__restrict__ sxx, const double *__restrict__ szz,
+                                   const double *__restrict__ sxz, const double *__restrict__ sxx0,
+                                   const double *__restrict__ szz0, const double *__restrict__ sxz0,
+                                   const double *__restrict__ gxx, const double *__restrict__ gzz,
+                                   const double *__restrict__ gxz, double *__restrict__ rxx, double *__restrict__ rzz,
+                                   double *__restrict__ rxz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nc) {
+        const bool on = tp[i] != 30 && tp[i] != 31;
+        rxx[i] = on ? (sxx[i] - sxx0[i]) - gxx[i] : 0.0;
+        rzz[i] = on ? (szz[i] - szz0[i]) - gzz[i] : 0.0;
+    } else if (i < nc + nv) {
+        const int j = i - nc;
+        rxz[j] = (tg[j] != 30) ? (sxz[j] - sxz0[j]) - gxz[j] : 0.0;
+    }
+}
+// old pressure on non-air cells, and later the remaining increment dp - dP_s
+__global__ void k_masked(int nc, const signed char *__restrict__ tag, const double *__restrict__ a,
+                         const double *__restrict__ b, double *__restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc) out[c] = (tag[c] != 30) ? (b ? a[c] - b[c] : a[c]) : 0.0;
+}
+// m = m + ms + mr on active markers
+__global__ void k_add_two(int n, const int *__restrict__ phase, const double *__restrict__ ms,
+                          const double *__restrict__ mr, double *__restrict__ m) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && phase[k] != -1) m[k] = m[k] + ms[k] + mr[k];
+}
+
 }  // namespace
 
 // staging buffers for host callers, one set per context (freed with it)
@@ -243,6 +293,7 @@ struct g2p_scratch {
     dbuf<int> ph;
     dbuf<double> in[6], inc, m[3];   // stress update: grid inputs, increments, marker stresses
     dbuf<double> e[8], ms[8];        // accumulated strain: grid rates, marker strains (host callers)
+    dbuf<double> sub[10], gsub[6], tab;   // subgrid diffusion: marker and grid temporaries, phase table
 };
 
 static g2p_scratch &g2p_of(mdg_ctx *c) {
@@ -259,6 +310,9 @@ void mdg_g2p_release(mdg_ctx *c) {
     for (auto &b : s->m) b.release();
     for (auto &b : s->e) b.release();
     for (auto &b : s->ms) b.release();
+    for (auto &b : s->sub) b.release();
+    for (auto &b : s->gsub) b.release();
+    s->tab.release();
     delete s;
     c->g2p = nullptr;
 }
@@ -485,4 +539,167 @@ extern "C" int mdg_rotate_particle_stress(mdg_ctx *c, int mode, double dt, int n
     for (int q = 0; q < 3; q++)
         if (dm[q] != user[q]) HIPCHK(c, hipMemcpyAsync(user[q], dm[q], (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
     return timed_end(c);
+}
+
+// Marker arrays: device pointers used in place, host arrays staged in and
+// copied back by the caller of this helper.
+static int stage_markers_rw(mdg_ctx *c, dbuf<double> *tmp, double *const *user, int nq, int n, double **dev) {
+    for (int q = 0; q < nq; q++) {
+        if (mdg_is_device_ptr(user[q])) {
+            dev[q] = user[q];
+        } else {
+            HIPCHK(c, tmp[q].reserve((size_t)(n > 0 ? n : 1)));
+            if (n > 0) HIPCHK(c, hipMemcpyAsync(tmp[q].p, user[q], (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+            dev[q] = tmp[q].p;
+        }
+    }
+    return MDG_OK;
+}
+
+struct local_timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~local_timer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+extern "C" int mdg_update_particle_stress_subgrid(mdg_ctx *c, double dt, const double *G_phase, int nph,
+                                                  int interp_stencil, const double *sxxd, const double *szzd,
+                                                  const double *sxz, const double *sxxd0, const double *szzd0,
+                                                  const double *sxz0, const double *eta_phys_s, int n,
+                                                  const double *x, const double *z, const int *phase, double *msxxd,
+                                                  double *mszzd, double *msxz) {
+    if (!c || !G_phase || nph < 1 || !sxxd || !szzd || !sxz || !sxxd0 || !szzd0 || !sxz0 || !eta_phys_s || !x || !z ||
+        !phase || !msxxd || !mszzd || !msxz || n < 0)
+        return mdg_fail(c, MDG_ERR_ARG, "bad argument");
+    if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
+    if (c->n_markers != n)
+        return mdg_fail(c, MDG_ERR_STATE, "the subgrid step needs these markers set (mdg_markers_set): %d set, %d given",
+                        c->n_markers, n);
+    if (n == 0) return MDG_OK;
+    g2p_scratch &s = g2p_of(c);
+    const int nc = (c->Nx - 1) * (c->Nz - 1), nv = c->Nx * c->Nz;
+    const double *src[7] = {sxxd, szzd, sxz, sxxd0, szzd0, sxz0, eta_phys_s}, *d[7];
+    const size_t len[7] = {(size_t)nc, (size_t)nc, (size_t)nv, (size_t)nc, (size_t)nc, (size_t)nv, (size_t)nv};
+    int rc;
+    for (int q = 0; q < 6; q++)
+        if ((rc = mdg_stage_in(c, src[q], len[q], s.in[q], &d[q]))) return rc;
+    if ((rc = mdg_stage_in(c, src[6], len[6], s.gsub[5], &d[6]))) return rc;
+    const double *dG, *dx_, *dz_;
+    const int *dph;
+    if ((rc = mdg_stage_in(c, G_phase, (size_t)nph, s.tab, &dG)) || (rc = mdg_stage_in(c, x, (size_t)n, s.x, &dx_)) ||
+        (rc = mdg_stage_in(c, z, (size_t)n, s.z, &dz_)) || (rc = mdg_stage_in(c, phase, (size_t)n, s.ph, &dph)))
+        return rc;
+    double *user[3] = {msxxd, mszzd, msxz}, *dm[3];
+    if ((rc = stage_markers_rw(c, s.m, user, 3, n, dm))) return rc;
+    for (int q = 0; q < 10; q++) HIPCHK(c, s.sub[q].reserve((size_t)n));
+    double *m0[3] = {s.sub[0].p, s.sub[1].p, s.sub[2].p}, *etam = s.sub[3].p;
+    double *ms[3] = {s.sub[4].p, s.sub[5].p, s.sub[6].p}, *mr[3] = {s.sub[7].p, s.sub[8].p, s.sub[9].p};
+    const size_t gl[5] = {(size_t)nc, (size_t)nc, (size_t)nv, (size_t)nc, (size_t)nc};
+    for (int q = 0; q < 5; q++) HIPCHK(c, s.gsub[q].reserve(gl[q]));
+    HIPCHK(c, s.inc.reserve((size_t)nv));
+    double *gs[3] = {s.gsub[0].p, s.gsub[1].p, s.gsub[2].p}, *gr[3] = {s.gsub[3].p, s.gsub[4].p, s.inc.p};
+    const cgrid gc = make_cgrid(c);
+    const vgrid gv = {c->Nx, c->Nz, c->dx, c->dz, c->xg.p, c->zg.p, c->BCg_t.p};
+    local_timer t;
+    HIPCHK(c, hipEventCreate(&t.a));
+    HIPCHK(c, hipEventCreate(&t.b));
+    HIPCHK(c, hipEventRecord(t.a, c->stream));
+    const dim3 gm(mdg_grid1d(n, 256)), bk(256);
+    // old stresses and the vertex viscosity at the markers
+    hipLaunchKernelGGL(k_g2p_centroids, gm, bk, 0, c->stream, gc, d[3], n, dx_, dz_, dph, m0[0]);
+    hipLaunchKernelGGL(k_g2p_centroids, gm, bk, 0, c->stream, gc, d[4], n, dx_, dz_, dph, m0[1]);
+    hipLaunchKernelGGL(k_g2p_vertices, gm, bk, 0, c->stream, gv, d[5], n, dx_, dz_, dph, m0[2]);
+    hipLaunchKernelGGL(k_g2p_vertices, gm, bk, 0, c->stream, gv, d[6], n, dx_, dz_, dph, etam);
+    relax3 ra{{dm[0], dm[1], dm[2]}, {m0[0], m0[1], m0[2]}, {ms[0], ms[1], ms[2]}};
+    hipLaunchKernelGGL(k_subgrid_relax, gm, bk, 0, c->stream, n, 3, dt, dG, dph, (const double *)etam, ra);
+    HIPCHK(c, hipGetLastError());
+    // subgrid increments onto the grid (P2Mastah, flag 1, arithmetic)
+    for (int q = 0; q < 3; q++) {
+        HIPCHK(c, hipMemsetAsync(gs[q], 0, (q < 2 ? (size_t)nc : (size_t)nv) * 8, c->stream));
+        if ((rc = mdg_p2mastah(c, ms[q], n, gs[q], q < 2 ? c->BCp_t.p : c->BCg_t.p, 1, 0, 0, q < 2 ? 1 : 0,
+                               interp_stencil, 0, nullptr)))
+            return rc;
+    }
+    hipLaunchKernelGGL(k_stress_remaining, dim3(mdg_grid1d(nc + nv, 256)), bk, 0, c->stream, nc, nv,
+                       (const signed char *)c->BCp_t.p, (const signed char *)c->BCg_t.p, d[0], d[1], d[2], d[3], d[4],
+                       d[5], (const double *)gs[0], (const double *)gs[1], (const double *)gs[2], gr[0], gr[1], gr[2]);
+    hipLaunchKernelGGL(k_g2p_centroids, gm, bk, 0, c->stream, gc, (const double *)gr[0], n, dx_, dz_, dph, mr[0]);
+    hipLaunchKernelGGL(k_g2p_centroids, gm, bk, 0, c->stream, gc, (const double *)gr[1], n, dx_, dz_, dph, mr[1]);
+    hipLaunchKernelGGL(k_g2p_vertices, gm, bk, 0, c->stream, gv, (const double *)gr[2], n, dx_, dz_, dph, mr[2]);
+    for (int q = 0; q < 3; q++)
+        hipLaunchKernelGGL(k_add_two, gm, bk, 0, c->stream, n, dph, (const double *)ms[q], (const double *)mr[q], dm[q]);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(t.b, c->stream));
+    for (int q = 0; q < 3; q++)
+        if (dm[q] != user[q]) HIPCHK(c, hipMemcpyAsync(user[q], dm[q], (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms_ = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms_, t.a, t.b));
+    c->last_ms = ms_;
+    return MDG_OK;
+}
+
+extern "C" int mdg_update_particle_pressure_subgrid(mdg_ctx *c, double dt, const double *beta_phase, int nph,
+                                                    const double *p_in, const double *p0_n, const double *eta_phys_n,
+                                                    double *dp, int n, const double *x, const double *z,
+                                                    const int *phase, double *P) {
+    if (!c || !beta_phase || nph < 1 || !p_in || !p0_n || !eta_phys_n || !x || !z || !phase || !P || n < 0)
+        return mdg_fail(c, MDG_ERR_ARG, "bad argument");
+    if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
+    if (c->n_markers != n)
+        return mdg_fail(c, MDG_ERR_STATE, "the subgrid step needs these markers set (mdg_markers_set): %d set, %d given",
+                        c->n_markers, n);
+    g2p_scratch &s = g2p_of(c);
+    const int nc = (c->Nx - 1) * (c->Nz - 1);
+    const double *dpin, *dp0, *deta, *dB, *dx_, *dz_;
+    const int *dph;
+    int rc;
+    if ((rc = mdg_stage_in(c, p_in, (size_t)nc, s.F, &dpin)) || (rc = mdg_stage_in(c, p0_n, (size_t)nc, s.p0, &dp0)) ||
+        (rc = mdg_stage_in(c, eta_phys_n, (size_t)nc, s.gsub[5], &deta)) ||
+        (rc = mdg_stage_in(c, beta_phase, (size_t)nph, s.tab, &dB)) || (rc = mdg_stage_in(c, x, (size_t)n, s.x, &dx_)) ||
+        (rc = mdg_stage_in(c, z, (size_t)n, s.z, &dz_)) || (rc = mdg_stage_in(c, phase, (size_t)n, s.ph, &dph)))
+        return rc;
+    double *ddp = dp;
+    if (!dp || !mdg_is_device_ptr(dp)) {
+        HIPCHK(c, s.dp.reserve((size_t)nc));
+        ddp = s.dp.p;
+    }
+    double *user[1] = {P}, *dP[1];
+    if ((rc = stage_markers_rw(c, s.m, user, 1, n, dP))) return rc;
+    for (int q = 0; q < 4; q++) HIPCHK(c, s.sub[q].reserve((size_t)(n > 0 ? n : 1)));
+    double *etam = s.sub[0].p, *Pm0 = s.sub[1].p, *dPms = s.sub[2].p, *dPmr = s.sub[3].p;
+    for (int q = 0; q < 3; q++) HIPCHK(c, s.gsub[q].reserve((size_t)nc));
+    double *Pg0 = s.gsub[0].p, *dPgs = s.gsub[1].p, *dPgr = s.gsub[2].p;
+    const cgrid gc = make_cgrid(c);
+    local_timer t;
+    HIPCHK(c, hipEventCreate(&t.a));
+    HIPCHK(c, hipEventCreate(&t.b));
+    HIPCHK(c, hipEventRecord(t.a, c->stream));
+    const dim3 gm(mdg_grid1d(n > 0 ? n : 1, 256)), gcl(mdg_grid1d(nc, 256)), bk(256);
+    const signed char *tag = c->BCp_t.p;
+    hipLaunchKernelGGL(k_pressure_increment, gcl, bk, 0, c->stream, nc, tag, dpin, dp0, ddp);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_g2p_centroids, gm, bk, 0, c->stream, gc, deta, n, dx_, dz_, dph, etam);
+        hipLaunchKernelGGL(k_masked, gcl, bk, 0, c->stream, nc, tag, dp0, (const double *)nullptr, Pg0);
+        hipLaunchKernelGGL(k_g2p_centroids, gm, bk, 0, c->stream, gc, (const double *)Pg0, n, dx_, dz_, dph, Pm0);
+        relax3 ra{{dP[0], nullptr, nullptr}, {Pm0, nullptr, nullptr}, {dPms, nullptr, nullptr}};
+        hipLaunchKernelGGL(k_subgrid_relax, gm, bk, 0, c->stream, n, 1, dt, dB, dph, (const double *)etam, ra);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemsetAsync(dPgs, 0, (size_t)nc * 8, c->stream));
+        if ((rc = mdg_p2mastah(c, dPms, n, dPgs, tag, 1, 0, 0, 1, 1, 0, nullptr))) return rc;
+        hipLaunchKernelGGL(k_masked, gcl, bk, 0, c->stream, nc, tag, (const double *)ddp, (const double *)dPgs, dPgr);
+        hipLaunchKernelGGL(k_g2p_centroids, gm, bk, 0, c->stream, gc, (const double *)dPgr, n, dx_, dz_, dph, dPmr);
+        hipLaunchKernelGGL(k_add_two, gm, bk, 0, c->stream, n, dph, (const double *)dPms, (const double *)dPmr, dP[0]);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(t.b, c->stream));
+    if (dP[0] != P && n > 0) HIPCHK(c, hipMemcpyAsync(P, dP[0], (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (dp && ddp != dp) HIPCHK(c, hipMemcpyAsync(dp, ddp, (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms_ = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms_, t.a, t.b));
+    c->last_ms = ms_;
+    return MDG_OK;
 }

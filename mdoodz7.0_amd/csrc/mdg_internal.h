@@ -88,6 +88,7 @@ struct mdg_ctx {
     struct advect_scratch *advect = nullptr;
     struct g2p_scratch *g2p = nullptr;
     struct thermal_scratch *thermal = nullptr;
+    struct reseed_scratch *reseed = nullptr;
     double      *static_scratch = nullptr;   // this device's static scratch (mdg_stokes_residual_static)
     // runtime
     hipStream_t  stream = nullptr;
@@ -138,6 +139,7 @@ void mdg_newton_release(mdg_ctx *c);
 void mdg_advect_release(mdg_ctx *c);
 void mdg_g2p_release(mdg_ctx *c);
 void mdg_thermal_release(mdg_ctx *c);
+void mdg_reseed_release(mdg_ctx *c);
 // Newton operator staged by mdg_newton_solver: y = J_gamma x (grid layout).
 bool mdg_newton_active(const mdg_ctx *c);
 int mdg_newton_apply_grid(mdg_ctx *c, double gamma, const double *x, double *y);

@@ -393,6 +393,240 @@ __global__ void k_scale_dinv(int n, T s, const T *__restrict__ dinv, const T *__
     if (i < n) d[i] = s * dinv[i] * r[i];
 }
 
+// ------------------------------------------ vertex-patch Schwarz smoother
+// The near-kernel of the grad-div term gamma C' W C is the divergence-free
+// fields, i.e. the curls of vertex stream functions, each of which lives on
+// the four faces around one vertex: Vx(a, b) below, Vx(a, b+1) above, Vz(a, b)
+// left and Vz(a+1, b) right of vertex (a, b).  Solving A_gamma exactly on
+// each such patch and adding the patch corrections (each face lies in two
+// patches) gives a smoother whose quality does not degrade with gamma, where
+// the diagonal one does.  M_p is A_gamma restricted to the free faces of
+// the patch; its inverse is stored as the 10 entries of the symmetric 4 x 4
+// (SoA, q-th entry of vertex v at pinv[q * Nx*Nz + v]; faces that are not
+// free get zero rows and columns).
+#define PATCH_Q 10
+__host__ __device__ inline int patch_q(int i, int j) {   // index of (i, j), i <= j, in the upper triangle
+    return i * 4 - i * (i - 1) / 2 + (j - i);
+}
+template <typename T>
+__global__ void k_patch_inv(sgrid g, T gamma, const T *__restrict__ en, const T *__restrict__ es,
+                            T *__restrict__ pinv) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int NV = g.Nx * g.Nz;
+    if (v >= NV) return;
+    const int a = v % g.Nx, b = v / g.Nx, Ncx = g.Nx - 1;
+    const bool fr[4] = {vx_free(g, a, b), vx_free(g, a, b + 1), vz_free(g, a, b), vz_free(g, a + 1, b)};
+    const double dV = g.dx * g.dz, idx2 = 1.0 / (g.dx * g.dx), idz2 = 1.0 / (g.dz * g.dz),
+                 idxz = 1.0 / (g.dx * g.dz), gm = (double)gamma;
+    const double D = (double)es[a + b * g.Nx];
+    auto w = [&](int ci, int cj) { return g.pen_const ? 1.0 : (double)en[ci + cj * Ncx]; };
+    double M[4][4] = {};
+    if (fr[0]) M[0][0] = (double)diag_vx<T>(g, a, b, en, es, gamma);
+    if (fr[1]) M[1][1] = (double)diag_vx<T>(g, a, b + 1, en, es, gamma);
+    if (fr[2]) M[2][2] = (double)diag_vz<T>(g, a, b, en, es, gamma);
+    if (fr[3]) M[3][3] = (double)diag_vz<T>(g, a + 1, b, en, es, gamma);
+    if (fr[0] && fr[1]) M[0][1] = -D * idz2 * dV;
+    if (fr[2] && fr[3]) M[2][3] = -D * idx2 * dV;
+    if (fr[0] && fr[2]) M[0][2] = (D + gm * w(a - 1, b - 1)) * idxz * dV;
+    if (fr[0] && fr[3]) M[0][3] = -(D + gm * w(a, b - 1)) * idxz * dV;
+    if (fr[1] && fr[2]) M[1][2] = -(D + gm * w(a - 1, b)) * idxz * dV;
+    if (fr[1] && fr[3]) M[1][3] = (D + gm * w(a, b)) * idxz * dV;
+    for (int i = 0; i < 4; i++) {
+        if (!fr[i]) M[i][i] = 1.0;
+        for (int j = 0; j < i; j++) M[i][j] = M[j][i];
+    }
+    // Gauss-Jordan on the SPD 4 x 4 (no pivoting needed)
+    double R[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    for (int p = 0; p < 4; p++) {
+        const double ip = 1.0 / M[p][p];
+        for (int j = 0; j < 4; j++) { M[p][j] *= ip; R[p][j] *= ip; }
+        for (int i = 0; i < 4; i++) {
+            if (i == p) continue;
+            const double f = M[i][p];
+            for (int j = 0; j < 4; j++) { M[i][j] -= f * M[p][j]; R[i][j] -= f * R[p][j]; }
+        }
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = i; j < 4; j++)
+            pinv[patch_q(i, j) * NV + v] = (fr[i] && fr[j]) ? (T)(0.5 * (R[i][j] + R[j][i])) : T(0);
+}
+
+// Face f's share of the patch correction of vertex (a, b), where f is
+// patch face `row`: sum_q Minv[row][q] r_q over the patch's free faces.
+template <typename T>
+__device__ inline T patch_row(const sgrid &g, const T *__restrict__ pinv, const T *__restrict__ r, int a, int b,
+                              int row) {
+    const int NV = g.Nx * g.Nz, v = a + b * g.Nx, nu = nvx(g), NxV = g.Nx + 1;
+    const int f[4] = {a + b * g.Nx, a + (b + 1) * g.Nx, nu + a + b * NxV, nu + (a + 1) + b * NxV};
+    const bool fr[4] = {vx_free(g, a, b), vx_free(g, a, b + 1), vz_free(g, a, b), vz_free(g, a + 1, b)};
+    T s = T(0);
+    for (int q = 0; q < 4; q++) {
+        if (!fr[q]) continue;
+        const int iq = row < q ? patch_q(row, q) : patch_q(q, row);
+        s += pinv[iq * NV + v] * r[f[q]];
+    }
+    return s;
+}
+// z = M_asm^-1 r on face i (0 on faces that are not free)
+template <typename T>
+__device__ inline T asm_face(const sgrid &g, const T *__restrict__ pinv, const T *__restrict__ r, int i) {
+    const int nu = nvx(g);
+    if (i < nu) {
+        const int k = i % g.Nx, l = i / g.Nx;
+        if (!vx_free(g, k, l)) return T(0);
+        return patch_row(g, pinv, r, k, l, 0) + patch_row(g, pinv, r, k, l - 1, 1);
+    }
+    const int j = i - nu, k = j % (g.Nx + 1), l = j / (g.Nx + 1);
+    if (!vz_free(g, k, l)) return T(0);
+    return patch_row(g, pinv, r, k, l, 2) + patch_row(g, pinv, r, k - 1, l, 3);
+}
+// d = s * M^-1 r
+template <typename T>
+__global__ void k_asm_scale(sgrid g, T s, const T *__restrict__ pinv, const T *__restrict__ r, T *__restrict__ d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nvx(g) + nvz(g)) d[i] = s * asm_face(g, pinv, r, i);
+}
+// x += d ; d = a*d + b*M^-1 r
+template <typename T>
+__global__ void k_asm_upd(sgrid g, T a, T b, const T *__restrict__ pinv, const T *__restrict__ r,
+                          T *__restrict__ d, T *__restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nvx(g) + nvz(g)) return;
+    const T di = d[i];
+    x[i] += di;
+    d[i] = a * di + b * asm_face(g, pinv, r, i);
+}
+
+// One fused step of the patch-preconditioned Chebyshev recurrence,
+//   x_{k+1} = x_k + a (x_k - x_{k-1}) + c M^-1 (b - A_g x_k),
+// M^-1 the vertex-patch additive Schwarz inverse, computed in the kernel:
+// a workgroup owns PC_TX x PC_TZ positions (k, l), i.e. the faces Vx(k, l)
+// and Vz(k, l); it evaluates the residual and the diagonal on the faces of
+// every patch it needs (one face of halo) into LDS, solves each vertex
+// patch's 4 x 4 system (Cholesky, the off-diagonal couplings from the
+// viscosities) into LDS, and each face adds its two patches' values.  The
+// HBM traffic is that of the Jacobi step (x_k, x_{k-1}, b, x_{k+1}, eta):
+// no patch inverses are stored.  FIRST: a = 0 and x_{k-1} is not read.
+#define PC_TX 64
+#define PC_TZ 8
+#define PC_THREADS 256
+struct pc_tiles {
+    int nbx, ntiles;
+};
+static inline pc_tiles make_pc_tiles(const sgrid &g) {
+    const int nbx = (g.Nx + 1 + PC_TX - 1) / PC_TX, nbz = (g.Nz + 1 + PC_TZ - 1) / PC_TZ;
+    return pc_tiles{nbx, nbx * nbz};
+}
+template <typename T, bool FIRST>
+__global__ void __launch_bounds__(PC_THREADS) k_pcheb(sgrid g, stc<T> cf, pc_tiles tg, T gamma, T a, T cc,
+                                                      const T *__restrict__ xc, T *__restrict__ xo,
+                                                      const T *__restrict__ b, const T *__restrict__ en,
+                                                      const T *__restrict__ es) {
+    constexpr int WX = PC_TX + 1, HX = PC_TZ + 2;   // Vx faces: k0-1 .. k0+TX-1, l0-1 .. l0+TZ
+    constexpr int WZ = PC_TX + 2, HZ = PC_TZ + 1;   // Vz faces: k0-1 .. k0+TX,   l0-1 .. l0+TZ-1
+    constexpr int WV = PC_TX + 1, HV = PC_TZ + 1;   // vertices: k0-1 .. k0+TX-1, l0-1 .. l0+TZ-1
+    __shared__ T rx[WX * HX], dxs[WX * HX], rz[WZ * HZ], dzs[WZ * HZ];
+    __shared__ T yv[4][WV * HV];
+    const int tile = xcd_tile(blockIdx.x, tg.ntiles);
+    const int tz = tile / tg.nbx, k0 = (tile - tz * tg.nbx) * PC_TX, l0 = tz * PC_TZ;
+    const int nu = nvx(g), NxV = g.Nx + 1;
+    const T *u = xc, *v = xc + nu;
+    // 1. residual and diagonal on the faces (0 and 1 outside the free set)
+    for (int q = threadIdx.x; q < WX * HX + WZ * HZ; q += PC_THREADS) {
+        if (q < WX * HX) {
+            const int k = k0 - 1 + q % WX, l = l0 - 1 + q / WX;
+            T r = T(0), dg = T(1);
+            if (k >= 0 && l >= 0 && k < g.Nx && l <= g.Nz && vx_free(g, k, l)) {
+                const T ax = apply_vx_rcp<T>(g, cf, k, l, u, v, en, es, gamma, &dg);
+                r = b[k + l * g.Nx] - ax;
+            }
+            rx[q] = r;
+            dxs[q] = dg;
+        } else {
+            const int p = q - WX * HX, k = k0 - 1 + p % WZ, l = l0 - 1 + p / WZ;
+            T r = T(0), dg = T(1);
+            if (k >= 0 && l >= 0 && k <= g.Nx && l < g.Nz && vz_free(g, k, l)) {
+                const T ax = apply_vz_rcp<T>(g, cf, k, l, u, v, en, es, gamma, &dg);
+                r = b[nu + k + l * NxV] - ax;
+            }
+            rz[p] = r;
+            dzs[p] = dg;
+        }
+    }
+    __syncthreads();
+    // 2. patch solves y_p = M_p^-1 r_p for the vertices the tile's faces use
+    for (int q = threadIdx.x; q < WV * HV; q += PC_THREADS) {
+        const int ia = q % WV, ib = q / WV, a_ = k0 - 1 + ia, b_ = l0 - 1 + ib;
+        T y[4] = {T(0), T(0), T(0), T(0)};
+        if (a_ >= 0 && b_ >= 0 && a_ < g.Nx && b_ < g.Nz) {
+            // faces xS = Vx(a, b), xN = Vx(a, b+1), zW = Vz(a, b), zE = Vz(a+1, b)
+            const int fx0 = ia + ib * WX, fx1 = ia + (ib + 1) * WX, fz0 = ia + ib * WZ, fz1 = ia + 1 + ib * WZ;
+            const bool fr[4] = {vx_free(g, a_, b_), vx_free(g, a_, b_ + 1), vz_free(g, a_, b_), vz_free(g, a_ + 1, b_)};
+            const T r[4] = {rx[fx0], rx[fx1], rz[fz0], rz[fz1]};
+            T M[4][4];
+            M[0][0] = fr[0] ? dxs[fx0] : T(1);
+            M[1][1] = fr[1] ? dxs[fx1] : T(1);
+            M[2][2] = fr[2] ? dzs[fz0] : T(1);
+            M[3][3] = fr[3] ? dzs[fz1] : T(1);
+            const T D = es[a_ + b_ * g.Nx];
+            const int nc = g.Nx - 1;
+            auto w = [&](int ci, int cj) { return g.pen_const ? T(1) : en[ci + cj * nc]; };
+            M[0][1] = (fr[0] && fr[1]) ? -D * cf.idz2 * cf.dV : T(0);
+            M[2][3] = (fr[2] && fr[3]) ? -D * cf.idx2 * cf.dV : T(0);
+            M[0][2] = (fr[0] && fr[2]) ? (D + gamma * w(a_ - 1, b_ - 1)) * cf.idxz * cf.dV : T(0);
+            M[0][3] = (fr[0] && fr[3]) ? -(D + gamma * w(a_, b_ - 1)) * cf.idxz * cf.dV : T(0);
+            M[1][2] = (fr[1] && fr[2]) ? -(D + gamma * w(a_ - 1, b_)) * cf.idxz * cf.dV : T(0);
+            M[1][3] = (fr[1] && fr[3]) ? (D + gamma * w(a_, b_)) * cf.idxz * cf.dV : T(0);
+            // Cholesky M = L L^T (lower triangle in place), then L L^T y = r
+            T L00 = sqrt(M[0][0]);
+            T L10 = M[0][1] / L00, L20 = M[0][2] / L00, L30 = M[0][3] / L00;
+            T L11 = sqrt(M[1][1] - L10 * L10);
+            T L21 = (M[1][2] - L20 * L10) / L11, L31 = (M[1][3] - L30 * L10) / L11;
+            T L22 = sqrt(M[2][2] - L20 * L20 - L21 * L21);
+            T L32 = (M[2][3] - L30 * L20 - L31 * L21) / L22;
+            T L33 = sqrt(M[3][3] - L30 * L30 - L31 * L31 - L32 * L32);
+            const T z0 = r[0] / L00;
+            const T z1 = (r[1] - L10 * z0) / L11;
+            const T z2 = (r[2] - L20 * z0 - L21 * z1) / L22;
+            const T z3 = (r[3] - L30 * z0 - L31 * z1 - L32 * z2) / L33;
+            y[3] = z3 / L33;
+            y[2] = (z2 - L32 * y[3]) / L22;
+            y[1] = (z1 - L21 * y[2] - L31 * y[3]) / L11;
+            y[0] = (z0 - L10 * y[1] - L20 * y[2] - L30 * y[3]) / L00;
+        }
+        for (int i = 0; i < 4; i++) yv[i][q] = y[i];
+    }
+    __syncthreads();
+    // 3. each position updates its Vx and Vz face
+    for (int q = threadIdx.x; q < PC_TX * PC_TZ; q += PC_THREADS) {
+        const int kk = q % PC_TX, ll = q / PC_TX, k = k0 + kk, l = l0 + ll;
+        if (k > g.Nx || l > g.Nz) continue;
+        const int vi = (kk + 1) + (ll + 1) * WV;   // vertex (k, l) in yv
+        if (k < g.Nx) {
+            const int i = k + l * g.Nx;
+            T y = T(0);
+            if (vx_free(g, k, l)) {
+                const T z = yv[0][vi] + yv[1][vi - WV];   // patch (k, l) as xS, patch (k, l-1) as xN
+                const T xi = u[i];
+                y = xi + cc * z;
+                if (!FIRST) y += a * (xi - xo[i]);
+            }
+            xo[i] = y;
+        }
+        if (l < g.Nz) {
+            const int j = k + l * NxV;
+            T y = T(0);
+            if (vz_free(g, k, l)) {
+                const T z = yv[2][vi] + yv[3][vi - 1];    // patch (k, l) as zW, patch (k-1, l) as zE
+                const T xi = v[j];
+                y = xi + cc * z;
+                if (!FIRST) y += a * (xi - xo[nu + j]);
+            }
+            xo[nu + j] = y;
+        }
+    }
+}
+
 // Linear interpolation weight of fine coordinate index f (position
 // (f + off_f) * hf) onto coarse index K (position (K + off_c) * hc), with the
 // coarse bracket clamped to [0, nc - 2].  Shared by P and R = P^T.
@@ -973,6 +1207,7 @@ struct mg_level {
     sgrid g;
     int nu = 0, nv = 0, n = 0;
     dbuf<T> en, es, dinv, x, b, r, d;
+    dbuf<T> pinv;          // vertex-patch inverses (smoother 2)
     double lmax = 1.0;
     double cheb_div = 30.0;  // smoothed range [lmax / cheb_div, lmax]
     const T *bsrc = nullptr;  // right-hand side read in place of b (the FP64 CG residual on level 0)
@@ -993,7 +1228,7 @@ struct mg_level {
     void release() {
         rt_f.release(); rt_w.release();
         en.release(); es.release(); dinv.release(); x.release(); b.release(); r.release();
-        d.release(); idx.release(); Ainv.release();
+        d.release(); idx.release(); Ainv.release(); pinv.release();
         px.release(); pb.release(); pr.release(); pd.release(); pdinv.release(); pAinv.release();
     }
 };
@@ -1007,6 +1242,7 @@ struct mg_hier {
     dbuf<double> scal, part;                // CG scalars (rz_old, rz_new, pq, rr) and partials
     double *h_scal = nullptr;               // pinned host copy of scal
     bool fused = true;                      // three-term Chebyshev kernels
+    bool patch = false;                     // vertex-patch Schwarz preconditioning in the smoother
     int deg_coarse = 0;                     // smoothing degree below the finest level
     ktimer *tm = nullptr;                   // times the fine-level smoother steps (graphs = 2)
     hipGraph_t graph = nullptr;             // one CG iteration (V-cycle included)
@@ -1048,6 +1284,7 @@ struct solver_state {
     dbuf<double> X, R, Rt, B, Zp, V, Z;   // outer FGCR (V, Z: restart slots)
     dbuf<double> Nu, Np, Ku, Kp;          // learned outer directions and K times them
     dbuf<double> Rm, Req;                 // residual of the returned solution (grid, equation numbering)
+    dbuf<double> Xr;                      // refinement correction (method 0 continuity bound)
     dbuf<double> etaf;                    // filtered viscosity of the Schur approximation (schur 2, 3)
     dbuf<double> ph_u, ph_f, ph_v, ph_s, ph_VV, ph_SS, ph_w, ph_y;   // KillerSolver iteration (method 1)
     ktimer tm;                            // fine-level smoother steps of the last graphs = 2 solve
@@ -1065,7 +1302,7 @@ void mdg_solver_release(mdg_ctx *c) {
     s->X.release(); s->R.release(); s->Rt.release(); s->B.release(); s->Zp.release();
     s->V.release(); s->Z.release(); s->part.release(); s->scal.release();
     s->Nu.release(); s->Np.release(); s->Ku.release(); s->Kp.release();
-    s->Rm.release(); s->Req.release(); s->etaf.release();
+    s->Rm.release(); s->Req.release(); s->Xr.release(); s->etaf.release();
     s->ph_u.release(); s->ph_f.release(); s->ph_v.release(); s->ph_s.release(); s->ph_VV.release(); s->ph_SS.release();
     s->ph_w.release(); s->ph_y.release();
     delete s;
@@ -1132,7 +1369,8 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
     H.drop_graph();
     for (auto &l : H.L) l.release();
     H.L.clear();
-    H.fused = (o->smoother == 0);
+    H.fused = (o->smoother == 0 || o->smoother == 3);
+    H.patch = (o->smoother == 2 || o->smoother == 3);
     H.deg_coarse = o->cheb_degree_coarse;
     const double gamma = o->penalty;
     sgrid g = c->sg;
@@ -1158,6 +1396,11 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
                                o->eta_avg, fl.en.p, fl.es.p, lv.en.p, lv.es.p);
         }
         hipLaunchKernelGGL(k_vel_diag_inv<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, g, (T)gamma, lv.en.p, lv.es.p, lv.dinv.p);
+        if (H.patch && !(H.fused && lv.n <= CHEB_BLOCK_MAX)) {
+            HIPCHK(c, lv.pinv.reserve((size_t)PATCH_Q * g.Nx * g.Nz));
+            hipLaunchKernelGGL(k_patch_inv<T>, dim3(nblk(g.Nx * g.Nz)), dim3(256), 0, c->stream, g, (T)gamma,
+                               (const T *)lv.en.p, (const T *)lv.es.p, lv.pinv.p);
+        }
         HIPCHK(c, hipGetLastError());
         if (ncx <= cmin || ncz <= cmin) break;
         // coarsen by 2, or by 4 once both sides are at most coarsen4 cells
@@ -1208,7 +1451,11 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
         for (int it = 0; it < 30; it++) {
             hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p,
                                (const T *)nullptr, lv.en.p, lv.es.p, lv.r.p);     // r = -A x
-            hipLaunchKernelGGL(k_scale_dinv<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.n, T(-1), lv.dinv.p, lv.r.p, lv.d.p);
+            if (H.patch && !(H.fused && lv.n <= CHEB_BLOCK_MAX))
+                hipLaunchKernelGGL(k_asm_scale<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, T(-1),
+                                   (const T *)lv.pinv.p, (const T *)lv.r.p, lv.d.p);
+            else
+                hipLaunchKernelGGL(k_scale_dinv<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.n, T(-1), lv.dinv.p, lv.r.p, lv.d.p);
             double yy, xx;
             int rc;
             if ((rc = dot_host<T>(c, s, lv.n, lv.d.p, lv.d.p, &yy)) || (rc = dot_host<T>(c, s, lv.n, lv.x.p, lv.x.p, &xx))) return rc;
@@ -1471,7 +1718,7 @@ static int bfbt_rhs(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solver
 // ----------------------------------------------------------------- V-cycle
 template <typename T>
 static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool zero_start, bool fused,
-                      ktimer *tm = nullptr) {
+                      ktimer *tm = nullptr, bool patch = false) {
     // x (+)= Chebyshev(deg) correction for A_g x = b
     const double lmax = lv.lmax, lmin = lmax / lv.cheb_div;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
@@ -1481,6 +1728,30 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
         hipLaunchKernelGGL(k_cheb_block<T>, dim3(1), dim3(CHEB_BLOCK), 0, c->stream, lv.g, make_stc<T>(lv.g), (T)gamma, deg,
                            theta, delta, zero_start ? 1 : 0, lv.x.p, lv.d.p, lv.rhs(), (const T *)lv.en.p,
                            (const T *)lv.es.p, (const T *)lv.dinv.p);
+        return;
+    }
+    if (fused && patch) {
+        // three-term form with the in-kernel vertex-patch inverse
+        const pc_tiles tg = make_pc_tiles(lv.g);
+        const stc<T> cf = make_stc<T>(lv.g);
+        T *cur = lv.d.p, *prev = lv.x.p;
+        if (zero_start)
+            hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, prev, T(0));
+        hipLaunchKernelGGL((k_pcheb<T, true>), dim3(tg.ntiles), dim3(PC_THREADS), 0, c->stream, lv.g, cf, tg, (T)gamma,
+                           T(0), (T)(1.0 / theta), (const T *)prev, cur, lv.rhs(), (const T *)lv.en.p,
+                           (const T *)lv.es.p);
+        for (int k = 0; k < deg - 1; k++) {
+            const double rho_new = 1.0 / (2.0 * sigma - rho);
+            if (tm) tm->begin(c->stream);
+            hipLaunchKernelGGL((k_pcheb<T, false>), dim3(tg.ntiles), dim3(PC_THREADS), 0, c->stream, lv.g, cf, tg,
+                               (T)gamma, (T)(rho_new * rho), (T)(2.0 * rho_new / delta), (const T *)cur, prev,
+                               lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
+            if (tm) tm->end(c->stream);
+            std::swap(cur, prev);
+            rho = rho_new;
+        }
+        if (cur != lv.x.p)
+            hipLaunchKernelGGL(k_copy<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (const T *)cur, lv.x.p);
         return;
     }
     if (fused) {
@@ -1516,12 +1787,22 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
         hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p,
                            lv.rhs(), lv.en.p, lv.es.p, lv.r.p);
     }
-    hipLaunchKernelGGL(k_scale_dinv<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)(1.0 / theta), lv.dinv.p, lv.r.p, lv.d.p);
+    if (patch)
+        hipLaunchKernelGGL(k_asm_scale<T>, dim3(nblk(n)), dim3(256), 0, c->stream, lv.g, (T)(1.0 / theta),
+                           (const T *)lv.pinv.p, (const T *)lv.r.p, lv.d.p);
+    else
+        hipLaunchKernelGGL(k_scale_dinv<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)(1.0 / theta), lv.dinv.p, lv.r.p, lv.d.p);
     for (int k = 0; k < deg - 1; k++) {
+        if (tm) tm->begin(c->stream);
         hipLaunchKernelGGL(k_cheb_res<T>, dim3(nblk(n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.d.p, lv.en.p, lv.es.p, lv.r.p);
         const double rho_new = 1.0 / (2.0 * sigma - rho);
-        hipLaunchKernelGGL(k_cheb_upd<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)(rho_new * rho),
-                           (T)(2.0 * rho_new / delta), lv.dinv.p, lv.r.p, lv.d.p, lv.x.p);
+        if (patch)
+            hipLaunchKernelGGL(k_asm_upd<T>, dim3(nblk(n)), dim3(256), 0, c->stream, lv.g, (T)(rho_new * rho),
+                               (T)(2.0 * rho_new / delta), (const T *)lv.pinv.p, (const T *)lv.r.p, lv.d.p, lv.x.p);
+        else
+            hipLaunchKernelGGL(k_cheb_upd<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, (T)(rho_new * rho),
+                               (T)(2.0 * rho_new / delta), lv.dinv.p, lv.r.p, lv.d.p, lv.x.p);
+        if (tm) tm->end(c->stream);
         rho = rho_new;
     }
     hipLaunchKernelGGL(k_axpby<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, T(1), lv.d.p, T(1), lv.x.p);
@@ -1540,7 +1821,7 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, i
     }
     mg_level<T> &cl = H.L[i + 1];
     ktimer *tm = (i == 0) ? H.tm : nullptr;
-    chebyshev(c, lv, gamma, deg, true, H.fused, tm);
+    chebyshev(c, lv, gamma, deg, true, H.fused, tm, H.patch);
     const int reps = (i + 2 == H.L.size()) ? 1 : cycle;
     for (int rep = 0; rep < reps; rep++) {
         hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p, lv.rhs(),
@@ -1554,7 +1835,7 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, i
         vcycle(c, H, i + 1, gamma, deg, cycle);
         hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, cl.x.p, lv.x.p);
     }
-    chebyshev(c, lv, gamma, deg, false, H.fused, tm);
+    chebyshev(c, lv, gamma, deg, false, H.fused, tm, H.patch);
 }
 
 // Records one CG iteration (preconditioning V-cycle, direction, operator
@@ -2077,7 +2358,7 @@ int mdg_solve_grid(mdg_ctx *c, mdg_solver_opts *o, const double *b, double *x) {
         return mdg_fail(c, MDG_ERR_STATE, "precond %d needs the network parameters (mdg_pcnet_set_params)", o->precond);
     if (o->schur < 0 || o->schur > 3) return mdg_fail(c, MDG_ERR_ARG, "schur must be 0, 1, 2 or 3");
     if (o->cycle != 1) return mdg_fail(c, MDG_ERR_UNSUPPORTED, "only the V-cycle (cycle = 1) is supported");
-    if (o->smoother < 0 || o->smoother > 1) return mdg_fail(c, MDG_ERR_ARG, "smoother must be 0 or 1");
+    if (o->smoother < 0 || o->smoother > 3) return mdg_fail(c, MDG_ERR_ARG, "smoother must be 0, 1, 2 or 3");
     if (o->graphs < 0 || o->graphs > 2) return mdg_fail(c, MDG_ERR_ARG, "graphs must be 0, 1 or 2");
     s->tm.ms = 0.0;
     s->tm.launches = 0;
@@ -2170,7 +2451,16 @@ extern "C" int mdg_killer_solver(mdg_ctx *c, mdg_solver_opts *o, const double *D
         return rc;
     rc = mdg_solve_grid(c, o, b, x);
     if (rc && rc != MDG_ERR_DIVERGED) return rc;
-    if (o->method == 0) {
+    const bool check = o->lin_abs_div > 0.0 || o->lin_rel_div > 0.0;
+    auto div_fails = [&]() {
+        return std::fabs(o->max_div) > 10.0 * o->lin_abs_div && o->max_div / o->max_div0 > o->lin_rel_div;
+    };
+    // method 0: the relative 2-norm it stops on weighs the continuity rows
+    // (dV * div) little next to the momentum rows, so a solution can pass it
+    // and still miss KillerSolver's continuity bounds; it is then refined
+    // (K dx = b - K x solved to the same relative tolerance, x += dx), at
+    // most three times, before the check below
+    for (int round = 0; o->method == 0 && round < 4; round++) {
         // signed maxima of the returned solution's residual b - K x
         // (equation numbering) against those of the zero start
         HIPCHK(c, s->Rm.reserve(n));
@@ -2190,6 +2480,18 @@ extern "C" int mdg_killer_solver(mdg_ctx *c, mdg_solver_opts *o, const double *D
             return rc3;
         o->max_mom0 = mom0;
         o->max_div0 = div0;
+        if (!check || !div_fails() || round == 3) break;
+        HIPCHK(c, s->Xr.reserve(n));
+        const int its = o->its, inner = o->inner_its;
+        const double ms = o->time_ms;
+        const int rc4 = mdg_solve_grid(c, o, s->Rm.p, s->Xr.p);
+        if (rc4 && rc4 != MDG_ERR_DIVERGED) return rc4;
+        o->its += its;
+        o->inner_its += inner;
+        o->time_ms += ms;
+        hipLaunchKernelGGL(k_axpby<double>, dim3(nblk((int)n)), dim3(256), 0, c->stream, (int)n, 1.0,
+                           (const double *)s->Xr.p, 1.0, x);
+        rc = rc4;
     }
     const int rc2 = mdg_grid_to_eqn(c, x, x + nvx(g), x + nvx(g) + nvz(g), c->eqbuf.p);
     if (rc2) return rc2;
@@ -2198,8 +2500,7 @@ extern "C" int mdg_killer_solver(mdg_ctx *c, mdg_solver_opts *o, const double *D
     // KillerSolver's a-posteriori incompressibility check (Solvers.c:1333-1336):
     // a missed momentum tolerance is carried on with, a continuity residual
     // above both bounds is not
-    if ((o->lin_abs_div > 0.0 || o->lin_rel_div > 0.0) && std::fabs(o->max_div) > 10.0 * o->lin_abs_div &&
-        o->max_div / o->max_div0 > o->lin_rel_div)
+    if (check && div_fails())
         return mdg_fail(c, MDG_ERR_INCOMPRESSIBLE,
                         "incompressibility not satisfied: max. div. residual %.3e (x %.3e of the first), bounds "
                         "10 x %.1e and %.1e",

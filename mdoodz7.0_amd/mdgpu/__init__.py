@@ -84,6 +84,11 @@ def lib():
         L.mdg_update_particle_pressure.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
         L.mdg_accumulated_strain.argtypes = [vp, C.c_double, vp, vp, C.c_int, vp, vp, vp, vp]
         L.mdg_rotate_particle_stress.argtypes = [vp, C.c_int, C.c_double, C.c_int, vp, vp, vp, vp, vp]
+        L.mdg_update_particle_stress_subgrid.argtypes = [vp, C.c_double, vp, C.c_int, C.c_int] + [vp] * 7 + \
+            [C.c_int] + [vp] * 6
+        L.mdg_update_particle_pressure_subgrid.argtypes = [vp, C.c_double, vp, C.c_int] + [vp] * 4 + [C.c_int] + [vp] * 4
+        L.mdg_count_part_cell.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int)]
         L.mdg_energy_step.argtypes = [vp, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_double, vp,
                                       C.POINTER(C.c_int)]
         _extra_signatures(L)
@@ -434,6 +439,47 @@ class Context:
             self.h, int(mode), dt, n, checked(phase, np.int32, n, "phase"), checked(wxzm, np.float64, n, "wxzm"),
             checked(sxxd, np.float64, n, "sxxd"), checked(szzd, np.float64, n, "szzd"),
             checked(sxz, np.float64, n, "sxz")))
+
+    def update_particle_stress_subgrid(self, dt, G_phase, sxxd, szzd, sxz, sxxd0, szzd0, sxz0, eta_phys_s, x, z,
+                                       phase, msxxd, mszzd, msxz, interp_stencil=1):
+        """UpdateParticleStress, subgrid_diffusion = 2 (RheologyParticles.c:
+        1811-1895): marker stresses updated in place."""
+        n, nc, nv = len(phase), (self.Nx - 1) * (self.Nz - 1), self.Nx * self.Nz
+        G = np.ascontiguousarray(G_phase, np.float64)
+        f = lambda a, m, name: checked(a, np.float64, m, name)
+        self._check(self._L.mdg_update_particle_stress_subgrid(
+            self.h, dt, ptr(G), G.size, int(interp_stencil), f(sxxd, nc, "sxxd"), f(szzd, nc, "szzd"),
+            f(sxz, nv, "sxz"), f(sxxd0, nc, "sxxd0"), f(szzd0, nc, "szzd0"), f(sxz0, nv, "sxz0"),
+            f(eta_phys_s, nv, "eta_phys_s"), n, f(x, n, "x"), f(z, n, "z"), checked(phase, np.int32, n, "phase"),
+            f(msxxd, n, "msxxd"), f(mszzd, n, "mszzd"), f(msxz, n, "msxz")))
+
+    def update_particle_pressure_subgrid(self, dt, beta_phase, p_in, p0_n, eta_phys_n, x, z, phase, P, dp=None):
+        """UpdateParticlePressure, subgrid_diffusion >= 2 (RheologyParticles.c:
+        1621-1695): P updated in place; returns dp."""
+        n, nc = len(phase), (self.Nx - 1) * (self.Nz - 1)
+        B = np.ascontiguousarray(beta_phase, np.float64)
+        dp = np.zeros(nc) if dp is None else dp
+        f = lambda a, m, name: checked(a, np.float64, m, name)
+        self._check(self._L.mdg_update_particle_pressure_subgrid(
+            self.h, dt, ptr(B), B.size, f(p_in, nc, "p_in"), f(p0_n, nc, "p0_n"), f(eta_phys_n, nc, "eta_phys_n"),
+            f(dp, nc, "dp"), n, f(x, n, "x"), f(z, n, "z"), checked(phase, np.int32, n, "phase"), f(P, n, "P")))
+        return dp
+
+    def count_part_cell(self, x, z, phase, n=None, min_part_cell=16, cell_tag=None):
+        """Re-seeding of CountPartCell_OLD (ParticleRoutines.c:3224): x, z,
+        phase (arrays of the marker capacity, the first n in use) are
+        updated in place; cell_tag is BCt.type on the cells (30 = air) or
+        None.  Returns (n_out, new_slot, new_src)."""
+        cap = len(x)
+        n = cap if n is None else int(n)
+        nq = 4 * (self.Nx - 1) * (self.Nz - 1)
+        slot, src = np.zeros(nq, np.int32), np.zeros(nq, np.int32)
+        nn, no = C.c_int(0), C.c_int(0)
+        self._check(self._L.mdg_count_part_cell(
+            self.h, int(min_part_cell), checked(cell_tag, np.int8, nq // 4, "cell_tag", optional=True), n, cap,
+            checked(x, np.float64, cap, "x"), checked(z, np.float64, cap, "z"),
+            checked(phase, np.int32, cap, "phase"), ptr(slot), ptr(src), C.byref(nn), C.byref(no)))
+        return no.value, slot[:nn.value], src[:nn.value]
 
     def energy_step(self, dt, rho_n, Cp, T, kx, kz, BCT_t, BCT_v, Qr=None, Wdiss=None, Ha=None, max_its=1000,
                     rel_tol=1e-8):

@@ -125,6 +125,15 @@ def lib():
         L.orc_killer_solve_newton.argtypes = [C.c_int, C.c_int] + [i32p, i32p, f64p] * 5 + \
             [f64p, f64p, f64p, f64p, C.POINTER(KillerOpts)]
         L.orc_killer_solve_newton.restype = C.c_int
+        L.orc_update_particle_stress_subgrid.argtypes = [gp, C.c_double, f64p, C.c_int] + [f64p] * 7 + \
+            [C.c_int, f64p, f64p, i32p, f64p, f64p, f64p]
+        L.orc_update_particle_stress_subgrid.restype = None
+        L.orc_update_particle_pressure_subgrid.argtypes = [gp, C.c_double, f64p, f64p, f64p, f64p, f64p, C.c_int,
+                                                           f64p, f64p, i32p, f64p]
+        L.orc_update_particle_pressure_subgrid.restype = None
+        L.orc_reseed.argtypes = [gp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, f64p, f64p, i32p, i32p, i32p,
+                                 C.POINTER(C.c_int)]
+        L.orc_reseed.restype = C.c_int
         L.orc_energy_matrix.argtypes = [gp, C.c_double, f64p, f64p, f64p, f64p, _bp_nd, f64p, i32p, i32p, f64p,
                                         f64p]
         L.orc_energy_matrix.restype = None
@@ -279,6 +288,46 @@ def accumulated_strain(g, dt, x, z, phase, rates, strains, div_u_pl=None):
     if bad >= 0:
         raise ValueError("negative strain increment at cell %d" % bad)
     return out
+
+
+def update_particle_stress_subgrid(g, dt, G_phase, sxxd, szzd, sxz, sxxd0, szzd0, sxz0, eta_phys_s, x, z, phase,
+                                   msxxd, mszzd, msxz, stencil=1):
+    """UpdateParticleStress, subgrid_diffusion = 2 (RheologyParticles.c:1811-1895): returns new
+    (msxxd, mszzd, msxz)."""
+    f = lambda a: np.ascontiguousarray(a, np.float64)
+    out = [np.array(a, np.float64) for a in (msxxd, mszzd, msxz)]
+    lib().orc_update_particle_stress_subgrid(g.c, dt, f(G_phase), int(stencil), f(sxxd), f(szzd), f(sxz), f(sxxd0),
+                                             f(szzd0), f(sxz0), f(eta_phys_s), len(x), f(x), f(z),
+                                             np.ascontiguousarray(phase, np.int32), *out)
+    return out
+
+
+def update_particle_pressure_subgrid(g, dt, beta_phase, p_in, p0_n, eta_phys_n, x, z, phase, P):
+    """UpdateParticlePressure, subgrid_diffusion >= 2 (RheologyParticles.c:1621-1695): returns
+    (new P, dp)."""
+    f = lambda a: np.ascontiguousarray(a, np.float64)
+    out = np.array(P, np.float64)
+    dp = np.zeros((g.Nx - 1) * (g.Nz - 1))
+    lib().orc_update_particle_pressure_subgrid(g.c, dt, f(beta_phase), f(p_in), f(p0_n), f(eta_phys_n), dp, len(x),
+                                               f(x), f(z), np.ascontiguousarray(phase, np.int32), out)
+    return out, dp
+
+
+def reseed(g, x, z, phase, min_part_cell=16, capacity=None, xmin=None, xmax=None):
+    """CountPartCell_OLD re-seeding (ParticleRoutines.c:3224-4050).  Returns
+    (x, z, phase) of length n_out, the slots and sources of the new markers,
+    and the status (number of new markers, or -1 / -2 as orc_reseed)."""
+    n = len(x)
+    cap = capacity if capacity is not None else n + 4 * g.Nx * g.Nz + 64
+    X = np.zeros(cap); Z = np.zeros(cap); P = np.full(cap, -1, np.int32)
+    X[:n] = x; Z[:n] = z; P[:n] = phase
+    nf = 4 * (g.Nx + 1) * (g.Nz + 1)
+    slot = np.zeros(nf, np.int32); src = np.zeros(nf, np.int32)
+    n_out = C.c_int(n)
+    rc = lib().orc_reseed(g.c, g.xg[0] if xmin is None else xmin, g.xg[-1] if xmax is None else xmax,
+                          int(min_part_cell), n, cap, X, Z, P, slot, src, C.byref(n_out))
+    m = max(rc, 0)
+    return X[:n_out.value], Z[:n_out.value], P[:n_out.value], slot[:m], src[:m], rc
 
 
 def energy_matrix(g, dt, rho_n, Cp, kx, kz, BCT_t, BCT_v):

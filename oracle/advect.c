@@ -378,3 +378,95 @@ void orc_rotate_stress(int mode, double dt, int n, const int *phase, const doubl
         }
     }
 }
+
+/* UpdateParticleStress with subgrid_diffusion = 2 (RheologyParticles.c:
+ * 1811-1895): old grid stresses and the vertex viscosity eta_phys_s at the
+ * markers; per active marker the subgrid relaxation
+ *   dtau_s = -(tau_m - tau0_m) (1 - exp(-d dt / (eta_m / G_p))),  d = 1;
+ * the subgrid increments back to the grid (P2Mastah, flag 1, arithmetic,
+ * with the BCp / BCg node tags), the remaining grid increments
+ * (tau - tau0) - dtau_s_grid on non-air nodes, those at the markers, and
+ * tau_m += dtau_s + dtau_r. */
+void orc_update_particle_stress_subgrid(const orc_grid *g, double dt, const double *G_phase, int stencil,
+                                        const double *sxxd, const double *szzd, const double *sxz,
+                                        const double *sxxd0, const double *szzd0, const double *sxz0,
+                                        const double *eta_phys_s, int n, const double *x, const double *z,
+                                        const int *phase, double *msxxd, double *mszzd, double *msxz) {
+    const int nc = (g->Nx - 1) * (g->Nz - 1), nv = g->Nx * g->Nz;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    double *m0[3], *ms[3], *mr[3], *etam = calloc(N, sizeof(double));
+    for (int q = 0; q < 3; q++) {
+        m0[q] = calloc(N, sizeof(double));
+        ms[q] = calloc(N, sizeof(double));
+        mr[q] = calloc(N, sizeof(double));
+    }
+    orc_g2p_centroids(g, sxxd0, n, x, z, phase, m0[0]);
+    orc_g2p_centroids(g, szzd0, n, x, z, phase, m0[1]);
+    orc_g2p_vertices(g, sxz0, n, x, z, phase, m0[2]);
+    orc_g2p_vertices(g, eta_phys_s, n, x, z, phase, etam);
+    double *tm[3] = {msxxd, mszzd, msxz};
+    const double d = 1.0;
+    for (int k = 0; k < n; k++) {
+        if (phase[k] == -1) continue;
+        const double dtaum = etam[k] / G_phase[phase[k]];
+        for (int q = 0; q < 3; q++) ms[q][k] = -(tm[q][k] - m0[q][k]) * (1.0 - exp(-d * dt / dtaum));
+    }
+    double *gs[3] = {calloc((size_t)nc, sizeof(double)), calloc((size_t)nc, sizeof(double)),
+                     calloc((size_t)nv, sizeof(double))};
+    double *gr[3] = {calloc((size_t)nc, sizeof(double)), calloc((size_t)nc, sizeof(double)),
+                     calloc((size_t)nv, sizeof(double))};
+    for (int q = 0; q < 3; q++)
+        orc_p2mastah(g, n, x, z, phase, ms[q], 1, 0, 0, q < 2 ? 1 : 0, stencil, 0, q < 2 ? g->BCp_t : g->BCg_t,
+                     gs[q], 0, NULL, 0, 1);
+    for (int c = 0; c < nc; c++) {
+        if (g->BCp_t[c] != 30 && g->BCp_t[c] != 31) gr[0][c] = (sxxd[c] - sxxd0[c]) - gs[0][c];
+        if (g->BCp_t[c] != 30 && g->BCp_t[c] != 31) gr[1][c] = (szzd[c] - szzd0[c]) - gs[1][c];
+    }
+    for (int c = 0; c < nv; c++)
+        if (g->BCg_t[c] != 30) gr[2][c] = (sxz[c] - sxz0[c]) - gs[2][c];
+    orc_g2p_centroids(g, gr[0], n, x, z, phase, mr[0]);
+    orc_g2p_centroids(g, gr[1], n, x, z, phase, mr[1]);
+    orc_g2p_vertices(g, gr[2], n, x, z, phase, mr[2]);
+    for (int k = 0; k < n; k++)
+        if (phase[k] != -1)
+            for (int q = 0; q < 3; q++) tm[q][k] = tm[q][k] + ms[q][k] + mr[q][k];
+    for (int q = 0; q < 3; q++) { free(m0[q]); free(ms[q]); free(mr[q]); free(gs[q]); free(gr[q]); }
+    free(etam);
+}
+
+/* UpdateParticlePressure with subgrid_diffusion >= 2 (RheologyParticles.c:
+ * 1621-1695): dp = p_in - p0_n on non-air cells; eta_phys_n and p0_n (BCt
+ * 30 cells left at 0; here the BCp tags, which mark the same air cells) at
+ * the markers; dP_s = -(P_m - P0_m) (1 - exp(-d dt / (eta_m / beta_p))) (the
+ * reference's relaxation time, kept as written); P2Mastah of dP_s onto the
+ * centres (stencil 1, as the reference hard-codes), remaining dp - dP_s_grid
+ * back to the markers, P_m += dP_s + dP_r. */
+void orc_update_particle_pressure_subgrid(const orc_grid *g, double dt, const double *beta_phase,
+                                          const double *p_in, const double *p0_n, const double *eta_phys_n,
+                                          double *dp, int n, const double *x, const double *z, const int *phase,
+                                          double *P) {
+    const int nc = (g->Nx - 1) * (g->Nz - 1);
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    for (int c = 0; c < nc; c++) dp[c] = (g->BCp_t[c] != 30) ? (p_in[c] - p0_n[c]) : 0.0;
+    double *etam = calloc(N, sizeof(double)), *Pm0 = calloc(N, sizeof(double)), *dPms = calloc(N, sizeof(double)),
+           *dPmr = calloc(N, sizeof(double));
+    double *Pg0 = calloc((size_t)nc, sizeof(double)), *dPgs = calloc((size_t)nc, sizeof(double)),
+           *dPgr = calloc((size_t)nc, sizeof(double));
+    orc_g2p_centroids(g, eta_phys_n, n, x, z, phase, etam);
+    for (int c = 0; c < nc; c++)
+        if (g->BCp_t[c] != 30) Pg0[c] = p0_n[c];
+    orc_g2p_centroids(g, Pg0, n, x, z, phase, Pm0);
+    const double d = 1.0;
+    for (int k = 0; k < n; k++) {
+        if (phase[k] == -1) continue;
+        const double dtm = etam[k] / (beta_phase[phase[k]]);
+        dPms[k] = -(P[k] - Pm0[k]) * (1.0 - exp(-d * dt / dtm));
+    }
+    orc_p2mastah(g, n, x, z, phase, dPms, 1, 0, 0, 1, 1, 0, g->BCp_t, dPgs, 0, NULL, 0, 1);
+    for (int c = 0; c < nc; c++)
+        if (g->BCp_t[c] != 30) dPgr[c] = dp[c] - dPgs[c];
+    orc_g2p_centroids(g, dPgr, n, x, z, phase, dPmr);
+    for (int k = 0; k < n; k++)
+        if (phase[k] != -1) P[k] = P[k] + dPms[k] + dPmr[k];
+    free(etam); free(Pm0); free(dPms); free(dPmr); free(Pg0); free(dPgs); free(dPgr);
+}

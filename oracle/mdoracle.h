@@ -107,6 +107,19 @@ void orc_update_particle_stress(const orc_grid *g, const double *sxxd, const dou
 void orc_update_particle_pressure(const orc_grid *g, const double *p_in, const double *p0_n, double *dp,
                                   int n, const double *x, const double *z, const int *phase, double *P);
 
+/* UpdateParticleStress / UpdateParticlePressure with subgrid diffusion
+ * (RheologyParticles.c:1811-1895, :1621-1695): per-phase G / beta tables,
+ * vertex / centre physical viscosity, P2Mastah interp_stencil (stress). */
+void orc_update_particle_stress_subgrid(const orc_grid *g, double dt, const double *G_phase, int stencil,
+                                        const double *sxxd, const double *szzd, const double *sxz,
+                                        const double *sxxd0, const double *szzd0, const double *sxz0,
+                                        const double *eta_phys_s, int n, const double *x, const double *z,
+                                        const int *phase, double *msxxd, double *mszzd, double *msxz);
+void orc_update_particle_pressure_subgrid(const orc_grid *g, double dt, const double *beta_phase,
+                                          const double *p_in, const double *p0_n, const double *eta_phys_n,
+                                          double *dp, int n, const double *x, const double *z, const int *phase,
+                                          double *P);
+
 /* StokesRoutines.c:415-~500 (ghost values from BC tags) */
 void orc_apply_bc(const orc_grid *g, double *u, double *v);
 
@@ -188,6 +201,12 @@ int orc_accumulated_strain(const orc_grid *g, double dt, const double *const *eI
  * mode 1 Jaumann rate, mode 2 analytical rotation, with the markers' spin. */
 void orc_rotate_stress(int mode, double dt, int n, const int *phase, const double *wxzm, double *sxxd, double *szzd,
                        double *sxz);
+
+/* CountPartCell_OLD re-seeding (ParticleRoutines.c:3224-4050), no free
+ * surface, no periodic x: returns the number of new markers, -1 for an
+ * empty 5 x 5 neighbourhood, -2 when capacity is exceeded. */
+int orc_reseed(const orc_grid *g, double xmin, double xmax, int min_part_cell, int n, int capacity, double *x,
+               double *z, int *phase, int *new_slot, int *new_src, int *n_out);
 
 /* EnergyDirectSolve (ThermalRoutines.c:74-470) matrix for a box without
  * air cells: CSR rows in cell order (<= 5 entries), boundary rhs bbc. */

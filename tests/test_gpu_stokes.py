@@ -73,7 +73,7 @@ def test_strain_rates_body_force_and_residual(oracle_mod):
     np.testing.assert_allclose(ng, nrm, rtol=1e-12)
 
 
-@pytest.mark.parametrize("schur,smoother", [(0, 0), (1, 0), (0, 1)])
+@pytest.mark.parametrize("schur,smoother", [(0, 0), (1, 0), (0, 1), (0, 2), (0, 3)])
 @pytest.mark.parametrize("precision", [mdgpu.MDG_PREC_FP64, mdgpu.MDG_PREC_MIXED])
 @pytest.mark.parametrize("bc", ["pure_shear", "solvi"])
 def test_killer_solver_matches_oracle(oracle_mod, bc, precision, schur, smoother):
@@ -116,7 +116,8 @@ def test_killer_solver_matches_oracle(oracle_mod, bc, precision, schur, smoother
 def test_solver_variants_agree_at_511():
     """511^2 ellipse field with a 1e6 viscosity contrast (the size at which an
     FP32 inner CG stops converging): FP64 and FP32 V-cycles, fused and
-    two-kernel smoothers and the graph-free launch all reach 1e-8 on the FP64
+    two-kernel smoothers (diagonal and vertex-patch Schwarz) and the
+    graph-free launch all reach 1e-8 on the FP64
     residual, checked independently with the library's Picard operator, and
     agree on the solution to 1e-6."""
     from mdgpu import learned as L
@@ -127,7 +128,9 @@ def test_solver_variants_agree_at_511():
     D11 = 2.0 * prob.eta_n
     F = prob.F
     ref = None
-    for kw in (dict(), dict(precision=mdgpu.MDG_PREC_MIXED), dict(smoother=1), dict(graphs=1, max_its=60)):
+    for kw in (dict(), dict(precision=mdgpu.MDG_PREC_MIXED), dict(smoother=1), dict(smoother=2),
+               dict(smoother=3), dict(smoother=3, precision=mdgpu.MDG_PREC_MIXED, penalty=100.0),
+               dict(graphs=1, max_its=60)):
         sol, o = prob.solve(ctx, rel_tol=1e-8, **kw)
         print(kw, "its", o.its, "inner", o.inner_its, "rel", o.rel_res, "ms %.1f" % o.time_ms)
         r = F - ctx.stokes_apply(D11, D11, prob.eta_s, sol)
@@ -156,7 +159,7 @@ def test_operator_transpose_is_pressure_flip(oracle_mod, bc):
 
 
 @pytest.mark.parametrize("opt,val,code", [("cycle", 2, -3), ("schur", 4, -1), ("precond", 4, -1),
-                                            ("smoother", 2, -1)])
+                                            ("smoother", 4, -1)])
 def test_solver_refuses_unsupported_options(oracle_mod, opt, val, code):
     """Options outside the implemented set are refused before any work: the
     W-cycle (it stalled on the ellipse workloads), unknown Schur
