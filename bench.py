@@ -75,6 +75,12 @@ def main():
                          "updates), REPS times each, arrays resident in HBM")
     ap.add_argument("--trace-mixed", action="store_true",
                     help="with --trace-solve: the FP32 V-cycle (MDG_PREC_MIXED) instead of FP64")
+    ap.add_argument("--p2g-cases", type=int, default=0, metavar="REPS",
+                    help="measurement mode: the P2G cases one field, two fused fields, phase "
+                         "fractions (2 and 9 phases), the 9-cell stencil, and a P2G right after "
+                         "the markers moved (re-sort and weight pass included), REPS times each")
+    ap.add_argument("--solver", default="{}", metavar="JSON",
+                    help="extra mdg_solver_opts fields for --trace-solve (e.g. '{\"smoother\": 3}')")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: the library runs on one GPU only (--gpus 1); multi-GPU sharding is "
@@ -139,6 +145,9 @@ def main():
     if a.kernels:
         kernels_only(ctx, a.kernels, step, f_cent, f_vert, D11, en, es, F, nm, sol, eta_s, rho_s, n)
         return
+    if a.p2g_cases:
+        p2g_cases(ctx, a.p2g_cases, wl, dx_, dz_, dph, eta_s, rho_s, en, rn, es, rs, n)
+        return
     if a.trace_solve:
         ctx.p2mastah_fused(1, 1, f_cent)
         ctx.p2mastah_fused(0, 1, f_vert)
@@ -148,7 +157,7 @@ def main():
         try:
             prec = mdgpu.MDG_PREC_MIXED if a.trace_mixed else mdgpu.MDG_PREC_FP64
             o = ctx.killer_solver_dev(D11, D11, es, F, nm, sol, max_its=a.trace_solve, graphs=1,
-                                      precision=prec)
+                                      precision=prec, **json.loads(a.solver))
         except mdgpu.MdgError as e:
             if e.code != -6:
                 raise
@@ -312,6 +321,58 @@ def kernels_only(ctx, reps, step, f_cent, f_vert, D11, en, es, F, nm, sol, eta_s
         ctx.scale_into(eta_s, 1.0, rho_s)
     out["copy_bytes"] = 16.0 * n
     out["markers"] = n
+    print(json.dumps(out), flush=True)
+
+
+def p2g_cases(ctx, reps, wl, dx_, dz_, dph, eta_s, rho_s, en, rn, es, rs, n):
+    """Kernel times (HIP events, median of reps) of the P2G cases: one field
+    and two fused fields onto centres and vertices, phase fractions with 2
+    and 9 phases, the 9-cell stencil, and the cost of P2G after the markers
+    moved (the re-sort and the per-node-set weight pass, which a step whose
+    markers were advected pays once)."""
+    import numpy as np
+    import mdgpu
+
+    def med(f):
+        t = []
+        for _ in range(reps):
+            f()
+            t.append(ctx.last_kernel_ms())
+        return sorted(t)[len(t) // 2]
+
+    srt = 1 | mdgpu.MDG_SRC_SORTED
+    out = {"markers": n}
+    for cen, dst, name in ((1, en, "centres"), (0, es, "vertices")):
+        out["one_field_%s_ms" % name] = med(lambda: ctx.p2mastah_fused(cen, 1, [dict(src=rho_s, dst=dst, flag=srt)]))
+        d2 = rn if cen == 1 else rs
+        out["two_fields_%s_ms" % name] = med(lambda: ctx.p2mastah_fused(
+            cen, 1, [dict(src=eta_s, dst=dst, flag=srt, avg=2), dict(src=rho_s, dst=d2, flag=srt, avg=0)]))
+        out["stencil9_%s_ms" % name] = med(lambda: ctx.p2mastah_fused(cen, 9, [dict(src=rho_s, dst=dst, flag=srt)]))
+    # phase fractions: the workload's 2 phases, then 9 phases assigned by marker index
+    nc = ctx.node_count(1)
+    for nph in (2, 9):
+        if nph == 9:
+            ph9 = np.where(wl["phase"] >= 0, np.arange(n) % 9, -1).astype(np.int32)
+            ctx.markers_set(dx_, dz_, ctx.device(n, np.int32, ph9))
+        perc = [ctx.device(nc) for _ in range(nph)]
+        out["phase_fractions_%d_ms" % nph] = med(lambda: ctx.p2mastah(np.zeros(nph), prop=1, centroid=1,
+                                                                      nb_phases=nph, phase_perc=perc))
+    # markers moved: re-sort, then the first P2G of each node set computes
+    # and caches the per-marker weights
+    t_sort, t_w, t_p = [], [], []
+    for _ in range(max(1, reps // 2)):
+        t0 = time.perf_counter()
+        ctx.markers_set(dx_, dz_, dph)
+        t_sort.append((time.perf_counter() - t0) * 1e3)
+        ctx.p2mastah_fused(1, 1, [dict(src=rho_s, dst=en, flag=srt)])
+        t_p.append(ctx.last_kernel_ms())
+        t_w.append(ctx.last_weights_ms())
+    out["moved_resort_wall_ms"] = sorted(t_sort)[len(t_sort) // 2]
+    out["moved_weights_ms"] = sorted(t_w)[len(t_w) // 2]
+    out["moved_first_p2g_ms"] = sorted(t_p)[len(t_p) // 2]
+    for k in list(out):
+        if k.endswith("_ms") and not k.startswith("moved_resort"):
+            out[k.replace("_ms", "_gmarkers_per_s")] = n / (out[k] * 1e-3) / 1e9
     print(json.dumps(out), flush=True)
 
 

@@ -137,6 +137,14 @@ int mdg_stokes_apply(mdg_ctx *ctx, const double *D11_n, const double *D22_n,
  * on the Vx (Nx*(Nz+1)) and Vz ((Nx+1)*Nz) arrays. */
 int mdg_apply_bc(mdg_ctx *ctx, double *u, double *v);
 
+/* ExtractSolutions2 (StokesRoutines.c:536-576, called by the defect step
+ * :858 and the line search :650, :680; file-local, no prototype in
+ * mdoodz-private.h): u_in, v_in, p_in += alpha * sol (equation numbering,
+ * [momentum | continuity]) on every node that carries an unknown, then
+ * ApplyBC.  The non-linear loop's defect step (alpha = -1) and its line-search
+ * trials; host or device arrays. */
+int mdg_extract_solution(mdg_ctx *ctx, const double *sol, double alpha, double *u, double *v, double *p);
+
 /* StrainRateComponents (RheologyDensity.c:2001-2097, incompressible;
  * prototype mdoodz-private.h:426): div, exxd, ezzd on cells and exz on
  * vertices from u_in, v_in. */
@@ -359,6 +367,11 @@ typedef struct {
     double smoother_ms;    /* graphs = 2: mean duration of the fine-level
                               smoother steps (k_cheb3) in this solve          */
     int    smoother_launches;
+    /* input: smoothers 2 and 3 use vertex patches on the finest patch_levels
+     * levels and the diagonal below them (0: every level, the default; the
+     * coarse levels need the patches too: 2 levels of patches take 3x the
+     * V-cycles of all levels at 2047^2)                                      */
+    int    patch_levels;
 } mdg_solver_opts;
 int mdg_killer_solver(mdg_ctx *ctx, mdg_solver_opts *opts, const double *D11_n,
                       const double *D22_n, const double *D33_s, const double *rhs_mom,
@@ -590,7 +603,8 @@ int mdg_pcnet_train_residual(mdg_ctx *ctx, int steps, int batch, float lr, unsig
                              double *losses);
 
 /* Times the dominant kernel of the solve: the fine-level fused Chebyshev
- * smoother step (FP64) of the last FP64 solve, averaged over reps
+ * smoother step (FP64) of the last FP64 solve (the vertex-patch step when
+ * that solve used smoother 3, the Jacobi step otherwise), averaged over reps
  * launches with HIP events on the context stream.  Returns ms per launch and
  * the algorithmic (compulsory) bytes per launch.  No counterpart in the
  * reference; it feeds the roofline figure of the benchmark. */

@@ -15,6 +15,7 @@
 #define P2G_MAXF 4
 #define P2G_MAXPH 32
 #define P2G_MAXTAB 1024
+#define P2G_TILE 256     // nodes per block of the row-tiled kernels
 
 struct p2g_fld {
     const double      *src;   // per-phase table (flag 0) or sorted per-marker (flag 1)
@@ -103,6 +104,100 @@ __global__ void __launch_bounds__(256) k_p2g(p2g_geo g, p2g_set fs, const unsign
     }
 }
 
+// The 9-cell stencil when every marker's nearest node is the one its sort
+// key implies (the node set's cached state says so).  A block owns
+// P2G_TILE consecutive nodes of one node row; the markers that reach them
+// lie in the 3 (cell-centred rows) or 4 (rows on cell edges) cell rows
+// around it, over the block's column span plus one or two cells.  The block
+// streams those rows in chunks with coalesced loads into LDS as (x, z, home
+// node, transformed value), each marker's home taken from its sort key and
+// its value transformed once per chunk instead of once per visit; every
+// node thread then walks, chunk by chunk, the sorted span of its own
+// neighbour cells and sums the markers whose home is one of its 3 x 3
+// neighbours, with the reference's weight expression (|X[home] + o dx - x|
+// for a neighbour at offset o).  Fixed order: bit-reproducible.
+template <int NF> struct tile9_cap { static constexpr int v = (NF <= 2) ? 1024 : 640; };
+
+template <int NF>
+__global__ void __launch_bounds__(P2G_TILE)
+k_p2g9_tile(p2g_geo g, p2g_set fs, const unsigned *__restrict__ qoff, const unsigned *__restrict__ keys,
+            const double *__restrict__ mx, const double *__restrict__ mz, const int *__restrict__ mph,
+            int nbx, int nblocks) {
+    constexpr int CAP = tile9_cap<NF>::v;
+    __shared__ double lx[CAP], lz[CAP], lv[NF][CAP];
+    __shared__ int lh[CAP];
+    const int b = blockIdx.x, xcd = b & 7, per = nblocks >> 3, rem = nblocks & 7;
+    const int tile = xcd * per + min(xcd, rem) + (b >> 3);
+    const int l = tile / nbx, k0 = (tile - l * nbx) * P2G_TILE;
+    const int kend = min(k0 + P2G_TILE, g.nx);
+    const int t = threadIdx.x, k = k0 + t;
+    const bool own = k < kend;
+    const bool xc = (g.ax.xkind == CELL_AXIS), zc = (g.ax.zkind == CELL_AXIS);
+    // cell columns a node (and the block) draws from; cell rows of node row l
+    const int bx0 = max(xc ? k0 - 1 + g.ax.xoff : k0 - 2, 0);
+    const int bx1 = min(xc ? kend + g.ax.xoff : kend, g.Ncx - 1);
+    const int tx0 = max(xc ? k - 1 + g.ax.xoff : k - 2, 0);
+    const int tx1 = min(xc ? k + 1 + g.ax.xoff : k + 1, g.Ncx - 1);
+    const int cz0 = max(zc ? l - 1 + g.ax.zoff : l - 2, 0);
+    const int cz1 = min(zc ? l + 1 + g.ax.zoff : l + 1, g.Ncz - 1);
+    const int node = own ? l * g.nx + k : -1;
+    double W = 0.0, BW[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) BW[f] = 0.0;
+    for (int cz = cz0; cz <= cz1 && bx0 <= bx1; cz++) {
+        const unsigned S = qoff[4u * (unsigned)(bx0 + cz * g.Ncx)];
+        const unsigned E = qoff[4u * (unsigned)(bx1 + cz * g.Ncx) + 4u];
+        unsigned ts = 0u, te = 0u;   // this node's span in the row
+        if (own && tx0 <= tx1) {
+            ts = qoff[4u * (unsigned)(tx0 + cz * g.Ncx)];
+            te = qoff[4u * (unsigned)(tx1 + cz * g.Ncx) + 4u];
+        }
+        for (unsigned cs = S; cs < E; cs += CAP) {
+            const unsigned ce = min(E, cs + (unsigned)CAP);
+            __syncthreads();   // previous chunk consumed
+            for (unsigned j = cs + t; j < ce; j += P2G_TILE) {
+                const unsigned q = j - cs;
+                lx[q] = mx[j];
+                lz[q] = mz[j];
+                const unsigned key = keys[j], cell = key >> 2, qd = key & 3u;
+                const int cx = (int)(cell % (unsigned)g.Ncx), czm = (int)(cell / (unsigned)g.Ncx);
+                const int hk = xc ? cx - g.ax.xoff : cx + (int)(qd & 1u);
+                const int hl = zc ? czm - g.ax.zoff : czm + (int)(qd >> 1);
+                lh[q] = hk + hl * g.nx;
+                const int p = mph[j];
+#pragma unroll
+                for (int f = 0; f < NF; f++) lv[f][q] = mark_val(fs.f[f], p, j);
+            }
+            __syncthreads();
+            const unsigned a = max(ts, cs), e = min(te, ce);
+            for (unsigned i = a; i < e; i++) {
+                const unsigned q = i - cs;
+                const int h = lh[q], hl = h / g.nx, hk = h - hl * g.nx;
+                const int ox = k - hk, oz = l - hl;
+                if (ox < -1 || ox > 1 || oz < -1 || oz > 1) continue;
+                const double x = lx[q], z = lz[q];
+                const double dxm = (ox == 0) ? fabs(g.X[hk] - x) : fabs(g.X[hk] + ox * g.dx - x);
+                const double dzm = (oz == 0) ? fabs(g.Z[hl] - z) : fabs(g.Z[hl] + oz * g.dz - z);
+                const double w = (1.0 - dxm * g.ix_itp) * (1.0 - dzm * g.iz_itp);
+                W += w;
+#pragma unroll
+                for (int f = 0; f < NF; f++) BW[f] += lv[f][q] * w;
+            }
+        }
+    }
+    if (!own) return;
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        const p2g_fld &F = fs.f[f];
+        if (fabs(W) < 1e-30) continue;
+        if (F.ntype && (F.ntype[node] == 30 || F.ntype[node] == 31)) continue;
+        double v = BW[f] / W;
+        if (F.avg == 1) v = 1.0 / v;
+        if (F.avg == 2) v = exp(v);
+        F.dst[node] = v;
+    }
+}
+
 // Per-marker weight of the 1-cell stencil for one node set, in sorted
 // order: the same nearest node and weight the kernels compute inline, kept
 // while the markers do not move (MDOODZ interpolates 15-25 fields per step
@@ -139,7 +234,6 @@ __global__ void __launch_bounds__(256) k_p2g_weights(p2g_geo g, int n, const uns
 // twice.  Consecutive tiles are handed to the same XCD (blocks are
 // dispatched round-robin over the 8 XCDs), so a cell row shared by two
 // blocks is read from HBM once and from that XCD's L2 the second time.
-#define P2G_TILE 256
 template <int NF> struct tile_cap { static constexpr int v = (NF <= 2) ? 1280 : 768; };
 
 template <int NF, int ROWS, bool PRE>
@@ -343,7 +437,15 @@ static int setup_geo(mdg_ctx *c, int centroid, int stencil, p2g_geo *g) {
 // Computes the cached 1-cell-stencil weights of g's node set if they are
 // not current (once per node set after each mdg_markers_set).
 static int ensure_weights(mdg_ctx *c, const p2g_geo &g) {
-    if (g.R != 0 || c->mw_state[g.set] != 0) return MDG_OK;
+    if (c->mw_state[g.set] != 0) return MDG_OK;
+    if (g.R != 0) {
+        // the 9-cell stencil needs the node set's stray state only; the
+        // cached weights are those of the 1-cell stencil
+        p2g_geo g1;
+        const int centroid = g.set == 0 ? 1 : g.set == 1 ? 0 : g.set == 2 ? -1 : -2;
+        int rc = setup_geo(c, centroid, 1, &g1);
+        return rc ? rc : ensure_weights(c, g1);
+    }
     const int n = c->n_markers;
     HIPCHK(c, c->mw[g.set].reserve((size_t)n + 1));
     HIPCHK(c, hipMemsetAsync(c->flag_dev.p, 0, sizeof(int), c->stream));
@@ -390,6 +492,21 @@ static int launch_fields(mdg_ctx *c, p2g_geo g, const p2g_set &s, int nf) {
         return MDG_OK;
     }
     dim3 grid(mdg_grid1d(nn, 256)), block(256);
+    if (g.R == 1 && g.S == 1 && c->mw_state[g.set] == 1) {
+        // 9-cell stencil on exact homes: row tiles
+        const int nbx = (g.nx + P2G_TILE - 1) / P2G_TILE, nb = nbx * g.nz;
+#define T9(NF) hipLaunchKernelGGL(k_p2g9_tile<NF>, dim3(nb), dim3(P2G_TILE), 0, c->stream, g, s, c->qoff.p, \
+                                  (const unsigned *)c->keys2.p, c->mx.p, c->mz.p, c->mphase.p, nbx, nb)
+        switch (nf) {
+        case 1: T9(1); break;
+        case 2: T9(2); break;
+        case 3: T9(3); break;
+        default: T9(4); break;
+        }
+#undef T9
+        HIPCHK(c, hipGetLastError());
+        return MDG_OK;
+    }
     switch (nf) {
     case 1: hipLaunchKernelGGL(k_p2g<1>, grid, block, 0, c->stream, g, s, c->qoff.p, c->mx.p, c->mz.p, c->mphase.p, c->flag_dev.p); break;
     case 2: hipLaunchKernelGGL(k_p2g<2>, grid, block, 0, c->stream, g, s, c->qoff.p, c->mx.p, c->mz.p, c->mphase.p, c->flag_dev.p); break;

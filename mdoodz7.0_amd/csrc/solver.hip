@@ -507,17 +507,32 @@ __global__ void k_asm_upd(sgrid g, T a, T b, const T *__restrict__ pinv, const T
 // viscosities) into LDS, and each face adds its two patches' values.  The
 // HBM traffic is that of the Jacobi step (x_k, x_{k-1}, b, x_{k+1}, eta):
 // no patch inverses are stored.  FIRST: a = 0 and x_{k-1} is not read.
-#define PC_TX 64
-#define PC_TZ 8
-#define PC_THREADS 256
+// 1/sqrt(x) for the patch pivots (the patch solve only preconditions the
+// smoother, so the last bits do not matter, but the map must stay fixed):
+// the hardware reciprocal square root in FP32; in FP64 the hardware
+// estimate refined by one Newton step y (3 - x y^2) / 2, written as
+// y + (y/2)(1 - x y^2), which squares the estimate's relative error.
+__device__ inline float prec_rsqrt(float x) { return rsqrtf(x); }
+__device__ inline double prec_rsqrt(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+// Tiles: 64 x 8 positions and 256 threads on the large levels; 16 x 4
+// positions and one wave on levels with fewer than PC_SMALL_BELOW large
+// tiles, where a few large workgroups would leave most CUs idle and each
+// workgroup's serial phases set the step's latency.  The tile shape does
+// not change any value computed.
+#define PC_SMALL_BELOW 1024
 struct pc_tiles {
     int nbx, ntiles;
 };
+template <int TX, int TZ>
 static inline pc_tiles make_pc_tiles(const sgrid &g) {
-    const int nbx = (g.Nx + 1 + PC_TX - 1) / PC_TX, nbz = (g.Nz + 1 + PC_TZ - 1) / PC_TZ;
+    const int nbx = (g.Nx + 1 + TX - 1) / TX, nbz = (g.Nz + 1 + TZ - 1) / TZ;
     return pc_tiles{nbx, nbx * nbz};
 }
-template <typename T, bool FIRST>
+template <typename T, bool FIRST, int PC_TX, int PC_TZ, int PC_THREADS>
 __global__ void __launch_bounds__(PC_THREADS) k_pcheb(sgrid g, stc<T> cf, pc_tiles tg, T gamma, T a, T cc,
                                                       const T *__restrict__ xc, T *__restrict__ xo,
                                                       const T *__restrict__ b, const T *__restrict__ en,
@@ -577,22 +592,24 @@ __global__ void __launch_bounds__(PC_THREADS) k_pcheb(sgrid g, stc<T> cf, pc_til
             M[0][3] = (fr[0] && fr[3]) ? -(D + gamma * w(a_, b_ - 1)) * cf.idxz * cf.dV : T(0);
             M[1][2] = (fr[1] && fr[2]) ? -(D + gamma * w(a_ - 1, b_)) * cf.idxz * cf.dV : T(0);
             M[1][3] = (fr[1] && fr[3]) ? (D + gamma * w(a_, b_)) * cf.idxz * cf.dV : T(0);
-            // Cholesky M = L L^T (lower triangle in place), then L L^T y = r
-            T L00 = sqrt(M[0][0]);
-            T L10 = M[0][1] / L00, L20 = M[0][2] / L00, L30 = M[0][3] / L00;
-            T L11 = sqrt(M[1][1] - L10 * L10);
-            T L21 = (M[1][2] - L20 * L10) / L11, L31 = (M[1][3] - L30 * L10) / L11;
-            T L22 = sqrt(M[2][2] - L20 * L20 - L21 * L21);
-            T L32 = (M[2][3] - L30 * L20 - L31 * L21) / L22;
-            T L33 = sqrt(M[3][3] - L30 * L30 - L31 * L31 - L32 * L32);
-            const T z0 = r[0] / L00;
-            const T z1 = (r[1] - L10 * z0) / L11;
-            const T z2 = (r[2] - L20 * z0 - L21 * z1) / L22;
-            const T z3 = (r[3] - L30 * z0 - L31 * z1 - L32 * z2) / L33;
-            y[3] = z3 / L33;
-            y[2] = (z2 - L32 * y[3]) / L22;
-            y[1] = (z1 - L21 * y[2] - L31 * y[3]) / L11;
-            y[0] = (z0 - L10 * y[1] - L20 * y[2] - L30 * y[3]) / L00;
+            // Cholesky M = L L^T with the reciprocal pivots i_k = 1 / L_kk
+            // (one reciprocal square root per pivot, no other division),
+            // then L L^T y = r
+            const T i0 = prec_rsqrt(M[0][0]);
+            const T L10 = M[0][1] * i0, L20 = M[0][2] * i0, L30 = M[0][3] * i0;
+            const T i1 = prec_rsqrt(M[1][1] - L10 * L10);
+            const T L21 = (M[1][2] - L20 * L10) * i1, L31 = (M[1][3] - L30 * L10) * i1;
+            const T i2 = prec_rsqrt(M[2][2] - L20 * L20 - L21 * L21);
+            const T L32 = (M[2][3] - L30 * L20 - L31 * L21) * i2;
+            const T i3 = prec_rsqrt(M[3][3] - L30 * L30 - L31 * L31 - L32 * L32);
+            const T z0 = r[0] * i0;
+            const T z1 = (r[1] - L10 * z0) * i1;
+            const T z2 = (r[2] - L20 * z0 - L21 * z1) * i2;
+            const T z3 = (r[3] - L30 * z0 - L31 * z1 - L32 * z2) * i3;
+            y[3] = z3 * i3;
+            y[2] = (z2 - L32 * y[3]) * i2;
+            y[1] = (z1 - L21 * y[2] - L31 * y[3]) * i1;
+            y[0] = (z0 - L10 * y[1] - L20 * y[2] - L30 * y[3]) * i0;
         }
         for (int i = 0; i < 4; i++) yv[i][q] = y[i];
     }
@@ -1207,7 +1224,8 @@ struct mg_level {
     sgrid g;
     int nu = 0, nv = 0, n = 0;
     dbuf<T> en, es, dinv, x, b, r, d;
-    dbuf<T> pinv;          // vertex-patch inverses (smoother 2)
+    dbuf<T> pinv;          // vertex-patch inverses (smoother 2; smoother 3's eigenvalue estimate)
+    bool patch = false;    // smoothed with vertex patches (smoother 2, 3 on the finest patch_levels levels)
     double lmax = 1.0;
     double cheb_div = 30.0;  // smoothed range [lmax / cheb_div, lmax]
     const T *bsrc = nullptr;  // right-hand side read in place of b (the FP64 CG residual on level 0)
@@ -1396,7 +1414,9 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
                                o->eta_avg, fl.en.p, fl.es.p, lv.en.p, lv.es.p);
         }
         hipLaunchKernelGGL(k_vel_diag_inv<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, g, (T)gamma, lv.en.p, lv.es.p, lv.dinv.p);
-        if (H.patch && !(H.fused && lv.n <= CHEB_BLOCK_MAX)) {
+        const int np = o->patch_levels > 0 ? o->patch_levels : 1 << 30;   // 0: every level
+        lv.patch = H.patch && (int)H.L.size() <= np && !(H.fused && lv.n <= CHEB_BLOCK_MAX);
+        if (lv.patch) {
             HIPCHK(c, lv.pinv.reserve((size_t)PATCH_Q * g.Nx * g.Nz));
             hipLaunchKernelGGL(k_patch_inv<T>, dim3(nblk(g.Nx * g.Nz)), dim3(256), 0, c->stream, g, (T)gamma,
                                (const T *)lv.en.p, (const T *)lv.es.p, lv.pinv.p);
@@ -1451,7 +1471,7 @@ static int build_hierarchy(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg
         for (int it = 0; it < 30; it++) {
             hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p,
                                (const T *)nullptr, lv.en.p, lv.es.p, lv.r.p);     // r = -A x
-            if (H.patch && !(H.fused && lv.n <= CHEB_BLOCK_MAX))
+            if (lv.patch)
                 hipLaunchKernelGGL(k_asm_scale<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, T(-1),
                                    (const T *)lv.pinv.p, (const T *)lv.r.p, lv.d.p);
             else
@@ -1732,20 +1752,34 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
     }
     if (fused && patch) {
         // three-term form with the in-kernel vertex-patch inverse
-        const pc_tiles tg = make_pc_tiles(lv.g);
+        const bool small = make_pc_tiles<64, 8>(lv.g).ntiles < PC_SMALL_BELOW;
+        const pc_tiles tg = small ? make_pc_tiles<16, 4>(lv.g) : make_pc_tiles<64, 8>(lv.g);
         const stc<T> cf = make_stc<T>(lv.g);
         T *cur = lv.d.p, *prev = lv.x.p;
+        auto step = [&](bool first, T a, T cc, const T *xc, T *xo) {
+            if (small) {
+                if (first)
+                    hipLaunchKernelGGL((k_pcheb<T, true, 16, 4, 64>), dim3(tg.ntiles), dim3(64), 0, c->stream, lv.g, cf,
+                                       tg, (T)gamma, a, cc, xc, xo, lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
+                else
+                    hipLaunchKernelGGL((k_pcheb<T, false, 16, 4, 64>), dim3(tg.ntiles), dim3(64), 0, c->stream, lv.g, cf,
+                                       tg, (T)gamma, a, cc, xc, xo, lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
+            } else {
+                if (first)
+                    hipLaunchKernelGGL((k_pcheb<T, true, 64, 8, 256>), dim3(tg.ntiles), dim3(256), 0, c->stream, lv.g,
+                                       cf, tg, (T)gamma, a, cc, xc, xo, lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
+                else
+                    hipLaunchKernelGGL((k_pcheb<T, false, 64, 8, 256>), dim3(tg.ntiles), dim3(256), 0, c->stream, lv.g,
+                                       cf, tg, (T)gamma, a, cc, xc, xo, lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
+            }
+        };
         if (zero_start)
             hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, prev, T(0));
-        hipLaunchKernelGGL((k_pcheb<T, true>), dim3(tg.ntiles), dim3(PC_THREADS), 0, c->stream, lv.g, cf, tg, (T)gamma,
-                           T(0), (T)(1.0 / theta), (const T *)prev, cur, lv.rhs(), (const T *)lv.en.p,
-                           (const T *)lv.es.p);
+        step(true, T(0), (T)(1.0 / theta), (const T *)prev, cur);
         for (int k = 0; k < deg - 1; k++) {
             const double rho_new = 1.0 / (2.0 * sigma - rho);
             if (tm) tm->begin(c->stream);
-            hipLaunchKernelGGL((k_pcheb<T, false>), dim3(tg.ntiles), dim3(PC_THREADS), 0, c->stream, lv.g, cf, tg,
-                               (T)gamma, (T)(rho_new * rho), (T)(2.0 * rho_new / delta), (const T *)cur, prev,
-                               lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
+            step(false, (T)(rho_new * rho), (T)(2.0 * rho_new / delta), (const T *)cur, prev);
             if (tm) tm->end(c->stream);
             std::swap(cur, prev);
             rho = rho_new;
@@ -1821,7 +1855,7 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, i
     }
     mg_level<T> &cl = H.L[i + 1];
     ktimer *tm = (i == 0) ? H.tm : nullptr;
-    chebyshev(c, lv, gamma, deg, true, H.fused, tm, H.patch);
+    chebyshev(c, lv, gamma, deg, true, H.fused, tm, lv.patch);
     const int reps = (i + 2 == H.L.size()) ? 1 : cycle;
     for (int rep = 0; rep < reps; rep++) {
         hipLaunchKernelGGL(k_vel_residual<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, (T)gamma, lv.x.p, lv.rhs(),
@@ -1835,7 +1869,7 @@ static void vcycle(mdg_ctx *c, mg_hier<T> &H, size_t i, double gamma, int deg, i
         vcycle(c, H, i + 1, gamma, deg, cycle);
         hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(lv.n)), dim3(256), 0, c->stream, lv.g, cl.g, lv.mu, lv.mv, cl.x.p, lv.x.p);
     }
-    chebyshev(c, lv, gamma, deg, false, H.fused, tm, H.patch);
+    chebyshev(c, lv, gamma, deg, false, H.fused, tm, lv.patch);
 }
 
 // Records one CG iteration (preconditioning V-cycle, direction, operator
@@ -2524,12 +2558,21 @@ extern "C" int mdg_time_smoother(mdg_ctx *c, int reps, double *ms, double *bytes
     if (c->solver->hd.cg_r.p)
         HIPCHK(c, hipMemcpyAsync(l0.b.p, c->solver->hd.cg_r.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
                                  c->stream));
+    // the fine-level step of the last FP64 solve's smoother: the fused
+    // vertex-patch step (smoother 3) or the fused Jacobi step
+    const bool patch = c->solver->hd.fused && l0.patch;
     const tile_grid tg = make_tiles(l0.g);
+    const pc_tiles pg = make_pc_tiles<64, 8>(l0.g);
     const stc<double> cf = make_stc<double>(l0.g);
     auto launch = [&]() {
-        hipLaunchKernelGGL((k_cheb3<double, false>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, l0.g, cf, tg, 10.0, 0.5,
-                           0.1, (const double *)l0.x.p, l0.d.p, (const double *)l0.b.p, (const double *)l0.en.p,
-                           (const double *)l0.es.p);
+        if (patch)
+            hipLaunchKernelGGL((k_pcheb<double, false, 64, 8, 256>), dim3(pg.ntiles), dim3(256), 0, c->stream, l0.g, cf,
+                               pg, 10.0, 0.5, 0.1, (const double *)l0.x.p, l0.d.p, (const double *)l0.b.p,
+                               (const double *)l0.en.p, (const double *)l0.es.p);
+        else
+            hipLaunchKernelGGL((k_cheb3<double, false>), dim3(tg.ntiles), dim3(CHEB_TX), 0, c->stream, l0.g, cf, tg, 10.0,
+                               0.5, 0.1, (const double *)l0.x.p, l0.d.p, (const double *)l0.b.p,
+                               (const double *)l0.en.p, (const double *)l0.es.p);
     };
     launch();
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));

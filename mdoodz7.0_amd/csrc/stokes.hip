@@ -366,6 +366,61 @@ extern "C" int mdg_apply_bc(mdg_ctx *c, double *u, double *v) {
     return MDG_OK;
 }
 
+// ExtractSolutions2 (StokesRoutines.c:536-576): u, v, p += alpha * the
+// correction on every node that carries an unknown (the reference's tag test
+// selects exactly the numbered nodes of a closed box), then ApplyBC.
+__global__ void k_extract_add(int nu, int nv, int nc, const int *__restrict__ eu, const int *__restrict__ ev,
+                              const int *__restrict__ ep, const double *__restrict__ x, double alpha,
+                              double *__restrict__ u, double *__restrict__ v, double *__restrict__ p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nu) {
+        const int e = eu[i];
+        if (e >= 0) u[i] = u[i] + alpha * x[e];
+    } else if (i < nu + nv) {
+        const int e = ev[i - nu];
+        if (e >= 0) v[i - nu] = v[i - nu] + alpha * x[e];
+    } else if (i < nu + nv + nc) {
+        const int e = ep[i - nu - nv];
+        if (e >= 0) p[i - nu - nv] = p[i - nu - nv] + alpha * x[e];
+    }
+}
+
+extern "C" int mdg_extract_solution(mdg_ctx *c, const double *sol, double alpha, double *u, double *v, double *p) {
+    if (!c || !sol || !u || !v || !p) return mdg_fail(c, MDG_ERR_ARG, "null array");
+    if (!c->have_bc) return mdg_fail(c, MDG_ERR_STATE, "mdg_set_bc has not been called");
+    const sgrid g = c->sg;
+    const int nu = nvx(g), nv = nvz(g), nc = ncell(g);
+    const double *dx;
+    int rc;
+    if ((rc = mdg_stage_in(c, sol, (size_t)c->neq, c->eqbuf, &dx))) return rc;
+    double *user[3] = {u, v, p}, *d[3];
+    const size_t sz[3] = {(size_t)nu, (size_t)nv, (size_t)nc};
+    const bool dev = mdg_is_device_ptr(u) && mdg_is_device_ptr(v) && mdg_is_device_ptr(p);
+    if (dev) {
+        for (int q = 0; q < 3; q++) d[q] = user[q];
+    } else {
+        HIPCHK(c, c->tmp_a.reserve(sz[0] + sz[1] + sz[2]));
+        d[0] = c->tmp_a.p; d[1] = d[0] + sz[0]; d[2] = d[1] + sz[1];
+        for (int q = 0; q < 3; q++)
+            HIPCHK(c, hipMemcpyAsync(d[q], user[q], sz[q] * 8, mdg_is_device_ptr(user[q]) ? hipMemcpyDeviceToDevice
+                                                                                          : hipMemcpyHostToDevice,
+                                     c->stream));
+    }
+    hipLaunchKernelGGL(k_extract_add, dim3(mdg_grid1d(nu + nv + nc, 256)), dim3(256), 0, c->stream, nu, nv, nc,
+                       c->eqn_u.p, c->eqn_v.p, c->eqn_p.p, dx, alpha, d[0], d[1], d[2]);
+    const int n = g.Nx > g.Nz ? g.Nx : g.Nz;
+    hipLaunchKernelGGL(k_apply_bc, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, g, c->BCu_t.p,
+                       c->BCv_t.p, c->BCu_v.p, c->BCv_v.p, d[0], d[1]);
+    HIPCHK(c, hipGetLastError());
+    if (!dev)
+        for (int q = 0; q < 3; q++)
+            HIPCHK(c, hipMemcpyAsync(user[q], d[q], sz[q] * 8, mdg_is_device_ptr(user[q]) ? hipMemcpyDeviceToDevice
+                                                                                          : hipMemcpyDeviceToHost,
+                                     c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDG_OK;
+}
+
 extern "C" int mdg_strain_rates(mdg_ctx *c, const double *u, const double *v, double *div,
                                 double *exxd, double *ezzd, double *exz) {
     if (!c || !u || !v || !div || !exxd || !ezzd || !exz) return mdg_fail(c, MDG_ERR_ARG, "null array");

@@ -116,7 +116,7 @@ class SolverOpts(C.Structure):
                 ("lin_abs_div", C.c_double), ("lin_rel_div", C.c_double), ("pc_tol", C.c_double),
                 ("max_mom", C.c_double), ("max_mom0", C.c_double), ("max_div", C.c_double),
                 ("max_div0", C.c_double), ("its_PH", C.c_int), ("its_KSP", C.c_int),
-                ("smoother_ms", C.c_double), ("smoother_launches", C.c_int)]
+                ("smoother_ms", C.c_double), ("smoother_launches", C.c_int), ("patch_levels", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -168,6 +168,7 @@ def _extra_signatures(L):
     L.mdg_stokes_apply.argtypes = [vp] * 6
     L.mdg_apply_bc.argtypes = [vp, vp, vp]
     L.mdg_strain_rates.argtypes = [vp] * 7
+    L.mdg_extract_solution.argtypes = [vp, vp, C.c_double, vp, vp, vp]
     L.mdg_eval_rhs.argtypes = [vp, C.c_double, C.c_double, vp, vp, vp, vp]
     L.mdg_stokes_residual.argtypes = [vp] * 11
     L.mdg_stokes_residual_static.argtypes = [vp] * 11
@@ -235,6 +236,11 @@ def checked(a, dtype, n, name, optional=False):
     raise TypeError("%s: expected a numpy array or DeviceArray, got %r" % (name, type(a)))
 
 
+def _flat(a):
+    """A DeviceArray as it is, anything else as a contiguous float64 array."""
+    return a if isinstance(a, DeviceArray) else np.ascontiguousarray(a, np.float64)
+
+
 class DeviceArray:
     """A flat device array allocated with mdg_dev_alloc on a context."""
 
@@ -260,6 +266,13 @@ class DeviceArray:
         out = np.empty(self.n, self.dtype)
         self.ctx._check(self.ctx._L.mdg_memcpy(self.ctx.h, out.ctypes.data, self.p, out.nbytes))
         return out
+
+    def copy_from(self, other):
+        """Device-to-device copy of another DeviceArray of the same size."""
+        if other.n != self.n or other.dtype != self.dtype:
+            raise ValueError("copy_from: %d x %s into %d x %s" % (other.n, other.dtype, self.n, self.dtype))
+        self.ctx._check(self.ctx._L.mdg_memcpy(self.ctx.h, self.p, other.p, self.n * self.dtype.itemsize))
+        return self
 
     def view(self, offset, n):
         """A non-owning sub-array starting at element offset."""
@@ -545,10 +558,17 @@ class Context:
     def apply_bc(self, u, v):
         self._check(self._L.mdg_apply_bc(self.h, ptr(u), ptr(v)))
 
-    def strain_rates(self, u, v):
+    def extract_solution(self, sol, alpha, u, v, p):
+        """ExtractSolutions2 (StokesRoutines.c:536-576) + ApplyBC, in place."""
+        self._check(self._L.mdg_extract_solution(self.h, ptr(sol), float(alpha), ptr(u), ptr(v), ptr(p)))
+
+    def strain_rates(self, u, v, out=None):
+        """StrainRateComponents' cell rates and vertex exz; out (optional)
+        holds div, exxd, ezzd, exz arrays to write into (e.g. DeviceArray)."""
         nc = (self.Nx - 1) * (self.Nz - 1)
-        out = dict(div=np.zeros(nc), exxd=np.zeros(nc), ezzd=np.zeros(nc),
-                   exz=np.zeros(self.Nx * self.Nz))
+        if out is None:
+            out = dict(div=np.zeros(nc), exxd=np.zeros(nc), ezzd=np.zeros(nc),
+                       exz=np.zeros(self.Nx * self.Nz))
         self._check(self._L.mdg_strain_rates(self.h, ptr(u), ptr(v), ptr(out["div"]),
                                              ptr(out["exxd"]), ptr(out["ezzd"]), ptr(out["exz"])))
         return out
@@ -630,12 +650,13 @@ class Context:
         self._check(self._L.mdg_c2v(self.h, int(periodic_x), ptr(cent), ptr(vert)))
         return vert
 
-    def strain_rates_nodes(self, exxd, ezzd, exz, div):
+    def strain_rates_nodes(self, exxd, ezzd, exz, div, out=None):
         """exz_n, exxd_s, ezzd_s, div_s of StrainRateComponents
-        (RheologyDensity.c:2066-2094)."""
+        (RheologyDensity.c:2066-2094); out as for strain_rates."""
         nc, nv = (self.Nx - 1) * (self.Nz - 1), self.Nx * self.Nz
-        out = dict(exz_n=np.zeros(nc), exxd_s=np.zeros(nv), ezzd_s=np.zeros(nv),
-                   div_s=np.zeros(nv))
+        if out is None:
+            out = dict(exz_n=np.zeros(nc), exxd_s=np.zeros(nv), ezzd_s=np.zeros(nv),
+                       div_s=np.zeros(nv))
         self._check(self._L.mdg_strain_rates_nodes(
             self.h, ptr(exxd), ptr(ezzd), ptr(exz), ptr(div), ptr(out["exz_n"]),
             ptr(out["exxd_s"]), ptr(out["ezzd_s"]), ptr(out["div_s"])))
@@ -654,10 +675,12 @@ class Context:
                                                                      "dil")]))
         return out
 
-    def tangent_grid(self, phases, prm, centroid, **arrays):
-        """Newton tangent coefficients: (8, cells) or (4, vertices)."""
+    def tangent_grid(self, phases, prm, centroid, D=None, **arrays):
+        """Newton tangent coefficients: (8, cells) or (4, vertices); D may be
+        a DeviceArray of that many values to write into."""
         n = self.node_count(centroid)
-        D = np.zeros(((8 if centroid else 4), n))
+        if D is None:
+            D = np.zeros(((8 if centroid else 4), n))
         f, keep = RheoNodes(), []
         for k in RHEO_IN:
             a = arrays.get(k)
@@ -677,15 +700,16 @@ class Context:
                                                ptr(np.ascontiguousarray(x)), ptr(y)))
         return y
 
-    def newton_solver(self, Dn, Ds, D11, D33, rhs_mom, rhs_cont, accept_diverged=False, **opts):
+    def newton_solver(self, Dn, Ds, D11, D33, rhs_mom, rhs_cont, accept_diverged=False, sol=None, **opts):
+        """mdg_newton_solver: returns (sol, SolverOpts); the arrays may be
+        DeviceArray (sol then receives the solution on the device)."""
         o = SolverOpts()
         for k, v in opts.items():
             setattr(o, k, v)
-        sol = np.zeros(len(rhs_mom) + len(rhs_cont))
-        rc = self._L.mdg_newton_solver(self.h, C.byref(o), ptr(np.ascontiguousarray(Dn)),
-                                       ptr(np.ascontiguousarray(Ds)), ptr(D11), ptr(D33),
-                                       ptr(np.ascontiguousarray(rhs_mom)),
-                                       ptr(np.ascontiguousarray(rhs_cont)), ptr(sol))
+        if sol is None:
+            sol = np.zeros(len(rhs_mom) + len(rhs_cont))
+        rc = self._L.mdg_newton_solver(self.h, C.byref(o), ptr(_flat(Dn)), ptr(_flat(Ds)), ptr(D11), ptr(D33),
+                                       ptr(_flat(rhs_mom)), ptr(_flat(rhs_cont)), ptr(sol))
         self._solved(rc, o, accept_diverged)
         return sol, o
 
@@ -754,17 +778,18 @@ class Context:
             return
         self._check(rc)
 
-    def killer_solver(self, D11, D22, D33, rhs_mom, rhs_cont, accept_diverged=False, **opts):
+    def killer_solver(self, D11, D22, D33, rhs_mom, rhs_cont, accept_diverged=False, sol=None, **opts):
         """Replacement of KillerSolver: returns (sol, SolverOpts).  With
         accept_diverged, a solve that misses its tolerance returns its last
-        iterate with o.status = MDG_ERR_DIVERGED instead of raising."""
+        iterate with o.status = MDG_ERR_DIVERGED instead of raising.  The
+        arrays may be DeviceArray (sol then receives the solution there)."""
         o = SolverOpts()
         for k, v in opts.items():
             setattr(o, k, v)
-        sol = np.zeros(len(rhs_mom) + len(rhs_cont))
+        if sol is None:
+            sol = np.zeros(len(rhs_mom) + len(rhs_cont))
         rc = self._L.mdg_killer_solver(self.h, C.byref(o), ptr(D11), ptr(D22), ptr(D33),
-                                       ptr(np.ascontiguousarray(rhs_mom)),
-                                       ptr(np.ascontiguousarray(rhs_cont)), ptr(sol))
+                                       ptr(_flat(rhs_mom)), ptr(_flat(rhs_cont)), ptr(sol))
         self._solved(rc, o, accept_diverged)
         return sol, o
 

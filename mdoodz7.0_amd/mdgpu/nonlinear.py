@@ -14,13 +14,24 @@ routines:
   convergence test                     Main_DOODZ.c:1003
   SolveStokesDefectDecoupled           -> Context.killer_solver, alpha = -1
                                           or LineSearchDecoupled's step
-  ExtractSolutions2 + ApplyBC          scatter below + Context.apply_bc
+  ExtractSolutions2 + ApplyBC          scatter below + Context.apply_bc, or
+                                          Context.extract_solution on the device
+
+With device=True the state is uploaded once per run() and every field of the
+loop stays in device memory (DeviceArray): velocity, pressure, strain rates,
+the rheology outputs, tangents, residual, scaling vector and correction.  The
+host sees three residual norms per evaluation and the solver statistics.
 """
+import time
 import warnings
 
 import numpy as np
 
-from . import MDG_ERR_DIVERGED, RheoParams
+from . import MDG_ERR_DIVERGED, RHEO_OUT, DeviceArray, RheoParams
+
+# fields of the state that the loop reads (uploaded once in device mode)
+_STATE_IN = ("u", "v", "p", "T", "mu_n", "mu_s", "perc_n", "perc_s", "sxxd0", "szzd0", "sxz0_n",
+             "sxxd0_s", "szzd0_s", "sxz0", "rho_n", "rho_s", "pref_n", "pref_s")
 
 
 def scatter(eqn_u, eqn_v, eqn_p, dx, u, v, p, alpha=-1.0):
@@ -40,15 +51,35 @@ class PicardLoop:
     (nph x cells), perc_s (nph x vertices), the old stresses sxxd0, szzd0,
     sxz0_n (centres) and sxxd0_s, szzd0_s, sxz0 (vertices), rho_n, rho_s."""
 
-    def __init__(self, ctx, eqn_u, eqn_v, eqn_p, phases, prm: RheoParams):
+    def __init__(self, ctx, eqn_u, eqn_v, eqn_p, phases, prm: RheoParams, device=False):
         self.ctx, self.phases, self.prm = ctx, phases, prm
         self.eqn = (eqn_u, eqn_v, eqn_p)
+        self.device = device
+        self.W = None          # device work arrays (device mode)
+
+    def _work(self):
+        """Device work arrays, allocated once per loop."""
+        if self.W is not None:
+            return self.W
+        c = self.ctx
+        nc, nv = (c.Nx - 1) * (c.Nz - 1), c.Nx * c.Nz
+        neq = c.eqn_count()[0]
+        nu, nw = c.Nx * (c.Nz + 1), (c.Nx + 1) * c.Nz
+        sizes = dict(div=nc, exxd=nc, ezzd=nc, exz=nv, exz_n=nc, exxd_s=nv, ezzd_s=nv, div_s=nv,
+                     T_s=nv, P_s=nv, D11=nc, D33=nv, eta_n=nc, F=neq, d=neq, sol=neq, rx=nu, rz=nw, u0=nu,
+                     v0=nw, p0=nc, Dn=8 * nc, Ds=4 * nv)
+        W = {k: DeviceArray(c, n) for k, n in sizes.items()}
+        W["cen"] = {k: DeviceArray(c, nc) for k in RHEO_OUT}
+        W["ver"] = {k: DeviceArray(c, nv) for k in RHEO_OUT}
+        self.W = W
+        return W
 
     def node_inputs(self, st, sr):
         """Rheology inputs on centres and vertices (NonNewtonianViscosityGrid
         arguments; T and p_in carried to the vertices, RheologyDensity.c:1150-1151)."""
-        c = self.ctx
-        T_s, P_s = c.c2v(st["T"]), c.c2v(st["p"])
+        c, W = self.ctx, self.W if self.device else None
+        T_s = c.c2v(st["T"], W["T_s"] if W else None)
+        P_s = c.c2v(st["p"], W["P_s"] if W else None)
         cin = dict(perc=st["perc_n"], G=st["mu_n"], T=st["T"], P=st["p"], exx=sr["exxd"],
                    ezz=sr["ezzd"], exz=sr["exz_n"], sxx0=st["sxxd0"], szz0=st["szzd0"],
                    sxz0=st["sxz0_n"])
@@ -60,20 +91,52 @@ class PicardLoop:
         return cin, vin
 
     def update_nonlinearity(self, st):
-        c = self.ctx
+        c, W = self.ctx, self.W if self.device else None
         c.apply_bc(st["u"], st["v"])
-        sr = c.strain_rates(st["u"], st["v"])
-        sr.update(c.strain_rates_nodes(sr["exxd"], sr["ezzd"], sr["exz"], sr["div"]))
+        if W:
+            sr = c.strain_rates(st["u"], st["v"], out={k: W[k] for k in ("div", "exxd", "ezzd", "exz")})
+            sr.update(c.strain_rates_nodes(sr["exxd"], sr["ezzd"], sr["exz"], sr["div"],
+                                           out={k: W[k] for k in ("exz_n", "exxd_s", "ezzd_s", "div_s")}))
+        else:
+            sr = c.strain_rates(st["u"], st["v"])
+            sr.update(c.strain_rates_nodes(sr["exxd"], sr["ezzd"], sr["exz"], sr["div"]))
         cin, vin = self.node_inputs(st, sr)
-        cen = c.viscosity_grid(self.phases, self.prm, 1, **cin)
-        ver = c.viscosity_grid(self.phases, self.prm, 0, **vin)
+        cen = c.viscosity_grid(self.phases, self.prm, 1, **cin, **(W["cen"] if W else {}))
+        ver = c.viscosity_grid(self.phases, self.prm, 0, **vin, **(W["ver"] if W else {}))
         self.last_inputs = (cin, vin)
         return sr, cen, ver
 
+    def _residual(self, cen, ver, sr, rx, rz, d=None):
+        F = self.W["F"] if self.device else None
+        return self.ctx.stokes_residual(cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"], sr["div"], rx, rz,
+                                        d=d, F=F)
+
+    def _step(self, st, dx, alpha):
+        """ExtractSolutions2 + ApplyBC on the state."""
+        if self.device:
+            self.ctx.extract_solution(dx, alpha, st["u"], st["v"], st["p"])
+        else:
+            scatter(*self.eqn, dx, st["u"], st["v"], st["p"], alpha)
+            self.ctx.apply_bc(st["u"], st["v"])
+
+    def _save(self, st):
+        if self.device:
+            W = self.W
+            for k, b in (("u", "u0"), ("v", "v0"), ("p", "p0")):
+                W[b].copy_from(st[k])
+            return W["u0"], W["v0"], W["p0"]
+        return [st[k].copy() for k in ("u", "v", "p")]
+
+    def _restore(self, st, saved):
+        for k, a in zip(("u", "v", "p"), saved):
+            if self.device:
+                st[k].copy_from(a)
+            else:
+                st[k][:] = a
+
     def residual_norms(self, st, d, rx, rz):
         sr, cen, ver = self.update_nonlinearity(st)
-        _, norms = self.ctx.stokes_residual(cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"],
-                                            sr["div"], rx, rz, d=d)
+        _, norms = self._residual(cen, ver, sr, rx, rz, d=d)
         return np.asarray(norms, dtype=float)
 
     def line_search(self, st, dx, d, rx, rz, res_f, min_step=0.0, let_res_grow=0, newton=False):
@@ -85,7 +148,7 @@ class PicardLoop:
         # Newton steps search [-1, -min_step], Picard steps [-2.5, 0] (:603-613)
         ntry = 6
         minalpha, maxalpha = (-1.0, -min_step) if newton else (-2.5, -0.0)
-        saved = [st[k].copy() for k in ("u", "v", "p")]
+        saved = self._save(st)
         dalpha = abs(maxalpha - minalpha) / (ntry - 1)
         trials, rows = [], []
         alpha = maxalpha
@@ -93,11 +156,10 @@ class PicardLoop:
             if kk > 0:
                 alpha -= dalpha
             trials.append(alpha)
-            st["u"][:], st["v"][:], st["p"][:] = saved
-            scatter(*self.eqn, dx, st["u"], st["v"], st["p"], alpha)
-            self.ctx.apply_bc(st["u"], st["v"])
+            self._restore(st, saved)
+            self._step(st, dx, alpha)
             rows.append(self.residual_norms(st, d, rx, rz))
-        st["u"][:], st["v"][:], st["p"][:] = saved
+        self._restore(st, saved)
         R = np.array(rows)
         comb = np.sqrt((R ** 2).sum(1))
         xz = np.hypot(R[:, 0], R[:, 1])
@@ -129,29 +191,48 @@ class PicardLoop:
         False (the reference's default Picard2Newton = 0, InputOutput.c:1193)
         every iteration is a Newton step.  Newton steps use the tangent
         coefficients and the line search (InputOutput.c:1334).
-        st["newton_steps"] lists the iteration numbers solved with Newton."""
+        st["newton_steps"] lists the iteration numbers solved with Newton,
+        st["iter_ms"] the wall time of each iteration (rheology, residual,
+        solve, line search and update).  In device mode st's fields are
+        uploaded once (st["upload_ms"]) and u, v, p, eta_n, eta_s come back
+        at the end."""
         c = self.ctx
         # the reference's continuity tolerances (InputOutput.c:1179-1180)
         # for KillerSolver's a-posteriori incompressibility check
         solver_opts.setdefault("lin_abs_div", 1e-9)
         solver_opts.setdefault("lin_rel_div", 1e-5)
-        rx, rz = c.eval_rhs(gx, gz, st["rho_n"], st["rho_s"])
+        W = None
+        S = st
+        if self.device:
+            t0 = time.perf_counter()
+            W = self._work()
+            S = {k: DeviceArray(c, np.size(st[k]), host=st[k]) for k in _STATE_IN if k in st}
+            st["upload_ms"] = 1e3 * (time.perf_counter() - t0)
+        rx, rz = c.eval_rhs(gx, gz, S["rho_n"], S["rho_s"], *((W["rx"], W["rz"]) if W else ()))
         abs_mom, rel_mom, abs_div, rel_div = tol
         hist, stats, res0 = [], [], None
         nm = c.eqn_count()[1]
         is_newton = bool(newton and not picard2newton)
         st["newton_steps"] = []
         st["diverged_solves"] = []
+        st["iter_ms"] = []
         for nit in range(nit_max + 1):
+            t_it = time.perf_counter()
             if newton and nit > 0 and not is_newton:
                 rr = hist[-1] / res0
                 if (rr < picard2newton_tol).any() or nit >= max_its_pic:
                     is_newton = True
-            sr, cen, ver = self.update_nonlinearity(st)
-            D11, D33 = 2.0 * cen["eta"], ver["eta"]
-            d = c.stokes_diag(D11, D11, D33) if diag_scaling else None
-            _, norms = c.stokes_residual(cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"],
-                                         sr["div"], rx, rz, d=d)
+            sr, cen, ver = self.update_nonlinearity(S)
+            if W:
+                # the line-search trials overwrite the rheology outputs: keep
+                # this evaluation's viscosities
+                D11, D33 = W["D11"], W["D33"].copy_from(ver["eta"])
+                W["eta_n"].copy_from(cen["eta"])
+                c.scale_into(cen["eta"], 2.0, D11)
+            else:
+                D11, D33 = 2.0 * cen["eta"], ver["eta"]
+            d = c.stokes_diag(D11, D11, D33, *((W["d"],) if W else ())) if diag_scaling else None
+            _, norms = self._residual(cen, ver, sr, rx, rz, d=d)
             norms = np.asarray(norms, dtype=float)
             if res0 is None:
                 res0 = norms.copy()
@@ -159,18 +240,20 @@ class PicardLoop:
             r = norms / res0
             if ((norms[0] < abs_mom or r[0] < rel_mom) and (norms[1] < abs_mom or r[1] < rel_mom)
                     and (norms[2] < abs_div or r[2] < rel_div)):
+                st["iter_ms"].append(1e3 * (time.perf_counter() - t_it))
                 break
-            F, _ = c.stokes_residual(cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"],
-                                     sr["div"], rx, rz)
+            F, _ = self._residual(cen, ver, sr, rx, rz)
+            Fm, Fc = (F.view(0, nm), F.view(nm, len(F) - nm)) if W else (F[:nm], F[nm:])
+            sol = W["sol"] if W else None
             if is_newton:
                 cin, vin = self.last_inputs
-                Dn = c.tangent_grid(self.phases, self.prm, 1, **cin)
-                Ds = c.tangent_grid(self.phases, self.prm, 0, **vin)
-                dx, o = c.newton_solver(Dn, Ds, D11, D33, F[:nm], F[nm:], accept_diverged=True,
+                Dn = c.tangent_grid(self.phases, self.prm, 1, D=W["Dn"] if W else None, **cin)
+                Ds = c.tangent_grid(self.phases, self.prm, 0, D=W["Ds"] if W else None, **vin)
+                dx, o = c.newton_solver(Dn, Ds, D11, D33, Fm, Fc, accept_diverged=True, sol=sol,
                                         **solver_opts)
                 st["newton_steps"].append(nit)
             else:
-                dx, o = c.killer_solver(D11, D11, D33, F[:nm], F[nm:], accept_diverged=True,
+                dx, o = c.killer_solver(D11, D11, D33, Fm, Fc, accept_diverged=True, sol=sol,
                                         **solver_opts)
             if o.status == MDG_ERR_DIVERGED:
                 # A missed momentum tolerance: KillerSolver carries on with the
@@ -184,11 +267,18 @@ class PicardLoop:
             stats.append(o)
             alpha, stagnated = -1.0, False
             if line_search or is_newton:
-                alpha, stagnated = self.line_search(st, dx, d, rx, rz, norms, newton=is_newton)
+                alpha, stagnated = self.line_search(S, dx, d, rx, rz, norms, newton=is_newton)
                 st.setdefault("alphas", []).append(alpha)
-            scatter(*self.eqn, dx, st["u"], st["v"], st["p"], alpha)
-            c.apply_bc(st["u"], st["v"])
+            self._step(S, dx, alpha)
+            st["iter_ms"].append(1e3 * (time.perf_counter() - t_it))
             if stagnated:
                 break
-        st["eta_n"], st["eta_s"] = cen["eta"], ver["eta"]
+        if W:
+            for k in ("u", "v", "p"):
+                st[k][:] = S[k].numpy()
+            st["eta_n"], st["eta_s"] = W["eta_n"].numpy(), W["D33"].numpy()
+            for a in S.values():
+                a.free()
+        else:
+            st["eta_n"], st["eta_s"] = cen["eta"], ver["eta"]
         return hist, len(stats), stats

@@ -12,8 +12,9 @@ from mdgpu import setups as S
 from mdgpu.nonlinear import PicardLoop
 
 
-def build(O, n, C):
-    """Returns (ctx, loop, st, info) for an n x n box with cohesion C."""
+def build(O, n, C, device=False):
+    """Returns (ctx, loop, st, info) for an n x n box with cohesion C; device
+    selects the loop that keeps its fields in device memory."""
     from oracle.driver import initial_fields
     box = (0.0, 1.0, -1.0, 0.0)
     g = O.Grid(n + 1, n + 1, *box)
@@ -40,6 +41,6 @@ def build(O, n, C):
     prm = mdgpu.RheoParams(average=0, step=1, elastic=1, dt=1.0, R=1.0, zeroC=0.0, min_eta=1e-4,
                            max_eta=1e4, eta_tol=1e-13, Tiimin=1e-5, scale_E=1.0, scale_eta=1.0)
     eu, ev, ep = g.numbering()[:3]
-    loop = PicardLoop(ctx, eu, ev, ep, [mdgpu.PhaseProp(**d) for d in S.vep_phases(C=C)], prm)
+    loop = PicardLoop(ctx, eu, ev, ep, [mdgpu.PhaseProp(**d) for d in S.vep_phases(C=C)], prm, device=device)
     info = dict(markers=x.size, weak=int(ph.sum()), pref_m=pref_m, nc=nc)
     return ctx, loop, st, info

@@ -79,6 +79,53 @@ def test_picard_loop_matches_oracle(oracle_mod, diag_scaling, line_search, C):
     assert rel(st_gpu["eta_n"], st["eta_n"]) < 1e-8
 
 
+@pytest.mark.parametrize("newton", [False, True])
+def test_device_resident_loop_matches_host_loop(oracle_mod, newton):
+    """The loop with every field in device memory (one upload per run, the
+    defect step and line-search trials by mdg_extract_solution) against the
+    host-staged loop, on the 31^2 visco-elasto-plastic box with the line
+    search (Picard) and on the power-law ShearTemplate case with Newton steps
+    from the first iteration: the same iterations, step lengths and solver
+    iteration counts, the residual history and the fields to 1e-10 (the
+    device update fuses u + alpha dx into one rounding where numpy rounds
+    twice)."""
+    if newton:
+        runs = []
+        for device in (False, True):
+            st, loop = _shear_template(oracle_mod, False)
+            loop.device = device
+            out = loop.run(st, 10, (1e-9, 1e-9, 1e-9, 1e-9), diag_scaling=0, newton=True, picard2newton=False,
+                           rel_tol=1e-11, max_its=400)
+            runs.append((st, out))
+    else:
+        n = 31
+        g, st0 = _problem(oracle_mod, n, seed=4)
+        phs = S.vep_phases(C=2.0)
+        common = dict(step=1, elastic=1, dt=1.0, R=1.0, zeroC=0.0, min_eta=1e-4, max_eta=1e4,
+                      eta_tol=1e-13, Tiimin=1e-5, scale_E=1.0, scale_eta=1.0)
+        ctx = mdgpu.Context(n + 1, n + 1, 0.0, 1.0, -1.0, 0.0)
+        ctx.set_bc(g.BCu_t, g.BCu_v, g.BCv_t, g.BCv_v, g.BCp_t, g.BCg_t)
+        eu, ev, ep = g.numbering()[:3]
+        runs = []
+        for device in (False, True):
+            st = {k: v.copy() for k, v in st0.items()}
+            loop = PicardLoop(ctx, eu, ev, ep, [mdgpu.PhaseProp(**d) for d in phs],
+                              mdgpu.RheoParams(average=0, **common), device=device)
+            out = loop.run(st, 10, TOL, diag_scaling=1, gz=-1.0, line_search=True, rel_tol=1e-11, max_its=400)
+            runs.append((st, out))
+    (sh, (hh, nh, oh)), (sd, (hd, nd, od)) = runs
+    print("host ms per iteration", [round(t, 1) for t in sh["iter_ms"]], "device", [round(t, 1) for t in sd["iter_ms"]],
+          "upload %.1f ms" % sd["upload_ms"], flush=True)
+    assert nd == nh and len(hd) == len(hh)
+    assert sd.get("alphas") == sh.get("alphas") and sd["newton_steps"] == sh["newton_steps"]
+    assert [o.its for o in od] == [o.its for o in oh]
+    np.testing.assert_allclose(np.array(hd), np.array(hh), rtol=1e-10, atol=1e-15)
+    for k in ("u", "v", "p", "eta_n", "eta_s"):
+        assert np.linalg.norm(sd[k] - sh[k]) <= 1e-10 * np.linalg.norm(sh[k]), k
+    if newton:
+        assert sd["newton_steps"]
+
+
 def _shear_template(O, linear):
     """The ShearTemplate cases of the reference's NewtonIterationConvergence
     test (TESTS/NewtonIterationConvergence.cpp:97-142, parameters
@@ -158,24 +205,30 @@ def test_shear_template_newton_iteration_counts(oracle_mod, linear):
 
 
 def test_unconverged_newton_solves_carry_on(oracle_mod):
-    """Newton on the heavily yielding shear-band box (127^2, C = 1): the
-    linear solve on the plastic Jacobian misses its tolerance. Like the
-    reference's KillerSolver after max_its (Solvers.c:1298-1334), the loop
-    warns, records the iteration and carries on with the last iterate, the
-    line search weighing the step; the Picard solves before the switch all
-    converge."""
-    import warnings
+    """Newton on the heavily yielding shear-band box (127^2, C = 1) with
+    KillerSolver's own iteration (method 1: GCR on the Jacobian preconditioned
+    by the Picard operator, as Solvers.c:1176-1193,1298 precondition it with
+    the Picard factor).  The Picard solves before the switch converge in a few
+    Powell-Hestenes iterations; on the plastic Jacobian GCR stops at its
+    iteration caps far from the momentum tolerance, which KillerSolver accepts
+    (only the continuity bound aborts, Solvers.c:1333-1336), and the loop
+    carries on with the steps the line search weighs.  The Powell-Hestenes
+    cap is the reference's default (max_its_PH = 10, InputOutput.c:1184):
+    with 3 the continuity misses 10 lin_abs_div on these solves, which the
+    reference would abort on as the GPU solver does."""
     from shearbox import build
     ctx, loop, st, info = build(oracle_mod, 127, C=1.0)
-    with warnings.catch_warnings(record=True) as w:
-        warnings.simplefilter("always")
-        hist, nsolve, stats = loop.run(st, 8, (1e-12, 1e-8, 1e-12, 1e-8), diag_scaling=1,
-                                       line_search=True, newton=True, rel_tol=1e-8, max_its=100)
+    hist, nsolve, stats = loop.run(st, 7, (1e-12, 1e-8, 1e-12, 1e-8), diag_scaling=1, line_search=True,
+                                   newton=True, method=1, max_its_PH=10, max_its_KSP=20)
     h = np.array(hist)
-    print("newton steps", st["newton_steps"], "unconverged", st["diverged_solves"],
-          "alphas", st["alphas"], "history", (h / h[0])[:, :3].tolist(), flush=True)
-    assert st["newton_steps"] and set(st["diverged_solves"]) <= set(st["newton_steps"])
-    assert len([x for x in w if "stopped at relative residual" in str(x.message)]) == len(st["diverged_solves"])
-    for nit, o in zip(range(nsolve), stats):
-        assert (o.status == mdgpu.MDG_ERR_DIVERGED) == (nit in st["diverged_solves"])
+    print("newton steps", st["newton_steps"], "alphas", st["alphas"], "history", (h / h[0])[:, :3].tolist(),
+          "rel", [o.rel_res for o in stats], flush=True)
+    assert st["newton_steps"]
+    for nit, o in enumerate(stats):
+        assert o.status == 0
+        if nit in st["newton_steps"]:
+            assert o.rel_res > 1e-3                      # the plastic Jacobian is not solved
+        else:
+            assert o.rel_res < 1e-8
+        assert abs(o.max_div) <= 10 * 1e-9 or o.max_div / o.max_div0 <= 1e-5
     assert np.isfinite(h).all() and h[-1, 0] < h[0, 0]

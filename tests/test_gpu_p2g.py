@@ -127,16 +127,45 @@ def test_device_resident_sorted_fields(oracle_mod):
     assert np.array_equal(out.numpy(), out2.numpy())
 
 
+@pytest.mark.parametrize("stencil", [1, 9])
+def test_p2mastah_on_ties(oracle_mod, stencil):
+    """Markers exactly on cell edges, node midpoints and the box walls, where
+    a marker's nearest node can differ from the one its sort key implies
+    (the node set then takes the path that recomputes every marker's node):
+    P2Mastah matches the oracle on every node set with both stencils, per
+    marker and through a phase table."""
+    box = (0.0, 8.0, 0.0, 6.0)
+    g = oracle_mod.Grid(9, 7, *box)
+    xs = np.repeat(np.arange(0, 17) * 0.5, 13)
+    zs = np.tile(np.arange(0, 13) * 0.5, 17)
+    ph = (np.arange(xs.size) % 2).astype(np.int32)
+    f = 1.5 + np.sin(xs) * np.cos(zs)
+    ctx = mdgpu.Context(9, 7, *box)
+    ctx.markers_set(xs, zs, ph)
+    for centroid in (1, 0, -1, -2):
+        for avg in (0, 2):
+            ref = oracle_mod.p2mastah(g, xs, zs, ph, f, flag=1, avg=avg, centroid=centroid, stencil=stencil)
+            got = ctx.p2mastah(f, flag=1, avg=avg, centroid=centroid, interp_stencil=stencil)
+            np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0, err_msg="centroid %d" % centroid)
+        table = np.array([2.0, 5.0])
+        ref = oracle_mod.p2mastah(g, xs, zs, ph, table, flag=0, avg=1, centroid=centroid, stencil=stencil)
+        got = ctx.p2mastah(table, flag=0, avg=1, centroid=centroid, interp_stencil=stencil)
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0, err_msg="table, centroid %d" % centroid)
+
+
 @pytest.mark.parametrize("centroid", [1, 0, -1, -2])
-def test_cached_weights_follow_new_markers(oracle_mod, centroid):
-    """The per-marker weights cached by the first 1-cell P2G call of a node
-    set are recomputed after mdg_markers_set: P2G on a second marker set
-    (moved markers, same context) matches the oracle on that set, and a
-    repeated call on it is bit-identical."""
+@pytest.mark.parametrize("stencil", [1, 9])
+def test_cached_weights_follow_new_markers(oracle_mod, centroid, stencil):
+    """The per-marker state cached by the first P2G call of a node set (the
+    1-cell weights and whether every marker sits in the home its sort key
+    names, which the 9-cell stencil's fast path relies on) is recomputed
+    after mdg_markers_set: P2G on a second marker set (moved markers, same
+    context) matches the oracle on that set, and a repeated call on it is
+    bit-identical."""
     g, ctx, x, z, phase = _case(oracle_mod)
     f = np.cos(3 * x) * np.sin(2 * z) + 2.0
     nn = g.node_count(centroid)
-    ctx.p2mastah(f, flag=1, avg=0, centroid=centroid, interp_stencil=1)
+    ctx.p2mastah(f, flag=1, avg=0, centroid=centroid, interp_stencil=stencil)
     rng = np.random.default_rng(3)
     x2 = x + 0.3 * g.dx * rng.standard_normal(x.size)
     z2 = z + 0.3 * g.dz * rng.standard_normal(z.size)
@@ -144,9 +173,9 @@ def test_cached_weights_follow_new_markers(oracle_mod, centroid):
     f2 = np.sin(5 * x2) + 3.0
     for avg in (0, 2):
         ref = oracle_mod.p2mastah(g, x2, z2, phase, f2, flag=1, avg=avg, centroid=centroid,
-                                  stencil=1)
-        got = ctx.p2mastah(f2, flag=1, avg=avg, centroid=centroid, interp_stencil=1)
-        again = ctx.p2mastah(f2, flag=1, avg=avg, centroid=centroid, interp_stencil=1)
+                                  stencil=stencil)
+        got = ctx.p2mastah(f2, flag=1, avg=avg, centroid=centroid, interp_stencil=stencil)
+        again = ctx.p2mastah(f2, flag=1, avg=avg, centroid=centroid, interp_stencil=stencil)
         assert got.shape == (nn,)
         np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0)
         assert np.array_equal(got, again)

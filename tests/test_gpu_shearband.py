@@ -17,8 +17,10 @@ a geometric mean of the marker values (inside their range, log-mean near
 0); every linear solve reaches its tolerance; the continuity residual ends
 at round-off; the momentum residuals fall by more than 10x; plastic strain
 rates appear on a localised part of the box (more than the seeds, well
-under half of it). The same loop is checked against the oracle at 31^2 in
-test_gpu_nonlinear.py.
+under half of it). The loop runs with its fields in device memory (uploaded
+once); the wall time of each iteration is printed. The same loop is checked
+against the oracle at 31^2 in test_gpu_nonlinear.py, and the device-resident
+loop against the host-staged one there too.
 """
 import time
 
@@ -32,7 +34,7 @@ pytestmark = pytest.mark.gpu
 
 def test_shear_band_box_1023(oracle_mod):
     t0 = time.time()
-    ctx, loop, st, info = build(oracle_mod, 1023, C=1.5)
+    ctx, loop, st, info = build(oracle_mod, 1023, C=1.5, device=True)
     nc, pref_m = info["nc"], info["pref_m"]
     print("markers", info["markers"], "in weak seeds", info["weak"], "set-up %.1fs" % (time.time() - t0),
           flush=True)
@@ -47,7 +49,10 @@ def test_shear_band_box_1023(oracle_mod):
     wall = time.time() - t0
     h = np.array(hist)
     rel = h / h[0]
-    print("linear solves", nsolve, "in %.1fs" % wall, "step lengths", st["alphas"], flush=True)
+    print("linear solves", nsolve, "in %.1fs" % wall, "(upload %.0f ms)" % st["upload_ms"], "step lengths",
+          st["alphas"], flush=True)
+    print("ms per iteration", [round(t) for t in st["iter_ms"]], "solve ms", [round(o.time_ms) for o in stats],
+          flush=True)
     print("outer its", [o.its for o in stats], "inner its", [o.inner_its for o in stats], flush=True)
     print("relative residual history (x, z, continuity)\n", rel[:, :3], flush=True)
     assert nsolve == len(stats) >= 10
@@ -55,6 +60,7 @@ def test_shear_band_box_1023(oracle_mod):
     assert rel[-1, 0] < 0.1 and rel[-1, 1] < 0.1
     assert rel[-1, 2] < 1e-8
     assert st["diverged_solves"] == []
+    loop.device = False     # one host-staged evaluation at the final state
     _, cen, ver = loop.update_nonlinearity(st)
     npl = int((cen["eII_pl"] > 0).sum())
     print("centres yielding", npl, "of", nc, flush=True)
