@@ -25,10 +25,14 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "mdoodz7.0_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the fine-level smoother step on 2047^2 cells,
-# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a copy kernel of
-# known size): profiles/hbm_traffic_p2g_smoother_2047.txt
-SMOOTHER_TRAFFIC_2047 = 336.2e6
+# The solver configuration measured: the vertex-patch Chebyshev smoother fused
+# into one kernel per step (smoother 3), penalty 100 and degree 12 on every
+# level -- the fastest of the configurations swept on this workload
+SOLVER = dict(smoother=3, penalty=100.0, cheb_degree=12, cheb_degree_coarse=12)
+# HBM bytes per launch of the fine-level smoother step (k_pcheb, FP64) on
+# 2047^2 cells, rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
+# copy kernel of known size): profiles/hbm_traffic_pcheb_2047.txt
+SMOOTHER_TRAFFIC_2047 = 343.9e6
 METRIC = "Stokes solve time @1e-8 resid, 2047², plain vs learned PC; P2G Mmarkers/s"
 
 
@@ -73,6 +77,9 @@ def main():
                     help="profiling mode: only the marker kernels on the --cells workload's markers "
                          "(RK4 advection, centre and vertex interpolation, pressure and stress "
                          "updates), REPS times each, arrays resident in HBM")
+    ap.add_argument("--pcnet", type=int, default=0, metavar="REPS",
+                    help="profiling mode: only the learned network's forward pass and gradient "
+                         "(BF16 MFMA convolutions) on a 1024 x 1024 image, REPS times each")
     ap.add_argument("--trace-mixed", action="store_true",
                     help="with --trace-solve: the FP32 V-cycle (MDG_PREC_MIXED) instead of FP64")
     ap.add_argument("--p2g-cases", type=int, default=0, metavar="REPS",
@@ -96,6 +103,9 @@ def main():
         return
     if a.markers:
         markers_only(a, a.markers)
+        return
+    if a.pcnet:
+        pcnet_only(a.pcnet)
         return
     t0 = time.time()
     wl = W.ellipse_workload(a.cells, a.nper, a.seed, n_ell=a.ellipses)
@@ -138,7 +148,7 @@ def main():
         ctx.eval_rhs(0.0, -9.81, rn, rs, rgx, rgz)
         ctx.stokes_residual(zc, zc, zv, zc, zc, rgx, rgz, F=F)
         ctx.scale_into(en, 2.0, D11)
-        o = ctx.killer_solver_dev(D11, D11, es, F, nm, sol)
+        o = ctx.killer_solver_dev(D11, D11, es, F, nm, sol, **SOLVER)
         stats.update(p2g_ms=[k1, k2], its=o.its, inner=o.inner_its, solve_ms=o.time_ms,
                      rel=o.rel_res, levels=o.levels)
 
@@ -174,8 +184,9 @@ def main():
         step()
         log("warm-up step: solve %.1f s" % (stats["solve_ms"] / 1e3))
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for i in range(a.steps):
         step()
+        log("timed step %d of %d: solve %.1f s" % (i + 1, a.steps, stats["solve_ms"] / 1e3))
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / a.steps
     log("timed steps: %.1f ms per step" % ms_step)
@@ -183,7 +194,7 @@ def main():
     # more, launched kernel by kernel with HIP events around every fine-level
     # smoother step (graphs = 2); algorithmic bytes per launch from the library
     sol_t = ctx.device(neq, host=np.zeros(neq))
-    o_t = ctx.killer_solver_dev(D11, D11, es, F, nm, sol_t, graphs=2)
+    o_t = ctx.killer_solver_dev(D11, D11, es, F, nm, sol_t, graphs=2, **SOLVER)
     k_ms_b2b, k_bytes = ctx.time_smoother(50)
     k_ms = o_t.smoother_ms
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
@@ -209,12 +220,15 @@ def main():
         "config": {"workload": "Stokes solve to 1e-8 relative residual, %d^2 cells, %d seeded "
                                "ellipses (1e6 viscosity contrast), free slip, with P2G of "
                                "viscosity and density from %d markers" % (a.cells, a.ellipses, n),
-                   "solver": "FGCR(100) + viscosity-weighted augmented Lagrangian (gamma 10) "
-                             "+ multigrid-preconditioned CG, FP64, plain (no learned PC)"},
+                   "solver": "FGCR(100) + viscosity-weighted augmented Lagrangian (gamma 100) "
+                             "+ CG preconditioned by a multigrid V-cycle with vertex-patch "
+                             "Chebyshev smoothing (degree 12 on every level), FP64, plain "
+                             "(no learned PC)", "solver_opts": SOLVER},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": SMOOTHER_TRAFFIC_2047 if a.cells == 2047 else None,
-                     "kernel": "k_cheb3<double, false> (fine-level fused Chebyshev smoother step)",
+                     "kernel": "k_pcheb<double, false, 64, 8, 256> (fine-level fused vertex-patch "
+                               "Chebyshev smoother step)",
                      "kernel_ms": k_ms, "alg_bytes": k_bytes,
                      "kernel_ms_source": "mean of the %d fine-level smoother launches of one FP64 "
                                          "solve (HIP events on the solver stream)" % o_t.smoother_launches,
@@ -231,7 +245,7 @@ def main():
         # around it), after the timed steps; its error against the FP64 solve
         x64 = sol.numpy().copy()
         sol2 = ctx.device(neq, host=np.zeros(neq))
-        o2 = ctx.killer_solver_dev(D11, D11, es, F, nm, sol2, precision=mdgpu.MDG_PREC_MIXED)
+        o2 = ctx.killer_solver_dev(D11, D11, es, F, nm, sol2, precision=mdgpu.MDG_PREC_MIXED, **SOLVER)
         x32 = sol2.numpy()
         log("mixed-precision solve: %.1f s" % (o2.time_ms / 1e3))
         res["mixed"] = {"precision": "FP32 multigrid V-cycle inside FP64 CG and FP64 FGCR",
@@ -274,7 +288,7 @@ def learned_variant(a, ctx, D11, es, F, nm, neq, x64, fp64_ms):
                         "loss_last": float(losses[-50:].mean()), "seconds": t_train}}
     for name, prec in (("fp64", mdgpu.MDG_PREC_FP64), ("mixed", mdgpu.MDG_PREC_MIXED)):
         sol3 = ctx.device(neq, host=np.zeros(neq))
-        o3 = ctx.killer_solver_dev(D11, D11, es, F, nm, sol3, precond=3, precision=prec)
+        o3 = ctx.killer_solver_dev(D11, D11, es, F, nm, sol3, precond=3, precision=prec, **SOLVER)
         x3 = sol3.numpy()
         log("learned %s solve: %.1f s" % (name, o3.time_ms / 1e3))
         out[name] = {"solve_ms": o3.time_ms, "outer_its": o3.its, "inner_its": o3.inner_its,
@@ -373,6 +387,32 @@ def p2g_cases(ctx, reps, wl, dx_, dz_, dph, eta_s, rho_s, en, rn, es, rs, n):
     for k in list(out):
         if k.endswith("_ms") and not k.startswith("moved_resort"):
             out[k.replace("_ms", "_gmarkers_per_s")] = n / (out[k] * 1e-3) / 1e9
+    print(json.dumps(out), flush=True)
+
+
+def pcnet_only(reps, H=1024, W=1024):
+    """The learned network alone: forward pass and loss gradient on an H x W
+    image (random parameters and inputs), REPS times each.  Prints the wall
+    time per call and the algorithmic convolution FLOPs (2 x pixels x 9 x
+    Cin x Cout per layer) of one forward pass; the kernel times come from a
+    kernel trace of the same command."""
+    import numpy as np
+    import mdgpu
+    ctx = mdgpu.Context(H + 1, W + 1, 0.0, 1.0, 0.0, 1.0)
+    rng = np.random.default_rng(5)
+    ctx.pcnet_set_params(0.1 * rng.standard_normal(ctx.pcnet_param_count()).astype(np.float32))
+    x = rng.standard_normal((H, W, 4)).astype(np.float32)
+    t = rng.standard_normal((H, W, 4)).astype(np.float32)
+    mask = np.ones((H, W, 4), np.float32)
+    layers = [(4, 32, 0), (32, 32, 1), (32, 64, 2), (64, 64, 3), (64, 32, 2), (32, 32, 1), (32, 3, 0)]
+    flops = sum(2.0 * (H >> l) * (W >> l) * 9 * ci * co for ci, co, l in layers)
+    out = {"H": H, "W": W, "conv_gflop_forward": flops / 1e9}
+    for name, f in (("forward", lambda: ctx.pcnet_forward(x)), ("grad", lambda: ctx.pcnet_grad(x, t, mask))):
+        f()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        out[name + "_wall_ms"] = 1e3 * (time.perf_counter() - t0) / reps
     print(json.dumps(out), flush=True)
 
 

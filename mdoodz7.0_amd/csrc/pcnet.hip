@@ -1,5 +1,6 @@
 // Learned preconditioner network: a 4-level U-Net of 3x3 convolutions run
-// and trained with BF16 MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulate).
+// and trained with BF16 MFMA (gfx950's v_mfma_f32_16x16x32_bf16, FP32
+// accumulate).
 // It has no counterpart in the reference.
 //
 // Network (level 0 = full image, H and W multiples of 8, NHWC activations):
@@ -14,16 +15,19 @@
 // convolution with flipped taps), both as [N rows][K] so one MFMA B fragment
 // is one 8-byte load.
 //
-// Convolution as implicit GEMM: a wavefront owns 16 consecutive pixels (M)
-// and all output channels (N = 16*NT); K = 9*Cin runs in chunks of 16, each
-// lane loading 4 consecutive channels of one tap (8 bytes, NHWC).  Lane l
-// holds A[l%16][4(l/16)..+3], B[4(l/16)..+3][l%16] and receives
+// Convolution as implicit GEMM: a wavefront owns 64 consecutive pixels of a
+// row (M, four 16-pixel tiles) and all output channels (N = 16*NT); K =
+// 9*Cin runs in chunks of 32, each lane reading two groups of 4 consecutive
+// channels (8 bytes each, NHWC, from the workgroup's LDS tile; each group of
+// one tap, as Cin is a multiple of 4).  Lane l holds
+// A[l%16][8(l/16)..+7], B[8(l/16)..+7][l%16] and receives
 // D[4(l/16)+i][l%16].  The weight gradient is the GEMM over pixels
 // dW[k][co] = sum_p X[p + off(k)][ci(k)] dZ[p][co], split over pixel chunks
 // whose partial sums are added in a fixed order (no atomics).
 #include "mdg_internal.h"
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16_t;
 
@@ -43,61 +47,100 @@ __device__ __forceinline__ bf16_t f2bf(float f) {
     return (bf16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
+// two groups of 4 bf16 bit patterns as one 8-element MFMA operand
+__device__ __forceinline__ bf16x8 pack8(bf16x4 lo, bf16x4 hi) {
+    typedef short s16x8 __attribute__((ext_vector_type(8)));
+    const s16x8 v = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    return __builtin_bit_cast(bf16x8, v);
+}
 
 // y = conv3x3(x) + bias (optional ReLU).  x: H*W*Cs_in (bf16 or fp32),
 // Wt: (16*NT) x Kp bf16, out: H*W*Cs_out (bf16 or fp32), channels < Cout.
+// A workgroup owns CT_H rows of CT_W pixels: it stages the rows and columns
+// the 3 x 3 taps reach (one pixel of halo, zero outside the image), all
+// input channels, once into LDS as bf16 (pixel stride Cs_in + 4 elements,
+// so the 16 pixels of an MFMA fragment fall on distinct banks); each wave
+// then computes one row as 4 M tiles of 16 pixels, every B fragment (one
+// 16-byte load per lane) feeding 4 MFMAs.
+#define CT_H 4
+#define CT_W 64
 template <int NT, bool IN_F32, bool RELU, bool OUT_F32>
 __global__ void __launch_bounds__(256)
 k_conv3x3(int H, int W, int Cs_in, int Kp, int Cout, int Cs_out, const void *__restrict__ in,
           const bf16_t *__restrict__ Wt, const float *__restrict__ bias, void *__restrict__ out) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long npix = (long)H * W;
-    const long p0 = ((long)blockIdx.x * 4 + wave) * 16;
-    if (p0 >= npix) return;
-    const int m = lane & 15, g = lane >> 4;
-    const long p = p0 + m;
-    const bool pv = p < npix;
-    const int py = pv ? (int)(p / W) : 0, px = pv ? (int)(p - (long)py * W) : 0;
-    const int K = 9 * Cs_in;
-    f32x4 acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int kc = 0; kc < Kp; kc += 16) {
-        const int k = kc + 4 * g;
-        bf16x4 a = {0, 0, 0, 0};
-        if (pv && k < K) {
-            const int tap = k / Cs_in, ci = k - tap * Cs_in;
-            const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                const long off = ((long)yy * W + xx) * Cs_in + ci;
-                if (IN_F32) {
-                    const float4 v = *(const float4 *)((const float *)in + off);
-                    a = bf16x4{(short)f2bf(v.x), (short)f2bf(v.y), (short)f2bf(v.z), (short)f2bf(v.w)};
-                } else {
-                    a = *(const bf16x4 *)((const bf16_t *)in + off);
-                }
+    extern __shared__ bf16_t tile[];
+    const int CS = Cs_in + 4, TW = CT_W + 2;
+    const int x0 = blockIdx.x * CT_W, y0 = blockIdx.y * CT_H;
+    const int t = threadIdx.x;
+    const int cq = Cs_in >> 2;                          // 4-channel groups per pixel
+    const int nel = (CT_H + 2) * TW * cq;
+    for (int e = t; e < nel; e += 256) {
+        const int q = e % cq, pix = e / cq, lx = pix % TW, ly = pix / TW;
+        const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+        bf16x4 v = {0, 0, 0, 0};
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const long off = ((long)gy * W + gx) * Cs_in + 4 * q;
+            if (IN_F32) {
+                const float4 f = *(const float4 *)((const float *)in + off);
+                v = bf16x4{(short)f2bf(f.x), (short)f2bf(f.y), (short)f2bf(f.z), (short)f2bf(f.w)};
+            } else {
+                v = *(const bf16x4 *)((const bf16_t *)in + off);
             }
         }
+        *(bf16x4 *)(tile + (ly * TW + lx) * CS + 4 * q) = v;
+    }
+    __syncthreads();
+    const int lane = t & 63, wave = t >> 6, m = lane & 15, g = lane >> 4;
+    const int K = 9 * Cs_in;
+    f32x4 acc[4][NT];
 #pragma unroll
-        for (int t = 0; t < NT; t++) {
-            const bf16x4 b = *(const bf16x4 *)(Wt + (long)(t * 16 + m) * Kp + kc + 4 * g);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) acc[mt][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < Kp; kc += 32) {
+        bf16x8 bw[NT];
+#pragma unroll
+        for (int n = 0; n < NT; n++) bw[n] = *(const bf16x8 *)(Wt + (long)(n * 16 + m) * Kp + kc + 8 * g);
+        int base[2];
+        bool kv[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int k = kc + 8 * g + 4 * h;
+            kv[h] = k < K;
+            const int tap = kv[h] ? k / Cs_in : 0, ci = kv[h] ? k - tap * Cs_in : 0;
+            // tap (dy, dx) of the wave's row and of fragment pixel m, tile coordinates
+            base[h] = ((wave + tap / 3) * TW + m + tap % 3) * CS + ci;
+        }
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            bf16x4 ah[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+                ah[h] = kv[h] ? *(const bf16x4 *)(tile + base[h] + 16 * mt * CS) : bf16x4{0, 0, 0, 0};
+            const bf16x8 a = pack8(ah[0], ah[1]);
+#pragma unroll
+            for (int n = 0; n < NT; n++) acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[n], acc[mt][n], 0, 0, 0);
         }
     }
+    const int y = y0 + wave;
+    if (y >= H) return;
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-        const int n = t * 16 + m;
-        if (n >= Cout) continue;
-        const float bn = bias ? bias[n] : 0.f;
+    for (int n = 0; n < NT; n++) {
+        const int co = n * 16 + m;
+        if (co >= Cout) continue;
+        const float bn = bias ? bias[co] : 0.f;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const long q = p0 + 4 * g + i;
-            if (q >= npix) continue;
-            float v = acc[t][i] + bn;
-            if (RELU) v = fmaxf(v, 0.f);
-            if (OUT_F32) ((float *)out)[q * Cs_out + n] = v;
-            else ((bf16_t *)out)[q * Cs_out + n] = f2bf(v);
-        }
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int x = x0 + 16 * mt + 4 * g + i;
+                if (x >= W) continue;
+                const long q = (long)y * W + x;
+                float v = acc[mt][n][i] + bn;
+                if (RELU) v = fmaxf(v, 0.f);
+                if (OUT_F32) ((float *)out)[q * Cs_out + co] = v;
+                else ((bf16_t *)out)[q * Cs_out + co] = f2bf(v);
+            }
     }
 }
 
@@ -118,11 +161,11 @@ k_conv_wgrad(int H, int W, int Cs_in, int Kp, int Cout, int Cs_g, int NTn,
     const int n = nt * 16 + m;
     const bool nv = n < Cout;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (long qb = q0; qb < q1; qb += 16) {
-        bf16x4 a, b;
+    for (long qb = q0; qb < q1; qb += 32) {
+        bf16x4 a4[2], b4[2];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const long q = qb + 4 * g + j;
+        for (int j = 0; j < 8; j++) {
+            const long q = qb + 8 * g + j;
             bf16_t av = 0, bv = 0;
             if (q < q1) {
                 if (kv) {
@@ -132,10 +175,10 @@ k_conv_wgrad(int H, int W, int Cs_in, int Kp, int Cout, int Cs_g, int NTn,
                 }
                 if (nv) bv = f2bf(G[q * Cs_g + n]);
             }
-            a[j] = (short)av;
-            b[j] = (short)bv;
+            a4[j >> 2][j & 3] = (short)av;
+            b4[j >> 2][j & 3] = (short)bv;
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pack8(a4[0], a4[1]), pack8(b4[0], b4[1]), acc, 0, 0, 0);
     }
     const int ld = 16 * NTn;
 #pragma unroll
@@ -156,22 +199,40 @@ __global__ void k_wgrad_reduce(int nchunk, int Kp, int ld, int Cin, int Cs_in, i
     for (int c = 0; c < nchunk; c++) s += part[((long)c * Kp + k) * ld + co];
     dW[i] = accumulate ? dW[i] + s : s;
 }
-__global__ void k_bias_part(long npix, int Cout, int Cs_g, const float *__restrict__ G, float *__restrict__ part) {
-    const int co = threadIdx.x;
-    if (co >= Cout) return;
+// Bias gradient: per-chunk channel sums, then the sum over chunks.  A
+// 256-thread block splits its pixels (or chunks) over 256 / Cout groups of
+// Cout channel threads and combines the groups in LDS in a fixed order.
+__global__ void __launch_bounds__(256) k_bias_part(long npix, int Cout, int Cs_g, const float *__restrict__ G,
+                                                   float *__restrict__ part) {
+    __shared__ float sh[256];
+    const int t = threadIdx.x, ng = 256 / Cout, co = t % Cout, pg = t / Cout;
     const long q0 = (long)blockIdx.x * WG_CHUNK;
     const long q1 = (q0 + WG_CHUNK < npix) ? q0 + WG_CHUNK : npix;
     float s = 0.f;
-    for (long q = q0; q < q1; q++) s += G[q * Cs_g + co];
-    part[(long)blockIdx.x * 64 + co] = s;
+    if (pg < ng)
+        for (long q = q0 + pg; q < q1; q += ng) s += G[q * Cs_g + co];
+    sh[t] = s;
+    __syncthreads();
+    if (t < Cout) {
+        float r = 0.f;
+        for (int k = 0; k < ng; k++) r += sh[k * Cout + t];
+        part[(long)blockIdx.x * 64 + t] = r;
+    }
 }
-__global__ void k_bias_reduce(int nchunk, int Cout, const float *__restrict__ part, float *__restrict__ db,
-                              int accumulate) {
-    const int co = threadIdx.x;
-    if (co >= Cout) return;
+__global__ void __launch_bounds__(256) k_bias_reduce(int nchunk, int Cout, const float *__restrict__ part,
+                                                     float *__restrict__ db, int accumulate) {
+    __shared__ float sh[256];
+    const int t = threadIdx.x, ng = 256 / Cout, co = t % Cout, pg = t / Cout;
     float s = 0.f;
-    for (int c = 0; c < nchunk; c++) s += part[(long)c * 64 + co];
-    db[co] = accumulate ? db[co] + s : s;
+    if (pg < ng)
+        for (int c = pg; c < nchunk; c += ng) s += part[(long)c * 64 + co];
+    sh[t] = s;
+    __syncthreads();
+    if (t < Cout) {
+        float r = 0.f;
+        for (int k = 0; k < ng; k++) r += sh[k * Cout + t];
+        db[t] = accumulate ? db[t] + r : r;
+    }
 }
 
 // 2x2 mean pool (fine H x W -> H/2 x W/2), bf16 in/out, fp32 arithmetic.
@@ -336,8 +397,8 @@ void layout_layers(pcnet_state *s) {
         l.cout = L_COUT[k];
         l.csi = rup(l.cin, 4);
         l.cso = rup(l.cout, 4);
-        l.kpf = rup(9 * l.csi, 16);
-        l.kpb = rup(9 * l.cso, 16);
+        l.kpf = rup(9 * l.csi, 32);
+        l.kpb = rup(9 * l.cso, 32);
         l.ntf = rup(l.cout, 16) / 16;
         l.ntb = rup(l.cin, 16) / 16;
         l.level = L_LEVEL[k];
@@ -419,9 +480,9 @@ float *Gb(pcnet_state *s, int i) { return s->gbuf.p + s->goff[i]; }
 template <int NT, bool IN_F32, bool RELU, bool OUT_F32>
 void conv_launch(mdg_ctx *c, int H, int W, int Cs_in, int Kp, int Cout, int Cs_out, const void *in,
                  const bf16_t *Wt, const float *bias, void *out) {
-    const long npix = (long)H * W;
-    const int blocks = (int)((npix + 63) / 64);
-    hipLaunchKernelGGL((k_conv3x3<NT, IN_F32, RELU, OUT_F32>), dim3(blocks), dim3(256), 0, c->stream, H, W,
+    const dim3 grid((W + CT_W - 1) / CT_W, (H + CT_H - 1) / CT_H);
+    const size_t lds = (size_t)(CT_H + 2) * (CT_W + 2) * (Cs_in + 4) * sizeof(bf16_t);
+    hipLaunchKernelGGL((k_conv3x3<NT, IN_F32, RELU, OUT_F32>), grid, dim3(256), lds, c->stream, H, W,
                        Cs_in, Kp, Cout, Cs_out, in, Wt, bias, out);
 }
 
@@ -460,8 +521,8 @@ void conv_bwd_weights(mdg_ctx *c, pcnet_state *s, int k, const bf16_t *x, const 
     hipLaunchKernelGGL(k_wgrad_reduce, dim3(nblk256(9L * l.cin * l.cout)), dim3(256), 0, c->stream, nch,
                        l.kpf, 16 * l.ntf, l.cin, l.csi, l.cout, (const float *)s->part.p,
                        s->G.p + l.woff, accumulate);
-    hipLaunchKernelGGL(k_bias_part, dim3(nch), dim3(64), 0, c->stream, npix, l.cout, l.cso, dz, s->part.p);
-    hipLaunchKernelGGL(k_bias_reduce, dim3(1), dim3(64), 0, c->stream, nch, l.cout, (const float *)s->part.p,
+    hipLaunchKernelGGL(k_bias_part, dim3(nch), dim3(256), 0, c->stream, npix, l.cout, l.cso, dz, s->part.p);
+    hipLaunchKernelGGL(k_bias_reduce, dim3(1), dim3(256), 0, c->stream, nch, l.cout, (const float *)s->part.p,
                        s->G.p + l.boff, accumulate);
 }
 

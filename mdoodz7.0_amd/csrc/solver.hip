@@ -1678,7 +1678,6 @@ template <typename T>
 static int poisson_solve(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solver_opts *o, int *its) {
     const int n = ncell(H.L[0].g);
     const bool graphs = (o->graphs == 0);
-    H.tm = (o->graphs == 2) ? &s->tm : nullptr;
     if (graphs && !H.pexec) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         poisson_iteration(c, H, o->cheb_degree);
@@ -1915,6 +1914,8 @@ static int inner_pcg(mdg_ctx *c, solver_state *s, mg_hier<T> &H, const mdg_solve
     HIPCHK(c, H.scal.reserve(8));
     if (!H.h_scal) HIPCHK(c, hipHostMalloc((void **)&H.h_scal, 8 * sizeof(double), hipHostMallocDefault));
     const bool graphs = (o->graphs == 0);
+    // graphs = 2: HIP events around every fine-level smoother step of the V-cycle
+    H.tm = (o->graphs == 2) ? &s->tm : nullptr;
     if (graphs && !H.exec) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         pcg_iteration(c, H, o);

@@ -4,7 +4,8 @@ lognormal (sigma = 0.5) per marker and reaches the nodes by P2Mastah's
 geometric average, Maxwell elasticity with preloaded old stresses,
 Drucker-Prager yield (friction 30 degrees) and 64 seeded weak circles of
 radius 2 cells with half the cohesion. Nondimensional: the matrix
-viscosity at unit strain rate is 1, G dt = 1."""
+viscosity at unit strain rate is 1, G dt = 1 by default; PHYSICAL holds
+the scaled values of the physical set-up."""
 import numpy as np
 
 import mdgpu
@@ -12,9 +13,21 @@ from mdgpu import setups as S
 from mdgpu.nonlinear import PicardLoop
 
 
-def build(O, n, C, device=False):
+# The shear-band benchmark's physical parameters, scaled as the reference
+# scales its inputs (ScaleMe, InputOutput.c:1835-1849) with a viscosity
+# scale of 1e22 Pa s and a strain-rate scale of 1e-15 1/s, i.e. a stress
+# scale of 1e7 Pa and a time scale of 1e15 s: shear modulus 1e10 Pa,
+# cohesion 1e7 Pa (seeds 5e6 Pa), friction 30 degrees, dt = 1e11 s,
+# background pure shear 1e-15 1/s, matrix viscosity 1e22 Pa s at that rate.
+PHYSICAL = dict(G=1e10 / 1e7, C=1e7 / 1e7, dt=1e11 / 1e15)
+
+
+def build(O, n, C, device=False, G=1.0, dt=1.0, pre=None):
     """Returns (ctx, loop, st, info) for an n x n box with cohesion C; device
-    selects the loop that keeps its fields in device memory."""
+    selects the loop that keeps its fields in device memory.  G and dt are
+    the (scaled) shear modulus and time step; the old deviatoric stresses
+    are 0.2 with a normal jitter of 0.05, or pre * C with a 5% jitter when
+    pre is given (a box loaded close to yield)."""
     from oracle.driver import initial_fields
     box = (0.0, 1.0, -1.0, 0.0)
     g = O.Grid(n + 1, n + 1, *box)
@@ -33,12 +46,13 @@ def build(O, n, C, device=False):
     pref_s = ctx.p2mastah(pref_m, flag=1, avg=2, centroid=0)
     nc, nv = n * n, (n + 1) ** 2
     u, v, p = initial_fields(g)
-    s0 = 0.2 + 0.05 * np.random.default_rng(3).standard_normal(nc)
-    st = dict(u=u, v=v, p=p, T=np.ones(nc), mu_n=np.ones(nc), mu_s=np.ones(nv), perc_n=perc_n,
+    noise = np.random.default_rng(3).standard_normal(nc)
+    s0 = 0.2 + 0.05 * noise if pre is None else pre * C * (1.0 + 0.05 * noise)
+    st = dict(u=u, v=v, p=p, T=np.ones(nc), mu_n=np.full(nc, G), mu_s=np.full(nv, G), perc_n=perc_n,
               perc_s=perc_s, sxxd0=s0, szzd0=-s0, sxz0_n=np.zeros(nc), sxxd0_s=ctx.c2v(s0),
               szzd0_s=ctx.c2v(-s0), sxz0=np.zeros(nv), rho_n=np.ones(nc), rho_s=np.ones(nv),
               pref_n=pref_n, pref_s=pref_s)
-    prm = mdgpu.RheoParams(average=0, step=1, elastic=1, dt=1.0, R=1.0, zeroC=0.0, min_eta=1e-4,
+    prm = mdgpu.RheoParams(average=0, step=1, elastic=1, dt=dt, R=1.0, zeroC=0.0, min_eta=1e-4,
                            max_eta=1e4, eta_tol=1e-13, Tiimin=1e-5, scale_E=1.0, scale_eta=1.0)
     eu, ev, ep = g.numbering()[:3]
     loop = PicardLoop(ctx, eu, ev, ep, [mdgpu.PhaseProp(**d) for d in S.vep_phases(C=C)], prm, device=device)

@@ -27,7 +27,7 @@ import time
 import numpy as np
 import pytest
 
-from shearbox import build
+from shearbox import PHYSICAL, build
 
 pytestmark = pytest.mark.gpu
 
@@ -66,3 +66,31 @@ def test_shear_band_box_1023(oracle_mod):
     print("centres yielding", npl, "of", nc, flush=True)
     seeds_cells = 64 * np.pi * 2.0 ** 2
     assert seeds_cells < npl < 0.5 * nc
+
+
+def test_shear_band_box_1023_physical_newton(oracle_mod):
+    """The box with the physical parameters (shear modulus 1e10 Pa, cohesion
+    1e7 Pa with 5e6 Pa in the seeds, friction 30 degrees, dt = 1e11 s,
+    pure shear at 1e-15 1/s, matrix viscosity 1e22 Pa s at that rate; scaled
+    by 1e22 Pa s and 1e-15 1/s as the reference scales its inputs), loaded
+    close to yield (old deviatoric stresses 0.8 C): Picard iterations with
+    the line search switch to Newton once the residual fell tenfold
+    (Main_DOODZ.c:884-896), and the loop reaches the 1e-8 non-linear
+    tolerance within its 10 iterations, the Newton steps converging
+    quadratically; the solver runs the benchmark's configuration (vertex-patch
+    smoother, penalty 100, degree 12) and every solve meets 1e-8."""
+    ctx, loop, st, info = build(oracle_mod, 1023, device=True, pre=0.8, **PHYSICAL)
+    t0 = time.time()
+    hist, nsolve, stats = loop.run(st, 10, (1e-12, 1e-8, 1e-12, 1e-8), diag_scaling=1, line_search=True,
+                                   newton=True, smoother=3, penalty=100.0, cheb_degree=12, cheb_degree_coarse=12)
+    h = np.array(hist)
+    rel = h / h[0]
+    print("solves", nsolve, "in %.1fs" % (time.time() - t0), "Newton steps", st["newton_steps"], "alphas",
+          st["alphas"], flush=True)
+    print("ms per iteration", [round(t) for t in st["iter_ms"]], "outer its", [o.its for o in stats], flush=True)
+    print("relative residual history (x, z, continuity)\n", rel[:, :3], flush=True)
+    assert st["newton_steps"] and nsolve <= 10
+    assert all(o.rel_res <= 1e-8 for o in stats)
+    assert rel[-1, 0] < 1e-8 and rel[-1, 1] < 1e-8 and rel[-1, 2] < 1e-8
+    # quadratic convergence of the last Newton steps
+    assert rel[-1, 0] < 1e-2 * rel[-2, 0]

@@ -116,8 +116,9 @@ def test_killer_solver_matches_oracle(oracle_mod, bc, precision, schur, smoother
 def test_solver_variants_agree_at_511():
     """511^2 ellipse field with a 1e6 viscosity contrast (the size at which an
     FP32 inner CG stops converging): FP64 and FP32 V-cycles, fused and
-    two-kernel smoothers (diagonal and vertex-patch Schwarz) and the
-    graph-free launch all reach 1e-8 on the FP64
+    two-kernel smoothers (diagonal and vertex-patch Schwarz), the benchmark's
+    configuration and the graph-free launches (with and without the timing
+    of the fine-level smoother steps) all reach 1e-8 on the FP64
     residual, checked independently with the library's Picard operator, and
     agree on the solution to 1e-6."""
     from mdgpu import learned as L
@@ -130,9 +131,15 @@ def test_solver_variants_agree_at_511():
     ref = None
     for kw in (dict(), dict(precision=mdgpu.MDG_PREC_MIXED), dict(smoother=1), dict(smoother=2),
                dict(smoother=3), dict(smoother=3, precision=mdgpu.MDG_PREC_MIXED, penalty=100.0),
-               dict(graphs=1, max_its=60)):
+               dict(smoother=3, penalty=100.0, cheb_degree=12, cheb_degree_coarse=12),
+               dict(graphs=1, max_its=60), dict(graphs=2), dict(graphs=2, smoother=3)):
         sol, o = prob.solve(ctx, rel_tol=1e-8, **kw)
-        print(kw, "its", o.its, "inner", o.inner_its, "rel", o.rel_res, "ms %.1f" % o.time_ms)
+        print(kw, "its", o.its, "inner", o.inner_its, "rel", o.rel_res, "ms %.1f" % o.time_ms,
+              "smoother steps timed", o.smoother_launches, "ms %.4f" % o.smoother_ms)
+        if kw.get("graphs") == 2:   # every fine-level smoother step timed with HIP events
+            assert o.smoother_launches >= o.inner_its and o.smoother_ms > 0.0, kw
+        else:
+            assert o.smoother_launches == 0
         r = F - ctx.stokes_apply(D11, D11, prob.eta_s, sol)
         assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(F), kw
         if ref is None:
