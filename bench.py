@@ -26,9 +26,9 @@ sys.path.insert(0, os.path.join(ROOT, "mdoodz7.0_amd"))
 
 HBM_PEAK_GBS = 8000.0
 # The solver configuration measured: the vertex-patch Chebyshev smoother fused
-# into one kernel per step (smoother 3), penalty 100 and degree 12 on every
+# into one kernel per step (smoother 3), penalty 500 and degree 12 on every
 # level -- the fastest of the configurations swept on this workload
-SOLVER = dict(smoother=3, penalty=100.0, cheb_degree=12, cheb_degree_coarse=12)
+SOLVER = dict(smoother=3, penalty=500.0, cheb_degree=12, cheb_degree_coarse=12)
 # HBM bytes per launch of the fine-level smoother step (k_pcheb, FP64) on
 # 2047^2 cells, rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
 # copy kernel of known size): profiles/hbm_traffic_pcheb_2047.txt
@@ -220,7 +220,7 @@ def main():
         "config": {"workload": "Stokes solve to 1e-8 relative residual, %d^2 cells, %d seeded "
                                "ellipses (1e6 viscosity contrast), free slip, with P2G of "
                                "viscosity and density from %d markers" % (a.cells, a.ellipses, n),
-                   "solver": "FGCR(100) + viscosity-weighted augmented Lagrangian (gamma 100) "
+                   "solver": "FGCR(100) + viscosity-weighted augmented Lagrangian (gamma 500) "
                              "+ CG preconditioned by a multigrid V-cycle with vertex-patch "
                              "Chebyshev smoothing (degree 12 on every level), FP64, plain "
                              "(no learned PC)", "solver_opts": SOLVER},

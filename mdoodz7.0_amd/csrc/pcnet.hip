@@ -161,15 +161,20 @@ k_conv_wgrad(int H, int W, int Cs_in, int Kp, int Cout, int Cs_g, int NTn,
     const int n = nt * 16 + m;
     const bool nv = n < Cout;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    // the lane's 8 consecutive pixels of each 32-pixel step start at
+    // qb + 8g; their (y, x) advance by 32 pixels a step (one division at
+    // the start, wrap-arounds after)
+    long qs = q0 + 8 * g;
+    int ys = (int)(qs / W), xs = (int)(qs - (long)ys * W);
     for (long qb = q0; qb < q1; qb += 32) {
         bf16x4 a4[2], b4[2];
+        int y = ys, x = xs;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const long q = qb + 8 * g + j;
             bf16_t av = 0, bv = 0;
             if (q < q1) {
                 if (kv) {
-                    const int y = (int)(q / W), x = (int)(q - (long)y * W);
                     const int yy = y + dy, xx = x + dx;
                     if (yy >= 0 && yy < H && xx >= 0 && xx < W) av = X[((long)yy * W + xx) * Cs_in + ci];
                 }
@@ -177,7 +182,10 @@ k_conv_wgrad(int H, int W, int Cs_in, int Kp, int Cout, int Cs_g, int NTn,
             }
             a4[j >> 2][j & 3] = (short)av;
             b4[j >> 2][j & 3] = (short)bv;
+            if (++x == W) { x = 0; y++; }
         }
+        xs += 32;
+        while (xs >= W) { xs -= W; ys++; }
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pack8(a4[0], a4[1]), pack8(b4[0], b4[1]), acc, 0, 0, 0);
     }
     const int ld = 16 * NTn;
@@ -188,16 +196,28 @@ k_conv_wgrad(int H, int W, int Cs_in, int Kp, int Cout, int Cs_g, int NTn,
     }
 }
 
-// dW[tap][ci][co] (+)= sum over chunks (fixed order); db[co] likewise.
-__global__ void k_wgrad_reduce(int nchunk, int Kp, int ld, int Cin, int Cs_in, int Cout,
-                               const float *__restrict__ part, float *__restrict__ dW, int accumulate) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 9 * Cin * Cout) return;
-    const int co = i % Cout, ci = (i / Cout) % Cin, tap = i / (Cout * Cin);
-    const int k = tap * Cs_in + ci;
+// dW[tap][ci][co] (+)= sum over chunks.  A 256-thread block owns 16
+// consecutive weights; 16 threads per weight sum every 16th chunk and the
+// 16 partial sums are combined in LDS in a fixed order.
+__global__ void __launch_bounds__(256) k_wgrad_reduce(int nchunk, int Kp, int ld, int Cin, int Cs_in, int Cout,
+                                                      const float *__restrict__ part, float *__restrict__ dW,
+                                                      int accumulate) {
+    __shared__ float sh[16][17];
+    const int w = threadIdx.x & 15, j = threadIdx.x >> 4;
+    const int i = blockIdx.x * 16 + w, nw = 9 * Cin * Cout;
     float s = 0.f;
-    for (int c = 0; c < nchunk; c++) s += part[((long)c * Kp + k) * ld + co];
-    dW[i] = accumulate ? dW[i] + s : s;
+    if (i < nw) {
+        const int co = i % Cout, ci = (i / Cout) % Cin, tap = i / (Cout * Cin);
+        const int k = tap * Cs_in + ci;
+        for (int c = j; c < nchunk; c += 16) s += part[((long)c * Kp + k) * ld + co];
+    }
+    sh[j][w] = s;
+    __syncthreads();
+    if (j == 0 && i < nw) {
+        float r = 0.f;
+        for (int q = 0; q < 16; q++) r += sh[q][w];
+        dW[i] = accumulate ? dW[i] + r : r;
+    }
 }
 // Bias gradient: per-chunk channel sums, then the sum over chunks.  A
 // 256-thread block splits its pixels (or chunks) over 256 / Cout groups of
@@ -518,7 +538,7 @@ void conv_bwd_weights(mdg_ctx *c, pcnet_state *s, int k, const bf16_t *x, const 
     const int nch = (int)((npix + WG_CHUNK - 1) / WG_CHUNK);
     hipLaunchKernelGGL(k_conv_wgrad, dim3(nch, (l.kpf / 16) * l.ntf), dim3(64), 0, c->stream, H, W, l.csi,
                        l.kpf, l.cout, l.cso, l.ntf, x, dz, s->part.p);
-    hipLaunchKernelGGL(k_wgrad_reduce, dim3(nblk256(9L * l.cin * l.cout)), dim3(256), 0, c->stream, nch,
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((int)((9L * l.cin * l.cout + 15) / 16)), dim3(256), 0, c->stream, nch,
                        l.kpf, 16 * l.ntf, l.cin, l.csi, l.cout, (const float *)s->part.p,
                        s->G.p + l.woff, accumulate);
     hipLaunchKernelGGL(k_bias_part, dim3(nch), dim3(256), 0, c->stream, npix, l.cout, l.cso, dz, s->part.p);
