@@ -173,38 +173,55 @@ int mdg_strain_rates_nodes(mdg_ctx *ctx, const double *exxd, const double *ezzd,
                            const double *exz, const double *div, double *exz_n,
                            double *exxd_s, double *ezzd_s, double *div_s);
 
-/* ---- rheology (NonNewtonianViscosityGrid subset) ------------------------ */
+/* ---- rheology (NonNewtonianViscosityGrid) -------------------------------- */
 
-/* One phase of mat_prop (mdoodz-private.h) for the implemented creep laws:
- * constant viscosity (cstv), dislocation power law (pwlv), Drucker-Prager
- * plasticity with viscoplastic regularisation (plast = 1); values in the
- * model's scaled units.  beta is the compressibility giving K = 1/beta. */
+/* One phase of mat_prop (mdoodz.h:158-170) for the implemented creep laws:
+ * constant viscosity (cstv), dislocation power law (pwlv), diffusion creep
+ * (linv), grain-boundary sliding (gbsv), regularised Peierls creep (expv),
+ * Drucker-Prager plasticity with viscoplastic regularisation (plast = 1);
+ * values in the model's scaled units, as ReadDataPowerLaw / ReadDataLinear /
+ * ReadDataGBS / ReadDataExponential (FlowLaws.c:46-842) leave them.  beta is
+ * the compressibility giving K = 1/beta.  A zero-filled tail (linv onward)
+ * reproduces the results of the laws before it exactly: the grain-size,
+ * fugacity and melt factors d^(m/n) f^(-r/n) exp(-a phi/n) are then 1. */
 typedef struct {
     int    cstv, pwlv, plast;
     double eta0;
     double pref_pwl, F_pwl, A_pwl, n_pwl, Q_pwl, V_pwl;
     double C, fric, dil, Slim, eta_vp, n_vp;
     double beta;
+    int    linv, gbsv, expv, reserved;
+    double m_pwl, r_pwl, f_pwl, a_pwl;                         /* mpwl rpwl fpwl apwl */
+    double F_lin, A_lin, n_lin, m_lin, r_lin, f_lin, a_lin, Q_lin, V_lin;
+    double F_gbs, A_gbs, n_gbs, m_gbs, r_gbs, f_gbs, a_gbs, Q_gbs, V_gbs;
+    double G_exp, S_exp, Q_exp, E_exp, n_exp;                  /* Gexp Sexp Qexp Eexp nexp */
+    double gs_ref;                                             /* grain size of post-processing */
 } mdg_phase_prop;
 
 /* Fields of params / scale used by ViscosityConcise.  average is
- * model->eta_average: 0 arithmetic, 1 harmonic, 2 geometric. */
+ * model->eta_average: 0 arithmetic, 1 harmonic, 2 geometric.  T_peierls is
+ * TmaxPeierls, (1200 + zeroC) / scaling->T (RheologyDensity.c:393): Peierls
+ * creep acts below it.  d_min is 1e-13 m / scaling->L: grain-size dependent
+ * diffusion creep refuses grain sizes below it (RheologyDensity.c:506-509). */
 typedef struct {
     int    step, elastic, average;
     double dt, R, zeroC, min_eta, max_eta, eta_tol, Tiimin, scale_E, scale_eta;
+    double T_peierls, d_min;
 } mdg_rheo_params;
 
 /* Node arrays of one node set (centres or vertices).  Inputs: perc is
  * nph x n (phase_perc_n / phase_perc_s, phase-major); G (mu), T, P (p_in or
  * P_s), the strain rates, the old stresses (NULL = 0), and the per-node
  * fric, C, dil (NULL = the phase's own value) and pref (a multiplier of
- * pref_pwl, NULL = 1; no counterpart in the reference).  Outputs: eta
- * (eta_n / eta_s, required), eta_phys, the stresses, p_corr, eII_pl
+ * pref_pwl, NULL = 1; no counterpart in the reference), grain size d
+ * (d0_n / d0_s) and melt fraction phi (phi0_n / phi0_s), NULL = 0.  Outputs:
+ * eta (eta_n / eta_s, required), eta_phys, the stresses, p_corr, eII_pl
  * (each may be NULL). */
 typedef struct {
     const double *perc, *G, *T, *P, *exx, *ezz, *exz, *sxx0, *szz0, *sxz0;
     const double *fric, *C, *dil, *pref;
     double *eta, *eta_phys, *sxx, *szz, *sxz, *p_corr, *eII_pl;
+    const double *d, *phi;
 } mdg_rheo_nodes;
 
 #define MDG_MAX_PHASES 64
@@ -235,7 +252,9 @@ int mdg_material_grid(mdg_ctx *ctx, const mdg_phase_soft *phases, int nph, int c
  * phase (fraction > 1e-13) ViscosityConcise (:385-1080), the phase average
  * of the effective and dissipative viscosities, and the stresses 2 eta E.
  * Air nodes (tags 30, and 31 on centres) get zeros.  Returns
- * MDG_ERR_DIVERGED where the reference exits on a failed local iteration. */
+ * MDG_ERR_DIVERGED where the reference exits on a failed local iteration
+ * and MDG_ERR_ARG where it exits on a grain size below d_min in a phase with
+ * grain-size dependent diffusion creep (m_lin > 0). */
 int mdg_viscosity_grid(mdg_ctx *ctx, const mdg_phase_prop *phases, int nph,
                        const mdg_rheo_params *prm, int centroid, const mdg_rheo_nodes *f);
 

@@ -26,7 +26,8 @@ __global__ void __launch_bounds__(256)
 k_tangent_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
                const signed char *__restrict__ tag, int centroid, mdg_rheo_nodes f,
                double *__restrict__ D, int *__restrict__ fail) {
-    __shared__ mdg_phase_prop sph[MDG_MAX_PHASES];
+    extern __shared__ double sph_raw[];     // nph phase records
+    mdg_phase_prop *sph = reinterpret_cast<mdg_phase_prop *>(sph_raw);
     for (int p = threadIdx.x; p < md.nph; p += blockDim.x) sph[p] = phases[p];
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -49,15 +50,16 @@ k_tangent_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
     const double tol = centroid ? 1e-5 : 1e-7;
     const double px = tol * fabs(Exx) + tol / 1e10, pz = tol * fabs(Ezz) + tol / 1e10;
     const double pg = tol * fabs(Exz) + tol / 1e10, pp = tol * fabs(P) + tol / 1e10;
-    bool ok = true;
+    const double dg = f.d ? f.d[i] : 0.0, phi = f.phi ? f.phi[i] : 0.0;
+    int bad = 0;
     double e = 0.0;
     for (int p = 0; p < md.nph; p++) {
         const double w = f.perc[(size_t)p * n + i];
         if (!(fabs(w) > 1e-13)) continue;
         const mdg_phase_prop &m = sph[p];
         double etaVE, Pc, Epl;
-        viscosity_concise(m, md, G, T, P, Exx, Ezz, Exz, f.fric ? f.fric[i] : m.fric,
-                          f.C ? f.C[i] : m.C, f.dil ? f.dil[i] : m.dil, pref, etaVE, Pc, Epl, ok);
+        viscosity_concise(m, md, G, T, P, dg, phi, Exx, Ezz, Exz, f.fric ? f.fric[i] : m.fric,
+                          f.C ? f.C[i] : m.C, f.dil ? f.dil[i] : m.dil, pref, etaVE, Pc, Epl, bad);
         if (md.average == 0) e += w * etaVE;
         else if (md.average == 1) e += w / etaVE;
         else e += w * log(etaVE);
@@ -80,13 +82,13 @@ k_tangent_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
             const double fr = f.fric ? f.fric[i] : m.fric, co = f.C ? f.C[i] : m.C, di = f.dil ? f.dil[i] : m.dil;
             double eb = 0.0, Pc, Epl;
             if (md.average != 0)   // the phase's own value at the base state
-                viscosity_concise(m, md, G, T, P, Exx, Ezz, Exz, fr, co, di, pref, eb, Pc, Epl, ok);
+                viscosity_concise(m, md, G, T, P, dg, phi, Exx, Ezz, Exz, fr, co, di, pref, eb, Pc, Epl, bad);
             double epm[2];
             for (int sgn = 0; sgn < 2; sgn++) {
                 const double d = sgn ? -h : h;
-                viscosity_concise(m, md, G, T, P + (q == 3 ? d : 0.0), Exx + (q == 0 ? d : 0.0),
+                viscosity_concise(m, md, G, T, P + (q == 3 ? d : 0.0), dg, phi, Exx + (q == 0 ? d : 0.0),
                                   Ezz + (q == 1 ? d : 0.0), Exz + (q == 2 ? d : 0.0), fr, co, di, pref, epm[sgn],
-                                  Pc, Epl, ok);
+                                  Pc, Epl, bad);
             }
             double v = w * (epm[0] - epm[1]) / den;
             if (md.average == 1) v /= eb * eb;
@@ -113,7 +115,7 @@ k_tangent_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
         D[2 * (size_t)n + i] = e + 2.0 * der[2] * Exz;
         D[3 * (size_t)n + i] = 2.0 * der[3] * Exz;
     }
-    if (!ok) atomicOr(fail, 1);
+    if (bad) atomicOr(fail, bad);
 }
 
 // ---------------------------------------------------------------------------
@@ -404,14 +406,14 @@ extern "C" int mdg_tangent_grid(mdg_ctx *c, const mdg_phase_prop *phases, int np
     const int nrow = centroid ? 8 : 4;
     newton_state *s = newton_of(c);
     // stage inputs through the rheology scratch of the grid update
-    std::vector<dbuf<double>> tmp(14);
+    std::vector<dbuf<double>> tmp(16);
     mdg_rheo_nodes d = {};
-    const double *const *ins[14] = {&f->perc, &f->G, &f->T, &f->P, &f->exx, &f->ezz, &f->exz,
-                                    &f->sxx0, &f->szz0, &f->sxz0, &f->fric, &f->C, &f->dil, &f->pref};
-    const double **din[14] = {&d.perc, &d.G, &d.T, &d.P, &d.exx, &d.ezz, &d.exz,
-                              &d.sxx0, &d.szz0, &d.sxz0, &d.fric, &d.C, &d.dil, &d.pref};
+    const double *const *ins[16] = {&f->perc, &f->G, &f->T, &f->P, &f->exx, &f->ezz, &f->exz, &f->sxx0,
+                                    &f->szz0, &f->sxz0, &f->fric, &f->C, &f->dil, &f->pref, &f->d, &f->phi};
+    const double **din[16] = {&d.perc, &d.G, &d.T, &d.P, &d.exx, &d.ezz, &d.exz, &d.sxx0,
+                              &d.szz0, &d.sxz0, &d.fric, &d.C, &d.dil, &d.pref, &d.d, &d.phi};
     int rc = MDG_OK;
-    for (int q = 0; q < 14 && !rc; q++)
+    for (int q = 0; q < 16 && !rc; q++)
         rc = mdg_stage_in(c, *ins[q], (size_t)n * (q == 0 ? nph : 1), tmp[q], din[q]);
     dbuf<double> dout;
     double *Dd = D;
@@ -422,14 +424,14 @@ extern "C" int mdg_tangent_grid(mdg_ctx *c, const mdg_phase_prop *phases, int np
     if (!rc && s->ph.reserve(nph) != hipSuccess) rc = mdg_fail(c, MDG_ERR_HIP, "allocation failed");
     int fail = 0;
     if (!rc) {
-        rheo_ctl md = {prm->step, prm->elastic, prm->average, nph, prm->dt, prm->R, prm->zeroC,
-                       prm->min_eta, prm->max_eta, prm->eta_tol, prm->Tiimin, prm->scale_E, prm->scale_eta};
+        const rheo_ctl md = rheo::make_ctl(prm, nph);
         if (hipMemcpyAsync(s->ph.p, phases, nph * sizeof(mdg_phase_prop), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
             c->flag_dev.reserve(4) != hipSuccess ||
             hipMemsetAsync(c->flag_dev.p, 0, sizeof(int), c->stream) != hipSuccess)
             rc = mdg_fail(c, MDG_ERR_HIP, "staging failed");
         else {
-            hipLaunchKernelGGL(k_tangent_grid, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, md,
+            hipLaunchKernelGGL(k_tangent_grid, dim3(mdg_grid1d(n, 256)), dim3(256), nph * sizeof(mdg_phase_prop),
+                               c->stream, md,
                                (const mdg_phase_prop *)s->ph.p, n, centroid ? c->BCp_t.p : c->BCg_t.p,
                                centroid ? 1 : 0, d, Dd, c->flag_dev.p);
             if (hipGetLastError() != hipSuccess) rc = mdg_fail(c, MDG_ERR_HIP, "tangent kernel launch failed");
@@ -445,8 +447,7 @@ extern "C" int mdg_tangent_grid(mdg_ctx *c, const mdg_phase_prop *phases, int np
     for (auto &b : tmp) b.release();
     dout.release();
     if (rc) return rc;
-    if (fail) return mdg_fail(c, MDG_ERR_DIVERGED, "visco-elastic local iterations failed to converge");
-    return MDG_OK;
+    return rheo::fail_status(c, fail);
 }
 
 // Stage the Newton coefficients for mdg_jacobian_apply / the Newton solve.

@@ -20,7 +20,8 @@ __global__ void __launch_bounds__(256)
 k_viscosity_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
                  const signed char *__restrict__ tag, int centroid, mdg_rheo_nodes f,
                  int *__restrict__ fail) {
-    __shared__ mdg_phase_prop sph[MDG_MAX_PHASES];
+    extern __shared__ double sph_raw[];     // nph phase records
+    mdg_phase_prop *sph = reinterpret_cast<mdg_phase_prop *>(sph_raw);
     for (int p = threadIdx.x; p < md.nph; p += blockDim.x) sph[p] = phases[p];
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -44,17 +45,18 @@ k_viscosity_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
     const double Exz = f.exz[i] + (f.sxz0 ? el * f.sxz0[i] / eta_e / 2.0 : 0.0);
     const double T = f.T[i], P = f.P[i];
     const double pref = f.pref ? f.pref[i] : 1.0;
+    const double dg = f.d ? f.d[i] : 0.0, phi = f.phi ? f.phi[i] : 0.0;
     double e = 0.0, ep = 0.0, pc = 0.0, epl = 0.0;
-    bool ok = true;
+    int bad = 0;
     for (int p = 0; p < md.nph; p++) {
         const double w = f.perc[(size_t)p * n + i];
         if (!(fabs(w) > 1e-13)) continue;                              // :1212-1215
         const mdg_phase_prop &m = sph[p];
         double etaVE, Pcorr, Eii_pl;
-        const double eta = viscosity_concise(m, md, G, T, P, Exx, Ezz, Exz,
+        const double eta = viscosity_concise(m, md, G, T, P, dg, phi, Exx, Ezz, Exz,
                                              f.fric ? f.fric[i] : m.fric, f.C ? f.C[i] : m.C,
                                              f.dil ? f.dil[i] : m.dil, pref, etaVE, Pcorr,
-                                             Eii_pl, ok);
+                                             Eii_pl, bad);
         if (md.average == 0) { e += w * etaVE; ep += w * eta; }
         else if (md.average == 1) { e += w / etaVE; ep += w / eta; }
         else { e += w * log(etaVE); ep += w * log(eta); }
@@ -70,7 +72,7 @@ k_viscosity_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
     if (f.sxz) f.sxz[i] = 2.0 * e * Exz;
     if (f.p_corr) f.p_corr[i] = pc;
     if (f.eII_pl) f.eII_pl[i] = epl;
-    if (!ok) atomicOr(fail, 1);
+    if (bad) atomicOr(fail, bad);
 }
 
 // InterpCentroidsToVerticesDouble (GridRoutines.c:108-212) without the padded
@@ -211,7 +213,7 @@ int out_dev(mdg_ctx *c, double *user, size_t n, dbuf<double> &tmp, double **o) {
 }  // namespace
 
 struct rheo_scratch {
-    dbuf<double> in[14], out[7];
+    dbuf<double> in[16], out[7];
     dbuf<mdg_phase_prop> ph;
     dbuf<mdg_phase_soft> soft;
 };
@@ -249,7 +251,7 @@ extern "C" int mdg_viscosity_grid(mdg_ctx *c, const mdg_phase_prop *phases, int 
         const mdg_phase_prop &m = phases[p];
         if (m.plast != 0 && m.plast != 1)
             return mdg_fail(c, MDG_ERR_UNSUPPORTED, "phase %d: plast = %d (only 0 and 1 are implemented)", p, m.plast);
-        if (!m.cstv && !m.pwlv && !prm->elastic)
+        if (!m.cstv && !m.pwlv && !m.linv && !m.expv && !prm->elastic)
             return mdg_fail(c, MDG_ERR_UNSUPPORTED, "phase %d has no creep law", p);
     }
     const int n = centroid ? (c->Nx - 1) * (c->Nz - 1) : c->Nx * c->Nz;
@@ -257,11 +259,11 @@ extern "C" int mdg_viscosity_grid(mdg_ctx *c, const mdg_phase_prop *phases, int 
     rheo_scratch *s = scratch_of(c);
     int rc;
     mdg_rheo_nodes d = {};
-    const double *const *ins[14] = {&f->perc, &f->G, &f->T, &f->P, &f->exx, &f->ezz, &f->exz,
-                                    &f->sxx0, &f->szz0, &f->sxz0, &f->fric, &f->C, &f->dil, &f->pref};
-    const double **din[14] = {&d.perc, &d.G, &d.T, &d.P, &d.exx, &d.ezz, &d.exz,
-                              &d.sxx0, &d.szz0, &d.sxz0, &d.fric, &d.C, &d.dil, &d.pref};
-    for (int q = 0; q < 14; q++)
+    const double *const *ins[16] = {&f->perc, &f->G, &f->T, &f->P, &f->exx, &f->ezz, &f->exz, &f->sxx0,
+                                    &f->szz0, &f->sxz0, &f->fric, &f->C, &f->dil, &f->pref, &f->d, &f->phi};
+    const double **din[16] = {&d.perc, &d.G, &d.T, &d.P, &d.exx, &d.ezz, &d.exz, &d.sxx0,
+                              &d.szz0, &d.sxz0, &d.fric, &d.C, &d.dil, &d.pref, &d.d, &d.phi};
+    for (int q = 0; q < 16; q++)
         if ((rc = mdg_stage_in(c, *ins[q], (size_t)n * (q == 0 ? nph : 1), s->in[q], din[q]))) return rc;
     double *const outs[7] = {f->eta, f->eta_phys, f->sxx, f->szz, f->sxz, f->p_corr, f->eII_pl};
     double **dout[7] = {&d.eta, &d.eta_phys, &d.sxx, &d.szz, &d.sxz, &d.p_corr, &d.eII_pl};
@@ -271,10 +273,10 @@ extern "C" int mdg_viscosity_grid(mdg_ctx *c, const mdg_phase_prop *phases, int 
     HIPCHK(c, hipMemcpyAsync(s->ph.p, phases, nph * sizeof(mdg_phase_prop), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, c->flag_dev.reserve(4));
     HIPCHK(c, hipMemsetAsync(c->flag_dev.p, 0, sizeof(int), c->stream));
-    rheo_ctl md = {prm->step, prm->elastic, prm->average, nph, prm->dt, prm->R, prm->zeroC,
-                   prm->min_eta, prm->max_eta, prm->eta_tol, prm->Tiimin, prm->scale_E, prm->scale_eta};
+    const rheo_ctl md = rheo::make_ctl(prm, nph);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_viscosity_grid, dim3(mdg_grid1d(n, 256)), dim3(256), 0, c->stream, md,
+    hipLaunchKernelGGL(k_viscosity_grid, dim3(mdg_grid1d(n, 256)), dim3(256), nph * sizeof(mdg_phase_prop),
+                       c->stream, md,
                        (const mdg_phase_prop *)s->ph.p, n, tag, centroid, d, c->flag_dev.p);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -287,8 +289,7 @@ extern "C" int mdg_viscosity_grid(mdg_ctx *c, const mdg_phase_prop *phases, int 
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
-    if (fail) return mdg_fail(c, MDG_ERR_DIVERGED, "visco-elastic local iterations failed to converge");
-    return MDG_OK;
+    return rheo::fail_status(c, fail);
 }
 
 extern "C" int mdg_c2v(mdg_ctx *c, int periodic_x, const double *cent, double *vert) {

@@ -127,26 +127,37 @@ MDG_ERR_DIVERGED = -6
 MDG_ERR_INCOMPRESSIBLE = -8
 
 
+LAW_TAIL = ("m_pwl", "r_pwl", "f_pwl", "a_pwl") + \
+    tuple(k + "_lin" for k in ("F", "A", "n", "m", "r", "f", "a", "Q", "V")) + \
+    tuple(k + "_gbs" for k in ("F", "A", "n", "m", "r", "f", "a", "Q", "V")) + \
+    ("G_exp", "S_exp", "Q_exp", "E_exp", "n_exp", "gs_ref")
+
+
 class PhaseProp(C.Structure):
-    """mdg_phase_prop: one phase of mat_prop for the implemented creep laws."""
+    """mdg_phase_prop: one phase of mat_prop for the implemented creep laws
+    (constant, power law, diffusion, grain-boundary sliding, Peierls)."""
     _fields_ = [("cstv", C.c_int), ("pwlv", C.c_int), ("plast", C.c_int),
                 ("eta0", C.c_double), ("pref_pwl", C.c_double), ("F_pwl", C.c_double),
                 ("A_pwl", C.c_double), ("n_pwl", C.c_double), ("Q_pwl", C.c_double),
                 ("V_pwl", C.c_double), ("C", C.c_double), ("fric", C.c_double),
                 ("dil", C.c_double), ("Slim", C.c_double), ("eta_vp", C.c_double),
-                ("n_vp", C.c_double), ("beta", C.c_double)]
+                ("n_vp", C.c_double), ("beta", C.c_double),
+                ("linv", C.c_int), ("gbsv", C.c_int), ("expv", C.c_int), ("reserved", C.c_int)] + \
+        [(k, C.c_double) for k in LAW_TAIL]
 
 
 class RheoParams(C.Structure):
-    """mdg_rheo_params: the params/scale fields ViscosityConcise reads."""
+    """mdg_rheo_params: the params/scale fields ViscosityConcise reads
+    (T_peierls = 0 leaves Peierls creep inactive)."""
     _fields_ = [("step", C.c_int), ("elastic", C.c_int), ("average", C.c_int),
                 ("dt", C.c_double), ("R", C.c_double), ("zeroC", C.c_double),
                 ("min_eta", C.c_double), ("max_eta", C.c_double), ("eta_tol", C.c_double),
-                ("Tiimin", C.c_double), ("scale_E", C.c_double), ("scale_eta", C.c_double)]
+                ("Tiimin", C.c_double), ("scale_E", C.c_double), ("scale_eta", C.c_double),
+                ("T_peierls", C.c_double), ("d_min", C.c_double)]
 
 
 RHEO_IN = ("perc", "G", "T", "P", "exx", "ezz", "exz", "sxx0", "szz0", "sxz0", "fric", "C",
-           "dil", "pref")
+           "dil", "pref", "d", "phi")
 RHEO_OUT = ("eta", "eta_phys", "sxx", "szz", "sxz", "p_corr", "eII_pl")
 
 
@@ -158,7 +169,8 @@ class PhaseSoft(C.Structure):
 
 
 class RheoNodes(C.Structure):
-    _fields_ = [(k, C.c_void_p) for k in RHEO_IN + RHEO_OUT]
+    """mdg_rheo_nodes: grain size d and melt fraction phi follow the outputs."""
+    _fields_ = [(k, C.c_void_p) for k in RHEO_IN[:14] + RHEO_OUT + RHEO_IN[14:]]
 
 
 def _extra_signatures(L):

@@ -1,6 +1,7 @@
-"""Dislocation-creep (power-law) flow-law database and the model scales.
+"""Creep flow-law databases and the model scales.
 
-Restates ReadDataPowerLaw (FlowLaws.c:46-588) and ScaleMe
+Restates ReadDataPowerLaw (FlowLaws.c:46-588), ReadDataLinear (:591-706),
+ReadDataGBS (:711-776), ReadDataExponential (:782-842) and ScaleMe
 (InputOutput.c:1835-1849).  A flow law is the published laboratory fit
 
     strain rate = A * stress^n * d^-m * f^r * exp(a phi) * exp(-(Q + P V) / (R T)),
@@ -16,8 +17,16 @@ where F converts the experiment's stress and strain-rate measures to the
 second invariants.  All values leave this module in the model's scaled
 units, like mat_prop after ReadInputFile.
 
-The table holds the published constants each case of ReadDataPowerLaw
-sets (same numbers, same case numbers, so input files keep their meaning).
+Diffusion (linear) creep and grain-boundary sliding use the same form and
+correction factor; Peierls creep is the exponential law
+
+    strain rate = E * exp(-Q / (R T) * (1 - sigma / S)^2) * sigma^n,
+
+regularised in ViscosityConcise (RheologyDensity.c:516-523) by its tangent
+power law at sigma = G * S.
+
+The tables hold the published constants each case of the readers sets
+(same numbers, same case numbers, so input files keep their meaning).
 """
 import math
 from dataclasses import dataclass
@@ -91,6 +100,33 @@ class Scale:
         self.rhoE = self.J / self.m * self.rho
 
 
+# number: (label, t, n, m, r, Q [J/mol], V [m^3/mol], A [Pa^-n m^m s^-1], f [Pa], a)
+LINEAR_LAWS = {
+    15: ("Calcite diffusion creep, Herwegh 2003", 1, 1.1, 3.3, 0.0, 200e3, 0.0, 1.7119e-19, 0.0, 0.0),
+    19: ("Anorthite 100 (0.007 wt% H2O), Rybacki & Dresen 2000, 2004", 1, 1.0, 3.0, 0.0, 170e3, 0.0, 5.0119e-23,
+         0.0, 0.0),
+    25: ("Plagioclase diffusion creep, Rybacki & Dresen 2000", 1, 1.0, 3.0, 0.0, 268e3, 0.0, 1.2589e-19, 0.0, 0.0),
+    40: ("Olivine dry diffusion creep, Hirth & Kohlstedt 2003", 1, 1.0, 3.0, 0.0, 375e3, 4.0e-6, 1.5000e-15, 0.0,
+         0.0),
+    41: ("Olivine wet diffusion creep, Hirth & Kohlstedt 2003", 1, 1.0, 3.0, 1.0, 375e3, 7e-6, 2.5000e-23, 1e9,
+         0.0),
+}
+
+# number: (label, t, n, m, r, Q, V, A, f, a), as LINEAR_LAWS
+GBS_LAWS = {
+    40: ("Olivine GBS T < 1250 C, Hirth & Kohlstedt 2003", 1, 3.5, 2.0, 0.0, 400e3, 0.0, 6.5000e-30, 0.0, 0.0),
+    41: ("Olivine GBS T > 1250 C, Hirth & Kohlstedt 2003", 1, 3.5, 2.0, 0.0, 600e3, 0.0, 4.7000e-23, 0.0, 0.0),
+}
+
+# number: (label, t, q, G (gamma), S [Pa], Q [J/mol], V [m^3/mol], E [Pa^-n s^-1], n)
+EXP_LAWS = {
+    25: ("Plagioclase Peierls creep, Azuma et al. 2014, regularised as Kameyama et al. 1999", 1, 2.0, 0.2, 9.831e9,
+         431.0e3, 0.0, 3.02e-9, 2.0),
+    40: ("Olivine Peierls creep, Evans & Goetze 1979, regularised as Kameyama et al. 1999", 1, 2.0, 0.1, 8.5e9,
+         5.4e5, 0.0, 5.7e11, 0.0),
+}
+
+
 def correction_factor(t, n):
     """F of ReadDataPowerLaw (FlowLaws.c:577-579): 1 for laws already in
     invariant form, the axial-compression and simple-shear conversions
@@ -141,22 +177,74 @@ def power_law(number, scaling, n=1.0, Q=0.0, A=0.0, a=0.0, t=0.0, eta0=0.0, eps0
     return out
 
 
+def _scaled_creep(law, number, scaling):
+    t_, n_, m_, r_, Q_, V_, A_, f_, a_ = (float(v) for v in law)
+    S, L = scaling.S, scaling.L
+    return {"t": t_, "n": n_, "m": m_, "r": r_, "Q": Q_ / scaling.J, "V": V_ / L ** 3,
+            "A": A_ / (math.pow(S, -n_) * math.pow(S, -r_) * math.pow(L, m_) / scaling.t),
+            "f": f_ / S, "a": a_, "F": 1.0 if number < 0 else correction_factor(t_, n_)}
+
+
+def linear_law(number, scaling, force_act_vol_ast=0, act_vol_dif_ast=0.0):
+    """ReadDataLinear for phase parameter linv = number: the scaled t, n, m,
+    r, Q, V, A, f, a and F (a negative number drops the correction factor).
+    force_act_vol_ast replaces the activation volume of law 40."""
+    k = abs(int(number))
+    if k not in LINEAR_LAWS:
+        raise ValueError("non-existing linear flow law number %d" % number)
+    law = LINEAR_LAWS[k][1:]
+    if k == 40 and force_act_vol_ast == 1:
+        law = law[:5] + (act_vol_dif_ast,) + law[6:]
+    return _scaled_creep(law, number, scaling)
+
+
+def gbs_law(number, scaling):
+    """ReadDataGBS for phase parameter gbsv = number (fields as linear_law)."""
+    k = abs(int(number))
+    if k not in GBS_LAWS:
+        raise ValueError("non-existing GBS flow law number %d" % number)
+    return _scaled_creep(GBS_LAWS[k][1:], number, scaling)
+
+
+def exponential_law(number, scaling):
+    """ReadDataExponential for phase parameter expv = number: the scaled
+    t, q, G, S, Q, V, E, n (a negative number sets t = 0, which the
+    viscosity does not use)."""
+    k = abs(int(number))
+    if k not in EXP_LAWS:
+        raise ValueError("non-existing exponential flow law number %d" % number)
+    t_, q_, G_, S_, Q_, V_, E_, n_ = (float(v) for v in EXP_LAWS[k][1:])
+    return {"t": 0.0 if number < 0 else t_, "q": q_, "G": G_, "S": S_ / scaling.S, "Q": Q_ / scaling.J,
+            "V": V_ / scaling.L ** 3, "E": E_ / (scaling.E * math.pow(scaling.S, -n_)), "n": n_}
+
+
 def phase_prop_fields(law, pref_pwl=1.0):
-    """The power-law fields of mdg_phase_prop for a law from power_law().
-    The kernels evaluate B = pref * F * A^(-1/n) * exp((Q + P V) / (n R T));
-    the constant factor f^(-r/n) of the wet laws is folded into pref here
-    (a product reordering of ViscosityConcise's, RheologyDensity.c:502).
-    Grain-size (m) and melt (a) terms need a grain size and a melt fraction
-    per node, which this path does not carry: m must be 0 (every tabulated
-    law has m = 0), and a != 0 is accepted only because the melt fraction
-    of a model without melting is zero, where exp(-a phi / n) = 1."""
-    if law["m"] != 0.0:
-        raise ValueError("grain-size dependent power laws (m != 0) are not supported")
-    pref = pref_pwl
-    if law["r"] != 0.0:
-        pref = pref * math.pow(law["f"], -law["r"] / law["n"])
-    return {"pwlv": 1, "pref_pwl": pref, "F_pwl": law["F"], "A_pwl": law["A"], "n_pwl": law["n"],
-            "Q_pwl": law["Q"], "V_pwl": law["V"]}
+    """The power-law fields of mdg_phase_prop for a law from power_law():
+    B = pref * F * A^(-1/n) * exp((Q + P V) / (n R T)) * d^(m/n) * f^(-r/n) * exp(-a phi / n)
+    with the node's grain size d and melt fraction phi."""
+    return {"pwlv": 1, "pref_pwl": pref_pwl, "F_pwl": law["F"], "A_pwl": law["A"], "n_pwl": law["n"],
+            "Q_pwl": law["Q"], "V_pwl": law["V"], "m_pwl": law["m"], "r_pwl": law["r"], "f_pwl": law["f"],
+            "a_pwl": law["a"]}
+
+
+def linear_prop_fields(law):
+    """The diffusion-creep fields of mdg_phase_prop for a law from linear_law()."""
+    out = {"linv": 1}
+    out.update({k + "_lin": law[k] for k in ("F", "A", "n", "m", "r", "f", "a", "Q", "V")})
+    return out
+
+
+def gbs_prop_fields(law):
+    """The grain-boundary-sliding fields of mdg_phase_prop for a law from gbs_law()."""
+    out = {"gbsv": 1}
+    out.update({k + "_gbs": law[k] for k in ("F", "A", "n", "m", "r", "f", "a", "Q", "V")})
+    return out
+
+
+def exponential_prop_fields(law):
+    """The Peierls-creep fields of mdg_phase_prop for a law from exponential_law()."""
+    return {"expv": 1, "G_exp": law["G"], "S_exp": law["S"], "Q_exp": law["Q"], "E_exp": law["E"],
+            "n_exp": law["n"]}
 
 
 def viscosity_si(law_si, Eii, T, P=0.0, R=8.314, pref=1.0):

@@ -146,7 +146,7 @@ _MODEL_KEYS = {
     "shear_heating": ("i", 1, None), "adiab_heating": ("i", 0, None),
     "bkg_strain_rate": ("d", 1e-30, "E"), "bkg_div_rate": ("d", 0.0, "E"), "bkg_pressure": ("d", 0.0, "S"),
     "bkg_temperature": ("d", 0.0, "T"),
-    "force_act_vol_ast": ("i", 0, None), "act_vol_dis_ast": ("d", 0.0, None),
+    "force_act_vol_ast": ("i", 0, None), "act_vol_dis_ast": ("d", 0.0, None), "act_vol_dif_ast": ("d", 0.0, None),
     "force_melt_weak": ("i", 0, None), "melt_weak": ("d", 0.0, None),
     "gx": ("d", 0.0, "a"), "gz": ("d", 0.0, "a"),
     "Nb_phases": ("i", 0, None),
@@ -207,6 +207,11 @@ class Phase:
                                     force_act_vol_ast=model["force_act_vol_ast"],
                                     act_vol_dis_ast=model["act_vol_dis_ast"],
                                     force_melt_weak=model["force_melt_weak"], melt_weak=model["melt_weak"])
+        self.gs_ref = d("gs_ref", 2.0e-3) / scl.L
+        self.lin = None if self.linv == 0 else FL.linear_law(self.linv, scl, model["force_act_vol_ast"],
+                                                              model["act_vol_dif_ast"])
+        self.gbs = None if self.gbsv == 0 else FL.gbs_law(self.gbsv, scl)
+        self.exp = None if self.expv == 0 else FL.exponential_law(self.expv, scl)
 
 
 class Model:
@@ -248,17 +253,22 @@ class Model:
         return self.params[key]
 
     def phase_props(self):
-        """mdg_phase_prop per phase (constant viscosity, power law, Drucker-Prager)."""
+        """mdg_phase_prop per phase (constant viscosity, power-law, diffusion,
+        grain-boundary-sliding and Peierls creep, Drucker-Prager)."""
         from . import PhaseProp
         out = []
         for ph in self.phases:
-            if ph.linv or ph.gbsv or ph.expv:
-                raise ValueError("linear, GBS and exponential creep are not implemented on this path")
             kw = dict(cstv=1 if ph.cstv else 0, pwlv=0, plast=ph.plast, eta0=ph.eta0, pref_pwl=1.0, F_pwl=1.0,
                       A_pwl=1.0, n_pwl=1.0, Q_pwl=0.0, V_pwl=0.0, C=ph.C, fric=ph.phi, dil=ph.psi, Slim=ph.Slim,
-                      eta_vp=ph.eta_vp, n_vp=ph.n_vp, beta=ph.bet)
+                      eta_vp=ph.eta_vp, n_vp=ph.n_vp, beta=ph.bet, gs_ref=ph.gs_ref)
             if ph.pwl is not None:
                 kw.update(FL.phase_prop_fields(ph.pwl, ph.pref_pwl))
+            if ph.lin is not None:
+                kw.update(FL.linear_prop_fields(ph.lin))
+            if ph.gbs is not None:
+                kw.update(FL.gbs_prop_fields(ph.gbs))
+            if ph.exp is not None:
+                kw.update(FL.exponential_prop_fields(ph.exp))
             out.append(PhaseProp(**kw))
         return out
 
@@ -271,13 +281,14 @@ class Model:
 
     def rheo_params(self, step=1, dt=None):
         """mdg_rheo_params for time step `step` (ViscosityConcise's constants,
-        RheologyDensity.c:427,496: Tiimin = 1e5 Pa, T floor 0 degC)."""
+        RheologyDensity.c:393,427,496,506: Peierls creep below 1200 degC,
+        Tiimin = 1e5 Pa, T floor 0 degC, smallest grain size 1e-13 m)."""
         from . import RheoParams
         s, p = self.scaling, self.params
         return RheoParams(step=step, elastic=p["elastic"], average=p["eta_average"],
                           dt=p["dt"] if dt is None else dt, R=self.R, zeroC=ZERO_C / s.T, min_eta=p["min_eta"],
                           max_eta=p["max_eta"], eta_tol=p["eta_tol"], Tiimin=1e5 / s.S, scale_E=s.E,
-                          scale_eta=s.eta)
+                          scale_eta=s.eta, T_peierls=(1200.0 + ZERO_C) / s.T, d_min=1e-13 / s.L)
 
     def solver_opts(self):
         """mdg_solver_opts running KillerSolver's own iteration (method 1)

@@ -67,7 +67,7 @@ class PicardLoop:
         nu, nw = c.Nx * (c.Nz + 1), (c.Nx + 1) * c.Nz
         sizes = dict(div=nc, exxd=nc, ezzd=nc, exz=nv, exz_n=nc, exxd_s=nv, ezzd_s=nv, div_s=nv,
                      T_s=nv, P_s=nv, D11=nc, D33=nv, eta_n=nc, F=neq, d=neq, sol=neq, rx=nu, rz=nw, u0=nu,
-                     v0=nw, p0=nc, Dn=8 * nc, Ds=4 * nv)
+                     v0=nw, p0=nc, u_start=nu, v_start=nw, p_start=nc, Dn=8 * nc, Ds=4 * nv)
         W = {k: DeviceArray(c, n) for k, n in sizes.items()}
         W["cen"] = {k: DeviceArray(c, nc) for k in RHEO_OUT}
         W["ver"] = {k: DeviceArray(c, nv) for k in RHEO_OUT}
@@ -119,12 +119,14 @@ class PicardLoop:
             scatter(*self.eqn, dx, st["u"], st["v"], st["p"], alpha)
             self.ctx.apply_bc(st["u"], st["v"])
 
-    def _save(self, st):
+    def _save(self, st, slot="0"):
+        """u, v, p into the work slot ("0": the line search's, "_start":
+        the time step's start for safe-mode restarts)."""
         if self.device:
             W = self.W
-            for k, b in (("u", "u0"), ("v", "v0"), ("p", "p0")):
-                W[b].copy_from(st[k])
-            return W["u0"], W["v0"], W["p0"]
+            for k in ("u", "v", "p"):
+                W[k + slot].copy_from(st[k])
+            return tuple(W[k + slot] for k in ("u", "v", "p"))
         return [st[k].copy() for k in ("u", "v", "p")]
 
     def _restore(self, st, saved):
@@ -139,16 +141,25 @@ class PicardLoop:
         _, norms = self._residual(cen, ver, sr, rx, rz, d=d)
         return np.asarray(norms, dtype=float)
 
-    def line_search(self, st, dx, d, rx, rz, res_f, min_step=0.0, let_res_grow=0, newton=False):
-        """LineSearchDecoupled (StokesRoutines.c:582-807), Picard branch: six
-        trial steps from alpha = -0 down to -2.5, each followed by
+    def line_search(self, st, dx, d, rx, rz, res_f, min_step=0.0, let_res_grow=0, newton=False, pc=False):
+        """LineSearchDecoupled (StokesRoutines.c:582-807): six trial steps
+        from alpha = maxalpha down to minalpha, each followed by
         UpdateNonLinearity and the residual norms, then the acceptance cascade
-        on the combined, xz, x and z norms.  Returns (alpha, stagnated) and
+        on the combined, xz, x and z norms.  With pc (pc_line_search,
+        :636-668) the step minalpha is tried first and taken without the scan
+        if it lowers the combined norm.  Returns (alpha, stagnated) and
         leaves u, v, p as they were."""
         # Newton steps search [-1, -min_step], Picard steps [-2.5, 0] (:603-613)
         ntry = 6
         minalpha, maxalpha = (-1.0, -min_step) if newton else (-2.5, -0.0)
         saved = self._save(st)
+        f0 = np.asarray(res_f, dtype=float)
+        if pc:
+            self._step(st, dx, minalpha)
+            r_pc = self.residual_norms(st, d, rx, rz)
+            self._restore(st, saved)
+            if np.sqrt((r_pc ** 2).sum()) < np.sqrt((f0 ** 2).sum()):
+                return minalpha, False
         dalpha = abs(maxalpha - minalpha) / (ntry - 1)
         trials, rows = [], []
         alpha = maxalpha
@@ -164,7 +175,6 @@ class PicardLoop:
         comb = np.sqrt((R ** 2).sum(1))
         xz = np.hypot(R[:, 0], R[:, 1])
         best = [int(np.argmin(a)) for a in (comb, xz, R[:, 0], R[:, 1])]
-        f0 = np.asarray(res_f, dtype=float)
         choice = None
         if comb[best[0]] < np.sqrt((f0 ** 2).sum()):
             choice = best[0]
@@ -181,7 +191,8 @@ class PicardLoop:
 
     def run(self, st, nit_max, tol, diag_scaling=1, gx=0.0, gz=0.0, line_search=False,
             newton=False, picard2newton=True, picard2newton_tol=1e-1, max_its_pic=10,
-            **solver_opts):
+            line_search_min=0.0, let_res_grow=0, pc_line_search=0, safe_mode=0, safe_dt_div=5.0,
+            max_num_stag=3, **solver_opts):
         """Iterates in place on st; tol = (abs_mom, rel_mom, abs_div, rel_div).
         Returns (residual norm history, number of linear solves, solver stats);
         with line_search the chosen steps are in st["alphas"].  With newton,
@@ -195,7 +206,15 @@ class PicardLoop:
         st["iter_ms"] the wall time of each iteration (rheology, residual,
         solve, line search and update).  In device mode st's fields are
         uploaded once (st["upload_ms"]) and u, v, p, eta_n, eta_s come back
-        at the end."""
+        at the end.
+
+        A stagnated line search ends the loop, unless safe_mode = 1 with
+        elasticity (Main_DOODZ.c:1021-1052): then the time step self.prm.dt
+        is divided by safe_dt_div, u, v, p return to their values at the call
+        and the iterations restart from 0 (st["safe_restarts"] lists the
+        iterations that stagnated); the max_num_stag-th stagnation raises
+        (the reference exits).  line_search_min, let_res_grow and
+        pc_line_search are the reference's keys of the same names."""
         c = self.ctx
         # the reference's continuity tolerances (InputOutput.c:1179-1180)
         # for KillerSolver's a-posteriori incompressibility check
@@ -216,7 +235,10 @@ class PicardLoop:
         st["newton_steps"] = []
         st["diverged_solves"] = []
         st["iter_ms"] = []
-        for nit in range(nit_max + 1):
+        st["safe_restarts"] = []
+        start = self._save(S, "_start") if safe_mode == 1 else None
+        nstag, nit = 0, 0
+        while nit <= nit_max:
             t_it = time.perf_counter()
             if newton and nit > 0 and not is_newton:
                 rr = hist[-1] / res0
@@ -267,12 +289,24 @@ class PicardLoop:
             stats.append(o)
             alpha, stagnated = -1.0, False
             if line_search or is_newton:
-                alpha, stagnated = self.line_search(S, dx, d, rx, rz, norms, newton=is_newton)
+                alpha, stagnated = self.line_search(S, dx, d, rx, rz, norms, min_step=line_search_min,
+                                                    let_res_grow=let_res_grow, newton=is_newton,
+                                                    pc=bool(pc_line_search))
                 st.setdefault("alphas", []).append(alpha)
             self._step(S, dx, alpha)
             st["iter_ms"].append(1e3 * (time.perf_counter() - t_it))
-            if stagnated:
+            if stagnated and safe_mode <= 0:
                 break
+            if stagnated and self.prm.elastic == 1 and safe_mode == 1:
+                st["safe_restarts"].append(nit)
+                self.prm.dt /= safe_dt_div
+                self._restore(S, start)
+                nstag += 1
+                if nstag == max_num_stag:
+                    raise RuntimeError("non-linear iterations stagnated %d times in safe mode" % nstag)
+                res0, nit = None, 0
+                continue
+            nit += 1
         if W:
             for k in ("u", "v", "p"):
                 st[k][:] = S[k].numpy()

@@ -37,21 +37,29 @@ class KillerOpts(C.Structure):
                 ("maxru", C.c_double), ("maxru0", C.c_double), ("status", C.c_int)]
 
 
+LAW_TAIL = ["m_pwl", "r_pwl", "f_pwl", "a_pwl"] + \
+    [k + "_lin" for k in ("F", "A", "n", "m", "r", "f", "a", "Q", "V")] + \
+    [k + "_gbs" for k in ("F", "A", "n", "m", "r", "f", "a", "Q", "V")] + \
+    ["G_exp", "S_exp", "Q_exp", "E_exp", "n_exp", "gs_ref"]
+
+
 class OrcPhase(C.Structure):
-    """One phase of mat_prop (mdoodz-private.h) restricted to the rheology subset."""
+    """One phase of mat_prop (mdoodz.h:158-170): the members ViscosityConcise reads."""
     _fields_ = [("cstv", C.c_int), ("pwlv", C.c_int), ("plast", C.c_int),
                 ("eta0", C.c_double), ("pref_pwl", C.c_double), ("F_pwl", C.c_double),
                 ("A_pwl", C.c_double), ("n_pwl", C.c_double), ("Q_pwl", C.c_double),
                 ("V_pwl", C.c_double), ("C", C.c_double), ("fric", C.c_double),
                 ("dil", C.c_double), ("Slim", C.c_double), ("eta_vp", C.c_double),
-                ("n_vp", C.c_double), ("beta_for_K", C.c_double)]
+                ("n_vp", C.c_double), ("beta_for_K", C.c_double),
+                ("linv", C.c_int), ("gbsv", C.c_int), ("expv", C.c_int), ("reserved", C.c_int)] + \
+        [(k, C.c_double) for k in LAW_TAIL]
 
 
 class OrcRheoModel(C.Structure):
     _fields_ = [("step", C.c_int), ("elastic", C.c_int), ("dt", C.c_double), ("R", C.c_double),
                 ("zeroC", C.c_double), ("min_eta", C.c_double), ("max_eta", C.c_double),
                 ("eta_tol", C.c_double), ("Tiimin", C.c_double), ("scale_E", C.c_double),
-                ("scale_eta", C.c_double)]
+                ("scale_eta", C.c_double), ("T_peierls", C.c_double), ("d_min", C.c_double)]
 
 
 class OrcPhaseSoft(C.Structure):
@@ -64,14 +72,14 @@ def phase(**kw):
     """OrcPhase with reference-like defaults (constant viscosity 1, no plasticity)."""
     d = dict(cstv=1, pwlv=0, plast=0, eta0=1.0, pref_pwl=1.0, F_pwl=1.0, A_pwl=1.0, n_pwl=1.0,
              Q_pwl=0.0, V_pwl=0.0, C=1e50, fric=0.0, dil=0.0, Slim=1e50, eta_vp=0.0, n_vp=1.0,
-             beta_for_K=1e-50)
+             beta_for_K=1e-50, gs_ref=2e-3)
     d.update(kw)
     return OrcPhase(**d)
 
 
 def rheo_model(**kw):
     d = dict(step=1, elastic=0, dt=1.0, R=8.314, zeroC=0.0, min_eta=1e-50, max_eta=1e50,
-             eta_tol=1e-13, Tiimin=1e-50, scale_E=1.0, scale_eta=1.0)
+             eta_tol=1e-13, Tiimin=1e-50, scale_E=1.0, scale_eta=1.0, T_peierls=1473.15, d_min=1e-13)
     d.update(kw)
     return OrcRheoModel(**d)
 
@@ -147,13 +155,13 @@ def lib():
         L.orc_rotate_stress.restype = None
         L.orc_set_band_order.argtypes = [gp, i32p, i32p, C.c_int]
         L.orc_viscosity_concise.argtypes = [C.POINTER(OrcPhase), C.POINTER(OrcRheoModel)] + \
-            [C.c_double] * 10 + [_dp, _dp, _dp, C.POINTER(C.c_int)]
+            [C.c_double] * 12 + [_dp, _dp, _dp, C.POINTER(C.c_int)]
         L.orc_viscosity_concise.restype = C.c_double
         L.orc_rheology_nodes.argtypes = [C.POINTER(OrcRheoModel), C.POINTER(OrcPhase), C.c_int,
-                                         C.c_int] + [C.c_void_p] * 14 + [C.c_void_p, C.c_int,
+                                         C.c_int] + [C.c_void_p] * 16 + [C.c_void_p, C.c_int,
                                          C.c_int] + [C.c_void_p] * 7
         L.orc_rheology_nodes.restype = C.c_int
-        L.orc_tangent_nodes.argtypes = L.orc_rheology_nodes.argtypes[:21] + [C.c_void_p]
+        L.orc_tangent_nodes.argtypes = L.orc_rheology_nodes.argtypes[:23] + [C.c_void_p]
         L.orc_tangent_nodes.restype = C.c_int
         L.orc_material_grid.argtypes = [C.POINTER(OrcPhaseSoft), C.c_int, C.c_int] + \
             [C.c_void_p] * 3 + [C.c_int, C.c_int] + [C.c_void_p] * 5
@@ -495,11 +503,12 @@ def killer_solve(g, num, A, B, Cm, bu, bp, penalty=1e5, max_its_PH=10, rel_tol_K
 
 
 def viscosity_concise(ph, md, G, T, P, Exx, Ezz, Exz, fric=None, Coh=None, dil=None,
-                      pref_mult=1.0):
-    """RheologyDensity.c:385-1080 on one (node, phase): returns
-    (eta_dissipative, etaVE, Pcorr, Eii_pl, status)."""
+                      pref_mult=1.0, d=0.0, phi=0.0):
+    """RheologyDensity.c:385-1080 on one (node, phase) at grain size d and
+    melt fraction phi: returns (eta_dissipative, etaVE, Pcorr, Eii_pl,
+    status)."""
     e = C.c_double(); pc = C.c_double(); epl = C.c_double(); st = C.c_int(0)
-    eta = lib().orc_viscosity_concise(C.byref(ph), C.byref(md), G, T, P, Exx, Ezz, Exz,
+    eta = lib().orc_viscosity_concise(C.byref(ph), C.byref(md), G, T, P, d, phi, Exx, Ezz, Exz,
                                       ph.fric if fric is None else fric,
                                       ph.C if Coh is None else Coh,
                                       ph.dil if dil is None else dil, pref_mult,
@@ -508,7 +517,8 @@ def viscosity_concise(ph, md, G, T, P, Exx, Ezz, Exz, fric=None, Coh=None, dil=N
 
 
 def rheology_nodes(md, phases, perc, G, T, P, exx, ezz, exz, sxx0=None, szz0=None, sxz0=None,
-                   fric=None, Coh=None, dil=None, pref=None, tag=None, centroid=1, average=0):
+                   fric=None, Coh=None, dil=None, pref=None, tag=None, centroid=1, average=0, d=None,
+                   phi=None):
     """NonNewtonianViscosityGrid (RheologyDensity.c:1131-1426) on one node set.
     perc has shape (nph, n).  Returns dict of eta, eta_phys, sxx, szz, sxz, p_corr,
     eII_pl and the status (-1 when a local iteration failed)."""
@@ -529,8 +539,8 @@ def rheology_nodes(md, phases, perc, G, T, P, exx, ezz, exz, sxx0=None, szz0=Non
     st = lib().orc_rheology_nodes(C.byref(md), ph, nph, n, perc.ctypes.data, p(G), p(T), p(P),
                                   p(exx), p(ezz), p(exz), p(z if sxx0 is None else sxx0),
                                   p(z if szz0 is None else szz0), p(z if sxz0 is None else sxz0),
-                                  p(fric), p(Coh), p(dil), p(pref), p(tag, np.int8), centroid,
-                                  average, *[outs[k].ctypes.data for k in
+                                  p(fric), p(Coh), p(dil), p(pref), p(d), p(phi), p(tag, np.int8),
+                                  centroid, average, *[outs[k].ctypes.data for k in
                                              ("eta", "eta_phys", "sxx", "szz", "sxz", "p_corr",
                                               "eII_pl")])
     outs["status"] = st
@@ -554,7 +564,8 @@ def material_grid(phases, perc, strain_pl=None, tag=None, centroid=1, style=0):
 
 
 def tangent_nodes(md, phases, perc, G, T, P, exx, ezz, exz, sxx0=None, szz0=None, sxz0=None,
-                  fric=None, Coh=None, dil=None, pref=None, tag=None, centroid=1, average=0):
+                  fric=None, Coh=None, dil=None, pref=None, tag=None, centroid=1, average=0, d=None,
+                  phi=None):
     """Newton tangent coefficients (StokesRoutines.c:250-375 with
     FD_Jacobian.c:48-323): array (8, n) on centres (D11..D14, D21..D24) or
     (4, n) on vertices (D31..D34)."""
@@ -574,8 +585,8 @@ def tangent_nodes(md, phases, perc, G, T, P, exx, ezz, exz, sxx0=None, szz0=None
     st = lib().orc_tangent_nodes(C.byref(md), ph, nph, n, perc.ctypes.data, p(G), p(T), p(P),
                                  p(exx), p(ezz), p(exz), p(z if sxx0 is None else sxx0),
                                  p(z if szz0 is None else szz0), p(z if sxz0 is None else sxz0),
-                                 p(fric), p(Coh), p(dil), p(pref), p(tag, np.int8), centroid,
-                                 average, D.ctypes.data)
+                                 p(fric), p(Coh), p(dil), p(pref), p(d), p(phi), p(tag, np.int8),
+                                 centroid, average, D.ctypes.data)
     if st:
         raise RuntimeError("local iterations failed in the tangent")
     return D

@@ -220,37 +220,49 @@ int orc_energy_step(const orc_grid *g, double dt, const double *rho_n, const dou
                     const double *Wdiss, const double *Ha, const double *kx, const double *kz, const signed char *BCT_t,
                     const double *BCT_v, int max_its, double rel_tol, double *T);
 
-/* Material parameters of one phase (mat_prop of mdoodz.h, the subset used
- * by ViscosityConcise below; units are the model's scaled units). */
+/* Material parameters of one phase (mat_prop of mdoodz.h:158-170, the
+ * members ViscosityConcise reads for the laws below; units are the model's
+ * scaled units). */
 typedef struct {
     int    cstv, pwlv, plast;
     double eta0;                                     /* constant viscosity   */
     double pref_pwl, F_pwl, A_pwl, n_pwl, Q_pwl, V_pwl; /* power-law creep   */
     double C, fric, dil, Slim, eta_vp, n_vp;         /* Drucker-Prager       */
     double beta_for_K;                               /* compressibility beta */
+    int    linv, gbsv, expv, reserved;
+    double m_pwl, r_pwl, f_pwl, a_pwl;               /* power-law d, f, phi  */
+    double F_lin, A_lin, n_lin, m_lin, r_lin, f_lin, a_lin, Q_lin, V_lin; /* diffusion */
+    double F_gbs, A_gbs, n_gbs, m_gbs, r_gbs, f_gbs, a_gbs, Q_gbs, V_gbs; /* GBS */
+    double G_exp, S_exp, Q_exp, E_exp, n_exp;        /* Peierls (Gexp ...)   */
+    double gs_ref;                                   /* reference grain size */
 } orc_phase;
 
 typedef struct {
     int    step, elastic;
     double dt, R, zeroC, min_eta, max_eta, eta_tol, Tiimin, scale_E, scale_eta;
+    double T_peierls, d_min;     /* TmaxPeierls, 1e-13 m scaled */
 } orc_rheo_model;
 
-/* RheologyDensity.c:385-1080 (subset: constant, power law, elasticity,
- * Drucker-Prager plast = 1).  pref_mult multiplies the power-law prefactor
- * (1 in the reference). */
+/* RheologyDensity.c:385-1080 (constant, power-law, diffusion, GBS and
+ * Peierls creep, elasticity, Drucker-Prager plast = 1) at grain size d and
+ * melt fraction phi.  pref_mult multiplies the power-law prefactor (1 in
+ * the reference).  *status gets -1 where the reference exits on failed
+ * local iterations, -2 where it exits on a grain size below d_min. */
 double orc_viscosity_concise(const orc_phase *m, const orc_rheo_model *md, double G, double T,
-                             double P, double Exx, double Ezz, double Exz, double fric,
-                             double C, double dil, double pref_mult, double *etaVE,
+                             double P, double d, double phi, double Exx, double Ezz, double Exz,
+                             double fric, double C, double dil, double pref_mult, double *etaVE,
                              double *Pcorr, double *Eii_pl, int *status);
 
 /* RheologyDensity.c:1157-1316 / 1320-1407 on one node set.  Nullable
- * arrays: fric, C, dil (phase values), pref (1), outputs except eta_out. */
+ * arrays: fric, C, dil (phase values), pref (1), d, phi (0), outputs
+ * except eta_out. */
 int orc_rheology_nodes(const orc_rheo_model *md, const orc_phase *phases, int nph, int n,
                        const double *perc, const double *G, const double *T, const double *P,
                        const double *exx, const double *ezz, const double *exz,
                        const double *sxx0, const double *szz0, const double *sxz0,
                        const double *fric, const double *C, const double *dil,
-                       const double *pref, const signed char *tag, int centroid, int average,
+                       const double *pref, const double *d, const double *phi,
+                       const signed char *tag, int centroid, int average,
                        double *eta_out, double *eta_phys, double *sxx, double *szz, double *sxz,
                        double *p_corr, double *eII_pl);
 
@@ -261,8 +273,8 @@ int orc_tangent_nodes(const orc_rheo_model *md, const orc_phase *phases, int nph
                       const double *exx, const double *ezz, const double *exz,
                       const double *sxx0, const double *szz0, const double *sxz0,
                       const double *fric, const double *C, const double *dil,
-                      const double *pref, const signed char *tag, int centroid, int average,
-                      double *D);
+                      const double *pref, const double *d, const double *phi,
+                      const signed char *tag, int centroid, int average, double *D);
 
 /* Per-phase elastic moduli and softening parameters (mat_prop: G, bet, C,
  * C_end, phi, phi_end, psi, psi_end, pls_start, pls_end, H_c, *_soft). */

@@ -59,14 +59,33 @@ def picard_diag(g, num, eta_n, eta_s):
 
 
 def line_search(g, num, st, dx, md, phases, average, d, rx, rz, res_f, min_step=0.0,
-                let_res_grow=0):
+                let_res_grow=0, pc=False):
     """LineSearchDecoupled (StokesRoutines.c:582-807) for Picard steps: six
     trial steps alpha in [-2.5, 0] (evenly spaced from the top), each
     followed by UpdateNonLinearity and the residual, then the acceptance
-    cascade on the combined, xz, x and z norms.  Returns (alpha, stagnated);
-    u, v, p are left unchanged."""
+    cascade on the combined, xz, x and z norms.  With pc (pc_line_search,
+    :636-668) the longest step is tried first and taken without the scan if
+    it lowers the combined norm.  Returns (alpha, stagnated); u, v, p are
+    left unchanged."""
     ntry, minalpha, maxalpha = 6, -2.5, -0.0
     saved = [st[k].copy() for k in ("u", "v", "p")]
+
+    def trial(alpha):
+        st["u"][:], st["v"][:], st["p"][:] = saved
+        scatter_solution(num, dx, st["u"], st["v"], st["p"], alpha)
+        apply_bc(g, st["u"], st["v"])
+        sr, cen, ver = update_nonlinearity(g, st, md, phases, average)
+        _, norms = residual(g, num, cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"],
+                            sr["div"], rx, rz, d)
+        return norms.copy()
+
+    frac = 1.0
+    fxzp0 = np.sqrt((np.asarray(res_f) ** 2).sum())
+    if pc:
+        r_pc = trial(minalpha)
+        st["u"][:], st["v"][:], st["p"][:] = saved
+        if np.sqrt((r_pc ** 2).sum()) < frac * fxzp0:
+            return minalpha, False
     dalpha = abs(maxalpha - minalpha) / (ntry - 1)
     alphav, R = [], []
     alpha = maxalpha
@@ -74,21 +93,14 @@ def line_search(g, num, st, dx, md, phases, average, d, rx, rz, res_f, min_step=
         if kk > 0:
             alpha -= dalpha
         alphav.append(alpha)
-        st["u"][:], st["v"][:], st["p"][:] = saved
-        scatter_solution(num, dx, st["u"], st["v"], st["p"], alpha)
-        apply_bc(g, st["u"], st["v"])
-        sr, cen, ver = update_nonlinearity(g, st, md, phases, average)
-        _, norms = residual(g, num, cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"],
-                            sr["div"], rx, rz, d)
-        R.append(norms.copy())
+        R.append(trial(alpha))
     st["u"][:], st["v"][:], st["p"][:] = saved
     R = np.array(R)
     fxzp = np.sqrt((R ** 2).sum(1))
     fxz = np.sqrt(R[:, 0] ** 2 + R[:, 1] ** 2)
     # first index of the minimum, as the strict "<" scan of the reference
     ixzp, ixz, ix, iz = (int(np.argmin(a)) for a in (fxzp, fxz, R[:, 0], R[:, 1]))
-    frac, success = 1.0, False
-    fxzp0 = np.sqrt((np.asarray(res_f) ** 2).sum())
+    success = False
     if fxzp[ixzp] < frac * fxzp0:
         alpha, success = alphav[ixzp], True
     if not success and R[ixz, 0] < frac * res_f[0] and R[ixz, 1] < frac * res_f[1]:
@@ -107,17 +119,30 @@ def line_search(g, num, st, dx, md, phases, average, d, rx, rz, res_f, min_step=
 
 
 def picard_loop(g, st, md, phases, nit_max, tol, average=0, diag_scaling=1, gx=0.0, gz=0.0,
-                line_search_on=False, **killer_kw):
+                line_search_on=False, line_search_min=0.0, let_res_grow=0, pc_line_search=0,
+                safe_mode=0, safe_dt_div=5.0, max_num_stag=3, _line_search=None, **killer_kw):
     """Runs the loop in place on st (u, v, p updated).  tol = (abs_mom,
     rel_mom, abs_div, rel_div).  Returns the list of residual norm triples
     (one per evaluated iteration) and the number of linear solves; with
-    line_search_on the chosen step lengths are in st["alphas"]."""
+    line_search_on the chosen step lengths are in st["alphas"].
+
+    A stagnated line search ends the loop, unless safe_mode = 1 with
+    elasticity (Main_DOODZ.c:1021-1052): then md.dt is divided by
+    safe_dt_div, u, v, p return to their values at the call and the
+    iterations restart from 0; the max_num_stag-th stagnation raises (the
+    reference exits).  st["safe_restarts"] lists the iteration numbers that
+    stagnated.  _line_search replaces line_search (tests of the control
+    flow)."""
     num = g.numbering()
     nm = num[3]
     rx, rz = eval_rhs(g, gx, gz, st["rho_n"], st["rho_s"])
     abs_mom, rel_mom, abs_div, rel_div = tol
     hist, nsolve, res0 = [], 0, None
-    for nit in range(nit_max + 1):
+    start = [st[k].copy() for k in ("u", "v", "p")]
+    ls = _line_search or line_search
+    st["safe_restarts"] = []
+    nstag, nit = 0, 0
+    while nit <= nit_max:
         sr, cen, ver = update_nonlinearity(g, st, md, phases, average)
         d = picard_diag(g, num, cen["eta"], ver["eta"]) if diag_scaling else None
         F, norms = residual(g, num, cen["sxx"], cen["szz"], ver["sxz"], cen["p_corr"],
@@ -140,13 +165,24 @@ def picard_loop(g, st, md, phases, nit_max, tol, average=0, diag_scaling=1, gx=0
         dx = np.concatenate([du, dp])
         alpha, stagnated = -1.0, False
         if line_search_on:
-            alpha, stagnated = line_search(g, num, st, dx, md, phases, average, d, rx, rz,
-                                           norms)
+            alpha, stagnated = ls(g, num, st, dx, md, phases, average, d, rx, rz, norms,
+                                  min_step=line_search_min, let_res_grow=let_res_grow,
+                                  pc=bool(pc_line_search))
             st.setdefault("alphas", []).append(alpha)
         scatter_solution(num, dx, st["u"], st["v"], st["p"], alpha)
         apply_bc(g, st["u"], st["v"])
         nsolve += 1
-        if stagnated:
+        if stagnated and safe_mode <= 0:
             break
+        if stagnated and md.elastic == 1 and safe_mode == 1:
+            st["safe_restarts"].append(nit)
+            md.dt /= safe_dt_div
+            st["u"][:], st["v"][:], st["p"][:] = start
+            nstag += 1
+            if nstag == max_num_stag:
+                raise RuntimeError("non-linear iterations stagnated %d times in safe mode" % nstag)
+            res0, nit = None, 0
+            continue
+        nit += 1
     st["eta_n"], st["eta_s"] = cen["eta"], ver["eta"]
     return hist, nsolve

@@ -1,14 +1,15 @@
 /*
- * Oracle: per-node visco-elasto-plastic rheology (subset used by the
- * measured workloads).  TEST INFRASTRUCTURE ONLY.
+ * Oracle: per-node visco-elasto-plastic rheology.  TEST INFRASTRUCTURE ONLY.
  *
  * ViscosityConcise (RheologyDensity.c:385-1080) for constant viscosity,
- * dislocation (power-law) creep, Maxwell elasticity and Drucker-Prager
- * plasticity (plast = 1, viscoplastic regularisation eta_vp), with
- * LocalIterationViscoElastic (RheologyDensity.c:152-197), and
- * NonNewtonianViscosityGrid (RheologyDensity.c:1131-1426) for the
- * phase-weighted grid update.  Grain-size, Peierls, diffusion, GBS, Popov
- * (plast = 2), kinetics and melting branches are not part of this subset.
+ * dislocation (power-law) creep with its grain-size, fugacity and melt
+ * factors, diffusion (linear) creep, grain-boundary sliding, regularised
+ * Peierls creep, Maxwell elasticity and Drucker-Prager plasticity (plast = 1,
+ * viscoplastic regularisation eta_vp), with LocalIterationViscoElastic
+ * (RheologyDensity.c:152-197), and NonNewtonianViscosityGrid
+ * (RheologyDensity.c:1131-1426) for the phase-weighted grid update.
+ * Grain-size evolution (gs), Popov plasticity (plast = 2), kinetics,
+ * chemical mixing and melting branches are not part of this restatement.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -18,26 +19,41 @@
 #define MAXV(a, b) ((a) > (b) ? (a) : (b))
 
 /* RheologyDensity.c:152-197: Newton iterations on eta_ve so that
- * E = elastic*T/(2 eta_el) + T/(2 eta_cst) + C_pwl T^n with T = 2 eta_ve E. */
-static int local_ve(double Eii, double *eta_ve, int elastic, double eta_el, int constant,
-                    double eta_cst, int dislocation, double C_pwl, double n_pwl, double *Eii_pwl_out) {
+ * E = elastic*T/(2 eta_el) + Eii_pwl + Eii_exp + Eii_lin + Eii_gbs + Eii_cst
+ * with T = 2 eta_ve E.  The reference's GBS term is C_gbs T^n_gbs with
+ * C_gbs left at zero by ViscosityConcise's initialiser (:613-642), so it is
+ * zero here. */
+typedef struct {
+    int    elastic, constant, dislocation, peierls, diffusion;
+    double eta_el, eta_cst, C_pwl, n_pwl, C_exp, ST, n_exp, C_lin, n_lin, m_lin, d;
+} ve_laws;
+
+static int local_ve(const ve_laws *L, double Eii, double *eta_ve, double *Eii_pwl_out,
+                    double *Eii_exp_out, double *Eii_lin_out) {
     const int nitmax = 20;
     const double tol = 1.0e-11;
     double eta = *eta_ve;
     for (int it = 0; it < nitmax; it++) {
         const double Tii = 2.0 * eta * Eii;
-        double Eii_cst = 0.0, Eii_pwl = 0.0;
-        if (constant) Eii_cst = Tii / 2.0 / eta_cst;
-        if (dislocation) Eii_pwl = C_pwl * pow(Tii, n_pwl);
-        const double r = Eii - elastic * Tii / (2.0 * eta_el) - (Eii_pwl + Eii_cst);
+        double Eii_cst = 0.0, Eii_pwl = 0.0, Eii_exp = 0.0, Eii_lin = 0.0, Eii_gbs = 0.0;
+        if (L->constant) Eii_cst = Tii / 2.0 / L->eta_cst;
+        if (L->dislocation) Eii_pwl = L->C_pwl * pow(Tii, L->n_pwl);
+        if (L->peierls) Eii_exp = L->C_exp * pow(Tii, L->ST + L->n_exp);
+        if (L->diffusion) Eii_lin = L->C_lin * pow(Tii, L->n_lin) * pow(L->d, -L->m_lin);
+        const double Eii_vis = Eii_pwl + Eii_exp + Eii_lin + Eii_gbs + Eii_cst;
+        const double r = Eii - L->elastic * Tii / (2.0 * L->eta_el) - Eii_vis;
         const double res = fabs(r / Eii);
         *Eii_pwl_out = Eii_pwl;
+        *Eii_exp_out = Eii_exp;
+        *Eii_lin_out = Eii_lin;
         if (res < tol / 100) break;
         if (it == nitmax - 1 && res > tol) return -1;   /* reference exits */
         double dfdeta = 0.0;
-        if (elastic) dfdeta += -Eii / eta_el;
-        if (dislocation) dfdeta += -Eii_pwl * n_pwl / eta;
-        if (constant) dfdeta += -Eii / eta_cst;
+        if (L->elastic) dfdeta += -Eii / L->eta_el;
+        if (L->peierls) dfdeta += -Eii_exp * (L->ST + L->n_exp) / eta;
+        if (L->diffusion) dfdeta += -Eii_lin * L->n_lin / eta;
+        if (L->dislocation) dfdeta += -Eii_pwl * L->n_pwl / eta;
+        if (L->constant) dfdeta += -Eii / L->eta_cst;
         eta -= r / dfdeta;
     }
     *eta_ve = eta;
@@ -47,13 +63,15 @@ static int local_ve(double Eii, double *eta_ve, int elastic, double eta_el, int 
 /* One phase at one node: returns the effective (VE or VEP) viscosity in
  * *etaVE, the dissipative viscosity as return value, and Pcorr. */
 double orc_viscosity_concise(const orc_phase *m, const orc_rheo_model *md, double G, double T,
-                             double P, double Exx, double Ezz, double Exz, double fric,
-                             double C, double dil, double pref_mult, double *etaVE,
+                             double P, double d, double phi, double Exx, double Ezz, double Exz,
+                             double fric, double C, double dil, double pref_mult, double *etaVE,
                              double *Pcorr, double *Eii_pl, int *status) {
     double dt = (md->step == 0) ? 1e100 : md->dt;
     int elastic = md->elastic, plastic = (m->plast == 1);
-    if (md->step == 0) { elastic = 0; plastic = 0; }
     const int constant = m->cstv != 0, dislocation = m->pwlv != 0;
+    const int peierls = m->expv != 0 && T < md->T_peierls;   /* :474, before the T floor */
+    const int diffusion = m->linv != 0, gbs = m->gbsv != 0;
+    if (md->step == 0) { elastic = 0; plastic = 0; }
     double K = 1.0 / m->beta_for_K;
     const double Eyy = -(Exx + Ezz);
     double Eii = sqrt(0.5 * (Exx * Exx + Ezz * Ezz + Eyy * Eyy) + Exz * Exz);
@@ -61,11 +79,31 @@ double orc_viscosity_concise(const orc_phase *m, const orc_rheo_model *md, doubl
     *Pcorr = P;
     if (!elastic) { G = 1e1; dil = 0.0; K = 1e10; }
     if (T < md->zeroC) T = md->zeroC;
-    double B_pwl = 0.0, C_pwl = 0.0;
+    /* precomputations, :501-523 */
+    double B_pwl = 0.0, C_pwl = 0.0, B_lin = 0.0, C_lin = 0.0, B_gbs = 0.0;
+    double ST = 0.0, B_exp = 0.0, C_exp = 0.0;
     if (dislocation) {
         B_pwl = m->pref_pwl * pref_mult * m->F_pwl * pow(m->A_pwl, -1.0 / m->n_pwl) *
-                exp((m->Q_pwl + P * m->V_pwl) / md->R / m->n_pwl / T);
+                exp((m->Q_pwl + P * m->V_pwl) / md->R / m->n_pwl / T) * pow(d, m->m_pwl / m->n_pwl) *
+                pow(m->f_pwl, -m->r_pwl / m->n_pwl) * exp(-m->a_pwl * phi / m->n_pwl);
         C_pwl = pow(2.0 * B_pwl, -m->n_pwl);
+    }
+    if (diffusion) {
+        if (m->m_lin > 0.0 && d < md->d_min) *status = -2;   /* reference exits */
+        B_lin = m->F_lin * pow(m->A_lin, -1.0 / m->n_lin) * exp((m->Q_lin + P * m->V_lin) / md->R / m->n_lin / T) *
+                pow(m->f_lin, -m->r_lin / m->n_lin) * exp(-m->a_lin * phi / m->n_lin);
+        C_lin = pow(2.0 * B_lin, -m->n_lin);
+    }
+    if (gbs)
+        B_gbs = m->F_gbs * pow(m->A_gbs, -1.0 / m->n_gbs) * exp((m->Q_gbs + P * m->V_gbs) / md->R / m->n_gbs / T) *
+                pow(d, m->m_gbs / m->n_gbs) * pow(m->f_gbs, -m->r_gbs / m->n_gbs) * exp(-m->a_gbs * phi / m->n_gbs);
+    if (peierls) {
+        const double gamma = m->G_exp, n_exp = m->n_exp;
+        ST = m->Q_exp / md->R / T * 2.0 * gamma * (1 - gamma);
+        const double Arr_exp = exp(-m->Q_exp / md->R / T * pow(1.0 - gamma, 2.0));
+        const double F_exp = pow(pow(2.0, 1.0 - ST - n_exp) / pow(sqrt(3.0), ST + n_exp + 1.0), 1.0 / (ST + n_exp));
+        B_exp = F_exp * (pow(gamma * m->S_exp, ST / (ST + n_exp)) / pow(m->E_exp * Arr_exp, 1.0 / (ST + n_exp)));
+        C_exp = pow(2.0 * B_exp, -(ST + n_exp));
     }
     const double cos_fric = cos(fric), sin_fric = sin(fric), sin_dil = sin(dil);
     double Tyield = 0.0;
@@ -73,17 +111,26 @@ double orc_viscosity_concise(const orc_phase *m, const orc_rheo_model *md, doubl
         Tyield = C * cos_fric + P * sin_fric;
         if (m->Slim < Tyield) { C = m->Slim; Tyield = m->Slim; }
     }
+    /* isolated viscosities, :559-580 */
     const double eta_el = G * dt;
-    double eta_cst = 0.0, eta_pwl = 0.0;
+    double eta_cst = 0.0, eta_pwl = 0.0, eta_lin = 0.0, eta_gbs = 0.0, eta_exp = 0.0;
     if (constant) eta_cst = m->eta0;
     if (dislocation) eta_pwl = B_pwl * pow(Eii, 1.0 / m->n_pwl - 1.0);
+    if (diffusion) eta_lin = B_lin * pow(Eii, 1.0 / m->n_lin - 1.0) * pow(d, m->m_lin / m->n_lin);
+    if (gbs) eta_gbs = B_gbs * pow(Eii, 1.0 / m->n_gbs - 1.0);
+    if (peierls) eta_exp = B_exp * pow(Eii, 1.0 / (ST + m->n_exp) - 1.0);
+    /* upper bound of the visco-elastic viscosity, :585-592 */
     double eta_up = 1.0e100 / md->scale_eta;
     if (constant) eta_up = MINV(eta_up, eta_cst);
     if (dislocation) eta_up = MINV(eta_up, eta_pwl);
     if (elastic) eta_up = MINV(eta_up, eta_el);
-    double eta_ve = eta_up, Eii_pwl = 0.0;
-    if (local_ve(Eii, &eta_ve, elastic, eta_el, constant, eta_cst, dislocation, C_pwl, m->n_pwl,
-                 &Eii_pwl))
+    if (peierls) eta_up = MINV(eta_up, eta_exp);
+    if (diffusion) eta_up = MINV(eta_up, eta_lin);
+    if (gbs) eta_up = MINV(eta_up, eta_gbs);
+    double eta_ve = eta_up, Eii_pwl = 0.0, Eii_exp = 0.0, Eii_lin = 0.0;
+    const ve_laws L = {elastic, constant, dislocation, peierls, diffusion, eta_el, eta_cst, C_pwl, m->n_pwl,
+                       C_exp, ST, m->n_exp, C_lin, m->n_lin, m->m_lin, d};
+    if (local_ve(&L, Eii, &eta_ve, &Eii_pwl, &Eii_exp, &Eii_lin) && *status == 0)
         *status = -1;
     double Tii = 2.0 * eta_ve * Eii;
     int is_pl = 0;
@@ -122,9 +169,17 @@ double orc_viscosity_concise(const orc_phase *m, const orc_rheo_model *md, doubl
     }
     *etaVE = is_pl ? eta_vep : eta_ve;
     *Eii_pl = is_pl ? gdot / 2.0 : 0.0;
-    /* dissipative viscosity (post-processing branch, RheologyDensity.c:1011-1017) */
+    /* dissipative viscosity (post-processing branch, RheologyDensity.c:987-1032):
+     * each law at its own strain-rate share; diffusion creep at the grain
+     * size d1 = gs_ref the reference sets for phases without grain-size
+     * evolution (:486) */
+    if (dislocation) eta_pwl = B_pwl * pow(Eii_pwl, 1.0 / m->n_pwl - 1.0);
+    if (diffusion) eta_lin = B_lin * pow(Eii_lin, 1.0 / m->n_lin - 1.0) * pow(m->gs_ref, m->m_lin / m->n_lin);
+    if (peierls) eta_exp = B_exp * pow(Eii_exp, 1.0 / (ST + m->n_exp) - 1.0);
     double inv = 0.0;
-    if (dislocation) inv += 1.0 / (B_pwl * pow(Eii_pwl, 1.0 / m->n_pwl - 1.0));
+    if (peierls) inv += 1.0 / eta_exp;
+    if (dislocation) inv += 1.0 / eta_pwl;
+    if (diffusion) inv += 1.0 / eta_lin;
     if (constant) inv += 1.0 / eta_cst;
     if (is_pl) inv += 1.0 / (Tii / gdot);
     double eta = 1.0 / inv;
@@ -143,7 +198,8 @@ int orc_rheology_nodes(const orc_rheo_model *md, const orc_phase *phases, int np
                        const double *exx, const double *ezz, const double *exz,
                        const double *sxx0, const double *szz0, const double *sxz0,
                        const double *fric, const double *C, const double *dil,
-                       const double *pref, const signed char *tag, int centroid, int average,
+                       const double *pref, const double *d, const double *phi,
+                       const signed char *tag, int centroid, int average,
                        double *eta_out, double *eta_phys, double *sxx, double *szz, double *sxz,
                        double *p_corr, double *eII_pl) {
     const double el = md->elastic ? 1.0 : 0.0;
@@ -169,7 +225,8 @@ int orc_rheology_nodes(const orc_rheo_model *md, const orc_phase *phases, int np
             const double f = perc[(size_t)p * n + i];
             if (!(fabs(f) > 1e-13)) continue;
             double etaVE, Pcorr, Eii_pl;
-            const double eta = orc_viscosity_concise(&phases[p], md, G[i], T[i], P[i], Exx, Ezz, Exz,
+            const double eta = orc_viscosity_concise(&phases[p], md, G[i], T[i], P[i], d ? d[i] : 0.0,
+                                                     phi ? phi[i] : 0.0, Exx, Ezz, Exz,
                                                      fric ? fric[i] : phases[p].fric,
                                                      C ? C[i] : phases[p].C,
                                                      dil ? dil[i] : phases[p].dil,
@@ -282,8 +339,8 @@ int orc_tangent_nodes(const orc_rheo_model *md, const orc_phase *phases, int nph
                       const double *exx, const double *ezz, const double *exz,
                       const double *sxx0, const double *szz0, const double *sxz0,
                       const double *fric, const double *C, const double *dil,
-                      const double *pref, const signed char *tag, int centroid, int average,
-                      double *D) {
+                      const double *pref, const double *d, const double *phi,
+                      const signed char *tag, int centroid, int average, double *D) {
     const double el = md->elastic ? 1.0 : 0.0;
     const double tol = centroid ? 1e-5 : 1e-7;
     const int nrow = centroid ? 8 : 4;
@@ -311,8 +368,8 @@ int orc_tangent_nodes(const orc_rheo_model *md, const orc_phase *phases, int nph
             eta_ph[p] = 0.0;
             if (!(fabs(f) > 1e-13)) continue;
             double etaVE, Pc, Epl;
-            orc_viscosity_concise(&phases[p], md, G[i], T[i], P[i], Exx, Ezz, Exz,
-                                  fric ? fric[i] : phases[p].fric, C ? C[i] : phases[p].C,
+            orc_viscosity_concise(&phases[p], md, G[i], T[i], P[i], d ? d[i] : 0.0, phi ? phi[i] : 0.0,
+                                  Exx, Ezz, Exz, fric ? fric[i] : phases[p].fric, C ? C[i] : phases[p].C,
                                   dil ? dil[i] : phases[p].dil, pref ? pref[i] : 1.0, &etaVE,
                                   &Pc, &Epl, &status);
             eta_ph[p] = etaVE;
@@ -337,7 +394,8 @@ int orc_tangent_nodes(const orc_rheo_model *md, const orc_phase *phases, int nph
                 for (int sgn = 0; sgn < 2; sgn++) {
                     const int s = 2 * q + sgn;
                     double Pc, Epl;
-                    orc_viscosity_concise(&phases[p], md, G[i], T[i], P[i] + st[s][3], Exx + st[s][0],
+                    orc_viscosity_concise(&phases[p], md, G[i], T[i], P[i] + st[s][3], d ? d[i] : 0.0,
+                                          phi ? phi[i] : 0.0, Exx + st[s][0],
                                           Ezz + st[s][1], Exz + st[s][2],
                                           fric ? fric[i] : phases[p].fric, C ? C[i] : phases[p].C,
                                           dil ? dil[i] : phases[p].dil, pref ? pref[i] : 1.0,

@@ -38,8 +38,62 @@ def _eta_scaled(law, scl, Eii, T, P, R=8.314):
     pp = FL.phase_prop_fields(law)
     Rs, Ts, Ps, Es = R / (scl.J / scl.T), T / scl.T, P / scl.S, Eii / scl.E
     B = pp["pref_pwl"] * pp["F_pwl"] * pp["A_pwl"] ** (-1.0 / pp["n_pwl"]) * \
-        math.exp((pp["Q_pwl"] + Ps * pp["V_pwl"]) / Rs / pp["n_pwl"] / Ts)
+        math.exp((pp["Q_pwl"] + Ps * pp["V_pwl"]) / Rs / pp["n_pwl"] / Ts) * \
+        pp["f_pwl"] ** (-pp["r_pwl"] / pp["n_pwl"])
     return B * Es ** (1.0 / pp["n_pwl"] - 1.0)
+
+
+def _rate_si(law_si, tau, T, P, d, R=8.314):
+    """Laboratory form: strain-rate invariant of a tabulated power / diffusion /
+    GBS law at stress invariant tau [Pa], grain size d [m], SI units."""
+    t, n, m, r, Q, V, A, f, a = law_si
+    return (tau / (2 * FL.correction_factor(t, n))) ** n * A * d ** -m * f ** r * math.exp(-(Q + P * V) / (R * T))
+
+
+@pytest.mark.parametrize("scl", SCALES)
+@pytest.mark.parametrize("kind,number", [("lin", k) for k in sorted(FL.LINEAR_LAWS)] +
+                         [("gbs", k) for k in sorted(FL.GBS_LAWS)])
+def test_scaled_grain_size_laws(kind, number, scl):
+    """Diffusion creep and GBS (ReadDataLinear :591-706, ReadDataGBS
+    :711-776): the scaled fields, used as ViscosityConcise uses them
+    (RheologyDensity.c:505-515), give the SI strain rate of the published
+    law at a stress, temperature, pressure and grain size, for any scales."""
+    table = FL.LINEAR_LAWS if kind == "lin" else FL.GBS_LAWS
+    law = (FL.linear_law if kind == "lin" else FL.gbs_law)(number, scl)
+    pp = (FL.linear_prop_fields if kind == "lin" else FL.gbs_prop_fields)(law)
+    g = lambda k: pp[k + "_" + kind]
+    tau, T, P, d, R = 3e7, 1300.0, 2e9, 2e-3, 8.314
+    Rs, Ts, Ps, ds, taus = R / (scl.J / scl.T), T / scl.T, P / scl.S, d / scl.L, tau / scl.S
+    B = g("F") * g("A") ** (-1.0 / g("n")) * math.exp((g("Q") + Ps * g("V")) / Rs / g("n") / Ts) * \
+        g("f") ** (-g("r") / g("n")) * ds ** (g("m") / g("n"))
+    rate_scaled = (taus / (2.0 * B)) ** g("n")
+    assert rate_scaled * scl.E == pytest.approx(_rate_si(table[number][1:], tau, T, P, d), rel=1e-11)
+    assert (FL.linear_law if kind == "lin" else FL.gbs_law)(-number, scl)["F"] == 1.0
+    with pytest.raises(ValueError):
+        (FL.linear_law if kind == "lin" else FL.gbs_law)(7, scl)
+
+
+@pytest.mark.parametrize("scl", SCALES)
+@pytest.mark.parametrize("number", sorted(FL.EXP_LAWS))
+def test_scaled_peierls_law(number, scl):
+    """ReadDataExponential (:782-842): at sigma = G S the regularised law
+    of ViscosityConcise (RheologyDensity.c:516-523), evaluated with the
+    scaled fields, gives the SI rate of the exponential law in an axial test
+    (sigma = sqrt(3) tau_II, E_II = sqrt(3)/2 rate)."""
+    law = FL.exponential_law(number, scl)
+    _, _, G, S, Q, V, E, n = FL.EXP_LAWS[number][1:]
+    T, R = 900.0, 8.314
+    sig = G * S
+    rate_si = math.sqrt(3.0) / 2.0 * E * math.exp(-Q / (R * T) * (1.0 - sig / S) ** 2) * sig ** n
+    Rs, Ts = R / (scl.J / scl.T), T / scl.T
+    ST = law["Q"] / Rs / Ts * 2.0 * law["G"] * (1 - law["G"])
+    Arr = math.exp(-law["Q"] / Rs / Ts * (1.0 - law["G"]) ** 2)
+    F = (2.0 ** (1.0 - ST - law["n"]) / math.sqrt(3.0) ** (ST + law["n"] + 1.0)) ** (1.0 / (ST + law["n"]))
+    B = F * (law["G"] * law["S"]) ** (ST / (ST + law["n"])) / (law["E"] * Arr) ** (1.0 / (ST + law["n"]))
+    tau_s = sig / math.sqrt(3.0) / scl.S
+    rate_scaled = (tau_s / (2.0 * B)) ** (ST + law["n"])
+    assert rate_scaled * scl.E == pytest.approx(rate_si, rel=1e-9)
+    assert FL.exponential_law(-number, scl)["t"] == 0.0
 
 
 @pytest.mark.parametrize("scl", SCALES)
@@ -68,8 +122,8 @@ def test_user_defined_laws_and_errors():
     for bad in (0, 4, 9, 35, 47):
         with pytest.raises(ValueError):
             FL.power_law(bad, scl)
-    with pytest.raises(ValueError):
-        FL.phase_prop_fields(dict(FL.power_law(40, scl), m=1.0))
+    # grain-size dependent power laws pass m through to the kernels
+    assert FL.phase_prop_fields(dict(FL.power_law(40, scl), m=1.0))["m_pwl"] == 1.0
 
 
 def test_scale_relations():

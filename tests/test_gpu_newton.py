@@ -64,6 +64,37 @@ def _tangents(O, g, st, o_ph, g_ph, common, ctx, average=0):
     return Dn_o, Ds_o, Dn_g, Ds_g
 
 
+@pytest.mark.parametrize("centroid", [1, 0])
+def test_tangent_all_creep_laws(oracle_mod, centroid):
+    """Tangent coefficients of phases with every creep law (power law with
+    grain-size and melt factors, diffusion, GBS, Peierls) on random node
+    fields against the oracle.  Tolerance: the local iterations stop at a
+    1e-13 relative residual, and a derivative by a 1e-5 (centres) or 1e-7
+    (vertices) relative perturbation turns that into up to 1e-8 or 1e-6 of
+    eta, raised by the stress exponent of the stiffest law (ST + n ~ 10 for
+    the Peierls phases here): 1e-6 of each row's largest value on centres,
+    2e-5 on vertices."""
+    from test_gpu_rheology import LAWS, _ctx, _fields, _phase_pair
+    O = oracle_mod
+    g, ctx = _ctx(O)
+    n = ctx.node_count(centroid)
+    pairs = [_phase_pair(O, d) for d in LAWS]
+    f, rng = _fields(n, len(LAWS), seed=77 + centroid)
+    f["d"] = rng.uniform(0.3, 1.5, n)
+    f["phi"] = rng.uniform(0.0, 0.05, n)
+    common = dict(step=3, elastic=1, dt=0.5, R=1.0, zeroC=0.6, min_eta=1e-3, max_eta=1e3,
+                  eta_tol=1e-12, Tiimin=1e-3, scale_E=1.0, scale_eta=1.0, T_peierls=1.0, d_min=1e-3)
+    md = O.OrcRheoModel(**common)
+    D_o = O.tangent_nodes(md, [p[0] for p in pairs], *[f[k] for k in ("perc", "G", "T", "P", "exx", "ezz", "exz",
+                                                                      "sxx0", "szz0", "sxz0")],
+                          d=f["d"], phi=f["phi"], tag=(g.BCp_t if centroid else g.BCg_t), centroid=centroid)
+    D_g = ctx.tangent_grid([p[1] for p in pairs], mdgpu.RheoParams(average=0, **common), centroid, **f)
+    for r in range(D_o.shape[0]):
+        scale = np.abs(D_o[r]).max() + 1e-300
+        np.testing.assert_allclose(D_g[r], D_o[r], rtol=0, atol=(1e-6 if centroid else 2e-5) * scale,
+                                   err_msg="row %d" % r)
+
+
 @pytest.mark.parametrize("average", [0, 1, 2])
 @pytest.mark.parametrize("C", [1.0, 50.0])
 def test_tangent_matches_oracle(oracle_mod, average, C):

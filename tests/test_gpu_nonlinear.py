@@ -79,6 +79,78 @@ def test_picard_loop_matches_oracle(oracle_mod, diag_scaling, line_search, C):
     assert rel(st_gpu["eta_n"], st["eta_n"]) < 1e-8
 
 
+def test_predictor_line_search_matches_oracle(oracle_mod):
+    """pc_line_search (StokesRoutines.c:636-668): the longest step is tried
+    first; the GPU loop takes the same steps as the oracle's, including the
+    iterations where the predictor wins over the scan's choice."""
+    O = oracle_mod
+    from oracle.nonlinear import picard_loop
+    n = 31
+    g, st = _problem(O, n, seed=4)
+    st_gpu = {k: v.copy() for k, v in st.items()}
+    phs = S.vep_phases(C=2.0)
+    common = dict(step=1, elastic=1, dt=1.0, R=1.0, zeroC=0.0, min_eta=1e-4, max_eta=1e4,
+                  eta_tol=1e-13, Tiimin=1e-5, scale_E=1.0, scale_eta=1.0)
+    o_ph = []
+    for d in phs:
+        d = dict(d)
+        b = d.pop("beta")
+        o_ph.append(O.OrcPhase(beta_for_K=b, **d))
+    hist, nsolve = picard_loop(g, st, O.OrcRheoModel(**common), o_ph, 10, TOL, diag_scaling=1, gz=-1.0,
+                               line_search_on=True, pc_line_search=1, penalty=1e3, max_its_PH=50,
+                               rel_tol_KSP=1e-12, max_its_KSP=200, lin_abs_div=1e-14, lin_rel_div=1e-12,
+                               lin_abs_mom=1e-14, lin_rel_mom=1e-12)
+    ctx = mdgpu.Context(n + 1, n + 1, 0.0, 1.0, -1.0, 0.0)
+    ctx.set_bc(g.BCu_t, g.BCu_v, g.BCv_t, g.BCv_v, g.BCp_t, g.BCg_t)
+    eu, ev, ep = g.numbering()[:3]
+    loop = PicardLoop(ctx, eu, ev, ep, [mdgpu.PhaseProp(**d) for d in phs], mdgpu.RheoParams(average=0, **common),
+                      device=True)
+    ghist, gsolve, _ = loop.run(st_gpu, 10, TOL, diag_scaling=1, gz=-1.0, line_search=True, pc_line_search=1,
+                                rel_tol=1e-11, max_its=400)
+    print("alphas", st["alphas"], flush=True)
+    assert -2.5 in st["alphas"]
+    assert st_gpu["alphas"] == st["alphas"] and gsolve == nsolve
+    np.testing.assert_allclose(np.array(ghist), np.array(hist), rtol=1e-6, atol=1e-15)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(st_gpu["u"], st["u"]) < 1e-8 and rel(st_gpu["p"], st["p"]) < 1e-8
+
+
+def test_safe_mode_restart_on_the_device(oracle_mod):
+    """Safe mode (Main_DOODZ.c:1021-1052) on the device-resident loop: a
+    stagnation (forced on the second line search from the host) divides dt by
+    safe_dt_div and restarts from the call's u, v, p; the run then equals a
+    fresh run at the reduced step (same steps, fields to 1e-12)."""
+    n = 31
+    g, st0 = _problem(oracle_mod, n, seed=4)
+    phs = S.vep_phases(C=2.0)
+    common = dict(step=1, elastic=1, R=1.0, zeroC=0.0, min_eta=1e-4, max_eta=1e4, eta_tol=1e-13, Tiimin=1e-5,
+                  scale_E=1.0, scale_eta=1.0)
+    ctx = mdgpu.Context(n + 1, n + 1, 0.0, 1.0, -1.0, 0.0)
+    ctx.set_bc(g.BCu_t, g.BCu_v, g.BCv_t, g.BCv_v, g.BCp_t, g.BCg_t)
+    eu, ev, ep = g.numbering()[:3]
+    runs = []
+    for dt, stagnate in ((1.0, True), (0.25, False)):
+        st = {k: v.copy() for k, v in st0.items()}
+        loop = PicardLoop(ctx, eu, ev, ep, [mdgpu.PhaseProp(**d) for d in phs],
+                          mdgpu.RheoParams(average=0, dt=dt, **common), device=True)
+        if stagnate:
+            real, calls = loop.line_search, []
+
+            def scripted(*a, **k):
+                calls.append(1)
+                return (0.0, True) if len(calls) == 2 else real(*a, **k)
+            loop.line_search = scripted
+        out = loop.run(st, 8, TOL, diag_scaling=1, gz=-1.0, line_search=True, safe_mode=1, safe_dt_div=4.0,
+                       rel_tol=1e-11, max_its=400)
+        runs.append((st, loop, out))
+    (s1, l1, (h1, n1, _)), (s2, l2, (h2, n2, _)) = runs
+    assert s1["safe_restarts"] == [1] and l1.prm.dt == 0.25 and s2["safe_restarts"] == []
+    assert n1 == n2 + 2 and s1["alphas"][2:] == s2["alphas"]
+    np.testing.assert_allclose(np.array(h1[2:]), np.array(h2), rtol=1e-10, atol=1e-15)
+    for k in ("u", "v", "p"):
+        assert np.linalg.norm(s1[k] - s2[k]) <= 1e-12 * np.linalg.norm(s2[k]), k
+
+
 @pytest.mark.parametrize("newton", [False, True])
 def test_device_resident_loop_matches_host_loop(oracle_mod, newton):
     """The loop with every field in device memory (one upload per run, the

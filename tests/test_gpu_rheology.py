@@ -1,4 +1,4 @@
-"""GPU parity of the grid rheology (NonNewtonianViscosityGrid subset), the
+"""GPU parity of the grid rheology (NonNewtonianViscosityGrid), the
 centre-to-vertex interpolation and the strain-rate node averages against the
 oracle.
 
@@ -107,6 +107,66 @@ def test_viscosity_grid_device_arrays_and_errors(oracle_mod):
     with pytest.raises(mdgpu.MdgError):
         ctx.viscosity_grid([bad], prm, 1, **{k: v for k, v in f.items() if k != "perc"},
                            perc=np.ones((1, n)))
+
+
+# every creep law: power law with grain-size, fugacity and melt factors,
+# diffusion (grain-size dependent), grain-boundary sliding and Peierls
+# creep, which acts below T_peierls = 1 (half the nodes)
+LAWS = [
+    dict(cstv=0, pwlv=1, n_pwl=3.0, A_pwl=0.5, Q_pwl=1.0, V_pwl=0.1, F_pwl=0.8, m_pwl=1.0, r_pwl=1.2,
+         f_pwl=2.0, a_pwl=20.0, linv=1, F_lin=0.7, A_lin=2.5, n_lin=1.0, m_lin=3.0, Q_lin=1.5, V_lin=0.2,
+         r_lin=1.0, f_lin=0.4, a_lin=2.0, gs_ref=0.7, expv=1, G_exp=0.2, S_exp=4.0, Q_exp=6.0, E_exp=3.0,
+         n_exp=2.0, plast=1, C=1.2, fric=0.4, dil=0.05, eta_vp=0.02, beta=5.0),
+    dict(cstv=1, eta0=2.0, gbsv=1, F_gbs=0.5, A_gbs=1e-3, n_gbs=3.5, m_gbs=2.0, Q_gbs=1.0, linv=1,
+         F_lin=1.0, A_lin=1.0, n_lin=1.0, m_lin=0.0, Q_lin=0.5, gs_ref=1.0),
+    dict(cstv=0, expv=1, G_exp=0.1, S_exp=8.0, Q_exp=10.0, E_exp=50.0, n_exp=0.0, pwlv=1, n_pwl=3.5,
+         A_pwl=1.0, plast=1, C=2.0, fric=0.5),
+]
+
+
+@pytest.mark.parametrize("centroid", [1, 0])
+@pytest.mark.parametrize("average", [0, 1, 2])
+@pytest.mark.parametrize("step", [0, 3])
+def test_viscosity_grid_all_creep_laws(oracle_mod, centroid, average, step):
+    g, ctx = _ctx(oracle_mod)
+    n = ctx.node_count(centroid)
+    pairs = [_phase_pair(oracle_mod, d) for d in LAWS]
+    f, rng = _fields(n, len(LAWS), seed=1000 + 10 * step + average + 100 * centroid)
+    f["d"] = rng.uniform(0.3, 1.5, n)
+    f["phi"] = rng.uniform(0.0, 0.05, n)
+    common = dict(step=step, elastic=1, dt=0.5, R=1.0, zeroC=0.6, min_eta=1e-3, max_eta=1e3,
+                  eta_tol=1e-12, Tiimin=1e-3, scale_E=1.0, scale_eta=1.0, T_peierls=1.0, d_min=1e-3)
+    md = oracle_mod.OrcRheoModel(**common)
+    prm = mdgpu.RheoParams(average=average, **common)
+    ref = oracle_mod.rheology_nodes(md, [p[0] for p in pairs], f["perc"], f["G"], f["T"], f["P"],
+                                    f["exx"], f["ezz"], f["exz"], f["sxx0"], f["szz0"], f["sxz0"],
+                                    d=f["d"], phi=f["phi"], tag=(g.BCp_t if centroid else g.BCg_t),
+                                    centroid=centroid, average=average)
+    assert ref["status"] == 0
+    out = ctx.viscosity_grid([p[1] for p in pairs], prm, centroid, **f)
+    if step > 0:
+        assert np.any(ref["eII_pl"] > 0) and np.any(ref["eII_pl"] == 0)
+    for k in mdgpu.RHEO_OUT:
+        np.testing.assert_allclose(out[k], ref[k], rtol=RTOL,
+                                   atol=RTOL * np.abs(ref[k]).max() + 1e-300, err_msg=k)
+
+
+def test_viscosity_grid_grain_size_floor(oracle_mod):
+    # grain-size dependent diffusion creep below d_min: the reference exits
+    # (RheologyDensity.c:506-509), the entry point returns MDG_ERR_ARG
+    g, ctx = _ctx(oracle_mod)
+    n = ctx.node_count(1)
+    f, _ = _fields(n, 1, seed=3)
+    f["perc"] = np.ones((1, n))
+    f["d"] = np.full(n, 0.5)
+    f["d"][n // 2] = 1e-6
+    prm = mdgpu.RheoParams(step=1, elastic=1, average=0, dt=0.5, R=1.0, zeroC=0.0, min_eta=1e-3,
+                           max_eta=1e3, eta_tol=1e-12, Tiimin=1e-3, scale_E=1.0, scale_eta=1.0, d_min=1e-3)
+    ph = _phase_pair(oracle_mod, LAWS[0])[1]
+    with pytest.raises(mdgpu.MdgError, match="grain size"):
+        ctx.viscosity_grid([ph], prm, 1, **f)
+    f["d"][n // 2] = 0.5
+    ctx.viscosity_grid([ph], prm, 1, **f)
 
 
 @pytest.mark.parametrize("periodic", [0, 1])

@@ -6,6 +6,7 @@ TESTS/SolViBenchmark/SolViRes101.txt), and on small texts for the lookup
 rules.  The olivine law of PowerLawCreep is checked end to end: reader,
 flow-law database and the oracle's ViscosityConcise give, back in Pa s, the
 viscosity of the published law written in SI units."""
+import math
 import os
 
 import pytest
@@ -109,3 +110,96 @@ def test_lookup_rules():
         f.mat("rho", 5, 1.0)
     assert IF.c_atof("  -3.5e-2abc") == -3.5e-2 and IF.c_atof("abc") == 0.0
     assert IF.c_atoi(" 000005") == 5 and IF.c_atoi("1e3") == 1
+
+
+# ---- the reference's creep tests (TESTS/RheologyCreep, RheologyCreepTests.cpp:104-213):
+# one phase of each file at the file's temperature (user0, degC), background
+# strain rate (pure shear) and grain size (SetGrainSize returns gs_ref,
+# RheologyCreepTests.cpp:49-51), against the published law in SI units
+
+def _creep_case(O, name, phase=0, lift_cap=True):
+    m = _model(name)
+    s = m.scaling
+    pp = m.phase_props()[phase]
+    md = m.rheo_params()
+    orc_md = O.rheo_model(**{k: getattr(md, k) for k, _ in md._fields_ if k != "average"})
+    if lift_cap:
+        orc_md.max_eta, orc_md.min_eta = 1e50, 1e-50
+    orc_ph = O.phase(**{k: getattr(pp, k) for k, _ in pp._fields_ if k != "beta"}, beta_for_K=pp.beta)
+    T_K, E = m["user0"] + IF.ZERO_C, m["bkg_strain_rate"] * s.E    # SI
+    out = O.viscosity_concise(orc_ph, orc_md, m.phases[phase].G, T_K / s.T, 0.0, E / s.E, -E / s.E, 0.0,
+                              d=m.phases[phase].gs_ref)
+    return m, out, T_K, E
+
+
+def _rate_si(law_si, tau, T, d):
+    t, n, m, r, Q, V, A, f, a = law_si
+    return (tau / (2 * FL.correction_factor(t, n))) ** n * A * d ** -m * math.exp(-Q / (IF.RG * T))
+
+
+def test_reference_creep_inputs_read():
+    m = _model("DiffusionCreepLinear.txt")
+    ph, pp = m.phases[0], m.phase_props()[0]
+    assert (ph.linv, ph.gbsv, ph.expv, pp.linv, pp.pwlv, pp.cstv) == (40, 0, 0, 1, 0, 0)
+    assert ph.gs_ref * m.scaling.L == pytest.approx(5e-3)
+    assert pp.n_lin == 1.0 and pp.m_lin == 3.0 and pp.V_lin == pytest.approx(4e-6 / m.scaling.L ** 3)
+    assert pp.F_lin == pytest.approx(1.0 / 3.0)
+    md = m.rheo_params()
+    assert md.T_peierls * m.scaling.T == pytest.approx(1473.15) and md.d_min * m.scaling.L == pytest.approx(1e-13)
+    pe = _model("PeierlsCreep.txt").phase_props()[0]
+    assert pe.expv == 1 and pe.G_exp == 0.1 and pe.n_exp == 0.0
+    assert _model("GBSCreep.txt").phase_props()[0].gbsv == 1
+    cp = _model("CompositePwlLin.txt").phase_props()[0]
+    assert cp.pwlv == 1 and cp.linv == 1
+
+
+def test_diffusion_creep_input(oracle_mod):
+    m, (eta, etaVE, _, _, st), T_K, E = _creep_case(oracle_mod, "DiffusionCreepLinear.txt")
+    assert st == 0
+    law = FL.LINEAR_LAWS[40][1:]
+    tau = 2 * etaVE * m.scaling.eta * E
+    assert _rate_si(law, tau, T_K, 5e-3) == pytest.approx(E, rel=1e-10)
+    assert eta == pytest.approx(etaVE, rel=1e-12)      # d = gs_ref
+    # with the file's cut-offs (read in Pa s like the power-law case) the run is capped
+    _, capped, *_ = _creep_case(oracle_mod, "DiffusionCreepLinear.txt", lift_cap=False)
+    assert capped[1] * m.scaling.eta == pytest.approx(1e6, rel=1e-14)
+
+
+def test_diffusion_creep_grain_size_input(oracle_mod):
+    # two olivine phases with gs_ref 5 mm and 1 mm: n = 1, m = 3
+    _, a, *_ = _creep_case(oracle_mod, "DiffusionCreepGrainSize.txt", 0)
+    _, b, *_ = _creep_case(oracle_mod, "DiffusionCreepGrainSize.txt", 1)
+    assert a[1] / b[1] == pytest.approx(5.0 ** 3, rel=1e-12)
+
+
+def test_peierls_creep_input(oracle_mod):
+    m, (eta, etaVE, _, _, st), T_K, E = _creep_case(oracle_mod, "PeierlsCreep.txt")
+    assert st == 0 and T_K < 1473.15
+    _, _, G, S, Q, _, E0, n = FL.EXP_LAWS[40][1:]
+    tau = 2 * etaVE * m.scaling.eta * E
+    s_ = 2 * G * (1 - G) * Q / (IF.RG * T_K)
+    sig = math.sqrt(3.0) * tau
+    rate = math.sqrt(3.0) / 2 * E0 * math.exp(-Q / (IF.RG * T_K) * (1 - G) ** 2) * (sig / (G * S)) ** s_ * sig ** n
+    assert rate == pytest.approx(E, rel=1e-9)
+    assert eta == pytest.approx(etaVE, rel=1e-12)
+
+
+def test_gbs_creep_input(oracle_mod):
+    # grain-boundary sliding adds no strain rate in ViscosityConcise (C_gbs = 0):
+    # with elasticity on, the phase is a Maxwell spring G dt, and its
+    # dissipative viscosity is 1/0, capped at max_eta
+    m, (eta, etaVE, _, _, st), _, _ = _creep_case(oracle_mod, "GBSCreep.txt")
+    assert m["elastic"] == 1 and st == 0
+    assert etaVE == pytest.approx(m.phases[0].G * m["dt"], rel=1e-12)
+    assert eta == 1e50
+
+
+def test_composite_creep_input(oracle_mod):
+    from scipy.optimize import brentq
+    m, (eta, etaVE, _, _, st), T_K, E = _creep_case(oracle_mod, "CompositePwlLin.txt")
+    assert st == 0
+    pw, li = FL.POWER_LAWS[40][1:], FL.LINEAR_LAWS[40][1:]
+    tau = brentq(lambda t: _rate_si(pw, t, T_K, 1.0) + _rate_si(li, t, T_K, 5e-3) - E, 1.0, 1e12, rtol=1e-15)
+    assert 2 * etaVE * m.scaling.eta * E == pytest.approx(tau, rel=1e-10)
+    # diffusion carries ~1 % of the strain rate at 5 mm: dropping it would move tau by ~3e-3
+    assert 1e-3 < _rate_si(li, tau, T_K, 5e-3) / E < 0.5
