@@ -298,3 +298,16 @@ class Model:
         return SolverOpts(method=1, penalty=p["penalty"], max_its_PH=p["max_its_PH"], rel_tol_KSP=p["rel_tol_KSP"],
                           max_its_KSP=p["max_its_KSP"], lin_abs_mom=p["lin_abs_mom"], lin_rel_mom=p["lin_rel_mom"],
                           lin_abs_div=p["lin_abs_div"], lin_rel_div=p["lin_rel_div"])
+
+    def loop_options(self):
+        """(nit_max, tol, keyword arguments) of PicardLoop.run for the file's
+        non-linear settings (Main_DOODZ.c:878-1062 reads the same keys):
+        tol = (nonlin_abs_mom, nonlin_rel_mom, nonlin_abs_div, nonlin_rel_div)."""
+        p = self.params
+        tol = (p["nonlin_abs_mom"], p["nonlin_rel_mom"], p["nonlin_abs_div"], p["nonlin_rel_div"])
+        kw = dict(line_search=bool(p["line_search"]), newton=bool(p["Newton"]),
+                  picard2newton=bool(p["Picard2Newton"]), picard2newton_tol=p["Picard2Newton_tol"],
+                  max_its_pic=p["max_its_Pic"], line_search_min=p["line_search_min"],
+                  let_res_grow=p["let_res_grow"], pc_line_search=p["pc_line_search"], safe_mode=p["safe_mode"],
+                  safe_dt_div=p["safe_dt_div"], max_num_stag=p["max_num_stag"])
+        return p["nit_max"], tol, kw

@@ -203,3 +203,17 @@ def test_composite_creep_input(oracle_mod):
     assert 2 * etaVE * m.scaling.eta * E == pytest.approx(tau, rel=1e-10)
     # diffusion carries ~1 % of the strain rate at 5 mm: dropping it would move tau by ~3e-3
     assert 1e-3 < _rate_si(li, tau, T_K, 5e-3) / E < 0.5
+
+
+def test_loop_options():
+    m = _model("GaussianDiffusionL2.txt")
+    nit_max, tol, kw = m.loop_options()
+    assert kw["newton"] and kw["line_search"]            # Newton = 1 forces the line search
+    assert (kw["safe_mode"], kw["safe_dt_div"], kw["max_num_stag"], kw["pc_line_search"]) == (0, 5.0, 3, 0)
+    assert nit_max == m["nit_max"] and tol == (m["nonlin_abs_mom"], m["nonlin_rel_mom"], m["nonlin_abs_div"],
+                                               m["nonlin_rel_div"])
+    f = IF.InputFile("safe_mode = 1\nsafe_dt_div = 2.5\nmax_num_stag = 4\npc_line_search = 1\nline_search_min = 0.1\n"
+                     "Nb_phases = 0\n")
+    _, _, kw = IF.Model(f).loop_options()
+    assert (kw["safe_mode"], kw["safe_dt_div"], kw["max_num_stag"], kw["pc_line_search"], kw["line_search_min"]) == \
+        (1, 2.5, 4, 1, 0.1)
