@@ -540,8 +540,15 @@ __global__ void __launch_bounds__(PC_THREADS) k_pcheb(sgrid g, stc<T> cf, pc_til
     constexpr int WX = PC_TX + 1, HX = PC_TZ + 2;   // Vx faces: k0-1 .. k0+TX-1, l0-1 .. l0+TZ
     constexpr int WZ = PC_TX + 2, HZ = PC_TZ + 1;   // Vz faces: k0-1 .. k0+TX,   l0-1 .. l0+TZ-1
     constexpr int WV = PC_TX + 1, HV = PC_TZ + 1;   // vertices: k0-1 .. k0+TX-1, l0-1 .. l0+TZ-1
-    __shared__ T rx[WX * HX], dxs[WX * HX], rz[WZ * HZ], dzs[WZ * HZ];
-    __shared__ T yv[4][WV * HV];
+    // The face residuals and diagonals (phases 1-2) and the patch solutions
+    // (phases 2-3) share one LDS buffer: the solutions stay in registers
+    // until every patch solve has read its faces.  Half the LDS of separate
+    // arrays, which sets the FP64 occupancy on the large tiles.
+    constexpr int NFACE = 2 * WX * HX + 2 * WZ * HZ, NVERT = 4 * WV * HV;
+    constexpr int NQ = (WV * HV + PC_THREADS - 1) / PC_THREADS;   // patch solves per thread
+    __shared__ T lds[NFACE > NVERT ? NFACE : NVERT];
+    T *const rx = lds, *const dxs = lds + WX * HX, *const rz = lds + 2 * WX * HX, *const dzs = rz + WZ * HZ;
+    T *const yv = lds;                              // yv[i * WV * HV + vertex], i = 0..3
     const int tile = xcd_tile(blockIdx.x, tg.ntiles);
     const int tz = tile / tg.nbx, k0 = (tile - tz * tg.nbx) * PC_TX, l0 = tz * PC_TZ;
     const int nu = nvx(g), NxV = g.Nx + 1;
@@ -570,10 +577,14 @@ __global__ void __launch_bounds__(PC_THREADS) k_pcheb(sgrid g, stc<T> cf, pc_til
     }
     __syncthreads();
     // 2. patch solves y_p = M_p^-1 r_p for the vertices the tile's faces use
-    for (int q = threadIdx.x; q < WV * HV; q += PC_THREADS) {
+    T ys[NQ][4];
+#pragma unroll
+    for (int t = 0; t < NQ; t++) {
+        const int q = threadIdx.x + t * PC_THREADS;
+        T *const y = ys[t];
+        y[0] = y[1] = y[2] = y[3] = T(0);
         const int ia = q % WV, ib = q / WV, a_ = k0 - 1 + ia, b_ = l0 - 1 + ib;
-        T y[4] = {T(0), T(0), T(0), T(0)};
-        if (a_ >= 0 && b_ >= 0 && a_ < g.Nx && b_ < g.Nz) {
+        if (q < WV * HV && a_ >= 0 && b_ >= 0 && a_ < g.Nx && b_ < g.Nz) {
             // faces xS = Vx(a, b), xN = Vx(a, b+1), zW = Vz(a, b), zE = Vz(a+1, b)
             const int fx0 = ia + ib * WX, fx1 = ia + (ib + 1) * WX, fz0 = ia + ib * WZ, fz1 = ia + 1 + ib * WZ;
             const bool fr[4] = {vx_free(g, a_, b_), vx_free(g, a_, b_ + 1), vz_free(g, a_, b_), vz_free(g, a_ + 1, b_)};
@@ -611,7 +622,13 @@ __global__ void __launch_bounds__(PC_THREADS) k_pcheb(sgrid g, stc<T> cf, pc_til
             y[1] = (z1 - L21 * y[2] - L31 * y[3]) * i1;
             y[0] = (z0 - L10 * y[1] - L20 * y[2] - L30 * y[3]) * i0;
         }
-        for (int i = 0; i < 4; i++) yv[i][q] = y[i];
+    }
+    __syncthreads();   // every patch solve has read the face arrays
+#pragma unroll
+    for (int t = 0; t < NQ; t++) {
+        const int q = threadIdx.x + t * PC_THREADS;
+        if (q < WV * HV)
+            for (int i = 0; i < 4; i++) yv[i * WV * HV + q] = ys[t][i];
     }
     __syncthreads();
     // 3. each position updates its Vx and Vz face
@@ -623,7 +640,7 @@ __global__ void __launch_bounds__(PC_THREADS) k_pcheb(sgrid g, stc<T> cf, pc_til
             const int i = k + l * g.Nx;
             T y = T(0);
             if (vx_free(g, k, l)) {
-                const T z = yv[0][vi] + yv[1][vi - WV];   // patch (k, l) as xS, patch (k, l-1) as xN
+                const T z = yv[vi] + yv[WV * HV + vi - WV];   // patch (k, l) as xS, patch (k, l-1) as xN
                 const T xi = u[i];
                 y = xi + cc * z;
                 if (!FIRST) y += a * (xi - xo[i]);
@@ -634,7 +651,7 @@ __global__ void __launch_bounds__(PC_THREADS) k_pcheb(sgrid g, stc<T> cf, pc_til
             const int j = k + l * NxV;
             T y = T(0);
             if (vz_free(g, k, l)) {
-                const T z = yv[2][vi] + yv[3][vi - 1];    // patch (k, l) as zW, patch (k-1, l) as zE
+                const T z = yv[2 * WV * HV + vi] + yv[3 * WV * HV + vi - 1];   // patch (k, l) as zW, (k-1, l) as zE
                 const T xi = v[j];
                 y = xi + cc * z;
                 if (!FIRST) y += a * (xi - xo[nu + j]);
