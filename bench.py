@@ -227,7 +227,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": SMOOTHER_TRAFFIC_2047 if a.cells == 2047 else None,
-                     "kernel": "k_pcheb<double, false, 64, 8, 256> (fine-level fused vertex-patch "
+                     "kernel": "k_pcheb<double, false, 64, 16, 512> (fine-level fused vertex-patch "
                                "Chebyshev smoother step)",
                      "kernel_ms": k_ms, "alg_bytes": k_bytes,
                      "kernel_ms_source": "mean of the %d fine-level smoother launches of one FP64 "

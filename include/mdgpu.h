@@ -384,7 +384,8 @@ typedef struct {
     int    its_PH, its_KSP;/* method 1: PH iterations, GCR iterations summed
                               as kspgcr counts them (Solvers.c:208-211)       */
     double smoother_ms;    /* graphs = 2: mean duration of the fine-level
-                              smoother steps (k_cheb3) in this solve          */
+                              smoother steps (k_cheb3; k_pcheb with smoother
+                              3) in this solve                                */
     int    smoother_launches;
     /* input: smoothers 2 and 3 use vertex patches on the finest patch_levels
      * levels and the diagonal below them (0: every level, the default; the

@@ -22,6 +22,7 @@ using rheo::viscosity_concise;
 
 namespace {
 
+template <bool EXT>
 __global__ void __launch_bounds__(256)
 k_tangent_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
                const signed char *__restrict__ tag, int centroid, mdg_rheo_nodes f,
@@ -58,7 +59,7 @@ k_tangent_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
         if (!(fabs(w) > 1e-13)) continue;
         const mdg_phase_prop &m = sph[p];
         double etaVE, Pc, Epl;
-        viscosity_concise(m, md, G, T, P, dg, phi, Exx, Ezz, Exz, f.fric ? f.fric[i] : m.fric,
+        viscosity_concise<EXT>(m, md, G, T, P, dg, phi, Exx, Ezz, Exz, f.fric ? f.fric[i] : m.fric,
                           f.C ? f.C[i] : m.C, f.dil ? f.dil[i] : m.dil, pref, etaVE, Pc, Epl, bad);
         if (md.average == 0) e += w * etaVE;
         else if (md.average == 1) e += w / etaVE;
@@ -82,11 +83,11 @@ k_tangent_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
             const double fr = f.fric ? f.fric[i] : m.fric, co = f.C ? f.C[i] : m.C, di = f.dil ? f.dil[i] : m.dil;
             double eb = 0.0, Pc, Epl;
             if (md.average != 0)   // the phase's own value at the base state
-                viscosity_concise(m, md, G, T, P, dg, phi, Exx, Ezz, Exz, fr, co, di, pref, eb, Pc, Epl, bad);
+                viscosity_concise<EXT>(m, md, G, T, P, dg, phi, Exx, Ezz, Exz, fr, co, di, pref, eb, Pc, Epl, bad);
             double epm[2];
             for (int sgn = 0; sgn < 2; sgn++) {
                 const double d = sgn ? -h : h;
-                viscosity_concise(m, md, G, T, P + (q == 3 ? d : 0.0), dg, phi, Exx + (q == 0 ? d : 0.0),
+                viscosity_concise<EXT>(m, md, G, T, P + (q == 3 ? d : 0.0), dg, phi, Exx + (q == 0 ? d : 0.0),
                                   Ezz + (q == 1 ? d : 0.0), Exz + (q == 2 ? d : 0.0), fr, co, di, pref, epm[sgn],
                                   Pc, Epl, bad);
             }
@@ -430,7 +431,8 @@ extern "C" int mdg_tangent_grid(mdg_ctx *c, const mdg_phase_prop *phases, int np
             hipMemsetAsync(c->flag_dev.p, 0, sizeof(int), c->stream) != hipSuccess)
             rc = mdg_fail(c, MDG_ERR_HIP, "staging failed");
         else {
-            hipLaunchKernelGGL(k_tangent_grid, dim3(mdg_grid1d(n, 256)), dim3(256), nph * sizeof(mdg_phase_prop),
+            hipLaunchKernelGGL(rheo::needs_ext(phases, nph) ? k_tangent_grid<true> : k_tangent_grid<false>,
+                               dim3(mdg_grid1d(n, 256)), dim3(256), nph * sizeof(mdg_phase_prop),
                                c->stream, md,
                                (const mdg_phase_prop *)s->ph.p, n, centroid ? c->BCp_t.p : c->BCg_t.p,
                                centroid ? 1 : 0, d, Dd, c->flag_dev.p);

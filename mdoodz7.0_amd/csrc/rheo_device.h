@@ -57,7 +57,12 @@ constexpr int FAIL_GRAIN_SIZE = 2;     // d < d_min with m_lin > 0 (reference ex
 
 // ViscosityConcise (RheologyDensity.c:385-1080) for one (node, phase) with
 // grain size d and melt fraction phi: returns the dissipative viscosity;
-// etaVE, Pcorr, Eii_pl by reference.
+// etaVE, Pcorr, Eii_pl by reference.  EXT = false compiles the diffusion,
+// GBS and Peierls laws and the power law's d, f, phi factors out (the
+// caller launches it when no phase uses them: the factors are then exactly
+// 1 and the laws absent, so the results are the same bits), which keeps
+// the register count, and the occupancy, of the constant + power-law case.
+template <bool EXT>
 __device__ inline double viscosity_concise(const mdg_phase_prop &m, const rheo_ctl &md, double G, double T,
                                            double P, double d, double phi, double Exx, double Ezz, double Exz,
                                            double fric, double C, double dil, double pref_mult, double &etaVE,
@@ -66,8 +71,8 @@ __device__ inline double viscosity_concise(const mdg_phase_prop &m, const rheo_c
     bool elastic = md.elastic != 0, plastic = (m.plast == 1);
     if (md.step == 0) { elastic = false; plastic = false; }          // :483
     const bool constant = m.cstv != 0, dislocation = m.pwlv != 0;
-    const bool peierls = m.expv != 0 && T < md.T_peierls;             // :474, before the T floor
-    const bool diffusion = m.linv != 0, gbs = m.gbsv != 0;
+    const bool peierls = EXT && m.expv != 0 && T < md.T_peierls;      // :474, before the T floor
+    const bool diffusion = EXT && m.linv != 0, gbs = EXT && m.gbsv != 0;
     double K = 1.0 / m.beta;
     const double Eyy = -(Exx + Ezz);
     double Eii = sqrt(0.5 * (Exx * Exx + Ezz * Ezz + Eyy * Eyy) + Exz * Exz);   // :457-459
@@ -78,8 +83,10 @@ __device__ inline double viscosity_concise(const mdg_phase_prop &m, const rheo_c
     double B_pwl = 0.0, C_pwl = 0.0;
     if (dislocation) {                                                 // :501-504
         B_pwl = m.pref_pwl * pref_mult * m.F_pwl * pow(m.A_pwl, -1.0 / m.n_pwl) *
-                exp((m.Q_pwl + P * m.V_pwl) / md.R / m.n_pwl / T) * pow(d, m.m_pwl / m.n_pwl) *
-                pow(m.f_pwl, -m.r_pwl / m.n_pwl) * exp(-m.a_pwl * phi / m.n_pwl);
+                exp((m.Q_pwl + P * m.V_pwl) / md.R / m.n_pwl / T);
+        if (EXT)
+            B_pwl = B_pwl * pow(d, m.m_pwl / m.n_pwl) * pow(m.f_pwl, -m.r_pwl / m.n_pwl) *
+                    exp(-m.a_pwl * phi / m.n_pwl);
         C_pwl = pow(2.0 * B_pwl, -m.n_pwl);
     }
     double B_lin = 0.0, C_lin = 0.0;
@@ -169,6 +176,15 @@ __device__ inline double viscosity_concise(const mdg_phase_prop &m, const rheo_c
     if (eta > md.max_eta) eta = md.max_eta;
     if (eta < md.min_eta) eta = md.min_eta;
     return eta;
+}
+
+// Whether any phase needs the EXT instantiation of viscosity_concise.
+inline bool needs_ext(const mdg_phase_prop *ph, int nph) {
+    for (int p = 0; p < nph; p++)
+        if (ph[p].linv || ph[p].gbsv || ph[p].expv || ph[p].m_pwl != 0.0 || ph[p].r_pwl != 0.0 ||
+            ph[p].a_pwl != 0.0)
+            return true;
+    return false;
 }
 
 // The kernel-side constants of one call.

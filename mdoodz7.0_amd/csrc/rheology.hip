@@ -16,6 +16,7 @@ using rheo::viscosity_concise;
 
 namespace {
 
+template <bool EXT>
 __global__ void __launch_bounds__(256)
 k_viscosity_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
                  const signed char *__restrict__ tag, int centroid, mdg_rheo_nodes f,
@@ -53,7 +54,7 @@ k_viscosity_grid(rheo_ctl md, const mdg_phase_prop *__restrict__ phases, int n,
         if (!(fabs(w) > 1e-13)) continue;                              // :1212-1215
         const mdg_phase_prop &m = sph[p];
         double etaVE, Pcorr, Eii_pl;
-        const double eta = viscosity_concise(m, md, G, T, P, dg, phi, Exx, Ezz, Exz,
+        const double eta = viscosity_concise<EXT>(m, md, G, T, P, dg, phi, Exx, Ezz, Exz,
                                              f.fric ? f.fric[i] : m.fric, f.C ? f.C[i] : m.C,
                                              f.dil ? f.dil[i] : m.dil, pref, etaVE, Pcorr,
                                              Eii_pl, bad);
@@ -275,7 +276,8 @@ extern "C" int mdg_viscosity_grid(mdg_ctx *c, const mdg_phase_prop *phases, int 
     HIPCHK(c, hipMemsetAsync(c->flag_dev.p, 0, sizeof(int), c->stream));
     const rheo_ctl md = rheo::make_ctl(prm, nph);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_viscosity_grid, dim3(mdg_grid1d(n, 256)), dim3(256), nph * sizeof(mdg_phase_prop),
+    hipLaunchKernelGGL(rheo::needs_ext(phases, nph) ? k_viscosity_grid<true> : k_viscosity_grid<false>,
+                       dim3(mdg_grid1d(n, 256)), dim3(256), nph * sizeof(mdg_phase_prop),
                        c->stream, md,
                        (const mdg_phase_prop *)s->ph.p, n, tag, centroid, d, c->flag_dev.p);
     HIPCHK(c, hipGetLastError());

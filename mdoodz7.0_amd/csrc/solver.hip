@@ -518,7 +518,9 @@ __device__ inline double prec_rsqrt(double x) {
     const double e = fma(-x * y, y, 1.0);
     return fma(0.5 * y, e, y);
 }
-// Tiles: 64 x 8 positions and 256 threads on the large levels; 16 x 4
+// Tiles: 64 x 16 positions and 512 threads on the large levels (the
+// one-face halo adds 14% Vx and 10% Vz faces and 8% patches; 64 x 8 on 256
+// threads adds 27%, 16% and 14% and measured 3% slower per solve); 16 x 4
 // positions and one wave on levels with fewer than PC_SMALL_BELOW large
 // tiles, where a few large workgroups would leave most CUs idle and each
 // workgroup's serial phases set the step's latency.  The tile shape does
@@ -1768,8 +1770,8 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
     }
     if (fused && patch) {
         // three-term form with the in-kernel vertex-patch inverse
-        const bool small = make_pc_tiles<64, 8>(lv.g).ntiles < PC_SMALL_BELOW;
-        const pc_tiles tg = small ? make_pc_tiles<16, 4>(lv.g) : make_pc_tiles<64, 8>(lv.g);
+        const bool small = make_pc_tiles<64, 16>(lv.g).ntiles < PC_SMALL_BELOW;
+        const pc_tiles tg = small ? make_pc_tiles<16, 4>(lv.g) : make_pc_tiles<64, 16>(lv.g);
         const stc<T> cf = make_stc<T>(lv.g);
         T *cur = lv.d.p, *prev = lv.x.p;
         auto step = [&](bool first, T a, T cc, const T *xc, T *xo) {
@@ -1782,10 +1784,10 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
                                        tg, (T)gamma, a, cc, xc, xo, lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
             } else {
                 if (first)
-                    hipLaunchKernelGGL((k_pcheb<T, true, 64, 8, 256>), dim3(tg.ntiles), dim3(256), 0, c->stream, lv.g,
+                    hipLaunchKernelGGL((k_pcheb<T, true, 64, 16, 512>), dim3(tg.ntiles), dim3(512), 0, c->stream, lv.g,
                                        cf, tg, (T)gamma, a, cc, xc, xo, lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
                 else
-                    hipLaunchKernelGGL((k_pcheb<T, false, 64, 8, 256>), dim3(tg.ntiles), dim3(256), 0, c->stream, lv.g,
+                    hipLaunchKernelGGL((k_pcheb<T, false, 64, 16, 512>), dim3(tg.ntiles), dim3(512), 0, c->stream, lv.g,
                                        cf, tg, (T)gamma, a, cc, xc, xo, lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
             }
         };
@@ -2580,11 +2582,11 @@ extern "C" int mdg_time_smoother(mdg_ctx *c, int reps, double *ms, double *bytes
     // vertex-patch step (smoother 3) or the fused Jacobi step
     const bool patch = c->solver->hd.fused && l0.patch;
     const tile_grid tg = make_tiles(l0.g);
-    const pc_tiles pg = make_pc_tiles<64, 8>(l0.g);
+    const pc_tiles pg = make_pc_tiles<64, 16>(l0.g);
     const stc<double> cf = make_stc<double>(l0.g);
     auto launch = [&]() {
         if (patch)
-            hipLaunchKernelGGL((k_pcheb<double, false, 64, 8, 256>), dim3(pg.ntiles), dim3(256), 0, c->stream, l0.g, cf,
+            hipLaunchKernelGGL((k_pcheb<double, false, 64, 16, 512>), dim3(pg.ntiles), dim3(512), 0, c->stream, l0.g, cf,
                                pg, 10.0, 0.5, 0.1, (const double *)l0.x.p, l0.d.p, (const double *)l0.b.p,
                                (const double *)l0.en.p, (const double *)l0.es.p);
         else
