@@ -32,7 +32,7 @@ SOLVER = dict(smoother=3, penalty=500.0, cheb_degree=12, cheb_degree_coarse=12)
 # HBM bytes per launch of the fine-level smoother step (k_pcheb, FP64) on
 # 2047^2 cells, rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
 # copy kernel of known size): profiles/hbm_traffic_pcheb_2047.txt
-SMOOTHER_TRAFFIC_2047 = 343.9e6
+SMOOTHER_TRAFFIC_2047 = 415.1e6
 METRIC = "Stokes solve time @1e-8 resid, 2047², plain vs learned PC; P2G Mmarkers/s"
 
 

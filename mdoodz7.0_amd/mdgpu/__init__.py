@@ -687,19 +687,30 @@ class Context:
                                                                      "dil")]))
         return out
 
+    @staticmethod
+    def _rheo_inputs(arrays, n, nph):
+        """mdg_rheo_nodes with the inputs of arrays (numpy arrays converted
+        to contiguous float64 and checked to hold n values, nph*n for perc;
+        DeviceArrays passed as they are)."""
+        f, keep = RheoNodes(), []
+        for k in RHEO_IN:
+            a = arrays.get(k)
+            if isinstance(a, np.ndarray):
+                a = np.ascontiguousarray(a, np.float64)
+                want = n * nph if k == "perc" else n
+                if a.size != want:
+                    raise ValueError("%s holds %d values, expected %d" % (k, a.size, want))
+                keep.append(a)
+            setattr(f, k, ptr(a))
+        return f, keep
+
     def tangent_grid(self, phases, prm, centroid, D=None, **arrays):
         """Newton tangent coefficients: (8, cells) or (4, vertices); D may be
         a DeviceArray of that many values to write into."""
         n = self.node_count(centroid)
         if D is None:
             D = np.zeros(((8 if centroid else 4), n))
-        f, keep = RheoNodes(), []
-        for k in RHEO_IN:
-            a = arrays.get(k)
-            if isinstance(a, np.ndarray):
-                a = np.ascontiguousarray(a, np.float64)
-                keep.append(a)
-            setattr(f, k, ptr(a))
+        f, keep = self._rheo_inputs(arrays, n, len(phases))
         ph = (PhaseProp * len(phases))(*phases)
         self._check(self._L.mdg_tangent_grid(self.h, ph, len(phases), C.byref(prm), int(centroid),
                                              C.byref(f), ptr(D)))
@@ -732,14 +743,8 @@ class Context:
         RHEO_OUT are allocated on the host and returned in a dict, or may be
         passed in arrays (e.g. as DeviceArray)."""
         n = self.node_count(centroid)
-        out, keep = {}, []
-        f = RheoNodes()
-        for k in RHEO_IN:
-            a = arrays.get(k)
-            if isinstance(a, np.ndarray):
-                a = np.ascontiguousarray(a, np.float64)
-                keep.append(a)
-            setattr(f, k, ptr(a))
+        out = {}
+        f, keep = self._rheo_inputs(arrays, n, len(phases))
         for k in RHEO_OUT:
             a = arrays.get(k)
             if a is None and k in outputs:
