@@ -518,14 +518,18 @@ __device__ inline double prec_rsqrt(double x) {
     const double e = fma(-x * y, y, 1.0);
     return fma(0.5 * y, e, y);
 }
-// Tiles: 64 x 16 positions and 512 threads on the large levels (the
+// Tiles: 64 x 16 positions and 512 threads on the largest levels (the
 // one-face halo adds 14% Vx and 10% Vz faces and 8% patches; 64 x 8 on 256
-// threads adds 27%, 16% and 14% and measured 3% slower per solve); 16 x 4
-// positions and one wave on levels with fewer than PC_SMALL_BELOW large
-// tiles, where a few large workgroups would leave most CUs idle and each
-// workgroup's serial phases set the step's latency.  The tile shape does
-// not change any value computed.
-#define PC_SMALL_BELOW 1024
+// threads adds 27%, 16% and 14% and measured 3% slower per solve); 32 x 8
+// on 256 threads on levels with fewer than PC_LARGE_FROM such tiles; 16 x 4
+// and one wave on levels with fewer than PC_MID_FROM 32 x 8 tiles, where
+// larger workgroups would leave most CUs idle and each workgroup's serial
+// phases set the step's latency (at 2047^2: 64 x 16 on the finest level,
+// 32 x 8 on 1023^2 and 511^2, 16 x 4 below; 1% faster per solve than 64 x 16
+// down to 1023^2 and 16 x 4 below).  The tile shape does not change any
+// value computed.
+#define PC_LARGE_FROM 2048
+#define PC_MID_FROM 512
 struct pc_tiles {
     int nbx, ntiles;
 };
@@ -533,6 +537,12 @@ template <int TX, int TZ>
 static inline pc_tiles make_pc_tiles(const sgrid &g) {
     const int nbx = (g.Nx + 1 + TX - 1) / TX, nbz = (g.Nz + 1 + TZ - 1) / TZ;
     return pc_tiles{nbx, nbx * nbz};
+}
+// 2: 64 x 16 tiles on 512 threads (levels with at least PC_LARGE_FROM of
+// them), 1: 32 x 8 on 256 (at least PC_MID_FROM of those), 0: 16 x 4 on 64
+static inline int pc_tier(const sgrid &g) {
+    if (make_pc_tiles<64, 16>(g).ntiles >= PC_LARGE_FROM) return 2;
+    return make_pc_tiles<32, 8>(g).ntiles >= PC_MID_FROM ? 1 : 0;
 }
 template <typename T, bool FIRST, int PC_TX, int PC_TZ, int PC_THREADS>
 __global__ void __launch_bounds__(PC_THREADS) k_pcheb(sgrid g, stc<T> cf, pc_tiles tg, T gamma, T a, T cc,
@@ -661,6 +671,19 @@ __global__ void __launch_bounds__(PC_THREADS) k_pcheb(sgrid g, stc<T> cf, pc_til
             xo[nu + j] = y;
         }
     }
+}
+
+// One patch-Chebyshev step on TX x TZ tiles of TH threads.
+template <typename T, int TX, int TZ, int TH>
+static void pc_step(hipStream_t s, const sgrid &g, const stc<T> &cf, T gamma, bool first, T a, T cc, const T *xc,
+                    T *xo, const T *b, const T *en, const T *es) {
+    const pc_tiles tg = make_pc_tiles<TX, TZ>(g);
+    if (first)
+        hipLaunchKernelGGL((k_pcheb<T, true, TX, TZ, TH>), dim3(tg.ntiles), dim3(TH), 0, s, g, cf, tg, gamma, a, cc, xc,
+                           xo, b, en, es);
+    else
+        hipLaunchKernelGGL((k_pcheb<T, false, TX, TZ, TH>), dim3(tg.ntiles), dim3(TH), 0, s, g, cf, tg, gamma, a, cc, xc,
+                           xo, b, en, es);
 }
 
 // Linear interpolation weight of fine coordinate index f (position
@@ -1770,26 +1793,17 @@ static void chebyshev(mdg_ctx *c, mg_level<T> &lv, double gamma, int deg, bool z
     }
     if (fused && patch) {
         // three-term form with the in-kernel vertex-patch inverse
-        const bool small = make_pc_tiles<64, 16>(lv.g).ntiles < PC_SMALL_BELOW;
-        const pc_tiles tg = small ? make_pc_tiles<16, 4>(lv.g) : make_pc_tiles<64, 16>(lv.g);
+        const int tier = pc_tier(lv.g);
         const stc<T> cf = make_stc<T>(lv.g);
         T *cur = lv.d.p, *prev = lv.x.p;
         auto step = [&](bool first, T a, T cc, const T *xc, T *xo) {
-            if (small) {
-                if (first)
-                    hipLaunchKernelGGL((k_pcheb<T, true, 16, 4, 64>), dim3(tg.ntiles), dim3(64), 0, c->stream, lv.g, cf,
-                                       tg, (T)gamma, a, cc, xc, xo, lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
-                else
-                    hipLaunchKernelGGL((k_pcheb<T, false, 16, 4, 64>), dim3(tg.ntiles), dim3(64), 0, c->stream, lv.g, cf,
-                                       tg, (T)gamma, a, cc, xc, xo, lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
-            } else {
-                if (first)
-                    hipLaunchKernelGGL((k_pcheb<T, true, 64, 16, 512>), dim3(tg.ntiles), dim3(512), 0, c->stream, lv.g,
-                                       cf, tg, (T)gamma, a, cc, xc, xo, lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
-                else
-                    hipLaunchKernelGGL((k_pcheb<T, false, 64, 16, 512>), dim3(tg.ntiles), dim3(512), 0, c->stream, lv.g,
-                                       cf, tg, (T)gamma, a, cc, xc, xo, lv.rhs(), (const T *)lv.en.p, (const T *)lv.es.p);
-            }
+            const T *en = (const T *)lv.en.p, *es = (const T *)lv.es.p;
+            if (tier == 2)
+                pc_step<T, 64, 16, 512>(c->stream, lv.g, cf, (T)gamma, first, a, cc, xc, xo, lv.rhs(), en, es);
+            else if (tier == 1)
+                pc_step<T, 32, 8, 256>(c->stream, lv.g, cf, (T)gamma, first, a, cc, xc, xo, lv.rhs(), en, es);
+            else
+                pc_step<T, 16, 4, 64>(c->stream, lv.g, cf, (T)gamma, first, a, cc, xc, xo, lv.rhs(), en, es);
         };
         if (zero_start)
             hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, c->stream, n, prev, T(0));

@@ -13,9 +13,10 @@ gz = -9.81, 256 seeded ellipses (full axes 5-60 cells, log10 viscosity
 U[-3, 3] against 0, density 3000 +- 300), 3 x 3 markers per cell
 (37,711,881 markers): marker indices of every marker bit-exact and P2G
 (geometric viscosity, arithmetic density) on centres and vertices within
-1e-12 of the oracle over all markers; the FP64 and mixed-precision solves
-reach 1e-8 on the FP64 residual recomputed from the oracle's assembled
-Picard matrices, and agree with each other to 1e-6.
+1e-12 of the oracle over all markers; the FP64 and mixed-precision solves,
+in the benchmark's solver configurations, reach 1e-8 on the FP64 residual
+recomputed from the oracle's assembled Picard matrices, and agree with each
+other to 1e-6.
 """
 import numpy as np
 import pytest
@@ -253,8 +254,12 @@ def test_ellipses_2047_solves(oracle_mod, ellipses_2047):
     nm = num[3]
     A, B, Cm = O.assemble_picard(g, num, en, es)
     sols = {}
-    for prec in (mdgpu.MDG_PREC_FP64, mdgpu.MDG_PREC_MIXED):
-        sol, o = ctx.killer_solver(2 * en, 2 * en, es, F[:nm], F[nm:], rel_tol=1e-8, precision=prec)
+    # the benchmark's solver configurations (vertex-patch smoother, degree 12
+    # on every level; penalty 700 with the FP64 V-cycle, 300 with the FP32 one)
+    patch = dict(smoother=3, cheb_degree=12, cheb_degree_coarse=12)
+    for prec, gamma in ((mdgpu.MDG_PREC_FP64, 700.0), (mdgpu.MDG_PREC_MIXED, 300.0)):
+        sol, o = ctx.killer_solver(2 * en, 2 * en, es, F[:nm], F[nm:], rel_tol=1e-8, precision=prec, penalty=gamma,
+                                   **patch)
         r = np.concatenate([F[:nm] - A @ sol[:nm] - B @ sol[nm:], F[nm:] - Cm @ sol[:nm]])
         rel = np.linalg.norm(r) / np.linalg.norm(F)
         print("2047^2 precision %d: %d outer, %d inner its, %.1f s, FP64 residual %.3e"

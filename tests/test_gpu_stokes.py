@@ -132,6 +132,8 @@ def test_solver_variants_agree_at_511():
     for kw in (dict(), dict(precision=mdgpu.MDG_PREC_MIXED), dict(smoother=1), dict(smoother=2),
                dict(smoother=3), dict(smoother=3, precision=mdgpu.MDG_PREC_MIXED, penalty=100.0),
                dict(smoother=3, penalty=100.0, cheb_degree=12, cheb_degree_coarse=12),
+               dict(smoother=3, penalty=700.0, cheb_degree=12, cheb_degree_coarse=12),
+               dict(smoother=3, penalty=300.0, cheb_degree=12, cheb_degree_coarse=12, precision=mdgpu.MDG_PREC_MIXED),
                dict(graphs=1, max_its=60), dict(graphs=2), dict(graphs=2, smoother=3)):
         sol, o = prob.solve(ctx, rel_tol=1e-8, **kw)
         print(kw, "its", o.its, "inner", o.inner_its, "rel", o.rel_res, "ms %.1f" % o.time_ms,
