@@ -31,7 +31,8 @@ from . import MDG_ERR_DIVERGED, RHEO_OUT, DeviceArray, RheoParams
 
 # fields of the state that the loop reads (uploaded once in device mode)
 _STATE_IN = ("u", "v", "p", "T", "mu_n", "mu_s", "perc_n", "perc_s", "sxxd0", "szzd0", "sxz0_n",
-             "sxxd0_s", "szzd0_s", "sxz0", "rho_n", "rho_s", "pref_n", "pref_s")
+             "sxxd0_s", "szzd0_s", "sxz0", "rho_n", "rho_s", "pref_n", "pref_s", "d0_n", "d0_s", "phi0_n",
+             "phi0_s")
 
 
 def scatter(eqn_u, eqn_v, eqn_p, dx, u, v, p, alpha=-1.0):
@@ -88,6 +89,10 @@ class PicardLoop:
                    sxz0=st["sxz0"])
         if "pref_n" in st:   # per-node multiplier of the power-law prefactor
             cin["pref"], vin["pref"] = st["pref_n"], st["pref_s"]
+        if "d0_n" in st:     # grain size (d0_n / d0_s of the reference's grid)
+            cin["d"], vin["d"] = st["d0_n"], st["d0_s"]
+        if "phi0_n" in st:   # melt fraction (phi0_n / phi0_s)
+            cin["phi"], vin["phi"] = st["phi0_n"], st["phi0_s"]
         return cin, vin
 
     def update_nonlinearity(self, st):

@@ -43,10 +43,12 @@ def update_nonlinearity(g, st, md, phases, average):
     P_s = c2v(g, st["p"])
     cen = rheology_nodes(md, phases, st["perc_n"], st["mu_n"], st["T"], st["p"], sr["exxd"],
                          sr["ezzd"], sr["exz_n"], st["sxxd0"], st["szzd0"], st["sxz0_n"],
-                         tag=g.BCp_t, centroid=1, average=average)
+                         pref=st.get("pref_n"), tag=g.BCp_t, centroid=1, average=average,
+                         d=st.get("d0_n"), phi=st.get("phi0_n"))
     ver = rheology_nodes(md, phases, st["perc_s"], st["mu_s"], T_s, P_s, sr["exxd_s"],
                          sr["ezzd_s"], sr["exz"], st["sxxd0_s"], st["szzd0_s"], st["sxz0"],
-                         tag=g.BCg_t, centroid=0, average=average)
+                         pref=st.get("pref_s"), tag=g.BCg_t, centroid=0, average=average,
+                         d=st.get("d0_s"), phi=st.get("phi0_s"))
     if cen["status"] or ver["status"]:
         raise RuntimeError("visco-elastic local iterations failed")
     return sr, cen, ver

@@ -79,6 +79,64 @@ def test_picard_loop_matches_oracle(oracle_mod, diag_scaling, line_search, C):
     assert rel(st_gpu["eta_n"], st["eta_n"]) < 1e-8
 
 
+# every creep law in the loop: the power law of the shear box with
+# grain-size and melt factors, plus diffusion creep (grain-size dependent)
+# and Peierls creep (active everywhere: T = 1 < T_peierls = 2)
+COMMON_LAWS = dict(step=1, elastic=1, dt=1.0, R=1.0, zeroC=0.0, min_eta=1e-4, max_eta=1e4, eta_tol=1e-13,
+                   Tiimin=1e-5, scale_E=1.0, scale_eta=1.0, T_peierls=2.0, d_min=1e-3)
+
+
+def _law_phases(C=2.0):
+    extra = dict(m_pwl=0.5, a_pwl=5.0, linv=1, F_lin=1.0, A_lin=0.5, n_lin=1.0, m_lin=2.0, gs_ref=1.0,
+                 expv=1, G_exp=0.2, S_exp=4.0, Q_exp=2.0, E_exp=1.0, n_exp=1.0)
+    return [dict(p, **extra) for p in S.vep_phases(C=C)]
+
+
+def _law_state(O, g, st, seed=11):
+    """Grain size (0.5-1.5) and melt fraction (0-0.02) on the centres, carried
+    to the vertices as the reference carries centre fields."""
+    rng = np.random.default_rng(seed)
+    nc = (g.Nx - 1) * (g.Nz - 1)
+    st["d0_n"] = rng.uniform(0.5, 1.5, nc)
+    st["phi0_n"] = rng.uniform(0.0, 0.02, nc)
+    st["d0_s"], st["phi0_s"] = O.c2v(g, st["d0_n"]), O.c2v(g, st["phi0_n"])
+
+
+def test_picard_loop_all_creep_laws_matches_oracle(oracle_mod):
+    """The loop with the line search on phases carrying every implemented
+    creep law and per-node grain size and melt fraction (device-resident
+    fields): the same steps, solves and residual history as the oracle's
+    loop, u, v, p to 1e-8."""
+    O = oracle_mod
+    from oracle.nonlinear import picard_loop
+    n = 31
+    g, st = _problem(O, n, seed=4)
+    _law_state(O, g, st)
+    st_gpu = {k: v.copy() for k, v in st.items()}
+    phs = _law_phases()
+    o_ph = []
+    for d in phs:
+        d = dict(d)
+        b = d.pop("beta")
+        o_ph.append(O.OrcPhase(beta_for_K=b, **d))
+    hist, nsolve = picard_loop(g, st, O.OrcRheoModel(**COMMON_LAWS), o_ph, 10, TOL, diag_scaling=1, gz=-1.0,
+                               line_search_on=True, penalty=1e3, max_its_PH=50, rel_tol_KSP=1e-12, max_its_KSP=200,
+                               lin_abs_div=1e-14, lin_rel_div=1e-12, lin_abs_mom=1e-14, lin_rel_mom=1e-12)
+    ctx = mdgpu.Context(n + 1, n + 1, 0.0, 1.0, -1.0, 0.0)
+    ctx.set_bc(g.BCu_t, g.BCu_v, g.BCv_t, g.BCv_v, g.BCp_t, g.BCg_t)
+    eu, ev, ep = g.numbering()[:3]
+    loop = PicardLoop(ctx, eu, ev, ep, [mdgpu.PhaseProp(**d) for d in phs],
+                      mdgpu.RheoParams(average=0, **COMMON_LAWS), device=True)
+    ghist, gsolve, _ = loop.run(st_gpu, 10, TOL, diag_scaling=1, gz=-1.0, line_search=True, rel_tol=1e-11,
+                                max_its=400)
+    print("alphas", st["alphas"], "history", np.array(hist)[:, 0] / hist[0][0], flush=True)
+    assert st_gpu["alphas"] == st["alphas"] and gsolve == nsolve and len(ghist) == len(hist)
+    np.testing.assert_allclose(np.array(ghist), np.array(hist), rtol=1e-6, atol=1e-15)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    for k in ("u", "v", "p", "eta_n"):
+        assert rel(st_gpu[k], st[k]) < 1e-8, k
+
+
 def test_predictor_line_search_matches_oracle(oracle_mod):
     """pc_line_search (StokesRoutines.c:636-668): the longest step is tried
     first; the GPU loop takes the same steps as the oracle's, including the
