@@ -77,8 +77,8 @@ def test_shear_band_box_1023_physical_newton(oracle_mod):
     the line search switch to Newton once the residual fell tenfold
     (Main_DOODZ.c:884-896), and the loop reaches the 1e-8 non-linear
     tolerance within its 10 iterations, the Newton steps converging
-    quadratically; the solver runs the benchmark's configuration (vertex-patch
-    smoother, penalty 100, degree 12) and every solve meets 1e-8."""
+    quadratically; the solver runs the vertex-patch smoother (penalty 100,
+    degree 12 on every level) and every solve meets 1e-8."""
     ctx, loop, st, info = build(oracle_mod, 1023, device=True, pre=0.8, **PHYSICAL)
     t0 = time.time()
     hist, nsolve, stats = loop.run(st, 10, (1e-12, 1e-8, 1e-12, 1e-8), diag_scaling=1, line_search=True,
